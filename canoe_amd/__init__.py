@@ -1,0 +1,381 @@
+"""canoe_amd -- MI355X-native canoe dynamical-core step.
+
+Python host mirror of the reference's operator interface for this path,
+over the C-ABI of libcanoe_gpu.so (include/canoe_gpu.h):
+
+  Hydro.calculate_fluxes        <- Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch)
+  Hydro.riemann_solver          <- Hydro::RiemannSolver  (src/snap/riemann/lmars.cpp:19-22)
+  Hydro.add_flux_divergence     <- Hydro::AddFluxDivergence
+  ImplicitSolver.solve_implicit <- ImplicitSolver::SolveImplicit3D (src/snap/implicit/solve_implicit_3d.cpp:19)
+  Thermo.forward                <- kintera ThermoY::forward (src/tasklist/implicit_hydro_tasks.cpp:308)
+  Run.step                      <- ImplicitHydroTasks RK3 step (src/tasklist/implicit_hydro_tasks.cpp:134-313)
+
+The HIP library is required: every entry point raises if it is missing or if
+no GPU is present; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
+
+NGHOST = 3
+MAX_NMASS = 3
+
+OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
+    ERR_SIZE, ERR_NODEV = range(9)
+BC_REFLECT, BC_PERIODIC, BC_OUTFLOW = 0, 1, 2
+RS_LMARS, RS_HLLC = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
+SATADJ_OFF, SATADJ_NEWTON, SATADJ_MLP = 0, 1, 2
+
+
+class Thermo(ctypes.Structure):
+    _fields_ = [
+        ("nmass", ctypes.c_int), ("nvapor", ctypes.c_int),
+        ("gammad", ctypes.c_double), ("rd", ctypes.c_double),
+        ("cv_ratio_m1", ctypes.c_double * MAX_NMASS),
+        ("inv_mu_ratio_m1", ctypes.c_double * MAX_NMASS),
+        ("ivap", ctypes.c_int), ("icld", ctypes.c_int),
+        ("t3", ctypes.c_double), ("p3", ctypes.c_double),
+        ("beta", ctypes.c_double), ("delta", ctypes.c_double),
+        ("latent", ctypes.c_double), ("tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+    ]
+
+
+class Block(ctypes.Structure):
+    _fields_ = [
+        ("nx1", ctypes.c_int), ("nx2", ctypes.c_int), ("nx3", ctypes.c_int),
+        ("nmass", ctypes.c_int),
+        ("x1min", ctypes.c_double), ("x2min", ctypes.c_double),
+        ("x3min", ctypes.c_double),
+        ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
+        ("dx3", ctypes.c_double),
+        ("grav", ctypes.c_double),
+        ("bc", ctypes.c_int * 6),
+    ]
+
+    # AthenaArray geometry
+    @property
+    def nc1(self):
+        return self.nx1 + 2 * NGHOST
+
+    @property
+    def nc2(self):
+        return self.nx2 + 2 * NGHOST if self.nx2 > 1 else 1
+
+    @property
+    def nc3(self):
+        return self.nx3 + 2 * NGHOST if self.nx3 > 1 else 1
+
+    @property
+    def N(self):
+        return 5 + self.nmass
+
+    def shape(self):
+        """(N, nc3, nc2, nc1) of a cell-centred array."""
+        return (self.N, self.nc3, self.nc2, self.nc1)
+
+    def flux_shapes(self):
+        N, c3, c2, c1 = self.shape()
+        return ((N, c3, c2, c1 + 1), (N, c3, c2 + 1, c1), (N, c3 + 1, c2, c1))
+
+    def interior(self):
+        """Slices (k, j, i) of the interior cells."""
+        g = NGHOST
+        sj = slice(g, g + self.nx2) if self.nx2 > 1 else slice(0, 1)
+        sk = slice(g, g + self.nx3) if self.nx3 > 1 else slice(0, 1)
+        return (sk, sj, slice(g, g + self.nx1))
+
+    def ncells(self):
+        return self.nx1 * self.nx2 * self.nx3
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("order_h", ctypes.c_int), ("order_v", ctypes.c_int),
+        ("rsolver", ctypes.c_int), ("implicit_flag", ctypes.c_int),
+        ("precision", ctypes.c_int), ("satadj", ctypes.c_int),
+        ("visc", ctypes.c_double), ("p0", ctypes.c_double),
+        ("heat_flux", ctypes.c_double),
+    ]
+
+
+class CanoeGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("canoe_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_SIGS = {
+    "cg_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "cg_destroy": (_I, [_P]),
+    "cg_synchronize": (_I, [_P]),
+    "cg_last_error": (ctypes.c_char_p, [_P]),
+    "cg_malloc": (_I, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "cg_free": (_I, [_P, _P]),
+    "cg_memcpy_h2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "cg_memcpy_d2h": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "cg_riemann": (_I, [_P, _P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
+                        ctypes.c_ssize_t]),
+    "cg_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "cg_add_flux_divergence": (_I, [_P, _P, _D, _P, _P, _P, _P]),
+    "cg_implicit_solve": (_I, [_P, _P, _P, _P, _P, _D, _I]),
+    "cg_cons_to_prim": (_I, [_P, _P, _P, _P, _P]),
+    "cg_prim_to_cons": (_I, [_P, _P, _P, _P, _P]),
+    "cg_apply_bc": (_I, [_P, _P, _P]),
+    "cg_saturation_adjust": (_I, [_P, _P, _I, _P, _P, _P, _P, _P]),
+    "cg_run_create": (_I, [_P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "cg_run_destroy": (_I, [_P]),
+    "cg_run_set_prim_host": (_I, [_P, _P]),
+    "cg_run_get_cons_host": (_I, [_P, _P]),
+    "cg_run_get_prim_host": (_I, [_P, _P]),
+    "cg_run_step": (_I, [_P, _D, _I, _I]),
+    "cg_run_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "cg_run_sums": (_I, [_P, _P]),
+    "cg_run_set_timing": (_I, [_P, _I]),
+    "cg_run_kernel_ms": (_I, [_P, _I, ctypes.POINTER(_D)]),
+    "cg_version": (ctypes.c_char_p, []),
+    "cg_apply_bc_host": (_I, [_P, _P]),
+    "cg_pgen_setup": (_I, [_I, _I, _I, _I, _P, _P, _P]),
+    "cg_pgen_init": (_I, [_I, ctypes.c_uint64, _P, _P, _P, _P]),
+}
+
+
+def lib():
+    """The loaded libcanoe_gpu.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CanoeGpuError(
+                ERR_NODEV, "%s not built (run python -m canoe_amd.build)"
+                % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _ref(s):
+    return ctypes.cast(ctypes.pointer(s), ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+def pgen_setup(config, nx1=0, nx2=0, nx3=0):
+    """(Block, Thermo, Config) of configuration `config` (1, 2 or 4)."""
+    b, t, c = Block(), Thermo(), Config()
+    r = lib().cg_pgen_setup(config, nx1, nx2, nx3, _ref(b), _ref(t), _ref(c))
+    if r:
+        raise CanoeGpuError(r, "cg_pgen_setup(%d)" % config)
+    return b, t, c
+
+
+def pgen_init(config, seed, blk, th, cfg):
+    """Seeded primitive state (N, nc3, nc2, nc1), ghosts filled."""
+    w = np.zeros(blk.shape(), dtype=np.float64)
+    r = lib().cg_pgen_init(config, seed, _ref(blk), _ref(th), _ref(cfg),
+                           _ptr(w))
+    if r:
+        raise CanoeGpuError(r, "cg_pgen_init(%d)" % config)
+    return w
+
+
+def apply_bc_host(blk, w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    r = lib().cg_apply_bc_host(_ref(blk), _ptr(w))
+    if r:
+        raise CanoeGpuError(r, "cg_apply_bc_host")
+    return w
+
+
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A device buffer owned by a Context, with the shape of a numpy array."""
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        ctx._check(lib().cg_malloc(ctx.h, max(self.nbytes, 8),
+                                   ctypes.byref(p)), "cg_malloc")
+        self.ptr = p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        self.ctx._check(lib().cg_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a),
+                                            self.nbytes), "h2d")
+        return self
+
+    def download(self):
+        a = np.empty(self.shape, dtype=self.dtype)
+        self.ctx._check(lib().cg_memcpy_d2h(self.ctx.h, _ptr(a), self.ptr,
+                                            self.nbytes), "d2h")
+        return a
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h is not None:
+            lib().cg_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+
+class Context:
+    """One HIP device + stream (cg_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        r = lib().cg_create(device, ctypes.byref(h))
+        if r:
+            raise CanoeGpuError(r, "cg_create(device=%d)" % device)
+        self.h = h
+
+    def _check(self, r, what):
+        if r:
+            msg = lib().cg_last_error(self.h)
+            raise CanoeGpuError(r, "%s: %s" % (what, msg.decode()
+                                               if msg else ""))
+
+    def array(self, a=None, shape=None, dtype=np.float64):
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=dtype)
+            return DeviceArray(self, a.shape, dtype).upload(a)
+        return DeviceArray(self, shape, dtype)
+
+    def close(self):
+        if self.h is not None:
+            lib().cg_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+# ---------------------------------------------------------------------------
+class Hydro:
+    """Flux sweep of one MeshBlock (host buffers in, host buffers out)."""
+
+    def __init__(self, ctx, blk, th, cfg):
+        self.ctx, self.blk, self.th, self.cfg = ctx, blk, th, cfg
+
+    def riemann_solver(self, ivx, wl, wr):
+        """Hydro::RiemannSolver on wl, wr (N, n_faces) -> flx (N, n_faces)."""
+        wl = np.ascontiguousarray(wl, np.float64)
+        wr = np.ascontiguousarray(wr, np.float64)
+        N, nf = wl.shape
+        dl, dr = self.ctx.array(wl), self.ctx.array(wr)
+        df = self.ctx.array(shape=(N, nf))
+        self.ctx._check(lib().cg_riemann(
+            self.ctx.h, _ref(self.th), self.cfg.rsolver, ivx, nf, dl.ptr,
+            dr.ptr, nf, df.ptr, nf), "cg_riemann")
+        out = df.download()
+        for a in (dl, dr, df):
+            a.free()
+        return out
+
+    def calculate_fluxes(self, w):
+        """Hydro::CalculateFluxes(w) -> (flux1, flux2, flux3)."""
+        dw = self.ctx.array(w)
+        fs = [self.ctx.array(np.zeros(s)) for s in self.blk.flux_shapes()]
+        self.ctx._check(lib().cg_calculate_fluxes(
+            self.ctx.h, _ref(self.blk), _ref(self.th), _ref(self.cfg), dw.ptr,
+            fs[0].ptr, fs[1].ptr, fs[2].ptr), "cg_calculate_fluxes")
+        out = tuple(f.download() for f in fs)
+        for a in [dw] + fs:
+            a.free()
+        return out
+
+    def add_flux_divergence(self, wght, fluxes, du):
+        dfs = [self.ctx.array(f) for f in fluxes]
+        ddu = self.ctx.array(du)
+        self.ctx._check(lib().cg_add_flux_divergence(
+            self.ctx.h, _ref(self.blk), wght, dfs[0].ptr, dfs[1].ptr,
+            dfs[2].ptr, ddu.ptr), "cg_add_flux_divergence")
+        out = ddu.download()
+        for a in dfs + [ddu]:
+            a.free()
+        return out
+
+
+class ImplicitSolver:
+    def __init__(self, ctx, blk, th, flag=9):
+        self.ctx, self.blk, self.th, self.flag = ctx, blk, th, flag
+
+    def solve_implicit_3d(self, du, w, dt):
+        """ImplicitSolver::SolveImplicit3D(du, w, dt): returns the new du."""
+        ddu, dw = self.ctx.array(du), self.ctx.array(w)
+        self.ctx._check(lib().cg_implicit_solve(
+            self.ctx.h, _ref(self.blk), _ref(self.th), ddu.ptr, dw.ptr, dt,
+            self.flag), "cg_implicit_solve")
+        out = ddu.download()
+        ddu.free()
+        dw.free()
+        return out
+
+
+class Run:
+    """A device-resident MeshBlock advanced by full SSPRK(3,3) steps."""
+
+    def __init__(self, ctx, blk, th, cfg):
+        self.ctx, self.blk, self.th, self.cfg = ctx, blk, th, cfg
+        h = ctypes.c_void_p()
+        ctx._check(lib().cg_run_create(ctx.h, _ref(blk), _ref(th), _ref(cfg),
+                                       ctypes.byref(h)), "cg_run_create")
+        self.h = h
+
+    def set_prim(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.shape == self.blk.shape()
+        self.ctx._check(lib().cg_run_set_prim_host(self.h, _ptr(w)), "set_prim")
+
+    def cons(self):
+        u = np.zeros(self.blk.shape())
+        self.ctx._check(lib().cg_run_get_cons_host(self.h, _ptr(u)), "get_cons")
+        return u
+
+    def prim(self):
+        w = np.zeros(self.blk.shape())
+        self.ctx._check(lib().cg_run_get_prim_host(self.h, _ptr(w)), "get_prim")
+        return w
+
+    def step(self, dt, nsteps=1, graph=True):
+        self.ctx._check(lib().cg_run_step(self.h, dt, nsteps, int(graph)),
+                        "cg_run_step")
+
+    def sums(self):
+        out = np.zeros(self.blk.N)
+        self.ctx._check(lib().cg_run_sums(self.h, _ptr(out)), "cg_run_sums")
+        return out
+
+    def set_timing(self, on):
+        self.ctx._check(lib().cg_run_set_timing(self.h, int(on)), "timing")
+
+    def kernel_ms(self, cls):
+        v = ctypes.c_double()
+        self.ctx._check(lib().cg_run_kernel_ms(self.h, cls, ctypes.byref(v)),
+                        "kernel_ms")
+        return v.value
+
+    def close(self):
+        if self.h is not None:
+            lib().cg_run_destroy(self.h)
+            self.h = None

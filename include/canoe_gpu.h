@@ -1,0 +1,203 @@
+/* canoe_gpu.h -- C-ABI of the MI355X-native canoe dynamical-core step.
+ *
+ * Plain C types only (no torch, no HIP types): a host program written
+ * against Athena++/canoe binds these through cgo/ctypes/a C++ shim.
+ *
+ * Conventions (mirroring the reference's AthenaArray contract):
+ *   - A cell-centred array is laid out (n, k, j, i) with i (x1, the vertical)
+ *     fastest, including NGHOST=3 ghost layers in every active direction:
+ *     nc1 = nx1 + 6, nc2 = nx2 > 1 ? nx2 + 6 : 1, nc3 = nx3 > 1 ? nx3 + 6 : 1.
+ *   - Variable index: IDN = 0 (total density in primitives, dry density in
+ *     conserved variables), 1..nmass the mass species (mass fractions in
+ *     primitives, partial densities in conserved), IVX = 1 + nmass, IVY, IVZ,
+ *     IPR = IEN = IVX + 3.  N = 5 + nmass.
+ *   - Face fluxes: flux1 (N, nc3, nc2, nc1 + 1), flux2 (N, nc3, nc2 + 1, nc1),
+ *     flux3 (N, nc3 + 1, nc2, nc1); face i lies between cells i-1 and i.
+ *   - Every pointer argument is a DEVICE pointer unless the function name ends
+ *     in _host.  Every function returns 0 (CG_OK) or a CG_ERR_* code and never
+ *     aborts the process.  A context owns one HIP stream; calls on different
+ *     contexts may run concurrently, calls on one context are serialised.
+ */
+#ifndef CANOE_GPU_H_
+#define CANOE_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CG_NGHOST 3
+#define CG_MAX_NMASS 3
+
+enum cg_status {
+  CG_OK = 0,
+  CG_ERR_ARG = 1,       /* bad argument / unsupported combination */
+  CG_ERR_HIP = 2,       /* HIP runtime error (launch, allocation, copy) */
+  CG_ERR_NAN = 3,       /* NaN produced (flux, state, solve) */
+  CG_ERR_NEGATIVE = 4,  /* negative density or pressure */
+  CG_ERR_PIVOT = 5,     /* singular 5x5 block in the implicit solve */
+  CG_ERR_CONVERGE = 6,  /* saturation-adjustment Newton did not converge */
+  CG_ERR_SIZE = 7,      /* size limit exceeded */
+  CG_ERR_NODEV = 8      /* no HIP device / library built without one */
+};
+
+enum cg_bc { CG_BC_REFLECT = 0, CG_BC_PERIODIC = 1, CG_BC_OUTFLOW = 2 };
+enum cg_rsolver { CG_RS_LMARS = 0, CG_RS_HLLC = 1 };
+enum cg_precision { CG_PREC_FP64 = 0, CG_PREC_FP32 = 1 };
+enum cg_satadj { CG_SATADJ_OFF = 0, CG_SATADJ_NEWTON = 1, CG_SATADJ_MLP = 2 };
+
+/* Species thermodynamics of the ideal-moist EOS (Li 2019 eq. 63).  Replaces
+ * the snap::IdealMoist / kintera tensors that get_gamma() reads
+ * (src/interface/eos.hpp:47-67) and the svp parameters of the condensation
+ * reaction (examples/2019-Li-snap/earth-moist.yaml:37). */
+typedef struct cg_thermo {
+  int nmass;   /* number of mass species (vapours first, then condensates) */
+  int nvapor;  /* species 1..nvapor are vapours, nvapor+1..nmass condensates */
+  double gammad;                          /* dry cp/cv */
+  double rd;                              /* dry gas constant, J/kg/K */
+  double cv_ratio_m1[CG_MAX_NMASS];       /* cv_n / cv_d - 1 */
+  double inv_mu_ratio_m1[CG_MAX_NMASS];   /* mu_d / mu_n - 1 (vapours) */
+  /* saturation adjustment: species ivap (vapour) <-> icld (condensate),
+   * 1-based species indices, 0 = no adjustment */
+  int ivap, icld;
+  double t3, p3, beta, delta; /* svp = p3 exp((1 - t3/T) beta - delta ln(T/t3)) */
+  double latent;              /* internal-energy offset of condensates, J/kg */
+  double tol;                 /* Newton tolerance on T, K */
+  int max_iter;
+} cg_thermo;
+
+/* One MeshBlock: uniform Cartesian spacing, constant gravity along x1. */
+typedef struct cg_block {
+  int nx1, nx2, nx3;         /* interior cells (x1 = vertical) */
+  int nmass;                 /* must equal cg_thermo.nmass */
+  double x1min, x2min, x3min;
+  double dx1, dx2, dx3;      /* for nx3 == 1, dx3 is the slab thickness */
+  double grav;               /* g1, acceleration along +x1 (negative = down) */
+  int bc[6];                 /* ix1, ox1, ix2, ox2, ix3, ox3 (enum cg_bc) */
+} cg_block;
+
+/* Numerics of one step (the reference's <time>/<hydro> input blocks). */
+typedef struct cg_config {
+  int order_h;        /* horizontal reconstruction: 2 = PLM, 5 = WENO5 */
+  int order_v;        /* vertical (x1) reconstruction: 2 = PLM, 5 = WENO5 */
+  int rsolver;        /* enum cg_rsolver (the RSOLVER link-time choice) */
+  int implicit_flag;  /* 0 = explicit, 9 = 5x5 full x1, 1 = 3x3 partial x1 */
+  int precision;      /* enum cg_precision: FP64, or FP32 fluxes + FP64 state */
+  int satadj;         /* enum cg_satadj (last RK stage) */
+  double visc;        /* constant viscosity/diffusivity K, m^2/s (0 = off) */
+  double p0;          /* reference pressure for potential temperature */
+  double heat_flux;   /* W/m^2 deposited in the lowest interior layer */
+} cg_config;
+
+typedef struct cg_ctx cg_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* New context on HIP device `device` with its own stream.  No reference
+ * counterpart (Athena++ state lives in MeshBlock; this owns the stream). */
+int cg_create(int device, cg_ctx** out);
+int cg_destroy(cg_ctx* ctx);
+int cg_synchronize(cg_ctx* ctx);
+/* Last error text of this context (static storage owned by the context). */
+const char* cg_last_error(cg_ctx* ctx);
+/* Device allocation helpers so a C host needs no HIP headers. */
+int cg_malloc(cg_ctx* ctx, size_t bytes, void** ptr);
+int cg_free(cg_ctx* ctx, void* ptr);
+int cg_memcpy_h2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cg_memcpy_d2h(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- Riemann plugin ----------------------------------------------------- *
+ * Replaces void Hydro::RiemannSolver(int k, int j, int il, int iu, int ivx,
+ *   AthenaArray<Real>& wl, AthenaArray<Real>& wr, AthenaArray<Real>& flx,
+ *   const AthenaArray<Real>& dxw)          (src/snap/riemann/lmars.cpp:19-22,
+ *                                           src/snap/riemann/hllc_transform.cpp:25-28)
+ * wl, wr: (N, ld) left/right primitive states of n_faces faces; flx: (N,
+ * flx_ld).  ivx in {IVX, IVY, IVZ} selects the normal velocity.  FP64. */
+int cg_riemann(cg_ctx* ctx, const cg_thermo* th, int solver, int ivx,
+               int n_faces, const double* wl, const double* wr, ptrdiff_t ld,
+               double* flx, ptrdiff_t flx_ld);
+
+/* ---- flux sweep --------------------------------------------------------- *
+ * Replaces Hydro::CalculateFluxes(w, b, bcc, order) as patched by canoe
+ * (patches/19.calculate_fluxes.patch:19-185): hydrostatic decomposition
+ * (Decomposition::ChangeToTemperature / RestoreFromTemperature,
+ * src/snap/decomposition/change_to_temperature.cpp:40-167) around the x1
+ * reconstruction, then the Riemann solver along x1, x2, x3.  `w` must have
+ * its ghost cells filled.  Pure: `w` is not modified (the reference mutates
+ * and restores it).  Fluxes are computed on every face of interior cells. */
+int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                        const cg_config* cfg, const double* w, double* flux1,
+                        double* flux2, double* flux3);
+
+/* Replaces Hydro::AddFluxDivergence(wght, du) (upstream; called from
+ * src/tasklist/implicit_hydro_tasks.cpp:163-164): du -= wght * div(F). */
+int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
+                           const double* flux1, const double* flux2,
+                           const double* flux3, double* du);
+
+/* ---- implicit vertical solve ------------------------------------------- *
+ * Replaces void ImplicitSolver::SolveImplicit3D(AthenaArray<Real>& du,
+ *   AthenaArray<Real>& w, Real dt)     (src/snap/implicit/solve_implicit_3d.cpp:19-20)
+ * for the x1 direction: flag 9 = FullCorrection 5x5 (full_correction.cpp),
+ * flag 1 = PartialCorrection 3x3 (partial_correction.cpp).  All columns of
+ * the block in one launch; du is updated in place. */
+int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                      double* du, const double* w, double dt, int flag);
+
+/* ---- EOS and boundaries ------------------------------------------------- *
+ * Replace EquationOfState::ConservedToPrimitive / PrimitiveToConserved
+ * (src/snap/eos/ideal_moist_hydro.cpp:47-81) on interior cells, in the
+ * closed form of src/snap/thermodynamics/thermodynamics_athena.hpp:153-160. */
+int cg_cons_to_prim(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const double* u, double* w);
+int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const double* w, double* u);
+/* Fills the ghost cells of a primitive array (reflecting / periodic /
+ * outflow), as the upstream BoundaryValues do after CONS2PRIM. */
+int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w);
+
+/* ---- saturation adjustment ---------------------------------------------- *
+ * Replaces kintera pthermo->forward(rho, ie, yfrac)
+ * (src/tasklist/implicit_hydro_tasks.cpp:300-312) on ncell cells: rho total
+ * density, ie internal energy density (in/out), y (nmass, ncell) mass
+ * fractions (in/out).  Newton on T to th->tol.  If mlp_params is non-NULL the
+ * Newton iteration starts from the network's guess (see cg_mlp_*).  iters
+ * (nullable) receives the Newton iteration count per cell. */
+int cg_saturation_adjust(cg_ctx* ctx, const cg_thermo* th, int ncell,
+                         const double* rho, double* ie, double* y,
+                         const void* mlp_params, int* iters);
+
+/* ---- whole step on a device-resident block ------------------------------ *
+ * A run owns u, w, u1, du and the scratch of one MeshBlock.  One step is
+ * the three SSPRK(3,3) stages of ImplicitHydroTasks
+ * (src/tasklist/implicit_hydro_tasks.cpp:134-313,
+ *  src/integrator/integrators.cpp:14-44), captured once as a hipGraph. */
+typedef struct cg_run cg_run;
+int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                  const cg_config* cfg, cg_run** out);
+int cg_run_destroy(cg_run* run);
+/* Primitive state (with or without ghosts filled) from host memory. */
+int cg_run_set_prim_host(cg_run* run, const double* w_host);
+int cg_run_get_cons_host(cg_run* run, double* u_host);
+int cg_run_get_prim_host(cg_run* run, double* w_host);
+/* nsteps steps of size dt (use_graph != 0: replay the captured graph). */
+int cg_run_step(cg_run* run, double dt, int nsteps, int use_graph);
+/* Device pointers of the run's state (for in-place use by a host code). */
+int cg_run_device_state(cg_run* run, double** u, double** w);
+/* FP64 global sums of the conserved variables over interior cells times the
+ * cell volume (the reference's `hst` history totals).  out: N doubles. */
+int cg_run_sums(cg_run* run, double* out_host);
+/* Average duration (ms) of the named kernel class over the last timed
+ * cg_run_step call with timing on: 0 = flux, 1 = implicit, 2 = update. */
+int cg_run_set_timing(cg_run* run, int on);
+int cg_run_kernel_ms(cg_run* run, int kernel_class, double* ms);
+
+/* Library build string (arch, flags). */
+const char* cg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CANOE_GPU_H_ */

@@ -1,0 +1,1376 @@
+// oracle.cpp -- CPU FP64 oracle of the canoe dynamical-core step.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg, never by the product library.
+//
+// Restates, in the reference's own loop structure (per-block (k,j) rows,
+// per-column block Thomas with a 5x5 partial-pivot inverse), the algorithms
+// of chengcli/canoe cited at each function.  Where the reference's arithmetic
+// lives in code that is not in /root/reference (athenapp's reconstruction and
+// flux divergence, snap's EOS, kintera's saturation adjustment) the function
+// states the definition adopted and why.
+//
+// Parity pins (tests/test_oracle_*.py): closed forms, invariants, brute-force
+// dense solves and the reference's WENO5 known answer.  Built with
+// -O2 -ffp-contract=off so that no multiply-add is fused.
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "orc_math.hpp"
+
+namespace orc {
+
+constexpr int NG = CG_NGHOST;
+constexpr double TINY_NUMBER = 1.0e-20;  // Athena++ athena.hpp
+
+static int g_threads = 1;
+
+struct Grid {
+  int nx1, nx2, nx3, nc1, nc2, nc3;
+  int is, ie, js, je, ks, ke;
+  int nmass, N, IVX, IVY, IVZ, IPR;
+  size_t ncell;
+  bool f2, f3;
+  explicit Grid(const cg_block& b) {
+    nx1 = b.nx1;
+    nx2 = b.nx2;
+    nx3 = b.nx3;
+    f2 = nx2 > 1;
+    f3 = nx3 > 1;
+    nc1 = nx1 + 2 * NG;
+    nc2 = f2 ? nx2 + 2 * NG : 1;
+    nc3 = f3 ? nx3 + 2 * NG : 1;
+    is = NG;
+    ie = NG + nx1 - 1;
+    js = f2 ? NG : 0;
+    je = js + nx2 - 1;
+    ks = f3 ? NG : 0;
+    ke = ks + nx3 - 1;
+    nmass = b.nmass;
+    N = 5 + nmass;
+    IVX = 1 + nmass;
+    IVY = IVX + 1;
+    IVZ = IVX + 2;
+    IPR = IVX + 3;
+    ncell = (size_t)nc1 * nc2 * nc3;
+  }
+  size_t at(int n, int k, int j, int i) const {
+    return (((size_t)n * nc3 + k) * nc2 + j) * nc1 + i;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// EOS helpers
+// get_gamma: src/interface/eos.hpp:47-67 (Li 2019 eq. 63)
+double get_gamma(const cg_thermo& th, const double* u, double rho = 1.) {
+  double fsig = 1., feps = 1.;
+  for (int n = 1; n <= th.nvapor; ++n) {
+    fsig += u[n] / rho * th.cv_ratio_m1[n - 1];
+    feps += u[n] / rho * th.inv_mu_ratio_m1[n - 1];
+  }
+  for (int n = 1 + th.nvapor; n <= th.nmass; ++n) {
+    fsig += u[n] / rho * th.cv_ratio_m1[n - 1];
+    feps -= u[n] / rho;
+  }
+  return 1. + (th.gammad - 1.) * feps / fsig;
+}
+
+// EquationOfState::SoundSpeed: src/snap/eos/ideal_moist_hydro.cpp:87-91
+double sound_speed(const cg_thermo& th, const double* prim, int IPR) {
+  double gamma = get_gamma(th, prim);
+  return std::sqrt(gamma * prim[IPR] / prim[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers
+
+// LMARS: src/snap/riemann/lmars.cpp:19-85 (Cartesian metric: vec_lower = id)
+void lmars(const cg_thermo& th, int ivx, int il, int iu, const double* wl,
+           const double* wr, ptrdiff_t ld, double* flx, ptrdiff_t fld) {
+  const int nmass = th.nmass, IVX = 1 + nmass, IPR = IVX + 3, N = 5 + nmass;
+  const int IDN = 0, IEN = IPR;
+  int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  double wli[5 + CG_MAX_NMASS], wri[5 + CG_MAX_NMASS];
+  for (int i = il; i <= iu; ++i) {
+    for (int n = 0; n < N; ++n) {
+      wli[n] = wl[n * ld + i];
+      wri[n] = wr[n * ld + i];
+    }
+    double kappal = 1. / (get_gamma(th, wli) - 1.);
+    double kappar = 1. / (get_gamma(th, wri) - 1.);
+    double hl = wli[IPR] / wli[IDN] * (kappal + 1.) +
+                0.5 * (wli[ivx] * wli[ivx] + wli[ivy] * wli[ivy] +
+                       wli[ivz] * wli[ivz]);
+    double hr = wri[IPR] / wri[IDN] * (kappar + 1.) +
+                0.5 * (wri[ivx] * wri[ivx] + wri[ivy] * wri[ivy] +
+                       wri[ivz] * wri[ivz]);
+    double rhobar = 0.5 * (wli[IDN] + wri[IDN]);
+    double cbar = std::sqrt(0.5 * (1. + (1. / kappar + 1. / kappal) / 2.) *
+                            (wli[IPR] + wri[IPR]) / rhobar);
+    double pbar = 0.5 * (wli[IPR] + wri[IPR]) +
+                  0.5 * (rhobar * cbar) * (wli[ivx] - wri[ivx]);
+    double ubar = 0.5 * (wli[ivx] + wri[ivx]) +
+                  0.5 / (rhobar * cbar) * (wli[IPR] - wri[IPR]);
+    double rdl = 1., rdr = 1.;
+    for (int n = 1; n < IVX; ++n) {
+      rdl -= wli[n];
+      rdr -= wri[n];
+    }
+    if (ubar > 0.) {
+      flx[IDN * fld + i] = ubar * wli[IDN] * rdl;
+      for (int n = 1; n < IVX; ++n) flx[n * fld + i] = ubar * wli[IDN] * wli[n];
+      flx[ivx * fld + i] = ubar * wli[IDN] * wli[ivx] + pbar;
+      flx[ivy * fld + i] = ubar * wli[IDN] * wli[ivy];
+      flx[ivz * fld + i] = ubar * wli[IDN] * wli[ivz];
+      flx[IEN * fld + i] = ubar * wli[IDN] * hl;
+    } else {
+      flx[IDN * fld + i] = ubar * wri[IDN] * rdr;
+      for (int n = 1; n < IVX; ++n) flx[n * fld + i] = ubar * wri[IDN] * wri[n];
+      flx[ivx * fld + i] = ubar * wri[IDN] * wri[ivx] + pbar;
+      flx[ivy * fld + i] = ubar * wri[IDN] * wri[ivy];
+      flx[ivz * fld + i] = ubar * wri[IDN] * wri[ivz];
+      flx[IEN * fld + i] = ubar * wri[IDN] * hr;
+    }
+  }
+}
+
+// HLLC: src/snap/riemann/hllc_transform.cpp:25-233 (GENERAL_EOS false,
+// no AFFINE / CUBED_SPHERE projection)
+void hllc(const cg_thermo& th, int ivx, int il, int iu, const double* wl,
+          const double* wr, ptrdiff_t ld, double* flx, ptrdiff_t fld) {
+  const int nmass = th.nmass, IVX = 1 + nmass, IVY = IVX + 1, IVZ = IVX + 2;
+  const int IPR = IVX + 3, IDN = 0, IEN = IPR;
+  int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  double wli[5 + CG_MAX_NMASS], wri[5 + CG_MAX_NMASS];
+  double flxi[5 + CG_MAX_NMASS], fl[5 + CG_MAX_NMASS], fr[5 + CG_MAX_NMASS];
+  for (int i = il; i <= iu; ++i) {
+    wli[IDN] = wl[IDN * ld + i];
+    wli[IVX] = wl[ivx * ld + i];
+    wli[IVY] = wl[ivy * ld + i];
+    wli[IVZ] = wl[ivz * ld + i];
+    wli[IPR] = wl[IPR * ld + i];
+    wri[IDN] = wr[IDN * ld + i];
+    wri[IVX] = wr[ivx * ld + i];
+    wri[IVY] = wr[ivy * ld + i];
+    wri[IVZ] = wr[ivz * ld + i];
+    wri[IPR] = wr[IPR * ld + i];
+    for (int n = 1; n < IVX; ++n) {
+      wli[n] = wl[n * ld + i];
+      wri[n] = wr[n * ld + i];
+    }
+    double gammal = get_gamma(th, wli);
+    double kappal = 1. / (gammal - 1.);
+    double gammar = get_gamma(th, wri);
+    double kappar = 1. / (gammar - 1.);
+
+    double cl = sound_speed(th, wli, IPR);
+    double cr = sound_speed(th, wri, IPR);
+    double KE_l = 0.5 * wli[IDN] *
+                  (sqr(wli[IVX]) + sqr(wli[IVY]) + sqr(wli[IVZ]));
+    double KE_r = 0.5 * wri[IDN] *
+                  (sqr(wri[IVX]) + sqr(wri[IVY]) + sqr(wri[IVZ]));
+    double el = wli[IPR] * kappal + KE_l;
+    double er = wri[IPR] * kappar + KE_r;
+    double rhoa = .5 * (wli[IDN] + wri[IDN]);
+    double ca = .5 * (cl + cr);
+    double pmid = .5 * (wli[IPR] + wri[IPR] + (wli[IVX] - wri[IVX]) * rhoa * ca);
+    double umid =
+        .5 * (wli[IVX] + wri[IVX] + (wli[IPR] - wri[IPR]) / (rhoa * ca));
+
+    double ql = (pmid <= wli[IPR])
+                    ? 1.0
+                    : std::sqrt(1.0 + (gammal + 1) / (2 * gammal) *
+                                          (pmid / wli[IPR] - 1.0));
+    double qr = (pmid <= wri[IPR])
+                    ? 1.0
+                    : std::sqrt(1.0 + (gammar + 1) / (2 * gammar) *
+                                          (pmid / wri[IPR] - 1.0));
+    double al = wli[IVX] - cl * ql;
+    double ar = wri[IVX] + cr * qr;
+    double bp = ar > 0.0 ? ar : (TINY_NUMBER);
+    double bm = al < 0.0 ? al : -(TINY_NUMBER);
+
+    double vxl = wli[IVX] - al;
+    double vxr = wri[IVX] - ar;
+    double tl = wli[IPR] + vxl * wli[IDN] * wli[IVX];
+    double tr = wri[IPR] + vxr * wri[IDN] * wri[IVX];
+    double ml = wli[IDN] * vxl;
+    double mr = -(wri[IDN] * vxr);
+    double am = (tl - tr) / (ml + mr);
+    double cp = (ml * tr + mr * tl) / (ml + mr);
+    cp = cp > 0.0 ? cp : 0.0;
+
+    vxl = wli[IVX] - bm;
+    vxr = wri[IVX] - bp;
+    double rdl = 1., rdr = 1.;
+    for (int n = 1; n < IVX; ++n) {
+      rdl -= wli[n];
+      rdr -= wri[n];
+    }
+    fl[IDN] = wli[IDN] * vxl * rdl;
+    fr[IDN] = wri[IDN] * vxr * rdr;
+    for (int n = 1; n < IVX; ++n) {
+      fl[n] = wli[IDN] * wli[n] * vxl;
+      fr[n] = wri[IDN] * wri[n] * vxr;
+    }
+    fl[IVX] = wli[IDN] * wli[IVX] * vxl + wli[IPR];
+    fr[IVX] = wri[IDN] * wri[IVX] * vxr + wri[IPR];
+    fl[IVY] = wli[IDN] * wli[IVY] * vxl;
+    fr[IVY] = wri[IDN] * wri[IVY] * vxr;
+    fl[IVZ] = wli[IDN] * wli[IVZ] * vxl;
+    fr[IVZ] = wri[IDN] * wri[IVZ] * vxr;
+    fl[IEN] = el * vxl + wli[IPR] * wli[IVX];
+    fr[IEN] = er * vxr + wri[IPR] * wri[IVX];
+
+    double sl, sr, sm;
+    if (am >= 0.0) {
+      sl = am / (am - bm);
+      sr = 0.0;
+      sm = -bm / (am - bm);
+    } else {
+      sl = 0.0;
+      sr = -am / (bp - am);
+      sm = bp / (bp - am);
+    }
+    for (int n = 0; n < IVX; ++n) flxi[n] = sl * fl[n] + sr * fr[n];
+    flxi[IVX] = sl * fl[IVX] + sr * fr[IVX] + sm * cp;
+    flxi[IVY] = sl * fl[IVY] + sr * fr[IVY];
+    flxi[IVZ] = sl * fl[IVZ] + sr * fr[IVZ];
+    flxi[IEN] = sl * fl[IEN] + sr * fr[IEN] + sm * cp * am;
+
+    for (int n = 0; n < IVX; ++n) flx[n * fld + i] = flxi[n];
+    flx[ivx * fld + i] = flxi[IVX];
+    flx[ivy * fld + i] = flxi[IVY];
+    flx[ivz * fld + i] = flxi[IVZ];
+    flx[IEN * fld + i] = flxi[IEN];
+  }
+}
+
+void riemann(const cg_thermo& th, int solver, int ivx, int il, int iu,
+             const double* wl, const double* wr, ptrdiff_t ld, double* flx,
+             ptrdiff_t fld) {
+  if (solver == CG_RS_HLLC)
+    hllc(th, ivx, il, iu, wl, wr, ld, flx, fld);
+  else
+    lmars(th, ivx, il, iu, wl, wr, ld, flx, fld);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction.  athenapp's PiecewiseLinearX* / Weno5X* are not in the
+// container; the definitions adopted (call sites patches/19:29-39):
+//  PLM: van Leer limited slope on a uniform mesh (Athena++ plm.cpp's
+//       uniform-Cartesian branch), face offset exactly 1/2.
+//  WENO5: Jiang & Shu (1996) on primitive variables, eps = 1e-6, both faces
+//       of a cell from the same three smoothness indicators.  Known answer:
+//       tests/test_glog.cpp:28-37 (1,2,3,4,5 -> 2.5 at the face between 2, 3).
+void plm_cell(double qm1, double q0, double qp1, double* qminus,
+              double* qplus) {
+  double dql = q0 - qm1;
+  double dqr = qp1 - q0;
+  double dq2 = dql * dqr;
+  double dqm = 2.0 * dq2 / (dql + dqr);
+  if (dq2 <= 0.0) dqm = 0.0;
+  *qplus = q0 + 0.5 * dqm;
+  *qminus = q0 - 0.5 * dqm;
+}
+
+void weno5_cell(double a, double b, double c, double d, double e,
+                double* qminus, double* qplus) {
+  const double eps = 1.0e-6;
+  double b0 = (13.0 / 12.0) * sqr(a - 2.0 * b + c) +
+              0.25 * sqr(a - 4.0 * b + 3.0 * c);
+  double b1 = (13.0 / 12.0) * sqr(b - 2.0 * c + d) + 0.25 * sqr(b - d);
+  double b2 = (13.0 / 12.0) * sqr(c - 2.0 * d + e) +
+              0.25 * sqr(3.0 * c - 4.0 * d + e);
+  double g0 = 1.0 / sqr(eps + b0);
+  double g1 = 1.0 / sqr(eps + b1);
+  double g2 = 1.0 / sqr(eps + b2);
+  // upper face c+1/2: linear weights (0.1, 0.6, 0.3)
+  double p0 = (2.0 * a - 7.0 * b + 11.0 * c) * (1.0 / 6.0);
+  double p1 = (-b + 5.0 * c + 2.0 * d) * (1.0 / 6.0);
+  double p2 = (2.0 * c + 5.0 * d - e) * (1.0 / 6.0);
+  double a0 = 0.1 * g0, a1 = 0.6 * g1, a2 = 0.3 * g2;
+  *qplus = (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+  // lower face c-1/2: mirrored weights (0.3, 0.6, 0.1)
+  double m0 = (2.0 * c + 5.0 * b - a) * (1.0 / 6.0);
+  double m1 = (-d + 5.0 * c + 2.0 * b) * (1.0 / 6.0);
+  double m2 = (2.0 * e - 7.0 * d + 11.0 * c) * (1.0 / 6.0);
+  double c0 = 0.3 * g0, c1 = 0.6 * g1, c2 = 0.1 * g2;
+  *qminus = (c0 * m0 + c1 * m1 + c2 * m2) / (c0 + c1 + c2);
+}
+
+void recon_cell(int order, const double* q, double* qm, double* qp) {
+  if (order == 5)
+    weno5_cell(q[0], q[1], q[2], q[3], q[4], qm, qp);
+  else
+    plm_cell(q[1], q[2], q[3], qm, qp);
+}
+
+// Reconstruct along a line: base points at cell 0 of the line, stride s.
+// For cells c in [cl, cu]: wl[c+1] = upper-face value, wr[c] = lower-face.
+void recon_line(int order, const double* base, ptrdiff_t s, int cl, int cu,
+                double* wl, double* wr) {
+  for (int c = cl; c <= cu; ++c) {
+    double q[5];
+    for (int m = -2; m <= 2; ++m) {
+      if (order != 5 && (m == -2 || m == 2)) {
+        q[m + 2] = 0.0;
+        continue;
+      }
+      q[m + 2] = base[(ptrdiff_t)(c + m) * s];
+    }
+    double qm, qp;
+    recon_cell(order, q, &qm, &qp);
+    wl[c + 1] = qp;
+    wr[c] = qm;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hydrostatic decomposition of one column (k,j):
+// Decomposition::ChangeToTemperature
+//   src/snap/decomposition/change_to_temperature.cpp:40-136
+// (single block: no neighbours; reflecting walls per block bcs)
+// Writes psf (nc1+1), psv (nc1), rdv (nc1) and, if wcol != nullptr,
+// overwrites wcol[IDN], wcol[IPR] (stride 1 columns) with T and p - psv.
+void decompose_column(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                      const double* w, int k, int j, double* psf, double* psv,
+                      double* rdv) {
+  const double grav = blk.grav;
+  const double Rd = th.rd;
+  const double dx = blk.dx1;
+  const int is = g.is, ie = g.ie;
+  auto W = [&](int n, int i) { return w[g.at(n, k, j, i)]; };
+  // isothermal extrapolation to the top face (:58-63)
+  {
+    double RdTv = W(g.IPR, ie) / W(0, ie);
+    psf[ie + 1] = W(g.IPR, ie) * cg_exp(grav * dx / (2 * RdTv));
+  }
+  // IntegrateDownwards(psf, w, grav, is, ie) (:30-38, :65)
+  for (int i = ie; i >= is; --i) psf[i] = psf[i + 1] - grav * W(0, i) * dx;
+  // bottom ghosts (:71-76)
+  double gb = (blk.bc[0] == CG_BC_REFLECT) ? -grav : grav;
+  for (int i = is - 1; i >= is - NG; --i)
+    psf[i] = psf[i + 1] - gb * W(0, i) * dx;
+  // top ghosts (:78-82): IntegrateUpwards
+  double gt = (blk.bc[1] == CG_BC_REFLECT) ? -grav : grav;
+  for (int i = ie + 1; i <= ie + NG; ++i)
+    psf[i + 1] = psf[i] + gt * W(0, i) * dx;
+  // (:92-111)
+  for (int i = is - NG; i <= ie + NG; ++i) {
+    if (std::fabs(psf[i] - psf[i + 1]) < 1.E-6)
+      psv[i] = 0.5 * (psf[i] + psf[i + 1]);
+    else
+      psv[i] = (psf[i] - psf[i + 1]) / cg_log(psf[i] / psf[i + 1]);
+    double feps = 1.;
+    for (int n = 1; n <= th.nvapor; ++n)
+      feps += W(n, i) * th.inv_mu_ratio_m1[n - 1];
+    rdv[i] = Rd * feps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch:19-185), pure.
+void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                      const cg_config& cfg, const double* w, double* f1,
+                      double* f2, double* f3) {
+  const int N = g.N, IPR = g.IPR, nc1 = g.nc1;
+  const bool decomp = blk.grav != 0.;
+  // ---- x1 ----
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j) {
+      std::vector<double> col((size_t)N * nc1), wl((size_t)N * (nc1 + 1)),
+          wr((size_t)N * (nc1 + 1)), psf(nc1 + 1), psv(nc1), rdv(nc1);
+      for (int n = 0; n < N; ++n)
+        for (int i = 0; i < nc1; ++i) col[n * nc1 + i] = w[g.at(n, k, j, i)];
+      if (decomp) {
+        decompose_column(g, blk, th, w, k, j, psf.data(), psv.data(),
+                         rdv.data());
+        // change density to temperature, pressure to perturbation (:107-110)
+        for (int i = g.is - NG; i <= g.ie + NG; ++i) {
+          col[0 * nc1 + i] = col[IPR * nc1 + i] / (col[0 * nc1 + i] * rdv[i]);
+          col[IPR * nc1 + i] -= psv[i];
+        }
+      }
+      for (int n = 0; n < N; ++n)
+        recon_line(cfg.order_v, &col[n * nc1], 1, g.is - 1, g.ie + 1,
+                   &wl[n * (nc1 + 1)], &wr[n * (nc1 + 1)]);
+      if (decomp) {
+        // RestoreFromTemperature faces (:155-164)
+        for (int i = g.is; i <= g.ie + 1; ++i) {
+          double* wrp = &wr[IPR * (nc1 + 1) + i];
+          double* wlp = &wl[IPR * (nc1 + 1) + i];
+          *wrp += psf[i];
+          if (*wrp < 0.) *wrp = psf[i];
+          double gamma = std::sqrt(rdv[i] * rdv[i - 1]);
+          wr[0 * (nc1 + 1) + i] = *wrp / (wr[0 * (nc1 + 1) + i] * gamma);
+          *wlp += psf[i];
+          if (*wlp < 0.) *wlp = psf[i];
+          wl[0 * (nc1 + 1) + i] = *wlp / (wl[0 * (nc1 + 1) + i] * gamma);
+        }
+      }
+      std::vector<double> fx((size_t)N * (nc1 + 1));
+      riemann(th, cfg.rsolver, g.IVX, g.is, g.ie + 1, wl.data(), wr.data(),
+              nc1 + 1, fx.data(), nc1 + 1);
+      for (int n = 0; n < N; ++n)
+        for (int i = g.is; i <= g.ie + 1; ++i)
+          f1[(((size_t)n * g.nc3 + k) * g.nc2 + j) * (nc1 + 1) + i] =
+              fx[n * (nc1 + 1) + i];
+    }
+  // ---- x2 ----
+  if (g.f2) {
+    const ptrdiff_t s2 = nc1;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int k = g.ks; k <= g.ke; ++k) {
+      int nf = g.nx2 + 1;
+      std::vector<double> wl((size_t)N * nf), wr((size_t)N * nf),
+          fx((size_t)N * nf), tl(g.nc2 + 1), tr(g.nc2 + 1);
+      for (int i = g.is; i <= g.ie; ++i) {
+        for (int n = 0; n < N; ++n) {
+          recon_line(cfg.order_h, &w[g.at(n, k, 0, i)], s2, g.js - 1, g.je + 1,
+                     tl.data(), tr.data());
+          for (int j = g.js; j <= g.je + 1; ++j) {
+            wl[n * nf + (j - g.js)] = tl[j];
+            wr[n * nf + (j - g.js)] = tr[j];
+          }
+        }
+        riemann(th, cfg.rsolver, g.IVY, 0, nf - 1, wl.data(), wr.data(), nf,
+                fx.data(), nf);
+        for (int n = 0; n < N; ++n)
+          for (int j = g.js; j <= g.je + 1; ++j)
+            f2[(((size_t)n * g.nc3 + k) * (g.nc2 + 1) + j) * nc1 + i] =
+                fx[n * nf + (j - g.js)];
+      }
+    }
+  }
+  // ---- x3 ----
+  if (g.f3) {
+    const ptrdiff_t s3 = (ptrdiff_t)nc1 * g.nc2;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int j = g.js; j <= g.je; ++j) {
+      int nf = g.nx3 + 1;
+      std::vector<double> wl((size_t)N * nf), wr((size_t)N * nf),
+          fx((size_t)N * nf), tl(g.nc3 + 1), tr(g.nc3 + 1);
+      for (int i = g.is; i <= g.ie; ++i) {
+        for (int n = 0; n < N; ++n) {
+          recon_line(cfg.order_h, &w[g.at(n, 0, j, i)], s3, g.ks - 1, g.ke + 1,
+                     tl.data(), tr.data());
+          for (int k = g.ks; k <= g.ke + 1; ++k) {
+            wl[n * nf + (k - g.ks)] = tl[k];
+            wr[n * nf + (k - g.ks)] = tr[k];
+          }
+        }
+        riemann(th, cfg.rsolver, g.IVZ, 0, nf - 1, wl.data(), wr.data(), nf,
+                fx.data(), nf);
+        for (int n = 0; n < N; ++n)
+          for (int k = g.ks; k <= g.ke + 1; ++k)
+            f3[(((size_t)n * (g.nc3 + 1) + k) * g.nc2 + j) * nc1 + i] =
+                fx[n * nf + (k - g.ks)];
+      }
+    }
+  }
+}
+
+// Hydro::AddFluxDivergence (upstream Athena++ add_flux_divergence.cpp, as
+// called at src/tasklist/implicit_hydro_tasks.cpp:163-164), Cartesian:
+//   dflx = A1 F1(i+1) - A1 F1(i) [+ (A2 F2(j+1) - A2 F2(j))] [+ (A3 ..)]
+//   du  -= wght * dflx / vol
+void add_flux_divergence(const Grid& g, const cg_block& blk, double wght,
+                         const double* f1, const double* f2, const double* f3,
+                         double* du) {
+  const double A1 = blk.dx2 * blk.dx3, A2 = blk.dx1 * blk.dx3,
+               A3 = blk.dx1 * blk.dx2;
+  const double vol = blk.dx1 * blk.dx2 * blk.dx3;
+  const int nc1 = g.nc1;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int n = 0; n < g.N; ++n)
+        for (int i = g.is; i <= g.ie; ++i) {
+          size_t b1 = (((size_t)n * g.nc3 + k) * g.nc2 + j) * (nc1 + 1);
+          double dflx = A1 * f1[b1 + i + 1] - A1 * f1[b1 + i];
+          if (g.f2) {
+            size_t b2 = (((size_t)n * g.nc3 + k) * (g.nc2 + 1) + j) * nc1;
+            dflx += A2 * f2[b2 + nc1 + i] - A2 * f2[b2 + i];
+          }
+          if (g.f3) {
+            size_t b3 = (((size_t)n * (g.nc3 + 1) + k) * g.nc2 + j) * nc1;
+            dflx += A3 * f3[b3 + (size_t)g.nc2 * nc1 + i] - A3 * f3[b3 + i];
+          }
+          du[g.at(n, k, j, i)] -= wght * dflx / vol;
+        }
+}
+
+// temperature of the ideal-moist gas, T = p / (rho Rd feps); snap "W->T"
+// (src/interface/eos.hpp:23-27), closed form of Li 2019.
+double temperature(const cg_thermo& th, const double* w, int IPR) {
+  double feps = 1.;
+  for (int n = 1; n <= th.nvapor; ++n) feps += w[n] * th.inv_mu_ratio_m1[n - 1];
+  for (int n = 1 + th.nvapor; n <= th.nmass; ++n) feps -= w[n];
+  return w[IPR] / (w[0] * th.rd * feps);
+}
+
+// Source terms (HydroSourceTerms::AddSourceTerms order: constant
+// acceleration, then the user source).
+//  gravity: upstream ConstantAcceleration as seen in the context lines of
+//    patches/25.constant_acc.patch (the patch itself is disabled,
+//    cmake/athena.cmake:4-19): src = dt rho g1; m1 += src; E += src v1.
+//  diffusion: examples/2019-Li-snap/straka.cpp:126-190 (x1-x2 plane).
+//  heating: config-2 bottom heat flux into the lowest interior layer.
+void add_sources(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                 const cg_config& cfg, double dt, const double* w,
+                 double* du) {
+  const int IVX = g.IVX, IVY = g.IVY, IEN = g.IPR;
+  const double g1 = blk.grav;
+  if (g1 != 0.0) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        for (int i = g.is; i <= g.ie; ++i) {
+          double src = dt * w[g.at(0, k, j, i)] * g1;
+          du[g.at(IVX, k, j, i)] += src;
+          du[g.at(IEN, k, j, i)] += src * w[g.at(IVX, k, j, i)];
+        }
+  }
+  if (cfg.visc != 0.0 && g.f2) {
+    const double K = cfg.visc, p0 = cfg.p0;
+    const double dx = blk.dx1, dy = blk.dx2;
+    const double cp = th.gammad / (th.gammad - 1.) * th.rd;
+    const double chi = (th.gammad - 1.) / th.gammad;
+    std::vector<double> theta(g.ncell), temp(g.ncell);
+    double wc[5 + CG_MAX_NMASS];
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js - 1; j <= g.je + 1; ++j)
+        for (int i = g.is - 1; i <= g.ie + 1; ++i) {
+          for (int n = 0; n < g.N; ++n) wc[n] = w[g.at(n, k, j, i)];
+          size_t c = g.at(0, k, j, i);
+          temp[c] = temperature(th, wc, g.IPR);
+          theta[c] = temp[c] * cg_pow(p0 / wc[g.IPR], chi);
+        }
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        for (int i = g.is; i <= g.ie; ++i) {
+          auto Wv = [&](int n, int jj, int ii) { return w[g.at(n, k, jj, ii)]; };
+          auto T = [&](int jj, int ii) { return theta[g.at(0, k, jj, ii)]; };
+          double rho = Wv(0, j, i);
+          du[g.at(IVX, k, j, i)] +=
+              dt * K * rho / (dx * dy) *
+              (Wv(IVX, j, i - 1) + Wv(IVX, j, i + 1) + Wv(IVX, j - 1, i) +
+               Wv(IVX, j + 1, i) - 4. * Wv(IVX, j, i));
+          du[g.at(IVY, k, j, i)] +=
+              dt * K * rho / (dx * dy) *
+              (Wv(IVY, j, i - 1) + Wv(IVY, j, i + 1) + Wv(IVY, j - 1, i) +
+               Wv(IVY, j + 1, i) - 4. * Wv(IVY, j, i));
+          double tc = temp[g.at(0, k, j, i)], thc = T(j, i);
+          du[g.at(IEN, k, j, i)] += dt * K * rho / (dx * dy) * cp * tc / thc *
+                                    (T(j + 1, i) + T(j - 1, i) + T(j, i + 1) +
+                                     T(j, i - 1) - 4. * thc);
+        }
+  }
+  if (cfg.heat_flux != 0.0) {
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        du[g.at(IEN, k, j, g.is)] += dt * cfg.heat_flux / blk.dx1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Implicit vertical solver (x1).  Small fixed-size matrices replace Eigen.
+template <int B>
+struct Mat {
+  double a[B][B];
+  void zero() {
+    for (int r = 0; r < B; ++r)
+      for (int c = 0; c < B; ++c) a[r][c] = 0.;
+  }
+  void ident() {
+    zero();
+    for (int r = 0; r < B; ++r) a[r][r] = 1.;
+  }
+};
+template <int B>
+struct Vec {
+  double v[B];
+};
+
+template <int B>
+Mat<B> mul(const Mat<B>& x, const Mat<B>& y) {
+  Mat<B> z;
+  for (int r = 0; r < B; ++r)
+    for (int c = 0; c < B; ++c) {
+      double s = x.a[r][0] * y.a[0][c];
+      for (int k = 1; k < B; ++k) s += x.a[r][k] * y.a[k][c];
+      z.a[r][c] = s;
+    }
+  return z;
+}
+template <int B>
+Vec<B> mul(const Mat<B>& x, const Vec<B>& y) {
+  Vec<B> z;
+  for (int r = 0; r < B; ++r) {
+    double s = x.a[r][0] * y.v[0];
+    for (int k = 1; k < B; ++k) s += x.a[r][k] * y.v[k];
+    z.v[r] = s;
+  }
+  return z;
+}
+
+// Inverse by Gauss-Jordan elimination with partial (row) pivoting: the
+// pivot of column c is the first row r >= c of largest |a[r][c]|; the pivot
+// row is scaled by the reciprocal of the pivot.  Eigen's .inverse() of the
+// reference (forward_backward.hpp:167,171,198) is a partial-pivot LU; the
+// two agree to round-off.  Returns false if a pivot is exactly zero.
+template <int B>
+bool inverse(Mat<B> m, Mat<B>* out) {
+  Mat<B> x;
+  x.ident();
+  for (int c = 0; c < B; ++c) {
+    int p = c;
+    for (int r = c + 1; r < B; ++r)
+      if (std::fabs(m.a[r][c]) > std::fabs(m.a[p][c])) p = r;
+    if (m.a[p][c] == 0.0) return false;
+    if (p != c)
+      for (int q = 0; q < B; ++q) {
+        std::swap(m.a[c][q], m.a[p][q]);
+        std::swap(x.a[c][q], x.a[p][q]);
+      }
+    double inv = 1.0 / m.a[c][c];
+    for (int q = 0; q < B; ++q) {
+      m.a[c][q] = m.a[c][q] * inv;
+      x.a[c][q] = x.a[c][q] * inv;
+    }
+    for (int r = 0; r < B; ++r) {
+      if (r == c) continue;
+      double f = m.a[r][c];
+      for (int q = 0; q < B; ++q) {
+        m.a[r][q] = m.a[r][q] - f * m.a[c][q];
+        x.a[r][q] = x.a[r][q] - f * x.a[c][q];
+      }
+    }
+  }
+  *out = x;
+  return true;
+}
+
+// RoeAverage: src/snap/implicit/flux_decomposition.hpp:34-67
+void roe_average(const Grid& g, double* prim, double gm1, const double* wl,
+                 const double* wr) {
+  const int IVX = g.IVX, IVY = g.IVY, IVZ = g.IVZ, IPR = g.IPR;
+  double sqrtdl = std::sqrt(wl[0]);
+  double sqrtdr = std::sqrt(wr[0]);
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  prim[0] = sqrtdl * sqrtdr;
+  prim[IVX] = (sqrtdl * wl[IVX] + sqrtdr * wr[IVX]) * isdlpdr;
+  prim[IVY] = (sqrtdl * wl[IVY] + sqrtdr * wr[IVY]) * isdlpdr;
+  prim[IVZ] = (sqrtdl * wl[IVZ] + sqrtdr * wr[IVZ]) * isdlpdr;
+  for (int i = 1; i < IVX; ++i)
+    prim[i] = (sqrtdl * wl[i] + sqrtdr * wr[i]) * isdlpdr;
+  double el = wl[IPR] / gm1 +
+              0.5 * wl[0] * (sqr(wl[IVX]) + sqr(wl[IVY]) + sqr(wl[IVZ]));
+  double er = wr[IPR] / gm1 +
+              0.5 * wr[0] * (sqr(wr[IVX]) + sqr(wr[IVY]) + sqr(wr[IVZ]));
+  double hbar = ((el + wl[IPR]) / sqrtdl + (er + wr[IPR]) / sqrtdr) * isdlpdr;
+  prim[IPR] =
+      (hbar - 0.5 * (sqr(prim[IVX]) + sqr(prim[IVY]) + sqr(prim[IVZ]))) * gm1 /
+      (gm1 + 1.) * prim[0];
+}
+
+// |A| = R |Lambda| R^-1 at a face: Eigenvalue / Eigenvector
+// (flux_decomposition.hpp:69-99), product order (R Lambda) R^-1.
+Mat<5> abs_roe_matrix(const Grid& g, const cg_thermo& th, const double* prim,
+                      double gm1) {
+  const int IPR = g.IPR;
+  double cs = sound_speed(th, prim, IPR);
+  double r = prim[0];
+  double u = prim[g.IVX];
+  double v = prim[g.IVY];
+  double w = prim[g.IVZ];
+  double p = prim[IPR];
+  Mat<5> L;
+  L.zero();
+  L.a[0][0] = std::fabs(u - cs);
+  L.a[1][1] = std::fabs(u);
+  L.a[2][2] = std::fabs(u + cs);
+  L.a[3][3] = std::fabs(u);
+  L.a[4][4] = std::fabs(u);
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / r;
+  double h = hp + ke;
+  Mat<5> R, Ri;
+  double Rv[25] = {1.,          1.,  1.,          0., 0.,  //
+                   u - cs,      u,   u + cs,      0., 0.,  //
+                   v,           v,   v,           1., 0.,  //
+                   w,           w,   w,           0., 1.,  //
+                   h - u * cs, ke, h + u * cs, v,  w};
+  double Riv[25] = {(cs * ke + u * hp) / (2. * cs * hp),
+                    (-hp - cs * u) / (2. * cs * hp),
+                    -v / (2. * hp),
+                    -w / (2. * hp),
+                    1. / (2. * hp),
+                    (hp - ke) / hp,
+                    u / hp,
+                    v / hp,
+                    w / hp,
+                    -1. / hp,
+                    (cs * ke - u * hp) / (2. * cs * hp),
+                    (hp - cs * u) / (2. * cs * hp),
+                    -v / (2. * hp),
+                    -w / (2. * hp),
+                    1. / (2. * hp),
+                    -v,
+                    0.,
+                    1.,
+                    0.,
+                    0.,
+                    -w,
+                    0.,
+                    0.,
+                    1.,
+                    0.};
+  for (int q = 0; q < 25; ++q) {
+    R.a[q / 5][q % 5] = Rv[q];
+    Ri.a[q / 5][q % 5] = Riv[q];
+  }
+  return mul(mul(R, L), Ri);
+}
+
+// FluxJacobian: flux_decomposition.hpp:101-120 (dir = X1)
+Mat<5> flux_jacobian(const Grid& g, double gm1, const double* w) {
+  double v1 = w[g.IVX];
+  double v2 = w[g.IVY];
+  double v3 = w[g.IVZ];
+  double rho = w[0];
+  double pres = w[g.IPR];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
+  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  double d[25] = {0,
+                  1.,
+                  0.,
+                  0.,
+                  0.,
+                  gm1 * s2 / 2 - v1 * v1,
+                  (2. - gm1) * v1,
+                  -gm1 * v2,
+                  -gm1 * v3,
+                  gm1,
+                  -v1 * v2,
+                  v2,
+                  v1,
+                  0.,
+                  0.,
+                  -v1 * v3,
+                  v3,
+                  0.,
+                  v1,
+                  0.,
+                  c1,
+                  c2,
+                  -gm1 * v2 * v1,
+                  -gm1 * v3 * v1,
+                  (gm1 + 1) * v1};
+  Mat<5> m;
+  for (int q = 0; q < 25; ++q) m.a[q / 5][q % 5] = d[q];
+  return m;
+}
+
+template <int B>
+Mat<B> sub5(const Mat<5>& m) {
+  Mat<B> s;
+  if (B == 5) {
+    for (int r = 0; r < B; ++r)
+      for (int c = 0; c < B; ++c) s.a[r][c] = m.a[r][c];
+  } else {
+    // PartialCorrection picks rows/cols (idn, ivx, ien)
+    // (src/snap/implicit/partial_correction.cpp:80-82, 96-97)
+    const int id[3] = {0, 1, 4};
+    for (int r = 0; r < B; ++r)
+      for (int c = 0; c < B; ++c) s.a[r][c] = m.a[id[r]][id[c]];
+  }
+  return s;
+}
+
+// FullCorrection (src/snap/implicit/full_correction.cpp:32-161) and
+// PartialCorrection (partial_correction.cpp:26-170) for one x1 column, with
+// ForwardSweep / BackwardSubstitution (forward_backward.hpp:14-137).
+// Single block (first_block = last_block = true), no pole.
+template <int B>
+bool correction_column(const Grid& g, const cg_block& blk,
+                       const cg_thermo& th, double* du, const double* w,
+                       double dt, int k, int j) {
+  const int is = g.is, ie = g.ie, nc = g.nc1, N = g.N;
+  const int IVX = g.IVX, IEN = g.IPR;
+  const double grav = blk.grav;
+  Mat<B> Phi, Dt, Bnd;
+  Phi.zero();
+  if (B == 5) {
+    Phi.a[1][0] = grav;
+    Phi.a[4][1] = grav;
+  } else {
+    Phi.a[1][0] = grav;
+    Phi.a[2][1] = grav;
+  }
+  Dt.zero();
+  for (int d = 0; d < B; ++d) Dt.a[d][d] = 1. / dt;
+  Bnd.ident();
+  Bnd.a[1][1] = -1;
+  const bool periodic = (blk.bc[0] == CG_BC_PERIODIC);
+  std::vector<Mat<B>> a(nc), b(nc), c(nc), dfdq(nc);
+  std::vector<Vec<B>> delta(nc);
+  const double gm1 = th.gammad - 1.;
+  double wl[5 + CG_MAX_NMASS], wr[5 + CG_MAX_NMASS], prim[5 + CG_MAX_NMASS];
+  auto load = [&](double* dst, int i) {
+    for (int n = 0; n < N; ++n) dst[n] = w[g.at(n, k, j, i)];
+  };
+  for (int i = is - 2; i <= ie + 1; ++i) {
+    load(wr, i);
+    dfdq[i] = sub5<B>(flux_jacobian(g, gm1, wr));
+  }
+  load(wl, is - 2);
+  load(wr, is - 1);
+  double gm = 0.5 * (gm1 + gm1);
+  roe_average(g, prim, gm, wl, wr);
+  Mat<B> Am = sub5<B>(abs_roe_matrix(g, th, prim, gm));
+  const double aleft = blk.dx2 * blk.dx3, aright = blk.dx2 * blk.dx3;
+  const double vol = blk.dx1 * blk.dx2 * blk.dx3;
+  for (int i = is - 1; i <= ie; ++i) {
+    load(wl, i);
+    load(wr, i + 1);
+    gm = 0.5 * (gm1 + gm1);
+    roe_average(g, prim, gm, wl, wr);
+    Mat<B> Ap = sub5<B>(abs_roe_matrix(g, th, prim, gm));
+    for (int r = 0; r < B; ++r)
+      for (int q = 0; q < B; ++q) {
+        a[i].a[r][q] = (Am.a[r][q] * aleft + Ap.a[r][q] * aright +
+                        (aright - aleft) * dfdq[i].a[r][q]) /
+                           (2. * vol) +
+                       Dt.a[r][q] - Phi.a[r][q];
+        b[i].a[r][q] = -(Am.a[r][q] + dfdq[i - 1].a[r][q]) * aleft / (2. * vol);
+        c[i].a[r][q] = -(Ap.a[r][q] - dfdq[i + 1].a[r][q]) * aright / (2. * vol);
+      }
+    Am = Ap;
+  }
+  if (!periodic) {
+    Mat<B> t = mul(b[is], Bnd);
+    for (int r = 0; r < B; ++r)
+      for (int q = 0; q < B; ++q) a[is].a[r][q] += t.a[r][q];
+    t = mul(c[ie], Bnd);
+    for (int r = 0; r < B; ++r)
+      for (int q = 0; q < B; ++q) a[ie].a[r][q] += t.a[r][q];
+  }
+  // ForwardSweep (forward_backward.hpp:14-84)
+  auto rhs_at = [&](int i) {
+    Vec<B> rhs;
+    double r0 = du[g.at(0, k, j, i)];
+    for (int n = 1; n < IVX; ++n) r0 += du[g.at(n, k, j, i)];
+    r0 /= dt;
+    rhs.v[0] = r0;
+    if (B == 5) {
+      rhs.v[1] = du[g.at(IVX, k, j, i)] / dt;
+      rhs.v[2] = du[g.at(IVX + 1, k, j, i)] / dt;
+      rhs.v[3] = du[g.at(IVX + 2, k, j, i)] / dt;
+      rhs.v[4] = du[g.at(IEN, k, j, i)] / dt;
+    } else {
+      rhs.v[1] = du[g.at(IVX, k, j, i)] / dt;
+      rhs.v[2] = du[g.at(IEN, k, j, i)] / dt;
+    }
+    return rhs;
+  };
+  {
+    Vec<B> rhs = rhs_at(is);
+    Mat<B> inv;
+    if (!inverse(a[is], &inv)) return false;
+    a[is] = inv;
+    delta[is] = mul(a[is], rhs);
+    a[is] = mul(a[is], c[is]);
+  }
+  for (int i = is + 1; i <= ie; ++i) {
+    Vec<B> rhs = rhs_at(i);
+    Mat<B> ba = mul(b[i], a[i - 1]);
+    Mat<B> m;
+    for (int r = 0; r < B; ++r)
+      for (int q = 0; q < B; ++q) m.a[r][q] = a[i].a[r][q] - ba.a[r][q];
+    Mat<B> inv;
+    if (!inverse(m, &inv)) return false;
+    a[i] = inv;
+    Vec<B> bd = mul(b[i], delta[i - 1]);
+    Vec<B> t;
+    for (int r = 0; r < B; ++r) t.v[r] = rhs.v[r] - bd.v[r];
+    delta[i] = mul(a[i], t);
+    a[i] = mul(a[i], c[i]);
+  }
+  // BackwardSubstitution (forward_backward.hpp:86-137)
+  for (int i = ie - 1; i >= is; --i) {
+    Vec<B> ad = mul(a[i], delta[i + 1]);
+    for (int r = 0; r < B; ++r) delta[i].v[r] = delta[i].v[r] - ad.v[r];
+  }
+  for (int i = is; i <= ie; ++i) {
+    double dens = du[g.at(0, k, j, i)];
+    for (int n = 1; n < IVX; ++n) dens += du[g.at(n, k, j, i)];
+    dens = delta[i].v[0] - dens;
+    du[g.at(0, k, j, i)] = delta[i].v[0];
+    for (int n = 1; n < IVX; ++n)
+      du[g.at(n, k, j, i)] += dens * w[g.at(n, k, j, i)];
+    if (B == 5) {
+      du[g.at(IVX, k, j, i)] = delta[i].v[1];
+      du[g.at(IVX + 1, k, j, i)] = delta[i].v[2];
+      du[g.at(IVX + 2, k, j, i)] = delta[i].v[3];
+      du[g.at(IEN, k, j, i)] = delta[i].v[4];
+    } else {
+      du[g.at(IVX, k, j, i)] = delta[i].v[1];
+      du[g.at(IEN, k, j, i)] = delta[i].v[2];
+    }
+    for (int n = 1; n < IVX; ++n)
+      du[g.at(0, k, j, i)] -= du[g.at(n, k, j, i)];
+  }
+  return true;
+}
+
+// ImplicitSolver::SolveImplicit3D, x1 branch (solve_implicit_3d.cpp:91-142)
+int implicit_solve(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                   double* du, const double* w, double dt, int flag) {
+  if (flag == 0) return CG_OK;
+  if ((flag & 6) != 0) return CG_ERR_ARG;  // x2/x3 implicit not supported
+  if (blk.bc[0] == CG_BC_PERIODIC) return CG_ERR_ARG;
+  int err = CG_OK;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j) {
+      bool ok = (flag & 8) ? correction_column<5>(g, blk, th, du, w, dt, k, j)
+                           : correction_column<3>(g, blk, th, du, w, dt, k, j);
+      if (!ok) {
+#pragma omp atomic write
+        err = CG_ERR_PIVOT;
+      }
+    }
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// EOS, closed form (src/snap/thermodynamics/thermodynamics_athena.hpp:153-160
+// IntEngToPres / PresToIntEng; snap's U->W / W->U are not in the container).
+void cons_to_prim_cell(const cg_thermo& th, int nmass, const double* u,
+                       double* w) {
+  const int IVX = 1 + nmass, IPR = IVX + 3;
+  double rho = u[0];
+  for (int n = 1; n <= nmass; ++n) rho += u[n];
+  for (int n = 1; n <= nmass; ++n) w[n] = u[n] / rho;
+  w[0] = rho;
+  double v1 = u[IVX] / rho, v2 = u[IVX + 1] / rho, v3 = u[IVX + 2] / rho;
+  double ke = 0.5 * (u[IVX] * v1 + u[IVX + 1] * v2 + u[IVX + 2] * v3);
+  double gamma = get_gamma(th, w);
+  w[IVX] = v1;
+  w[IVX + 1] = v2;
+  w[IVX + 2] = v3;
+  w[IPR] = (gamma - 1.) * (u[IPR] - ke);
+}
+
+void prim_to_cons_cell(const cg_thermo& th, int nmass, const double* w,
+                       double* u) {
+  const int IVX = 1 + nmass, IPR = IVX + 3;
+  double rho = w[0];
+  double qd = 1.;
+  for (int n = 1; n <= nmass; ++n) qd -= w[n];
+  u[0] = rho * qd;
+  for (int n = 1; n <= nmass; ++n) u[n] = rho * w[n];
+  double v1 = w[IVX], v2 = w[IVX + 1], v3 = w[IVX + 2];
+  u[IVX] = rho * v1;
+  u[IVX + 1] = rho * v2;
+  u[IVX + 2] = rho * v3;
+  double gamma = get_gamma(th, w);
+  u[IPR] = w[IPR] / (gamma - 1.) + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
+}
+
+void cons_to_prim(const Grid& g, const cg_thermo& th, const double* u,
+                  double* w) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        double uc[5 + CG_MAX_NMASS], wc[5 + CG_MAX_NMASS];
+        for (int n = 0; n < g.N; ++n) uc[n] = u[g.at(n, k, j, i)];
+        cons_to_prim_cell(th, g.nmass, uc, wc);
+        for (int n = 0; n < g.N; ++n) w[g.at(n, k, j, i)] = wc[n];
+      }
+}
+
+void prim_to_cons(const Grid& g, const cg_thermo& th, const double* w,
+                  double* u) {
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        double uc[5 + CG_MAX_NMASS], wc[5 + CG_MAX_NMASS];
+        for (int n = 0; n < g.N; ++n) wc[n] = w[g.at(n, k, j, i)];
+        prim_to_cons_cell(th, g.nmass, wc, uc);
+        for (int n = 0; n < g.N; ++n) u[g.at(n, k, j, i)] = uc[n];
+      }
+}
+
+// Ghost cells of primitives (upstream BoundaryValues: reflecting flips the
+// normal velocity; x1 first over interior (j,k), then x2 over all i, then x3
+// over all i, j).
+void apply_bc(const Grid& g, const cg_block& blk, double* w) {
+  const int N = g.N;
+  auto fill = [&](int dir, int bc_in, int bc_out, int lo, int hi, auto idx,
+                  int vn) {
+    // idx(n, c) -> address along direction `dir` at coordinate c
+    for (int n = 0; n < N; ++n) {
+      double sgn = (n == vn) ? -1.0 : 1.0;
+      for (int gh = 1; gh <= NG; ++gh) {
+        if (bc_in == CG_BC_REFLECT)
+          w[idx(n, lo - gh)] = sgn * w[idx(n, lo + gh - 1)];
+        else if (bc_in == CG_BC_PERIODIC)
+          w[idx(n, lo - gh)] = w[idx(n, hi - gh + 1)];
+        else
+          w[idx(n, lo - gh)] = w[idx(n, lo)];
+        if (bc_out == CG_BC_REFLECT)
+          w[idx(n, hi + gh)] = sgn * w[idx(n, hi - gh + 1)];
+        else if (bc_out == CG_BC_PERIODIC)
+          w[idx(n, hi + gh)] = w[idx(n, lo + gh - 1)];
+        else
+          w[idx(n, hi + gh)] = w[idx(n, hi)];
+      }
+    }
+    (void)dir;
+  };
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      fill(0, blk.bc[0], blk.bc[1], g.is, g.ie,
+           [&](int n, int c) { return g.at(n, k, j, c); }, g.IVX);
+  if (g.f2)
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int i = 0; i < g.nc1; ++i)
+        fill(1, blk.bc[2], blk.bc[3], g.js, g.je,
+             [&](int n, int c) { return g.at(n, k, c, i); }, g.IVY);
+  if (g.f3)
+    for (int j = 0; j < g.nc2; ++j)
+      for (int i = 0; i < g.nc1; ++i)
+        fill(2, blk.bc[4], blk.bc[5], g.ks, g.ke,
+             [&](int n, int c) { return g.at(n, c, j, i); }, g.IVZ);
+}
+
+// ---------------------------------------------------------------------------
+// Saturation adjustment.  The live one is kintera's ThermoY::forward
+// (external, src/tasklist/implicit_hydro_tasks.cpp:300-312); the legacy
+// EquilibrateUV (src/snap/thermodynamics/equilibrate_uv.cpp:12-73) fixes
+// its semantics: at fixed density and internal energy, move mass between
+// vapour and condensate until the vapour is saturated (or no condensate
+// is left).  Definition adopted (kintera's exact model is unavailable):
+//   sensible internal energy  ie = rho cv_mix(q) T            (p = (g-1) ie)
+//   moist internal energy     ie - rho L q_c  is held fixed
+//   saturation                q_sat = e_s(T) / (rho R_v T),
+//   e_s = p3 exp((1 - t3/T) beta - delta ln(T/t3))  (the "ideal" formula,
+//   earth-moist.yaml:37; delta = 0 is Clausius-Clapeyron with L = beta R_v t3)
+// Newton on T to |dT| < tol; ie is returned with the latent release added.
+struct SatParams {
+  double cvd, cvv, cvc, rv, L, t3, p3, beta, delta;
+  int iv, ic;
+};
+
+SatParams sat_params(const cg_thermo& th) {
+  SatParams s;
+  s.iv = th.ivap - 1;
+  s.ic = th.icld - 1;
+  s.cvd = th.rd / (th.gammad - 1.);
+  s.cvv = s.cvd * (1. + th.cv_ratio_m1[s.iv]);
+  s.cvc = s.cvd * (1. + th.cv_ratio_m1[s.ic]);
+  s.rv = th.rd * (1. + th.inv_mu_ratio_m1[s.iv]);
+  s.L = th.latent;
+  s.t3 = th.t3;
+  s.p3 = th.p3;
+  s.beta = th.beta;
+  s.delta = th.delta;
+  return s;
+}
+
+double qsat(const SatParams& s, double T, double rho) {
+  double es = s.p3 * cg_exp((1. - s.t3 / T) * s.beta - s.delta * cg_log(T / s.t3));
+  return es / (rho * s.rv * T);
+}
+
+// one cell; q (nmass) in/out, returns iterations or -1 if not converged
+int adjust_cell(const cg_thermo& th, const SatParams& s, double rho,
+                double* ie, double* q, const double* tguess) {
+  double cvd = s.cvd;
+  double qd = 1.;
+  for (int n = 0; n < th.nmass; ++n) qd -= q[n];
+  double cfix = cvd * qd;
+  for (int n = 0; n < th.nmass; ++n) {
+    if (n == s.iv || n == s.ic) continue;
+    cfix += cvd * (1. + th.cv_ratio_m1[n]) * q[n];
+  }
+  double qt = q[s.iv] + q[s.ic];
+  double em = *ie / rho - s.L * q[s.ic];
+  double tu = em / (cfix + s.cvv * qt);
+  double qc, T;
+  int it = 0;
+  if (qt <= qsat(s, tu, rho)) {
+    qc = 0.;
+    T = tu;
+  } else {
+    T = (tguess != nullptr) ? *tguess : tu;
+    bool conv = false;
+    for (it = 1; it <= th.max_iter; ++it) {
+      double qs = qsat(s, T, rho);
+      double dqs = qs * ((s.beta * s.t3 / T - s.delta) / T - 1. / T);
+      double qcn = qt - qs;
+      double cvm = cfix + s.cvv * qs + s.cvc * qcn;
+      double gT = cvm * T - s.L * qcn - em;
+      double gp = cvm + dqs * (s.L - (s.cvc - s.cvv) * T);
+      double dT = gT / gp;
+      T = T - dT;
+      if (std::fabs(dT) < th.tol) {
+        conv = true;
+        break;
+      }
+    }
+    if (!conv) return -1;
+    qc = qt - qsat(s, T, rho);
+  }
+  *ie = *ie + rho * s.L * (qc - q[s.ic]);
+  q[s.iv] = qt - qc;
+  q[s.ic] = qc;
+  return it;
+}
+
+int saturation_adjust(const cg_thermo& th, int ncell, const double* rho,
+                      double* ie, double* y, const double* tguess,
+                      int* iters) {
+  if (th.ivap <= 0 || th.icld <= 0) return CG_OK;
+  SatParams s = sat_params(th);
+  int err = CG_OK;
+  for (int c = 0; c < ncell; ++c) {
+    double q[CG_MAX_NMASS];
+    for (int n = 0; n < th.nmass; ++n) q[n] = y[(size_t)n * ncell + c];
+    int it = adjust_cell(th, s, rho[c], &ie[c], q,
+                         tguess ? &tguess[c] : nullptr);
+    if (it < 0) err = CG_ERR_CONVERGE;
+    for (int n = 0; n < th.nmass; ++n) y[(size_t)n * ncell + c] = q[n];
+    if (iters) iters[c] = it;
+  }
+  return err;
+}
+
+// UpdateAllConserved (implicit_hydro_tasks.cpp:290-313) on interior cells.
+int update_all_conserved(const Grid& g, const cg_thermo& th, double* u) {
+  if (th.ivap <= 0 || th.icld <= 0) return CG_OK;
+  SatParams s = sat_params(th);
+  int err = CG_OK;
+  const int IVX = g.IVX, IEN = g.IPR;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        double rho = u[g.at(0, k, j, i)];
+        for (int n = 1; n <= g.nmass; ++n) rho += u[g.at(n, k, j, i)];
+        double m1 = u[g.at(IVX, k, j, i)], m2 = u[g.at(IVX + 1, k, j, i)],
+               m3 = u[g.at(IVX + 2, k, j, i)];
+        double ke = 0.5 * (m1 * (m1 / rho) + m2 * (m2 / rho) + m3 * (m3 / rho));
+        double ie = u[g.at(IEN, k, j, i)] - ke;
+        double q[CG_MAX_NMASS];
+        for (int n = 0; n < g.nmass; ++n) q[n] = u[g.at(n + 1, k, j, i)] / rho;
+        int it = adjust_cell(th, s, rho, &ie, q, nullptr);
+        if (it < 0) {
+#pragma omp atomic write
+          err = CG_ERR_CONVERGE;
+        }
+        for (int n = 0; n < g.nmass; ++n) u[g.at(n + 1, k, j, i)] = q[n] * rho;
+        u[g.at(IEN, k, j, i)] = ie + ke;
+      }
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// One SSPRK(3,3) step of ImplicitHydroTasks
+// (src/tasklist/implicit_hydro_tasks.cpp:134-313; weights
+//  src/integrator/integrators.cpp:14-44):
+//   stage s: F = F(w); u1 = u1 + delta u; u = g1 u + g2 u1;
+//            du = -beta dt div F + S(w); du = Implicit(du, w, beta dt);
+//            u += du; [last stage: saturation adjustment]; w = W(u) + bcs.
+int step(const cg_block& blk, const cg_thermo& th, const cg_config& cfg,
+         double dt, int nsteps, double* u, double* w) {
+  Grid g(blk);
+  const size_t nt = g.N * g.ncell;
+  std::vector<double> u1(nt, 0.0), du(nt, 0.0);
+  std::vector<double> f1((size_t)g.N * g.nc3 * g.nc2 * (g.nc1 + 1), 0.0);
+  std::vector<double> f2((size_t)g.N * g.nc3 * (g.nc2 + 1) * g.nc1, 0.0);
+  std::vector<double> f3((size_t)g.N * (g.nc3 + 1) * g.nc2 * g.nc1, 0.0);
+  const double sdelta[3] = {1.0, 0.0, 0.0};
+  const double sg1[3] = {0.0, 0.25, 2. / 3.};
+  const double sg2[3] = {1.0, 0.75, 1. / 3.};
+  const double sbeta[3] = {1.0, 0.25, 2. / 3.};
+  for (int step = 0; step < nsteps; ++step) {
+    std::fill(u1.begin(), u1.end(), 0.0);
+    for (int s = 0; s < 3; ++s) {
+      calculate_fluxes(g, blk, th, cfg, w, f1.data(), f2.data(), f3.data());
+      for (int n = 0; n < g.N; ++n)
+        for (int k = g.ks; k <= g.ke; ++k)
+          for (int j = g.js; j <= g.je; ++j)
+            for (int i = g.is; i <= g.ie; ++i) {
+              size_t c = g.at(n, k, j, i);
+              u1[c] = u1[c] + sdelta[s] * u[c];
+              if (s == 0)
+                u[c] = u1[c];
+              else
+                u[c] = sg1[s] * u[c] + sg2[s] * u1[c];
+              du[c] = 0.0;
+            }
+      double wght = sbeta[s] * dt;
+      add_flux_divergence(g, blk, wght, f1.data(), f2.data(), f3.data(),
+                          du.data());
+      add_sources(g, blk, th, cfg, wght, w, du.data());
+      int err = implicit_solve(g, blk, th, du.data(), w, wght,
+                               cfg.implicit_flag);
+      if (err) return err;
+      for (int n = 0; n < g.N; ++n)
+        for (int k = g.ks; k <= g.ke; ++k)
+          for (int j = g.js; j <= g.je; ++j)
+            for (int i = g.is; i <= g.ie; ++i) {
+              size_t c = g.at(n, k, j, i);
+              u[c] = u[c] + du[c];
+            }
+      if (s == 2 && cfg.satadj != CG_SATADJ_OFF) {
+        err = update_all_conserved(g, th, u);
+        if (err) return err;
+      }
+      cons_to_prim(g, th, u, w);
+      apply_bc(g, blk, w);
+    }
+  }
+  return CG_OK;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C ABI of the oracle
+using namespace orc;
+
+extern "C" {
+
+double or_exp(double x) { return cg_exp(x); }
+double or_log(double x) { return cg_log(x); }
+double or_get_gamma(const cg_thermo* th, const double* w) {
+  return get_gamma(*th, w);
+}
+
+void or_recon_cell(int order, const double* q, double* qm, double* qp) {
+  recon_cell(order, q, qm, qp);
+}
+
+int or_riemann(const cg_thermo* th, int solver, int ivx, int n,
+               const double* wl, const double* wr, ptrdiff_t ld, double* flx,
+               ptrdiff_t fld) {
+  riemann(*th, solver, ivx, 0, n - 1, wl, wr, ld, flx, fld);
+  return CG_OK;
+}
+
+int or_decompose_column(const cg_block* blk, const cg_thermo* th,
+                        const double* w, int k, int j, double* psf,
+                        double* psv, double* rdv) {
+  Grid g(*blk);
+  decompose_column(g, *blk, *th, w, k, j, psf, psv, rdv);
+  return CG_OK;
+}
+
+int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
+                        const cg_config* cfg, const double* w, double* flux1,
+                        double* flux2, double* flux3) {
+  Grid g(*blk);
+  calculate_fluxes(g, *blk, *th, *cfg, w, flux1, flux2, flux3);
+  return CG_OK;
+}
+
+int or_add_flux_divergence(const cg_block* blk, double wght,
+                           const double* flux1, const double* flux2,
+                           const double* flux3, double* du) {
+  Grid g(*blk);
+  add_flux_divergence(g, *blk, wght, flux1, flux2, flux3, du);
+  return CG_OK;
+}
+
+int or_add_sources(const cg_block* blk, const cg_thermo* th,
+                   const cg_config* cfg, double dt, const double* w,
+                   double* du) {
+  Grid g(*blk);
+  add_sources(g, *blk, *th, *cfg, dt, w, du);
+  return CG_OK;
+}
+
+int or_implicit_solve(const cg_block* blk, const cg_thermo* th, double* du,
+                      const double* w, double dt, int flag) {
+  Grid g(*blk);
+  return implicit_solve(g, *blk, *th, du, w, dt, flag);
+}
+
+int or_cons_to_prim(const cg_block* blk, const cg_thermo* th, const double* u,
+                    double* w) {
+  Grid g(*blk);
+  cons_to_prim(g, *th, u, w);
+  return CG_OK;
+}
+
+int or_prim_to_cons(const cg_block* blk, const cg_thermo* th, const double* w,
+                    double* u) {
+  Grid g(*blk);
+  prim_to_cons(g, *th, w, u);
+  return CG_OK;
+}
+
+int or_apply_bc(const cg_block* blk, double* w) {
+  Grid g(*blk);
+  apply_bc(g, *blk, w);
+  return CG_OK;
+}
+
+int or_saturation_adjust(const cg_thermo* th, int ncell, const double* rho,
+                         double* ie, double* y, const double* t_guess,
+                         int* iters) {
+  return saturation_adjust(*th, ncell, rho, ie, y, t_guess, iters);
+}
+
+int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+            double dt, int nsteps, double* u, double* w) {
+  g_threads = 1;
+  return step(*blk, *th, *cfg, dt, nsteps, u, w);
+}
+
+int or_step_omp(const cg_block* blk, const cg_thermo* th,
+                const cg_config* cfg, double dt, int nsteps, double* u,
+                double* w, int nthreads) {
+#ifdef _OPENMP
+  g_threads = nthreads > 0 ? nthreads : omp_get_num_procs();
+#else
+  g_threads = 1;
+#endif
+  int r = step(*blk, *th, *cfg, dt, nsteps, u, w);
+  g_threads = 1;
+  return r;
+}
+
+int or_sums(const cg_block* blk, const double* u, double* out) {
+  Grid g(*blk);
+  const double vol = blk->dx1 * blk->dx2 * blk->dx3;
+  for (int n = 0; n < g.N; ++n) {
+    double s = 0.0;
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        for (int i = g.is; i <= g.ie; ++i) s += u[g.at(n, k, j, i)] * vol;
+    out[n] = s;
+  }
+  return CG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+/* oracle.h -- CPU FP64 oracle of the canoe dynamical-core step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load liboracle.so, and only as the checker
+ * or the timed CPU baseline -- never on the product path.
+ *
+ * Every function restates the reference algorithm on host memory laid out
+ * as in include/canoe_gpu.h (AthenaArray (n,k,j,i), NGHOST = 3) and cites the
+ * reference file:line it follows.  Arithmetic is plain IEEE double without
+ * FMA contraction (built with -ffp-contract=off).
+ */
+#ifndef CANOE_ORACLE_H_
+#define CANOE_ORACLE_H_
+
+#include <stddef.h>
+
+#include "../include/canoe_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* math */
+double or_exp(double x);
+double or_log(double x);
+double or_get_gamma(const cg_thermo* th, const double* w);
+
+/* reconstruction of one cell from its stencil: order 2 uses q[1..3] of a
+ * 5-value window q[0..4] centred on q[2]; order 5 uses all five.  qm is the
+ * value at the cell's lower face, qp at its upper face. */
+void or_recon_cell(int order, const double* q, double* qm, double* qp);
+
+/* Riemann solvers on n faces: wl, wr (N, ld), flx (N, fld). */
+int or_riemann(const cg_thermo* th, int solver, int ivx, int n,
+               const double* wl, const double* wr, ptrdiff_t ld, double* flx,
+               ptrdiff_t fld);
+
+/* Hydrostatic decomposition of one column of nc1 cells (ChangeToTemperature):
+ * outputs psf (nc1 + 1), psv (nc1), rdv (nc1, the R used for T). */
+int or_decompose_column(const cg_block* blk, const cg_thermo* th,
+                        const double* w, int k, int j, double* psf,
+                        double* psv, double* rdv);
+
+int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
+                        const cg_config* cfg, const double* w, double* flux1,
+                        double* flux2, double* flux3);
+int or_add_flux_divergence(const cg_block* blk, double wght,
+                           const double* flux1, const double* flux2,
+                           const double* flux3, double* du);
+int or_add_sources(const cg_block* blk, const cg_thermo* th,
+                   const cg_config* cfg, double dt, const double* w,
+                   double* du);
+int or_implicit_solve(const cg_block* blk, const cg_thermo* th, double* du,
+                      const double* w, double dt, int flag);
+int or_cons_to_prim(const cg_block* blk, const cg_thermo* th, const double* u,
+                    double* w);
+int or_prim_to_cons(const cg_block* blk, const cg_thermo* th, const double* w,
+                    double* u);
+int or_apply_bc(const cg_block* blk, double* w);
+int or_saturation_adjust(const cg_thermo* th, int ncell, const double* rho,
+                         double* ie, double* y, const double* t_guess,
+                         int* iters);
+/* nsteps full SSPRK(3,3) steps of ImplicitHydroTasks on (u, w): u interior is
+ * the state, w (with ghosts) is recomputed from u at the end of each stage. */
+int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+            double dt, int nsteps, double* u, double* w);
+/* Same step with OpenMP over rows/columns (nthreads <= 0: all cores). */
+int or_step_omp(const cg_block* blk, const cg_thermo* th,
+                const cg_config* cfg, double dt, int nsteps, double* u,
+                double* w, int nthreads);
+int or_sums(const cg_block* blk, const double* u, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CANOE_ORACLE_H_ */

@@ -1,0 +1,154 @@
+"""ctypes loader of the CPU oracle (oracle/lib/liboracle.so).
+
+Test infrastructure: the oracle is the checker, never the thing measured.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "lib", "liboracle.so")
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_SIGS = {
+    "or_exp": (_D, [_D]),
+    "or_log": (_D, [_D]),
+    "or_get_gamma": (_D, [_P, _P]),
+    "or_recon_cell": (None, [_I, _P, _P, _P]),
+    "or_riemann": (_I, [_P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
+                        ctypes.c_ssize_t]),
+    "or_decompose_column": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
+    "or_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "or_add_flux_divergence": (_I, [_P, _D, _P, _P, _P, _P]),
+    "or_add_sources": (_I, [_P, _P, _P, _D, _P, _P]),
+    "or_implicit_solve": (_I, [_P, _P, _P, _P, _D, _I]),
+    "or_cons_to_prim": (_I, [_P, _P, _P, _P]),
+    "or_prim_to_cons": (_I, [_P, _P, _P, _P]),
+    "or_apply_bc": (_I, [_P, _P]),
+    "or_saturation_adjust": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+    "or_step": (_I, [_P, _P, _P, _D, _I, _P, _P]),
+    "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
+    "or_sums": (_I, [_P, _P, _P]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        for name, (res, args) in _SIGS.items():
+            try:
+                f = getattr(L, name)
+            except AttributeError:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def ref(s):
+    return ctypes.cast(ctypes.pointer(s), ctypes.c_void_p)
+
+
+def riemann(th, solver, ivx, wl, wr):
+    wl = np.ascontiguousarray(wl, np.float64)
+    wr = np.ascontiguousarray(wr, np.float64)
+    N, nf = wl.shape
+    out = np.zeros_like(wl)
+    lib().or_riemann(ref(th), solver, ivx, nf, ptr(wl), ptr(wr), nf, ptr(out),
+                     nf)
+    return out
+
+
+def calculate_fluxes(blk, th, cfg, w):
+    w = np.ascontiguousarray(w, np.float64)
+    fs = [np.zeros(s) for s in blk.flux_shapes()]
+    lib().or_calculate_fluxes(ref(blk), ref(th), ref(cfg), ptr(w), ptr(fs[0]),
+                              ptr(fs[1]), ptr(fs[2]))
+    return fs
+
+
+def add_flux_divergence(blk, wght, fs, du):
+    du = np.array(du, dtype=np.float64, copy=True)
+    fs = [np.ascontiguousarray(f, np.float64) for f in fs]
+    lib().or_add_flux_divergence(ref(blk), wght, ptr(fs[0]), ptr(fs[1]),
+                                 ptr(fs[2]), ptr(du))
+    return du
+
+
+def add_sources(blk, th, cfg, dt, w, du):
+    du = np.array(du, dtype=np.float64, copy=True)
+    w = np.ascontiguousarray(w, np.float64)
+    lib().or_add_sources(ref(blk), ref(th), ref(cfg), dt, ptr(w), ptr(du))
+    return du
+
+
+def implicit_solve(blk, th, du, w, dt, flag):
+    du = np.array(du, dtype=np.float64, copy=True)
+    w = np.ascontiguousarray(w, np.float64)
+    r = lib().or_implicit_solve(ref(blk), ref(th), ptr(du), ptr(w), dt, flag)
+    return du, r
+
+
+def cons_to_prim(blk, th, u, w=None):
+    u = np.ascontiguousarray(u, np.float64)
+    w = np.zeros_like(u) if w is None else np.array(w, copy=True)
+    lib().or_cons_to_prim(ref(blk), ref(th), ptr(u), ptr(w))
+    return w
+
+
+def prim_to_cons(blk, th, w):
+    w = np.ascontiguousarray(w, np.float64)
+    u = np.zeros_like(w)
+    lib().or_prim_to_cons(ref(blk), ref(th), ptr(w), ptr(u))
+    return u
+
+
+def apply_bc(blk, w):
+    w = np.array(w, dtype=np.float64, copy=True)
+    lib().or_apply_bc(ref(blk), ptr(w))
+    return w
+
+
+def step(blk, th, cfg, dt, nsteps, u, w, threads=1):
+    u = np.array(u, dtype=np.float64, copy=True)
+    w = np.array(w, dtype=np.float64, copy=True)
+    if threads == 1:
+        r = lib().or_step(ref(blk), ref(th), ref(cfg), dt, nsteps, ptr(u),
+                          ptr(w))
+    else:
+        r = lib().or_step_omp(ref(blk), ref(th), ref(cfg), dt, nsteps, ptr(u),
+                              ptr(w), threads)
+    assert r == 0, r
+    return u, w
+
+
+def saturation_adjust(th, rho, ie, y, tguess=None):
+    rho = np.ascontiguousarray(rho, np.float64)
+    ie = np.array(ie, dtype=np.float64, copy=True)
+    y = np.array(y, dtype=np.float64, copy=True)
+    n = rho.size
+    it = np.zeros(n, dtype=np.int32)
+    tg = None if tguess is None else ptr(np.ascontiguousarray(tguess,
+                                                              np.float64))
+    r = lib().or_saturation_adjust(ref(th), n, ptr(rho), ptr(ie), ptr(y), tg,
+                                   ptr(it))
+    return ie, y, it, r

@@ -1,0 +1,121 @@
+"""GPU parity of the FP64 plain path against the CPU oracle.
+
+Every comparison is on the same seeded inputs; the bar is <= 1e-12 per
+entry (helpers.relerr).  The FP64 device code evaluates the same
+expressions in the same order without contraction, so the errors observed
+are expected to be exactly 0.
+"""
+import numpy as np
+import pytest
+
+import canoe_amd as ca
+import oracle_lib as orl
+from helpers import cfl_dt, perturbed_state, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+CASES = [(1, (24, 40, 1)), (2, (20, 12, 10)), (4, (20, 12, 10))]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("nmass", [0, 3])
+@pytest.mark.parametrize("solver", [ca.RS_LMARS, ca.RS_HLLC])
+def test_riemann(ctx, nmass, solver):
+    cfgid = 2 if nmass == 0 else 4
+    blk, th, cfg = ca.pgen_setup(cfgid, 8, 8, 8)
+    cfg.rsolver = solver
+    rng = np.random.default_rng(11 + nmass + 7 * solver)
+    N = 5 + nmass
+    nf = 4099
+    ivx = 1 + nmass
+
+    def states():
+        w = np.zeros((N, nf))
+        w[0] = rng.uniform(0.2, 1.5, nf)
+        for n in range(1, 1 + nmass):
+            w[n] = rng.uniform(0, 0.01, nf)
+        for d in range(3):
+            w[ivx + d] = rng.uniform(-60, 60, nf)
+        w[ivx + 3] = rng.uniform(2e4, 1.1e5, nf)
+        return w
+    wl, wr = states(), states()
+    h = ca.Hydro(ctx, blk, th, cfg)
+    for d in range(3):
+        _say("riemann nmass=%d solver=%d dir=%d" % (nmass, solver, d))
+        got = h.riemann_solver(ivx + d, wl, wr)
+        ref = orl.riemann(th, solver, ivx + d, wl, wr)
+        assert relerr(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("combo", [(2, 2, ca.RS_LMARS), (5, 2, ca.RS_HLLC),
+                                   (5, 5, ca.RS_LMARS), (2, 5, ca.RS_HLLC)])
+def test_calculate_fluxes(ctx, case, combo):
+    config, dims = case
+    blk, th, cfg, w = perturbed_state(config, dims)
+    cfg.order_h, cfg.order_v, cfg.rsolver = combo
+    _say("fluxes config=%d dims=%s combo=%s" % (config, dims, combo))
+    got = ca.Hydro(ctx, blk, th, cfg).calculate_fluxes(w)
+    ref = orl.calculate_fluxes(blk, th, cfg, w)
+    for d in range(3):
+        e = relerr(got[d], ref[d])
+        _say("  dir %d relerr %.3e" % (d, e))
+        assert e <= TOL
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_flux_divergence(ctx, case):
+    config, dims = case
+    blk, th, cfg, w = perturbed_state(config, dims)
+    _say("divergence config=%d" % config)
+    fs = orl.calculate_fluxes(blk, th, cfg, w)
+    du0 = np.random.default_rng(3).uniform(-1, 1, blk.shape())
+    got = ca.Hydro(ctx, blk, th, cfg).add_flux_divergence(0.37, fs, du0)
+    ref = orl.add_flux_divergence(blk, 0.37, fs, du0)
+    assert relerr(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2]])
+@pytest.mark.parametrize("flag", [9, 1])
+def test_implicit_solve(ctx, case, flag):
+    config, dims = case
+    blk, th, cfg, w = perturbed_state(config, dims)
+    fs = orl.calculate_fluxes(blk, th, cfg, w)
+    dt = 0.8
+    du = orl.add_flux_divergence(blk, dt, fs, np.zeros(blk.shape()))
+    du = orl.add_sources(blk, th, cfg, dt, w, du)
+    _say("implicit config=%d flag=%d" % (config, flag))
+    got = ca.ImplicitSolver(ctx, blk, th, flag).solve_implicit_3d(du, w, dt)
+    ref, r = orl.implicit_solve(blk, th, du, w, dt, flag)
+    assert r == 0
+    sk, sj, si = blk.interior()
+    e = relerr(got[:, sk, sj, si], ref[:, sk, sj, si])
+    _say("  relerr %.3e" % e)
+    assert e <= TOL
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("graph", [False, True])
+def test_step(ctx, case, graph):
+    config, dims = case
+    blk, th, cfg, w = perturbed_state(config, dims, vel=1.0, dp=1e-3,
+                                      cloud=True)
+    u = orl.prim_to_cons(blk, th, w)
+    dt = cfl_dt(blk, th, cfg, w, 0.6)
+    nsteps = 3
+    _say("step config=%d graph=%s" % (config, graph))
+    run = ca.Run(ctx, blk, th, cfg)
+    run.set_prim(w)
+    run.step(dt, nsteps, graph=graph)
+    ug, wg = run.cons(), run.prim()
+    run.close()
+    uo, wo = orl.step(blk, th, cfg, dt, nsteps, u, w)
+    sk, sj, si = blk.interior()
+    eu = relerr(ug[:, sk, sj, si], uo[:, sk, sj, si])
+    ew = relerr(wg[:, sk, sj, si], wo[:, sk, sj, si])
+    _say("  relerr u %.3e w %.3e" % (eu, ew))
+    assert eu <= TOL and ew <= TOL
