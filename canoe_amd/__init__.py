@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 
 NGHOST = 3
 MAX_NMASS = 3
+NUM_KERNEL_IDS = 13
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV = range(9)
@@ -142,7 +143,9 @@ _SIGS = {
     "cg_run_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "cg_run_sums": (_I, [_P, _P]),
     "cg_run_set_timing": (_I, [_P, _I]),
-    "cg_run_kernel_ms": (_I, [_P, _I, ctypes.POINTER(_D)]),
+    "cg_run_kernel_stats": (_I, [_P, _I, ctypes.POINTER(_D),
+                                 ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_char_p)]),
     "cg_version": (ctypes.c_char_p, []),
     "cg_apply_bc_host": (_I, [_P, _P]),
     "cg_pgen_setup": (_I, [_I, _I, _I, _I, _P, _P, _P]),
@@ -369,11 +372,18 @@ class Run:
     def set_timing(self, on):
         self.ctx._check(lib().cg_run_set_timing(self.h, int(on)), "timing")
 
-    def kernel_ms(self, cls):
-        v = ctypes.c_double()
-        self.ctx._check(lib().cg_run_kernel_ms(self.h, cls, ctypes.byref(v)),
-                        "kernel_ms")
-        return v.value
+    def kernel_stats(self):
+        """{kernel name: (total ms, launches)} of the timed steps."""
+        out = {}
+        for kid in range(NUM_KERNEL_IDS):
+            ms, n = ctypes.c_double(), ctypes.c_longlong()
+            name = ctypes.c_char_p()
+            self.ctx._check(lib().cg_run_kernel_stats(
+                self.h, kid, ctypes.byref(ms), ctypes.byref(n),
+                ctypes.byref(name)), "kernel_stats")
+            if n.value:
+                out[name.value.decode()] = (ms.value, n.value)
+        return out
 
     def close(self):
         if self.h is not None:

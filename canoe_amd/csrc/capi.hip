@@ -14,6 +14,7 @@
 
 #include "../../include/canoe_gpu.h"
 #include "kernels.hpp"
+#include "cg_timer.hpp"
 #include "flux32.hpp"
 
 using namespace cg;
@@ -119,6 +120,10 @@ KThermo make_thermo(const cg_thermo& t) {
 
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+}  // namespace
+
+namespace {
+
 size_t cell_doubles(const KGrid& g, int N) { return (size_t)N * g.sn; }
 size_t f1_doubles(const KGrid& g, int N) {
   return (size_t)N * g.nc3 * g.nc2 * (g.nc1 + 1);
@@ -134,59 +139,73 @@ size_t f3_doubles(const KGrid& g, int N) {
 template <int NM>
 void launch_fluxes(const KGrid& g, const KThermo& th, const cg_config& cfg,
                    const double* w, double* psf, double* psv, double* f1,
-                   double* f2, double* f3, hipStream_t s) {
+                   double* f2, double* f3, hipStream_t s, KTimer* tm = nullptr) {
   const long long ncol = (long long)g.nx2 * g.nx3;
   const int decomp = g.grav != 0.0;
   if (decomp)
-    k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, th, w, psf, psv);
+    TIMED(tm, KID_SCAN,
+          (k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, th, w, psf, psv)));
   long long n1 = ncol * (g.nx1 + 1);
   if (cfg.order_v == 5)
-    k_flux_x1<NM, 5><<<nblk(n1, 256), 256, 0, s>>>(g, th, cfg.rsolver, decomp,
-                                                    w, psf, psv, f1);
+    TIMED(tm, KID_FLUX_X1,
+          (k_flux_x1<NM, 5><<<nblk(n1, 256), 256, 0, s>>>(g, th, cfg.rsolver, decomp,
+                                                           w, psf, psv, f1)));
   else
-    k_flux_x1<NM, 2><<<nblk(n1, 256), 256, 0, s>>>(g, th, cfg.rsolver, decomp,
-                                                    w, psf, psv, f1);
+    TIMED(tm, KID_FLUX_X1,
+          (k_flux_x1<NM, 2><<<nblk(n1, 256), 256, 0, s>>>(g, th, cfg.rsolver, decomp,
+                                                           w, psf, psv, f1)));
   if (g.f2) {
     long long n2 = (long long)g.nx1 * (g.nx2 + 1) * g.nx3;
     if (cfg.order_h == 5)
-      k_flux_h<NM, 5, 1><<<nblk(n2, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f2);
+      TIMED(tm, KID_FLUX_X2,
+            (k_flux_h<NM, 5, 1><<<nblk(n2, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f2)));
     else
-      k_flux_h<NM, 2, 1><<<nblk(n2, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f2);
+      TIMED(tm, KID_FLUX_X2,
+            (k_flux_h<NM, 2, 1><<<nblk(n2, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f2)));
   }
   if (g.f3) {
     long long n3 = (long long)g.nx1 * g.nx2 * (g.nx3 + 1);
     if (cfg.order_h == 5)
-      k_flux_h<NM, 5, 2><<<nblk(n3, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f3);
+      TIMED(tm, KID_FLUX_X3,
+            (k_flux_h<NM, 5, 2><<<nblk(n3, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f3)));
     else
-      k_flux_h<NM, 2, 2><<<nblk(n3, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f3);
+      TIMED(tm, KID_FLUX_X3,
+            (k_flux_h<NM, 2, 2><<<nblk(n3, 256), 256, 0, s>>>(g, th, cfg.rsolver, w, f3)));
   }
 }
 
 template <int NM>
 void launch_implicit(const KGrid& g, const KThermo& th, double dt, int flag,
                      double* du, const double* w, double* arch, int* status,
-                     hipStream_t s) {
+                     hipStream_t s, KTimer* tm = nullptr) {
   KImplicit ip;
   ip.dt = dt;
   ip.ncol = (long long)g.nx2 * g.nx3;
   ip.nlev = g.nx1;
   unsigned nb = nblk(ip.ncol, 64);
   if (flag & 8) {
-    k_implicit_fwd<NM, 5><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status);
-    k_implicit_bwd<NM, 5><<<nb, 64, 0, s>>>(g, ip, w, arch, du);
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 5><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd<NM, 5><<<nb, 64, 0, s>>>(g, ip, w, arch, du)));
   } else {
-    k_implicit_fwd<NM, 3><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status);
-    k_implicit_bwd<NM, 3><<<nb, 64, 0, s>>>(g, ip, w, arch, du);
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 3><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd<NM, 3><<<nb, 64, 0, s>>>(g, ip, w, arch, du)));
   }
 }
 
 template <int NM>
-void launch_bc(const KGrid& g, double* w, hipStream_t s) {
-  k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, s>>>(g, w);
+void launch_bc(const KGrid& g, double* w, hipStream_t s, KTimer* tm = nullptr) {
+  TIMED(tm, KID_BC_X1,
+        (k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, s>>>(g, w)));
   if (g.f2)
-    k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, s>>>(g, w);
+    TIMED(tm, KID_BC_X2,
+          (k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, s>>>(g, w)));
   if (g.f3)
-    k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, s>>>(g, w);
+    TIMED(tm, KID_BC_X3,
+          (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, s>>>(g, w)));
 }
 
 size_t arch_doubles(const KGrid& g) {
@@ -212,9 +231,7 @@ struct cg_run {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double graph_dt = 0.0;
-  int timing = 0;
-  hipEvent_t ev[8];
-  double kms[3] = {0, 0, 0};
+  KTimer tm;
 };
 
 namespace {
@@ -235,30 +252,29 @@ int run_stage(cg_run* r, double dt, int s) {
   src.cp = r->th.gammad / (r->th.gammad - 1.) * r->th.rd;
   src.chi = (r->th.gammad - 1.) / r->th.gammad;
   src.heat_flux = r->cfg.heat_flux;
-  if (r->timing) hipEventRecord(r->ev[0], st);
+  KTimer* tm = r->tm.on ? &r->tm : nullptr;
   if (r->cfg.precision == CG_PREC_FP32) {
     launch_fluxes32<NM>(g, r->kt, r->cfg, src, r->w, r->wf, r->psf, r->psv,
-                        r->du, st);
+                        r->du, st, tm);
   } else {
     launch_fluxes<NM>(g, r->kt, r->cfg, r->w, r->psf, r->psv, r->f1, r->f2,
-                      r->f3, st);
-    k_divergence<NM><<<nblk(ncell, 256), 256, 0, st>>>(g, r->kt, src, r->f1,
-                                                       r->f2, r->f3, r->w, r->du);
+                      r->f3, st, tm);
+    TIMED(tm, KID_DIVERGENCE,
+          (k_divergence<NM><<<nblk(ncell, 256), 256, 0, st>>>(
+              g, r->kt, src, r->f1, r->f2, r->f3, r->w, r->du)));
   }
-  if (r->timing) hipEventRecord(r->ev[1], st);
   if (r->cfg.implicit_flag)
     launch_implicit<NM>(g, r->kt, wght, r->cfg.implicit_flag, r->du, r->w,
-                        r->arch, r->ctx->d_status, st);
-  if (r->timing) hipEventRecord(r->ev[2], st);
+                        r->arch, r->ctx->d_status, st, tm);
   KUpdate up;
   up.stage = s;
   up.g1 = sg1[s];
   up.g2 = sg2[s];
   up.satadj = (s == 2) && (r->cfg.satadj != CG_SATADJ_OFF);
-  k_update<NM><<<nblk(ncell, 256), 256, 0, st>>>(g, r->kt, up, r->u, r->u1,
-                                                 r->du, r->w, r->ctx->d_status);
-  launch_bc<NM>(g, r->w, st);
-  if (r->timing) hipEventRecord(r->ev[3], st);
+  TIMED(tm, KID_UPDATE,
+        (k_update<NM><<<nblk(ncell, 256), 256, 0, st>>>(g, r->kt, up, r->u, r->u1,
+                                                        r->du, r->w, r->ctx->d_status)));
+  launch_bc<NM>(g, r->w, st, tm);
   return CG_OK;
 }
 
@@ -266,16 +282,6 @@ int run_one_step(cg_run* r, double dt) {
   for (int s = 0; s < 3; ++s) {
     int e = (r->N == 5) ? run_stage<0>(r, dt, s) : run_stage<3>(r, dt, s);
     if (e) return e;
-    if (r->timing) {
-      float a = 0, b = 0, c = 0;
-      hipEventSynchronize(r->ev[3]);
-      hipEventElapsedTime(&a, r->ev[0], r->ev[1]);
-      hipEventElapsedTime(&b, r->ev[1], r->ev[2]);
-      hipEventElapsedTime(&c, r->ev[2], r->ev[3]);
-      r->kms[0] += a;
-      r->kms[1] += b;
-      r->kms[2] += c;
-    }
   }
   return CG_OK;
 }
@@ -572,7 +578,8 @@ int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
          hipMemset(r->u1, 0, sizeof(double) * nc) == hipSuccess &&
          hipMemset(r->du, 0, sizeof(double) * nc) == hipSuccess;
   }
-  for (int q = 0; q < 8 && ok; ++q) ok = hipEventCreate(&r->ev[q]) == hipSuccess;
+  r->tm.s = ctx->stream;
+  if (ok) ok = hipEventCreate(&r->tm.e0) == hipSuccess && hipEventCreate(&r->tm.e1) == hipSuccess;
   if (!ok) {
     cg_run_destroy(r);
     return set_err(ctx, CG_ERR_HIP, "allocation failed");
@@ -591,8 +598,8 @@ int cg_run_destroy(cg_run* r) {
   for (double* p : ps)
     if (p) hipFree(p);
   if (r->wf) hipFree(r->wf);
-  for (int q = 0; q < 8; ++q)
-    if (r->ev[q]) hipEventDestroy(r->ev[q]);
+  if (r->tm.e0) hipEventDestroy(r->tm.e0);
+  if (r->tm.e1) hipEventDestroy(r->tm.e1);
   delete r;
   return CG_OK;
 }
@@ -635,14 +642,20 @@ int cg_run_device_state(cg_run* r, double** u, double** w) {
 
 int cg_run_set_timing(cg_run* r, int on) {
   if (!r) return CG_ERR_ARG;
-  r->timing = on;
-  r->kms[0] = r->kms[1] = r->kms[2] = 0.0;
+  r->tm.on = on != 0;
+  for (int q = 0; q < KID_COUNT; ++q) {
+    r->tm.ms[q] = 0.0;
+    r->tm.cnt[q] = 0;
+  }
   return CG_OK;
 }
 
-int cg_run_kernel_ms(cg_run* r, int kernel_class, double* ms) {
-  if (!r || !ms || kernel_class < 0 || kernel_class > 2) return CG_ERR_ARG;
-  *ms = r->kms[kernel_class];
+int cg_run_kernel_stats(cg_run* r, int kernel_id, double* total_ms,
+                        long long* launches, const char** name) {
+  if (!r || kernel_id < 0 || kernel_id >= KID_COUNT) return CG_ERR_ARG;
+  if (total_ms) *total_ms = r->tm.ms[kernel_id];
+  if (launches) *launches = r->tm.cnt[kernel_id];
+  if (name) *name = kKernelNames[kernel_id];
   return CG_OK;
 }
 
@@ -651,7 +664,7 @@ int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
   if (!(dt > 0.0) || nsteps < 0) return set_err(r->ctx, CG_ERR_ARG, "bad dt/nsteps");
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (use_graph && !r->timing) {
+  if (use_graph && !r->tm.on) {
     if (!r->gexec || r->graph_dt != dt) {
       if (r->gexec) {
         hipGraphExecDestroy(r->gexec);

@@ -1,13 +1,16 @@
 // flux32.hpp -- FP32 reconstruction + Riemann fluxes with FP64 accumulation.
 #pragma once
 #include "kernels.hpp"
+#include "cg_timer.hpp"
 
 namespace cg {
 
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* wf,
-                     double* psf, double* psv, double* du, hipStream_t s) {
+                     double* psf, double* psv, double* du, hipStream_t s,
+                     KTimer* tm) {
+  (void)tm;
   (void)g; (void)th; (void)cfg; (void)src; (void)w; (void)wf; (void)psf;
   (void)psv; (void)du; (void)s;
 }

@@ -188,10 +188,15 @@ int cg_run_device_state(cg_run* run, double** u, double** w);
 /* FP64 global sums of the conserved variables over interior cells times the
  * cell volume (the reference's `hst` history totals).  out: N doubles. */
 int cg_run_sums(cg_run* run, double* out_host);
-/* Average duration (ms) of the named kernel class over the last timed
- * cg_run_step call with timing on: 0 = flux, 1 = implicit, 2 = update. */
+/* Per-kernel timing: with timing on, cg_run_step launches without the graph
+ * and records HIP events around every kernel on the run's stream.
+ * cg_run_kernel_stats returns the accumulated milliseconds, the number of
+ * launches and the kernel name of kernel id 0..CG_NUM_KERNEL_IDS-1 (ids
+ * with no launches report 0).  cg_run_set_timing resets the counters. */
+#define CG_NUM_KERNEL_IDS 13
 int cg_run_set_timing(cg_run* run, int on);
-int cg_run_kernel_ms(cg_run* run, int kernel_class, double* ms);
+int cg_run_kernel_stats(cg_run* run, int kernel_id, double* total_ms,
+                        long long* launches, const char** name);
 
 /* Library build string (arch, flags). */
 const char* cg_version(void);
