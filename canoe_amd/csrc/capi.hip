@@ -64,7 +64,9 @@ int check_block(cg_ctx* ctx, const cg_block* b, const cg_thermo* th) {
   return CG_OK;
 }
 
-KGrid make_grid(const cg_block& b) {
+// Grid of a block in the AthenaArray layout (lvl = 0, the C-ABI's caller
+// arrays) or in the level-major layout of a run's resident state (lvl = 1).
+KGrid make_grid(const cg_block& b, int lvl) {
   KGrid g;
   g.nx1 = b.nx1;
   g.nx2 = b.nx2;
@@ -80,13 +82,46 @@ KGrid make_grid(const cg_block& b) {
   g.je = g.js + b.nx2 - 1;
   g.ks = g.f3 ? NG : 0;
   g.ke = g.ks + b.nx3 - 1;
-  g.sn = (long long)g.nc1 * g.nc2 * g.nc3;
+  g.lvl = lvl;
+  const long long c1 = g.nc1, c2 = g.nc2, c3 = g.nc3;
+  g.sn = c1 * c2 * c3;
+  long long ncol = (long long)b.nx2 * b.nx3;
+  // extents of cells (e1, e2, e3) -> strides
+  auto mk = [&](long long e1, long long e2, long long e3) {
+    Strides st;
+    if (lvl) {
+      st.j = 1;
+      st.k = e2;
+      st.i = e2 * e3;
+    } else {
+      st.i = 1;
+      st.j = e1;
+      st.k = e1 * e2;
+    }
+    st.n = e1 * e2 * e3;
+    return st;
+  };
+  g.c = mk(c1, c2, c3);
+  g.f[0] = mk(c1 + 1, c2, c3);
+  g.f[1] = mk(c1, c2 + 1, c3);
+  g.f[2] = mk(c1, c2, c3 + 1);
+  if (lvl) {
+    g.ps_i = ncol;
+    g.ps_c = 1;
+  } else {
+    g.ps_i = 1;
+    g.ps_c = c1 + 1;
+  }
   g.dx1 = b.dx1;
   g.dx2 = b.dx2;
   g.dx3 = b.dx3;
   g.grav = b.grav;
   for (int s = 0; s < 6; ++s) g.bc[s] = b.bc[s];
   return g;
+}
+
+size_t psf_doubles(const KGrid& g) {
+  return (size_t)(g.nc1 + 1) * g.nx2 * g.nx3;
 }
 
 KThermo make_thermo(const cg_thermo& t) {
@@ -139,10 +174,11 @@ size_t f3_doubles(const KGrid& g, int N) {
 template <int NM>
 void launch_fluxes(const KGrid& g, const KThermo& th, const cg_config& cfg,
                    const double* w, double* psf, double* psv, double* f1,
-                   double* f2, double* f3, hipStream_t s, KTimer* tm = nullptr) {
+                   double* f2, double* f3, hipStream_t s, KTimer* tm = nullptr,
+                   bool scan = true) {
   const long long ncol = (long long)g.nx2 * g.nx3;
   const int decomp = g.grav != 0.0;
-  if (decomp)
+  if (decomp && scan)
     TIMED(tm, KID_SCAN,
           (k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, th, w, psf, psv)));
   long long n1 = ncol * (g.nx1 + 1);
@@ -181,7 +217,6 @@ void launch_implicit(const KGrid& g, const KThermo& th, double dt, int flag,
   KImplicit ip;
   ip.dt = dt;
   ip.ncol = (long long)g.nx2 * g.nx3;
-  ip.nlev = g.nx1;
   unsigned nb = nblk(ip.ncol, 64);
   if (flag & 8) {
     TIMED(tm, KID_IMPLICIT_FWD,
@@ -208,6 +243,33 @@ void launch_bc(const KGrid& g, double* w, hipStream_t s, KTimer* tm = nullptr) {
           (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, s>>>(g, w)));
 }
 
+// Forward sweep + fused downward sweep of a run (back substitution, stage
+// update, x1 ghosts, next stage's hydrostatic pressures).
+template <int NM>
+void launch_implicit_update(const KGrid& g, const KThermo& th, double dt,
+                            int flag, const KUpdate& up, const double* du,
+                            double* u, double* u1, double* w, double* arch,
+                            double* psf, double* psv, int* status,
+                            hipStream_t s, KTimer* tm) {
+  KImplicit ip;
+  ip.dt = dt;
+  ip.ncol = (long long)g.nx2 * g.nx3;
+  unsigned nb = nblk(ip.ncol, 64);
+  if (flag & 8) {
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 5><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd_update<NM, 5><<<nb, 64, 0, s>>>(
+              g, th, ip, up, arch, du, u, u1, w, psf, psv, status)));
+  } else {
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 3><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd_update<NM, 3><<<nb, 64, 0, s>>>(
+              g, th, ip, up, arch, du, u, u1, w, psf, psv, status)));
+  }
+}
+
 size_t arch_doubles(const KGrid& g) {
   return (size_t)g.nx1 * 30 * (size_t)g.nx2 * g.nx3;
 }
@@ -220,13 +282,15 @@ struct cg_run {
   cg_block blk;
   cg_thermo th;
   cg_config cfg;
-  KGrid g;
+  KGrid g;   // level-major resident state
+  KGrid ga;  // AthenaArray layout of host transfers
   KThermo kt;
   int N = 5;
   double *u = nullptr, *w = nullptr, *u1 = nullptr, *du = nullptr;
   double *f1 = nullptr, *f2 = nullptr, *f3 = nullptr;
   double *psf = nullptr, *psv = nullptr, *arch = nullptr;
-  float* wf = nullptr;  // FP32 mirror for the mixed path
+  float* ff = nullptr;     // FP32 face fluxes of the mixed form
+  double* pref = nullptr;  // per-level reference pressure of the mixed form
   double* part = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -253,28 +317,39 @@ int run_stage(cg_run* r, double dt, int s) {
   src.chi = (r->th.gammad - 1.) / r->th.gammad;
   src.heat_flux = r->cfg.heat_flux;
   KTimer* tm = r->tm.on ? &r->tm : nullptr;
-  if (r->cfg.precision == CG_PREC_FP32) {
-    launch_fluxes32<NM>(g, r->kt, r->cfg, src, r->w, r->wf, r->psf, r->psv,
-                        r->du, st, tm);
-  } else {
-    launch_fluxes<NM>(g, r->kt, r->cfg, r->w, r->psf, r->psv, r->f1, r->f2,
-                      r->f3, st, tm);
-    TIMED(tm, KID_DIVERGENCE,
-          (k_divergence<NM><<<nblk(ncell, 256), 256, 0, st>>>(
-              g, r->kt, src, r->f1, r->f2, r->f3, r->w, r->du)));
-  }
-  if (r->cfg.implicit_flag)
-    launch_implicit<NM>(g, r->kt, wght, r->cfg.implicit_flag, r->du, r->w,
-                        r->arch, r->ctx->d_status, st, tm);
+  const bool imp = r->cfg.implicit_flag != 0;
   KUpdate up;
   up.stage = s;
   up.g1 = sg1[s];
   up.g2 = sg2[s];
   up.satadj = (s == 2) && (r->cfg.satadj != CG_SATADJ_OFF);
-  TIMED(tm, KID_UPDATE,
-        (k_update<NM><<<nblk(ncell, 256), 256, 0, st>>>(g, r->kt, up, r->u, r->u1,
-                                                        r->du, r->w, r->ctx->d_status)));
-  launch_bc<NM>(g, r->w, st, tm);
+  if (r->cfg.precision == CG_PREC_FP32) {
+    launch_fluxes32<NM>(g, r->kt, r->cfg, src, r->w, r->ff, r->psf, r->psv,
+                        r->pref, r->du, st, tm, !imp);
+  } else {
+    // implicit runs get psf / psv from the previous downward sweep
+    launch_fluxes<NM>(g, r->kt, r->cfg, r->w, r->psf, r->psv, r->f1, r->f2,
+                      r->f3, st, tm, !imp);
+    TIMED(tm, KID_DIVERGENCE,
+          (k_divergence<NM><<<nblk(ncell, 256), 256, 0, st>>>(
+              g, r->kt, src, r->f1, r->f2, r->f3, r->w, r->du)));
+  }
+  if (imp) {
+    launch_implicit_update<NM>(g, r->kt, wght, r->cfg.implicit_flag, up, r->du,
+                               r->u, r->u1, r->w, r->arch, r->psf, r->psv,
+                               r->ctx->d_status, st, tm);
+    if (g.f2)
+      TIMED(tm, KID_BC_X2,
+            (k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, st>>>(g, r->w)));
+    if (g.f3)
+      TIMED(tm, KID_BC_X3,
+            (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, st>>>(g, r->w)));
+  } else {
+    TIMED(tm, KID_UPDATE,
+          (k_update<NM><<<nblk(ncell, 256), 256, 0, st>>>(g, r->kt, up, r->u, r->u1,
+                                                          r->du, r->w, r->ctx->d_status)));
+    launch_bc<NM>(g, r->w, st, tm);
+  }
   return CG_OK;
 }
 
@@ -409,12 +484,12 @@ int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if ((cfg->order_h != 2 && cfg->order_h != 5) ||
       (cfg->order_v != 2 && cfg->order_v != 5))
     return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
   double *psf = nullptr, *psv = nullptr;
   long long ncol = (long long)g.nx2 * g.nx3;
   HIPCHK(ctx, hipMallocAsync((void**)&psf, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
-  HIPCHK(ctx, hipMallocAsync((void**)&psv, sizeof(double) * ncol * g.nc1, ctx->stream));
+  HIPCHK(ctx, hipMallocAsync((void**)&psv, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
   if (blk->nmass == 0)
     launch_fluxes<0>(g, kt, *cfg, w, psf, psv, flux1, flux2, flux3, ctx->stream);
   else
@@ -430,7 +505,7 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
                            const double* flux1, const double* flux2,
                            const double* flux3, double* du) {
   if (!ctx || !blk) return CG_ERR_ARG;
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   if (blk->nmass == 0)
     k_add_divergence<0><<<nblk(ncell, 256), 256, 0, ctx->stream>>>(g, wght, flux1, flux2, flux3, du);
@@ -453,7 +528,7 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC)
     return set_err(ctx, CG_ERR_ARG, "periodic x1 implicit solve not supported");
   if (!(dt > 0.0)) return set_err(ctx, CG_ERR_ARG, "dt must be positive");
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
   double* arch = nullptr;
   HIPCHK(ctx, hipMallocAsync((void**)&arch, sizeof(double) * arch_doubles(g), ctx->stream));
@@ -472,7 +547,7 @@ int cg_cons_to_prim(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if (!ctx) return CG_ERR_ARG;
   int e = check_block(ctx, blk, th);
   if (e) return e;
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
   long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   if (blk->nmass == 0)
@@ -489,7 +564,7 @@ int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if (!ctx) return CG_ERR_ARG;
   int e = check_block(ctx, blk, th);
   if (e) return e;
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
   long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   if (blk->nmass == 0)
@@ -503,7 +578,7 @@ int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
 
 int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w) {
   if (!ctx || !blk) return CG_ERR_ARG;
-  KGrid g = make_grid(*blk);
+  KGrid g = make_grid(*blk, 0);
   if (blk->nmass == 0)
     launch_bc<0>(g, w, ctx->stream);
   else if (blk->nmass == 3)
@@ -554,7 +629,8 @@ int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   r->blk = *blk;
   r->th = *th;
   r->cfg = *cfg;
-  r->g = make_grid(*blk);
+  r->g = make_grid(*blk, 1);
+  r->ga = make_grid(*blk, 0);
   r->kt = make_thermo(*th);
   r->N = 5 + blk->nmass;
   const KGrid& g = r->g;
@@ -564,13 +640,17 @@ int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     return hipMalloc((void**)p, sizeof(double) * (n > 0 ? n : 1)) == hipSuccess;
   };
   bool ok = alloc(&r->u, nc) && alloc(&r->w, nc) && alloc(&r->u1, nc) &&
-            alloc(&r->du, nc) && alloc(&r->psf, ncol * (g.nc1 + 1)) &&
-            alloc(&r->psv, ncol * g.nc1) && alloc(&r->part, 1024 * 8);
+            alloc(&r->du, nc) && alloc(&r->psf, psf_doubles(g)) &&
+            alloc(&r->psv, psf_doubles(g)) && alloc(&r->part, 1024 * 8);
   if (ok && cfg->precision == CG_PREC_FP64)
     ok = alloc(&r->f1, f1_doubles(g, r->N)) && alloc(&r->f2, f2_doubles(g, r->N)) &&
          alloc(&r->f3, f3_doubles(g, r->N));
   if (ok && cfg->precision == CG_PREC_FP32)
-    ok = hipMalloc((void**)&r->wf, sizeof(float) * nc) == hipSuccess;
+    ok = hipMalloc((void**)&r->ff, sizeof(float) * (f1_doubles(g, r->N) +
+                                                    f2_doubles(g, r->N) +
+                                                    f3_doubles(g, r->N))) ==
+             hipSuccess &&
+         alloc(&r->pref, g.nc1);
   if (ok && cfg->implicit_flag) ok = alloc(&r->arch, arch_doubles(g));
   if (ok) {
     ok = hipMemset(r->u, 0, sizeof(double) * nc) == hipSuccess &&
@@ -597,7 +677,8 @@ int cg_run_destroy(cg_run* r) {
                   r->arch, r->part};
   for (double* p : ps)
     if (p) hipFree(p);
-  if (r->wf) hipFree(r->wf);
+  if (r->ff) hipFree(r->ff);
+  if (r->pref) hipFree(r->pref);
   if (r->tm.e0) hipEventDestroy(r->tm.e0);
   if (r->tm.e1) hipEventDestroy(r->tm.e1);
   delete r;
@@ -608,29 +689,57 @@ int cg_run_set_prim_host(cg_run* r, const double* w_host) {
   if (!r || !w_host) return CG_ERR_ARG;
   cg_ctx* ctx = r->ctx;
   size_t nc = cell_doubles(r->g, r->N);
-  HIPCHK(ctx, hipMemcpyAsync(r->w, w_host, sizeof(double) * nc,
+  HIPCHK(ctx, hipMemcpyAsync(r->du, w_host, sizeof(double) * nc,
                              hipMemcpyHostToDevice, ctx->stream));
+  k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(r->ga, r->g, r->N,
+                                                               r->du, r->w);
   long long ncell = (long long)r->g.nx1 * r->g.nx2 * r->g.nx3;
+  long long ncol = (long long)r->g.nx2 * r->g.nx3;
   if (r->N == 5) {
     k_eos<0, 0><<<nblk(ncell, 256), 256, 0, ctx->stream>>>(r->g, r->kt, r->w, r->u);
     launch_bc<0>(r->g, r->w, ctx->stream);
+    if (r->pref)
+      k_level_mean_p<0><<<r->g.nc1, 256, 0, ctx->stream>>>(r->g, r->w, r->pref);
+    if (r->g.grav != 0.0)
+      k_hydrostatic_scan<0><<<nblk(ncol, 256), 256, 0, ctx->stream>>>(r->g, r->kt, r->w,
+                                                                     r->psf, r->psv);
   } else {
     k_eos<3, 0><<<nblk(ncell, 256), 256, 0, ctx->stream>>>(r->g, r->kt, r->w, r->u);
     launch_bc<3>(r->g, r->w, ctx->stream);
+    if (r->pref)
+      k_level_mean_p<3><<<r->g.nc1, 256, 0, ctx->stream>>>(r->g, r->w, r->pref);
+    if (r->g.grav != 0.0)
+      k_hydrostatic_scan<3><<<nblk(ncol, 256), 256, 0, ctx->stream>>>(r->g, r->kt, r->w,
+                                                                     r->psf, r->psv);
   }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
 }
 
+static int run_download(cg_run* r, const double* src, double* host) {
+  cg_ctx* ctx = r->ctx;
+  size_t nc = cell_doubles(r->g, r->N);
+  double* tmp = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&tmp, sizeof(double) * nc, ctx->stream));
+  k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(r->g, r->ga, r->N,
+                                                               src, tmp);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(host, tmp, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipFreeAsync(tmp, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
 int cg_run_get_cons_host(cg_run* r, double* u_host) {
   if (!r || !u_host) return CG_ERR_ARG;
-  return cg_memcpy_d2h(r->ctx, u_host, r->u, sizeof(double) * cell_doubles(r->g, r->N));
+  return run_download(r, r->u, u_host);
 }
 
 int cg_run_get_prim_host(cg_run* r, double* w_host) {
   if (!r || !w_host) return CG_ERR_ARG;
-  return cg_memcpy_d2h(r->ctx, w_host, r->w, sizeof(double) * cell_doubles(r->g, r->N));
+  return run_download(r, r->w, w_host);
 }
 
 int cg_run_device_state(cg_run* r, double** u, double** w) {

@@ -13,19 +13,51 @@ namespace cg {
 
 constexpr int NG = 3;
 
-// Grid of one MeshBlock in the AthenaArray layout (n, k, j, i).
+// Element strides of an array indexed (n, k, j, i).
+struct Strides {
+  long long n, k, j, i;
+  CG_HD long long at(int nn, int kk, int jj, int ii) const {
+    return (long long)nn * n + (long long)kk * k + (long long)jj * j +
+           (long long)ii * i;
+  }
+};
+
+// Grid of one MeshBlock.  Cell arrays use the strides `c`: the AthenaArray
+// layout (i unit stride, the C-ABI's caller arrays) or the level-major
+// layout of a run's resident state (j unit stride, so that column-per-lane
+// kernels read whole rows per wavefront).  Face arrays use f[0..2];
+// hydrostatic face/cell pressures use psf[i * ps_i + col * ps_c] with col =
+// (k - ks) nx2 + (j - js).
 struct KGrid {
   int nx1, nx2, nx3, nc1, nc2, nc3;
   int is, ie, js, je, ks, ke;
   int f2, f3;
-  long long sn;  // stride of the variable index n = nc1 nc2 nc3
+  int lvl;  // 1: level-major mapping of lanes (j fastest), 0: i fastest
+  long long sn;
+  Strides c, f[3];
+  long long ps_i, ps_c;
   double dx1, dx2, dx3, grav;
   int bc[6];
-  CG_HD long long at(int n, int k, int j, int i) const {
-    return ((long long)n * nc3 + k) * ((long long)nc2 * nc1) +
-           (long long)j * nc1 + i;
+  CG_HD long long at(int n, int k, int j, int i) const { return c.at(n, k, j, i); }
+  CG_HD long long pcol(int k, int j) const {
+    return (long long)(k - ks) * nx2 + (j - js);
   }
-  CG_HD long long col(int k, int j) const { return (long long)k * nc2 + j; }
+  // (i, j, k) offsets of item t of a box of extents (n1, n2, n3), lanes
+  // running along the unit-stride index
+  CG_HD void decode(long long t, int n1, int n2, int n3, int& i, int& j,
+                    int& k) const {
+    if (lvl) {
+      j = (int)(t % n2);
+      long long r = t / n2;
+      k = (int)(r % n3);
+      i = (int)(r / n3);
+    } else {
+      i = (int)(t % n1);
+      long long r = t / n1;
+      j = (int)(r % n2);
+      k = (int)(r / n2);
+    }
+  }
 };
 
 // Thermodynamic constants (cg_thermo plus derived saturation constants).

@@ -1,18 +1,422 @@
-// flux32.hpp -- FP32 reconstruction + Riemann fluxes with FP64 accumulation.
+// flux32.hpp -- FP32 reconstruction + Riemann fluxes, FP64 accumulation.
+//
+// The mixed form of the step: face states and fluxes in float, the flux
+// divergence, sources and the state in double.  Pressure enters the float
+// path only as a perturbation p' = p - pb about a base pb that is the same
+// on both sides of a face:
+//   x1 faces: pb = psf, the hydrostatic face pressure of the decomposition
+//             (FP64, change_to_temperature.cpp:58-82); the reconstruction
+//             already works on p - psv and T, as in the reference;
+//   x2/x3:    pb = pref(i), a per-level reference pressure (FP64, horizontal
+//             mean of the initial state); it is uniform across every x2/x3
+//             face of a level, so its flux divergence vanishes exactly.
+// The Riemann solvers are written in perturbation form: every difference of
+// pressures uses p' (exact), absolute pressures (sound speed, enthalpy) use
+// pb + p' in float, and the normal-momentum flux is returned without pb.
+// The divergence kernel adds A1 (psf(i+1) - psf(i)) in FP64 to the x1
+// normal-momentum flux difference, which is the part that balances gravity.
 #pragma once
-#include "kernels.hpp"
+
 #include "cg_timer.hpp"
+#include "kernels.hpp"
 
 namespace cg {
 
+// get_gamma in float (src/interface/eos.hpp:47-67)
+template <int NM>
+__device__ inline float gamma_f(const KThermo& th, const float* u) {
+  float fsig = 1.f, feps = 1.f;
+#pragma unroll
+  for (int n = 1; n <= NM; ++n)
+    if (n <= th.nvapor) {
+      fsig += u[n] * (float)th.cv_ratio_m1[n - 1];
+      feps += u[n] * (float)th.inv_mu_ratio_m1[n - 1];
+    }
+#pragma unroll
+  for (int n = 1; n <= NM; ++n)
+    if (n > th.nvapor) {
+      fsig += u[n] * (float)th.cv_ratio_m1[n - 1];
+      feps -= u[n];
+    }
+  return 1.f + ((float)th.gammad - 1.f) * feps / fsig;
+}
+
+// LMARS (src/snap/riemann/lmars.cpp:19-85) in perturbation form: wl[IPR],
+// wr[IPR] hold p - pb.  Normal-momentum flux excludes pb.
+template <int NM, int DIR>
+__device__ inline void lmars_p(const KThermo& th, const float* wli,
+                               const float* wri, float pb, float* flx) {
+  constexpr int IVX = 1 + NM, IPR = IVX + 3, IDN = 0, IEN = IPR;
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  float pl = wli[IPR] + pb, pr = wri[IPR] + pb;
+  float kappal = 1.f / (gamma_f<NM>(th, wli) - 1.f);
+  float kappar = 1.f / (gamma_f<NM>(th, wri) - 1.f);
+  float hl = pl / wli[IDN] * (kappal + 1.f) +
+             0.5f * (wli[ivx] * wli[ivx] + wli[ivy] * wli[ivy] +
+                     wli[ivz] * wli[ivz]);
+  float hr = pr / wri[IDN] * (kappar + 1.f) +
+             0.5f * (wri[ivx] * wri[ivx] + wri[ivy] * wri[ivy] +
+                     wri[ivz] * wri[ivz]);
+  float rhobar = 0.5f * (wli[IDN] + wri[IDN]);
+  float cbar = sqrtf(0.5f * (1.f + (1.f / kappar + 1.f / kappal) * 0.5f) *
+                     (pl + pr) / rhobar);
+  float rc = rhobar * cbar;
+  float pbar = 0.5f * (wli[IPR] + wri[IPR]) + 0.5f * rc * (wli[ivx] - wri[ivx]);
+  float ubar = 0.5f * (wli[ivx] + wri[ivx]) +
+               0.5f / rc * (wli[IPR] - wri[IPR]);
+  float rdl = 1.f, rdr = 1.f;
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) {
+    rdl -= wli[n];
+    rdr -= wri[n];
+  }
+  const bool up = ubar > 0.f;
+  float ws[5 + NM];
+#pragma unroll
+  for (int n = 0; n < 5 + NM; ++n) ws[n] = up ? wli[n] : wri[n];
+  float rd = up ? rdl : rdr;
+  float h = up ? hl : hr;
+  float m = ubar * ws[IDN];
+  flx[IDN] = m * rd;
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) flx[n] = m * ws[n];
+  flx[ivx] = m * ws[ivx] + pbar;
+  flx[ivy] = m * ws[ivy];
+  flx[ivz] = m * ws[ivz];
+  flx[IEN] = m * h;
+}
+
+// HLLC (src/snap/riemann/hllc_transform.cpp:25-233) in perturbation form.
+template <int NM, int DIR>
+__device__ inline void hllc_p(const KThermo& th, const float* wl,
+                              const float* wr, float pb, float* flx) {
+  constexpr int IVX = 1 + NM, IVY = IVX + 1, IVZ = IVX + 2, IPR = IVX + 3;
+  constexpr int IDN = 0, IEN = IPR, N = 5 + NM;
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  const float TINY = 1.0e-20f;
+  float wli[N], wri[N], fl[N], fr[N], flxi[N];
+  wli[IDN] = wl[IDN];
+  wli[IVX] = wl[ivx];
+  wli[IVY] = wl[ivy];
+  wli[IVZ] = wl[ivz];
+  wli[IPR] = wl[IPR];
+  wri[IDN] = wr[IDN];
+  wri[IVX] = wr[ivx];
+  wri[IVY] = wr[ivy];
+  wri[IVZ] = wr[ivz];
+  wri[IPR] = wr[IPR];
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) {
+    wli[n] = wl[n];
+    wri[n] = wr[n];
+  }
+  float pl = wli[IPR] + pb, pr = wri[IPR] + pb;
+  float gammal = gamma_f<NM>(th, wli), gammar = gamma_f<NM>(th, wri);
+  float kappal = 1.f / (gammal - 1.f), kappar = 1.f / (gammar - 1.f);
+  float cl = sqrtf(gammal * pl / wli[IDN]);
+  float cr = sqrtf(gammar * pr / wri[IDN]);
+  float KE_l = 0.5f * wli[IDN] *
+               (wli[IVX] * wli[IVX] + wli[IVY] * wli[IVY] + wli[IVZ] * wli[IVZ]);
+  float KE_r = 0.5f * wri[IDN] *
+               (wri[IVX] * wri[IVX] + wri[IVY] * wri[IVY] + wri[IVZ] * wri[IVZ]);
+  float el = pl * kappal + KE_l;
+  float er = pr * kappar + KE_r;
+  float rhoa = 0.5f * (wli[IDN] + wri[IDN]);
+  float ca = 0.5f * (cl + cr);
+  float pmid = 0.5f * (pl + pr + (wli[IVX] - wri[IVX]) * rhoa * ca);
+  float ql = (pmid <= pl) ? 1.f
+                          : sqrtf(1.f + (gammal + 1.f) / (2.f * gammal) *
+                                            (pmid / pl - 1.f));
+  float qr = (pmid <= pr) ? 1.f
+                          : sqrtf(1.f + (gammar + 1.f) / (2.f * gammar) *
+                                            (pmid / pr - 1.f));
+  float al = wli[IVX] - cl * ql;
+  float ar = wri[IVX] + cr * qr;
+  float bp = ar > 0.f ? ar : TINY;
+  float bm = al < 0.f ? al : -TINY;
+  float vxl = wli[IVX] - al;
+  float vxr = wri[IVX] - ar;
+  float tl = wli[IPR] + vxl * wli[IDN] * wli[IVX];  // t - pb
+  float tr = wri[IPR] + vxr * wri[IDN] * wri[IVX];
+  float ml = wli[IDN] * vxl;
+  float mr = -(wri[IDN] * vxr);
+  float am = (tl - tr) / (ml + mr);
+  float cp = (ml * tr + mr * tl) / (ml + mr);  // contact pressure - pb
+  cp = cp > -pb ? cp : -pb;
+  vxl = wli[IVX] - bm;
+  vxr = wri[IVX] - bp;
+  float rdl = 1.f, rdr = 1.f;
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) {
+    rdl -= wli[n];
+    rdr -= wri[n];
+  }
+  fl[IDN] = wli[IDN] * vxl * rdl;
+  fr[IDN] = wri[IDN] * vxr * rdr;
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) {
+    fl[n] = wli[IDN] * wli[n] * vxl;
+    fr[n] = wri[IDN] * wri[n] * vxr;
+  }
+  fl[IVX] = wli[IDN] * wli[IVX] * vxl + wli[IPR];
+  fr[IVX] = wri[IDN] * wri[IVX] * vxr + wri[IPR];
+  fl[IVY] = wli[IDN] * wli[IVY] * vxl;
+  fr[IVY] = wri[IDN] * wri[IVY] * vxr;
+  fl[IVZ] = wli[IDN] * wli[IVZ] * vxl;
+  fr[IVZ] = wri[IDN] * wri[IVZ] * vxr;
+  fl[IEN] = el * vxl + pl * wli[IVX];
+  fr[IEN] = er * vxr + pr * wri[IVX];
+  float sl, sr, sm;
+  if (am >= 0.f) {
+    sl = am / (am - bm);
+    sr = 0.f;
+    sm = -bm / (am - bm);
+  } else {
+    sl = 0.f;
+    sr = -am / (bp - am);
+    sm = bp / (bp - am);
+  }
+#pragma unroll
+  for (int n = 0; n < IVX; ++n) flxi[n] = sl * fl[n] + sr * fr[n];
+  flxi[IVX] = sl * fl[IVX] + sr * fr[IVX] + sm * cp;
+  flxi[IVY] = sl * fl[IVY] + sr * fr[IVY];
+  flxi[IVZ] = sl * fl[IVZ] + sr * fr[IVZ];
+  flxi[IEN] = sl * fl[IEN] + sr * fr[IEN] + sm * (cp + pb) * am;
+#pragma unroll
+  for (int n = 0; n < IVX; ++n) flx[n] = flxi[n];
+  flx[ivx] = flxi[IVX];
+  flx[ivy] = flxi[IVY];
+  flx[ivz] = flxi[IVZ];
+  flx[IEN] = flxi[IEN];
+}
+
+template <int NM, int DIR>
+__device__ inline void riemann_p(const KThermo& th, int solver, const float* wl,
+                                 const float* wr, float pb, float* flx) {
+  if (solver == 1)
+    hllc_p<NM, DIR>(th, wl, wr, pb, flx);
+  else
+    lmars_p<NM, DIR>(th, wl, wr, pb, flx);
+}
+
+template <int ORDER>
+__device__ inline void recon_pair_f(const float* q, float* l, float* r) {
+  float qm, qp;
+  if (ORDER == 5) {
+    weno5_cell<float>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
+    *l = qp;
+    weno5_cell<float>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
+    *r = qm;
+  } else {
+    plm_cell<float>(q[1], q[2], q[3], &qm, &qp);
+    *l = qp;
+    plm_cell<float>(q[2], q[3], q[4], &qm, &qp);
+    *r = qm;
+  }
+}
+
+// x1 faces in float: decomposition in FP64 (T, p - psv), reconstruction and
+// LMARS/HLLC in float about pb = psf(i).
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_flux32_x1(KGrid g, KThermo th, int rsolver, int decomp,
+                const double* __restrict__ w, const double* __restrict__ psf,
+                const double* __restrict__ psv, float* __restrict__ f1) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long pc = g.pcol(k, j) * g.ps_c;
+  float q[N][6];
+  double rdv[6];
+#pragma unroll
+  for (int m = LO; m <= HI; ++m) {
+    double wc[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i - 3 + m)];
+    if (decomp) {
+      rdv[m] = decomp_r<NM>(th, wc);
+      wc[0] = wc[IPR] / (wc[0] * rdv[m]);
+      wc[IPR] = wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i];
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
+  }
+  float wl[N], wr[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
+  float pb = 0.f;
+  if (decomp) {
+    double ps = psf[pc + i * g.ps_i];
+    pb = (float)ps;
+    float gam = (float)sqrt(rdv[3] * rdv[2]);
+    // RestoreFromTemperature (:155-164): p = psf + p', clamped at psf
+    if (wr[IPR] + pb < 0.f) wr[IPR] = 0.f;
+    if (wl[IPR] + pb < 0.f) wl[IPR] = 0.f;
+    wr[0] = (float)((ps + (double)wr[IPR]) / ((double)wr[0] * gam));
+    wl[0] = (float)((ps + (double)wl[IPR]) / ((double)wl[0] * gam));
+  }
+  float fx[N];
+  riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
+#pragma unroll
+  for (int n = 0; n < N; ++n) f1[g.f[0].at(n, k, j, i)] = fx[n];
+}
+
+// x2 / x3 faces in float about pb = pref(i).
+template <int NM, int ORDER, int DIR>
+__global__ void __launch_bounds__(256)
+    k_flux32_h(KGrid g, KThermo th, int rsolver, const double* __restrict__ w,
+               const double* __restrict__ pref, float* __restrict__ fh) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
+  const int n2 = (DIR == 1) ? g.nx2 + 1 : g.nx2;
+  const int n3 = (DIR == 2) ? g.nx3 + 1 : g.nx3;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx1 * n2 * n3) return;
+  int i, j, k;
+  g.decode(t, g.nx1, n2, n3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long s = (DIR == 1) ? g.c.j : g.c.k;
+  const long long base = g.at(0, k, j, i);
+  const double pr0 = pref[i];
+  float q[N][6];
+#pragma unroll
+  for (int m = LO; m <= HI; ++m)
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double v = w[base + n * g.c.n + (m - 3) * s];
+      if (n == IPR) v -= pr0;
+      q[n][m] = (float)v;
+    }
+  float wl[N], wr[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
+  float fx[N];
+  riemann_p<NM, DIR>(th, rsolver, wl, wr, (float)pr0, fx);
+#pragma unroll
+  for (int n = 0; n < N; ++n) fh[g.f[DIR].at(n, k, j, i)] = fx[n];
+}
+
+// du = 0 - wght div F with float fluxes accumulated in FP64; the hydrostatic
+// face pressure difference enters the x1 normal-momentum flux in FP64.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_divergence32(KGrid g, KThermo th, KSource src,
+                   const float* __restrict__ f1, const float* __restrict__ f2,
+                   const float* __restrict__ f3, const double* __restrict__ psf,
+                   int decomp, const double* __restrict__ w,
+                   double* __restrict__ du) {
+  constexpr int N = 5 + NM, IVX = 1 + NM;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
+  const double vol = g.dx1 * g.dx2 * g.dx3;
+  double d[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    double dflx = A1 * (double)f1[g.f[0].at(n, k, j, i + 1)] -
+                  A1 * (double)f1[g.f[0].at(n, k, j, i)];
+    if (n == IVX && decomp) {
+      const long long pc = g.pcol(k, j) * g.ps_c;
+      dflx += A1 * (psf[pc + (i + 1) * g.ps_i] - psf[pc + i * g.ps_i]);
+    }
+    if (g.f2)
+      dflx += A2 * (double)f2[g.f[1].at(n, k, j + 1, i)] -
+              A2 * (double)f2[g.f[1].at(n, k, j, i)];
+    if (g.f3)
+      dflx += A3 * (double)f3[g.f[2].at(n, k + 1, j, i)] -
+              A3 * (double)f3[g.f[2].at(n, k, j, i)];
+    d[n] = 0.0 - src.wght * dflx / vol;
+  }
+  add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+#pragma unroll
+  for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
+}
+
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
-                     const KSource& src, const double* w, float* wf,
-                     double* psf, double* psv, double* du, hipStream_t s,
-                     KTimer* tm) {
-  (void)tm;
-  (void)g; (void)th; (void)cfg; (void)src; (void)w; (void)wf; (void)psf;
-  (void)psv; (void)du; (void)s;
+                     const KSource& src, const double* w, float* ff,
+                     double* psf, double* psv, const double* pref, double* du,
+                     hipStream_t s, KTimer* tm, bool scan) {
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+  const int decomp = g.grav != 0.0;
+  float* f1 = ff;
+  float* f2 = f1 + (size_t)(5 + NM) * g.f[0].n;
+  float* f3 = f2 + (size_t)(5 + NM) * g.f[1].n;
+  if (decomp && scan)
+    TIMED(tm, KID_SCAN,
+          (k_hydrostatic_scan<NM><<<(unsigned)((ncol + 255) / 256), 256, 0, s>>>(
+              g, th, w, psf, psv)));
+  const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + 255) / 256);
+  if (cfg.order_v == 5)
+    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, 256, 0, s>>>(
+                               g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+  else
+    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, 256, 0, s>>>(
+                               g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+  if (g.f2) {
+    const unsigned b2 =
+        (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + 255) / 256);
+    if (cfg.order_h == 5)
+      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 5, 1><<<b2, 256, 0, s>>>(
+                                 g, th, cfg.rsolver, w, pref, f2)));
+    else
+      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 2, 1><<<b2, 256, 0, s>>>(
+                                 g, th, cfg.rsolver, w, pref, f2)));
+  }
+  if (g.f3) {
+    const unsigned b3 =
+        (unsigned)(((long long)g.nx1 * g.nx2 * (g.nx3 + 1) + 255) / 256);
+    if (cfg.order_h == 5)
+      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 5, 2><<<b3, 256, 0, s>>>(
+                                 g, th, cfg.rsolver, w, pref, f3)));
+    else
+      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, 256, 0, s>>>(
+                                 g, th, cfg.rsolver, w, pref, f3)));
+  }
+  TIMED(tm, KID_DIVERGENCE,
+        (k_divergence32<NM><<<(unsigned)((ncell + 255) / 256), 256, 0, s>>>(
+            g, th, src, f1, f2, f3, psf, decomp, w, du)));
+}
+
+// per-level reference pressure: horizontal mean of p over interior columns
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_level_mean_p(KGrid g, const double* __restrict__ w,
+                   double* __restrict__ pref) {
+  constexpr int IPR = 4 + NM;
+  __shared__ double sh[256];
+  int i = blockIdx.x;  // 0 .. nc1-1
+  double s = 0.0;
+  long long ncol = (long long)g.nx2 * g.nx3;
+  for (long long c = threadIdx.x; c < ncol; c += blockDim.x) {
+    int j = g.js + (int)(c % g.nx2), k = g.ks + (int)(c / g.nx2);
+    s += w[g.at(IPR, k, j, i)];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pref[i] = sh[0] / (double)ncol;
 }
 
 }  // namespace cg

@@ -1,20 +1,38 @@
 // kernels.hpp -- device kernels of the canoe dynamical-core step (gfx950).
 //
-// FP64 plain path.  Thread mappings put the unit-stride index i (x1, the
-// vertical) on adjacent lanes wherever a kernel is cell- or face-parallel,
-// so every global load and store of a wavefront is one contiguous segment
-// per variable.  The column-sequential kernels (hydrostatic scan, block
-// Thomas) put one column per lane.
+// FP64 plain path.  Cell- and face-parallel kernels put the unit-stride
+// index on adjacent lanes (KGrid::decode), so every wavefront load or store
+// of a variable is one contiguous segment in either layout.  Column kernels
+// (hydrostatic scan, block Thomas) put one column per lane, j fastest: in
+// the level-major layout of a run a wavefront then reads one contiguous row
+// per level.
 #pragma once
 
 #include "cg_physics.hpp"
 
 namespace cg {
 
+// R = Rd (1 + sum_vap q inv_mu_ratio_m1) of the decomposition
+// (src/snap/decomposition/change_to_temperature.cpp:100-104)
+template <int NM>
+CG_HD double decomp_r(const KThermo& th, const double* wc) {
+  double feps = 1.;
+#pragma unroll
+  for (int n = 1; n <= NM; ++n)
+    if (n <= th.nvapor) feps += wc[n] * th.inv_mu_ratio_m1[n - 1];
+  return th.rd * feps;
+}
+
+// log-mean of two face pressures (change_to_temperature.cpp:94-98)
+CG_HD double log_mean(double lo, double hi) {
+  if (fabs(lo - hi) < 1.E-6) return 0.5 * (lo + hi);
+  return (lo - hi) / log_d(lo / hi);
+}
+
 // ---------------------------------------------------------------------------
 // K2: hydrostatic face pressure psf and log-mean cell pressure psv per
-// column (Decomposition::ChangeToTemperature,
-// src/snap/decomposition/change_to_temperature.cpp:40-111).
+// column (Decomposition::ChangeToTemperature, :40-111).  One column per
+// lane.
 template <int NM>
 __global__ void __launch_bounds__(256)
     k_hydrostatic_scan(KGrid g, KThermo th, const double* __restrict__ w,
@@ -27,56 +45,40 @@ __global__ void __launch_bounds__(256)
   int k = g.ks + (int)(t / g.nx2);
   const double grav = g.grav, dx = g.dx1;
   const int is = g.is, ie = g.ie;
-  double* pf = psf + t * (g.nc1 + 1);
-  double* pv = psv + t * g.nc1;
+  const long long pc = t * g.ps_c;
+  auto PF = [&](int i) -> double& { return psf[pc + i * g.ps_i]; };
   {
     double RdTv = w[g.at(IPR, k, j, ie)] / w[g.at(0, k, j, ie)];
-    pf[ie + 1] = w[g.at(IPR, k, j, ie)] * exp_d(grav * dx / (2 * RdTv));
+    PF(ie + 1) = w[g.at(IPR, k, j, ie)] * exp_d(grav * dx / (2 * RdTv));
   }
-  double cur = pf[ie + 1];
+  double cur = PF(ie + 1);
   for (int i = ie; i >= is; --i) {
     cur = cur - grav * w[g.at(0, k, j, i)] * dx;
-    pf[i] = cur;
+    PF(i) = cur;
   }
   double gb = (g.bc[0] == 0) ? -grav : grav;
   for (int i = is - 1; i >= is - NG; --i) {
     cur = cur - gb * w[g.at(0, k, j, i)] * dx;
-    pf[i] = cur;
+    PF(i) = cur;
   }
   double gt = (g.bc[1] == 0) ? -grav : grav;
-  cur = pf[ie + 1];
+  cur = PF(ie + 1);
   for (int i = ie + 1; i <= ie + NG; ++i) {
     cur = cur + gt * w[g.at(0, k, j, i)] * dx;
-    pf[i + 1] = cur;
+    PF(i + 1) = cur;
   }
-  double lo = pf[0];
+  double lo = PF(0);
   for (int i = 0; i < g.nc1; ++i) {
-    double hi = pf[i + 1];
-    double v;
-    if (fabs(lo - hi) < 1.E-6)
-      v = 0.5 * (lo + hi);
-    else
-      v = (lo - hi) / log_d(lo / hi);
-    pv[i] = v;
+    double hi = PF(i + 1);
+    psv[pc + i * g.ps_i] = log_mean(lo, hi);
     lo = hi;
   }
-}
-
-// R = Rd (1 + sum_vap q inv_mu_ratio_m1) of the decomposition (:100-104)
-template <int NM>
-CG_HD double decomp_r(const KThermo& th, const double* wc) {
-  double feps = 1.;
-#pragma unroll
-  for (int n = 1; n <= NM; ++n)
-    if (n <= th.nvapor) feps += wc[n] * th.inv_mu_ratio_m1[n - 1];
-  return th.rd * feps;
 }
 
 // ---------------------------------------------------------------------------
 // K1a: x1 face fluxes with the hydrostatic decomposition around the
 // reconstruction (patches/19.calculate_fluxes.patch:23-47;
-// change_to_temperature.cpp:107-110 and :155-164).  One lane per face,
-// faces of a column on adjacent lanes.
+// change_to_temperature.cpp:107-110 and :155-164).  One lane per face.
 template <int NM, int ORDER>
 __global__ void __launch_bounds__(256)
     k_flux_x1(KGrid g, KThermo th, int rsolver, int decomp,
@@ -84,16 +86,14 @@ __global__ void __launch_bounds__(256)
               const double* __restrict__ psv, double* __restrict__ f1) {
   constexpr int N = 5 + NM, IPR = 4 + NM;
   constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  const int nf = g.nx1 + 1;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long ncol = (long long)g.nx2 * g.nx3;
-  if (t >= ncol * nf) return;
-  int fi = (int)(t % nf);
-  long long cidx = t / nf;
-  int j = g.js + (int)(cidx % g.nx2);
-  int k = g.ks + (int)(cidx / g.nx2);
-  int i = g.is + fi;
-  // window cells i-3 .. i+2 -> q[n][0..5]
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long pc = g.pcol(k, j) * g.ps_c;
   double q[N][6];
   double rdv[6];
 #pragma unroll
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256)
       rdv[m] = decomp_r<NM>(th, wc);
       double T = wc[IPR] / (wc[0] * rdv[m]);
       wc[0] = T;
-      wc[IPR] = wc[IPR] - psv[cidx * g.nc1 + (i - 3 + m)];
+      wc[IPR] = wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i];
     }
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = wc[n];
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (decomp) {
-    double ps = psf[cidx * (g.nc1 + 1) + i];
+    double ps = psf[pc + i * g.ps_i];
     wr[IPR] += ps;
     if (wr[IPR] < 0.) wr[IPR] = ps;
     double gamma = sqrt(rdv[3] * rdv[2]);
@@ -138,35 +138,33 @@ __global__ void __launch_bounds__(256)
   }
   double fx[N];
   riemann<NM, 0, double>(th, rsolver, wl, wr, fx);
-  const long long fs = (long long)g.nc3 * g.nc2 * (g.nc1 + 1);
-  const long long b = ((long long)k * g.nc2 + j) * (g.nc1 + 1) + i;
 #pragma unroll
-  for (int n = 0; n < N; ++n) f1[n * fs + b] = fx[n];
+  for (int n = 0; n < N; ++n) f1[g.f[0].at(n, k, j, i)] = fx[n];
 }
 
-// K1b/K1c: x2 and x3 face fluxes.  One lane per face, i on adjacent lanes.
+// K1b/K1c: x2 and x3 face fluxes.  One lane per face.
 template <int NM, int ORDER, int DIR>
 __global__ void __launch_bounds__(256)
     k_flux_h(KGrid g, KThermo th, int rsolver, const double* __restrict__ w,
              double* __restrict__ fh) {
   constexpr int N = 5 + NM;
   constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  const int n1 = g.nx1;
-  const int na = (DIR == 1) ? g.nx2 + 1 : g.nx2;  // j range
-  const int nb = (DIR == 2) ? g.nx3 + 1 : g.nx3;  // k range
+  const int n2 = (DIR == 1) ? g.nx2 + 1 : g.nx2;
+  const int n3 = (DIR == 2) ? g.nx3 + 1 : g.nx3;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n1 * na * nb) return;
-  int i = g.is + (int)(t % n1);
-  long long r = t / n1;
-  int j = g.js + (int)(r % na);
-  int k = g.ks + (int)(r / na);
-  const long long s = (DIR == 1) ? (long long)g.nc1 : (long long)g.nc1 * g.nc2;
+  if (t >= (long long)g.nx1 * n2 * n3) return;
+  int i, j, k;
+  g.decode(t, g.nx1, n2, n3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long s = (DIR == 1) ? g.c.j : g.c.k;
   const long long base = g.at(0, k, j, i);
   double q[N][6];
 #pragma unroll
   for (int m = LO; m <= HI; ++m)
 #pragma unroll
-    for (int n = 0; n < N; ++n) q[n][m] = w[base + n * g.sn + (m - 3) * s];
+    for (int n = 0; n < N; ++n) q[n][m] = w[base + n * g.c.n + (m - 3) * s];
   double wl[N], wr[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
@@ -185,16 +183,8 @@ __global__ void __launch_bounds__(256)
   }
   double fx[N];
   riemann<NM, DIR, double>(th, rsolver, wl, wr, fx);
-  long long fs, b;
-  if (DIR == 1) {
-    fs = (long long)g.nc3 * (g.nc2 + 1) * g.nc1;
-    b = ((long long)k * (g.nc2 + 1) + j) * g.nc1 + i;
-  } else {
-    fs = (long long)(g.nc3 + 1) * g.nc2 * g.nc1;
-    b = ((long long)k * g.nc2 + j) * g.nc1 + i;
-  }
 #pragma unroll
-  for (int n = 0; n < N; ++n) fh[n * fs + b] = fx[n];
+  for (int n = 0; n < N; ++n) fh[g.f[DIR].at(n, k, j, i)] = fx[n];
 }
 
 // Riemann solver on caller-owned face arrays (cg_riemann).
@@ -233,50 +223,25 @@ CG_HD double temperature_d(const KThermo& th, const double* wc) {
 }
 
 struct KSource {
-  double wght;       // beta dt
+  double wght;  // beta dt
   double visc, p0, cp, chi, heat_flux;
 };
 
-// K4a: du = 0 - wght div F (Hydro::AddFluxDivergence) then the source terms
-// in HydroSourceTerms order: constant acceleration (upstream form, see
-// patches/25.constant_acc.patch context), the straka diffusion
-// (examples/2019-Li-snap/straka.cpp:126-190), the bottom heating.
+// Source terms of one interior cell added to d[] in HydroSourceTerms order:
+// constant acceleration (upstream form, see patches/25.constant_acc.patch
+// context lines), the straka diffusion (examples/2019-Li-snap/straka.cpp:
+// 126-190), the bottom heating.
 template <int NM>
-__global__ void __launch_bounds__(256)
-    k_divergence(KGrid g, KThermo th, KSource src,
-                 const double* __restrict__ f1, const double* __restrict__ f2,
-                 const double* __restrict__ f3, const double* __restrict__ w,
-                 double* __restrict__ du) {
+__device__ inline void add_sources_cell(const KGrid& g, const KThermo& th,
+                                        const KSource& src,
+                                        const double* __restrict__ w, int k,
+                                        int j, int i, double* d) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IVY = IVX + 1, IEN = 4 + NM;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
-  int i = g.is + (int)(t % g.nx1);
-  long long r = t / g.nx1;
-  int j = g.js + (int)(r % g.nx2);
-  int k = g.ks + (int)(r / g.nx2);
-  const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
-  const double vol = g.dx1 * g.dx2 * g.dx3;
-  const long long fs1 = (long long)g.nc3 * g.nc2 * (g.nc1 + 1);
-  const long long fs2 = (long long)g.nc3 * (g.nc2 + 1) * g.nc1;
-  const long long fs3 = (long long)(g.nc3 + 1) * g.nc2 * g.nc1;
-  const long long b1 = ((long long)k * g.nc2 + j) * (g.nc1 + 1) + i;
-  const long long b2 = ((long long)k * (g.nc2 + 1) + j) * g.nc1 + i;
-  const long long b3 = ((long long)k * g.nc2 + j) * g.nc1 + i;
-  double d[N];
-#pragma unroll
-  for (int n = 0; n < N; ++n) {
-    double dflx = A1 * f1[n * fs1 + b1 + 1] - A1 * f1[n * fs1 + b1];
-    if (g.f2) dflx += A2 * f2[n * fs2 + b2 + g.nc1] - A2 * f2[n * fs2 + b2];
-    if (g.f3)
-      dflx += A3 * f3[n * fs3 + b3 + (long long)g.nc2 * g.nc1] -
-              A3 * f3[n * fs3 + b3];
-    d[n] = 0.0 - src.wght * dflx / vol;
-  }
   const long long c0 = g.at(0, k, j, i);
   if (g.grav != 0.0) {
     double s = src.wght * w[c0] * g.grav;
     d[IVX] += s;
-    d[IEN] += s * w[c0 + IVX * g.sn];
+    d[IEN] += s * w[c0 + IVX * g.c.n];
   }
   if (src.visc != 0.0 && g.f2) {
     const double K = src.visc, dx = g.dx1, dy = g.dx2, dt = src.wght;
@@ -305,8 +270,38 @@ __global__ void __launch_bounds__(256)
   }
   if (src.heat_flux != 0.0 && i == g.is)
     d[IEN] += src.wght * src.heat_flux / g.dx1;
+}
+
+// K4a: du = 0 - wght div F (Hydro::AddFluxDivergence) plus the sources.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_divergence(KGrid g, KThermo th, KSource src,
+                 const double* __restrict__ f1, const double* __restrict__ f2,
+                 const double* __restrict__ f3, const double* __restrict__ w,
+                 double* __restrict__ du) {
+  constexpr int N = 5 + NM;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
+  const double vol = g.dx1 * g.dx2 * g.dx3;
+  double d[N];
 #pragma unroll
-  for (int n = 0; n < N; ++n) du[c0 + n * g.sn] = d[n];
+  for (int n = 0; n < N; ++n) {
+    double dflx = A1 * f1[g.f[0].at(n, k, j, i + 1)] - A1 * f1[g.f[0].at(n, k, j, i)];
+    if (g.f2)
+      dflx += A2 * f2[g.f[1].at(n, k, j + 1, i)] - A2 * f2[g.f[1].at(n, k, j, i)];
+    if (g.f3)
+      dflx += A3 * f3[g.f[2].at(n, k + 1, j, i)] - A3 * f3[g.f[2].at(n, k, j, i)];
+    d[n] = 0.0 - src.wght * dflx / vol;
+  }
+  add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+#pragma unroll
+  for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 }
 
 // Flux divergence only, accumulated into a caller du (cg_add_flux_divergence)
@@ -318,27 +313,21 @@ __global__ void __launch_bounds__(256)
   constexpr int N = 5 + NM;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
-  int i = g.is + (int)(t % g.nx1);
-  long long r = t / g.nx1;
-  int j = g.js + (int)(r % g.nx2);
-  int k = g.ks + (int)(r / g.nx2);
+  int i, j, k;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
   const double vol = g.dx1 * g.dx2 * g.dx3;
-  const long long fs1 = (long long)g.nc3 * g.nc2 * (g.nc1 + 1);
-  const long long fs2 = (long long)g.nc3 * (g.nc2 + 1) * g.nc1;
-  const long long fs3 = (long long)(g.nc3 + 1) * g.nc2 * g.nc1;
-  const long long b1 = ((long long)k * g.nc2 + j) * (g.nc1 + 1) + i;
-  const long long b2 = ((long long)k * (g.nc2 + 1) + j) * g.nc1 + i;
-  const long long b3 = ((long long)k * g.nc2 + j) * g.nc1 + i;
-  const long long c0 = g.at(0, k, j, i);
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    double dflx = A1 * f1[n * fs1 + b1 + 1] - A1 * f1[n * fs1 + b1];
-    if (g.f2) dflx += A2 * f2[n * fs2 + b2 + g.nc1] - A2 * f2[n * fs2 + b2];
+    double dflx = A1 * f1[g.f[0].at(n, k, j, i + 1)] - A1 * f1[g.f[0].at(n, k, j, i)];
+    if (g.f2)
+      dflx += A2 * f2[g.f[1].at(n, k, j + 1, i)] - A2 * f2[g.f[1].at(n, k, j, i)];
     if (g.f3)
-      dflx += A3 * f3[n * fs3 + b3 + (long long)g.nc2 * g.nc1] -
-              A3 * f3[n * fs3 + b3];
-    du[c0 + n * g.sn] -= wght * dflx / vol;
+      dflx += A3 * f3[g.f[2].at(n, k + 1, j, i)] - A3 * f3[g.f[2].at(n, k, j, i)];
+    du[g.at(n, k, j, i)] -= wght * dflx / vol;
   }
 }
 
@@ -398,10 +387,7 @@ __device__ inline bool minv(Mt<B> m, Mt<B>& x) {
         p = r;
       }
     }
-    // pivot value is m[p][c]; zero pivot -> singular
-#pragma unroll
-    for (int r = c; r < B; ++r)
-      if (p == r && m.a[r][c] == 0.0) ok = false;
+    if (best == 0.0) ok = false;
 #pragma unroll
     for (int r = c + 1; r < B; ++r) {
       bool sw = (p == r);
@@ -463,7 +449,8 @@ __device__ inline void roe_average(double* prim, double gm1, const double* wl,
 // |A| = (R |Lambda|) R^-1 restricted to the B variables of the correction
 // (flux_decomposition.hpp:69-99; rows/cols (idn, ivx, ien) for B = 3,
 // partial_correction.cpp:94-97).  R |Lambda| is R with column c scaled by
-// |lambda_c| (the diagonal product adds only exact zeros).
+// |lambda_c|: the diagonal product of the reference adds only exact zeros,
+// so every nonzero entry is the same number.
 template <int NM, int B>
 __device__ inline void abs_roe(const KThermo& th, const double* prim,
                                double gm1, Mt<B>& out) {
@@ -492,18 +479,11 @@ __device__ inline void abs_roe(const KThermo& th, const double* prim,
                       -w / (2. * hp), 1. / (2. * hp)},
                      {-v, 0., 1., 0., 0.},
                      {-w, 0., 0., 1., 0.}};
-  // RL = R * diag(lam) as a full product: sum over k of R[r][k] L[k][c]
-  // with L[k][c] = 0 for k != c; computed as the same sum.
   double RL[5][5];
 #pragma unroll
   for (int rr = 0; rr < 5; ++rr)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double s = R[rr][0] * (c == 0 ? lam[0] : 0.0);
-#pragma unroll
-      for (int k = 1; k < 5; ++k) s += R[rr][k] * (c == k ? lam[k] : 0.0);
-      RL[rr][c] = s;
-    }
+    for (int c = 0; c < 5; ++c) RL[rr][c] = R[rr][c] * lam[c];
   const int id3[3] = {0, 1, 4};
 #pragma unroll
   for (int a = 0; a < B; ++a)
@@ -547,13 +527,63 @@ __device__ inline void flux_jac(double gm1, const double* w, Mt<B>& out) {
 struct KImplicit {
   double dt;
   long long ncol;  // interior columns nx2 nx3
-  int nlev;        // nx1
 };
+
+// |A| at a face, evaluated with the zero/one structure of R and R^-1 of
+// flux_decomposition.hpp:91-98 spelled out (terms whose factor is an exact
+// zero are dropped, unit factors taken exactly) and fused multiply-adds.
+template <int NM, int B>
+__device__ inline void abs_roe_fma(const KThermo& th, const double* prim,
+                                   double gm1, Mt<B>& out) {
+  constexpr int IVX = 1 + NM, IPR = IVX + 3;
+  double cs = sound_speed<NM, double>(th, prim);
+  double u = prim[IVX], v = prim[IVX + 1], w = prim[IVX + 2];
+  double hp = (gm1 + 1.) / gm1 * prim[IPR] / prim[0];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double h = hp + ke;
+  double l0 = fabs(u - cs), l1 = fabs(u), l2 = fabs(u + cs);
+  double r2h = 1. / (2. * hp), rhp = 1. / hp, rch = 1. / (2. * cs * hp);
+  // rows of R^-1 (0..2 full; 3 = (-v, 0, 1, 0, 0); 4 = (-w, 0, 0, 1, 0))
+  double Ri0[5] = {(cs * ke + u * hp) * rch, (-hp - cs * u) * rch, -v * r2h,
+                   -w * r2h, r2h};
+  double Ri1[5] = {(hp - ke) * rhp, u * rhp, v * rhp, w * rhp, -rhp};
+  double Ri2[5] = {(cs * ke - u * hp) * rch, (hp - cs * u) * rch, -v * r2h,
+                   -w * r2h, r2h};
+  // rows of R |Lambda|
+  double RL[5][5] = {{l0, l1, l2, 0., 0.},
+                     {(u - cs) * l0, u * l1, (u + cs) * l2, 0., 0.},
+                     {v * l0, v * l1, v * l2, l1, 0.},
+                     {w * l0, w * l1, w * l2, 0., l1},
+                     {(h - u * cs) * l0, ke * l1, (h + u * cs) * l2, v * l1,
+                      w * l1}};
+  const int id3[3] = {0, 1, 4};
+#pragma unroll
+  for (int a = 0; a < B; ++a) {
+    const int ra = (B == 5) ? a : id3[a];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      const int cc = (B == 5) ? c : id3[c];
+      double s = RL[ra][0] * Ri0[cc];
+      s = fma(RL[ra][1], Ri1[cc], s);
+      s = fma(RL[ra][2], Ri2[cc], s);
+      if (ra >= 2) {  // RL[.][3], RL[.][4] vanish for rows 0, 1
+        if (cc == 0) s = fma(RL[ra][3], -v, fma(RL[ra][4], -w, s));
+        if (cc == 2) s = s + RL[ra][3];
+        if (cc == 3) s = s + RL[ra][4];
+      }
+      out.a[a][c] = s;
+    }
+  }
+}
 
 // Forward sweep of one column per lane: assembles a_i, b_i, c_i
 // (full_correction.cpp:221-288), folds the reflecting walls (:286-288) and
 // runs ForwardSweep (forward_backward.hpp:14-84), archiving a'_i (B x B)
 // and delta_i (B) per level, lane-contiguous: arch[(lvl*(B*B+B)+e)*ncol+col].
+// a'_i = M^-1 c_i and delta_i = M^-1 (rhs_i - b_i delta_{i-1}) with
+// M = a_i - b_i a'_{i-1} come from one Gauss-Jordan elimination of
+// [M | c_i | rhs] with partial pivoting (the reference forms M^-1 and then
+// multiplies; the two agree to round-off).
 template <int NM, int B>
 __global__ void __launch_bounds__(64)
     k_implicit_fwd(KGrid g, KThermo th, KImplicit ip,
@@ -566,122 +596,164 @@ __global__ void __launch_bounds__(64)
   int j = g.js + (int)(col % g.nx2);
   int k = g.ks + (int)(col / g.nx2);
   const int is = g.is, ie = g.ie;
-  const double dt = ip.dt, grav = g.grav;
+  const double grav = g.grav;
   const double gm1 = th.gammad - 1.;
   const double gm = 0.5 * (gm1 + gm1);
   const double aleft = g.dx2 * g.dx3, aright = g.dx2 * g.dx3;
   const double vol = g.dx1 * g.dx2 * g.dx3;
+  const double s2v = 1. / (2. * vol);
+  const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
+  const double idt = 1. / ip.dt;
   const long long c0 = g.at(0, k, j, 0);
+  const long long sn = g.c.n, si = g.c.i;
   auto load = [&](double* dst, int i) {
 #pragma unroll
-    for (int n = 0; n < N; ++n) dst[n] = w[c0 + n * g.sn + i];
+    for (int n = 0; n < N; ++n) dst[n] = w[c0 + n * sn + i * si];
   };
   double wm[N], wc[N], wp[N], prim[N];
   load(wm, is - 1);
   load(wc, is);
   Mt<B> Am, Ap, dm, dc, dp;
   roe_average<NM>(prim, gm, wm, wc);
-  abs_roe<NM, B>(th, prim, gm, Am);
+  abs_roe_fma<NM, B>(th, prim, gm, Am);
   flux_jac<NM, B>(gm1, wm, dm);
   flux_jac<NM, B>(gm1, wc, dc);
-  Mt<B> ap;   // a'_{i-1}
-  Vt<B> dl;   // delta_{i-1}
+  Mt<B> ap;  // a'_{i-1}
+  Vt<B> dl;  // delta_{i-1}
   bool ok = true;
   for (int i = is; i <= ie; ++i) {
     load(wp, i + 1);
     roe_average<NM>(prim, gm, wc, wp);
-    abs_roe<NM, B>(th, prim, gm, Ap);
+    abs_roe_fma<NM, B>(th, prim, gm, Ap);
     flux_jac<NM, B>(gm1, wp, dp);
-    Mt<B> a, b, c;
+    Mt<B> M, b, c;
+    double t[B];
+    // rhs (forward_backward.hpp:30-36)
+    {
+      const long long ci = c0 + i * si;
+      double r0 = du[ci];
+#pragma unroll
+      for (int n = 1; n < IVX; ++n) r0 += du[ci + n * sn];
+      t[0] = r0 * idt;
+      if (B == 5) {
+        t[1] = du[ci + IVX * sn] * idt;
+        t[2] = du[ci + (IVX + 1) * sn] * idt;
+        t[3] = du[ci + (IVX + 2) * sn] * idt;
+        t[4] = du[ci + IEN * sn] * idt;
+      } else {
+        t[1] = du[ci + IVX * sn] * idt;
+        t[2] = du[ci + IEN * sn] * idt;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        double dtq = (r == q) ? 1. / dt : 0.0;
         double phi = 0.0;
         if (B == 5) {
           if ((r == 1 && q == 0) || (r == 4 && q == 1)) phi = grav;
         } else {
           if ((r == 1 && q == 0) || (r == 2 && q == 1)) phi = grav;
         }
-        a.a[r][q] = (Am.a[r][q] * aleft + Ap.a[r][q] * aright +
-                     (aright - aleft) * dc.a[r][q]) /
-                        (2. * vol) +
-                    dtq - phi;
-        b.a[r][q] = -(Am.a[r][q] + dm.a[r][q]) * aleft / (2. * vol);
-        c.a[r][q] = -(Ap.a[r][q] - dp.a[r][q]) * aright / (2. * vol);
+        double a = fma(Am.a[r][q], sl, fma(Ap.a[r][q], sr, dc.a[r][q] * sd));
+        if (r == q) a += idt;
+        M.a[r][q] = a - phi;
+        b.a[r][q] = -(Am.a[r][q] + dm.a[r][q]) * sl;
+        c.a[r][q] = -(Ap.a[r][q] - dp.a[r][q]) * sr;
       }
-    // reflecting walls: a_is += b_is Bnd, a_ie += c_ie Bnd
-    if ((i == is && g.bc[0] != 1) || (i == ie && g.bc[1] != 1)) {
-      Mt<B> bnd, t;
+    // reflecting walls: a_is += b_is diag(1,-1,1,..), a_ie += c_ie diag(..)
+    if (i == is && g.bc[0] != 1) {
 #pragma unroll
       for (int r = 0; r < B; ++r)
 #pragma unroll
         for (int q = 0; q < B; ++q)
-          bnd.a[r][q] = (r == q) ? ((r == 1) ? -1. : 1.) : 0.0;
-      if (i == is && g.bc[0] != 1) {
-        mmul<B>(b, bnd, t);
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-          for (int q = 0; q < B; ++q) a.a[r][q] += t.a[r][q];
-      }
-      if (i == ie && g.bc[1] != 1) {
-        mmul<B>(c, bnd, t);
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-          for (int q = 0; q < B; ++q) a.a[r][q] += t.a[r][q];
-      }
+          M.a[r][q] += (q == 1) ? -b.a[r][q] : b.a[r][q];
     }
-    // rhs (forward_backward.hpp:30-36)
-    Vt<B> rhs;
-    {
-      double r0 = du[c0 + i];
-#pragma unroll
-      for (int n = 1; n < IVX; ++n) r0 += du[c0 + n * g.sn + i];
-      r0 /= dt;
-      rhs.v[0] = r0;
-      if (B == 5) {
-        rhs.v[1] = du[c0 + IVX * g.sn + i] / dt;
-        rhs.v[2] = du[c0 + (IVX + 1) * g.sn + i] / dt;
-        rhs.v[3] = du[c0 + (IVX + 2) * g.sn + i] / dt;
-        rhs.v[4] = du[c0 + IEN * g.sn + i] / dt;
-      } else {
-        rhs.v[1] = du[c0 + IVX * g.sn + i] / dt;
-        rhs.v[2] = du[c0 + IEN * g.sn + i] / dt;
-      }
-    }
-    Mt<B> inv;
-    Vt<B> dn;
-    if (i == is) {
-      ok = minv<B>(a, inv) && ok;
-      mvec<B>(inv, rhs, dn);
-    } else {
-      Mt<B> ba;
-      mmul<B>(b, ap, ba);
+    if (i == ie && g.bc[1] != 1) {
 #pragma unroll
       for (int r = 0; r < B; ++r)
 #pragma unroll
-        for (int q = 0; q < B; ++q) a.a[r][q] = a.a[r][q] - ba.a[r][q];
-      ok = minv<B>(a, inv) && ok;
-      Vt<B> bd, tv;
-      mvec<B>(b, dl, bd);
-#pragma unroll
-      for (int r = 0; r < B; ++r) tv.v[r] = rhs.v[r] - bd.v[r];
-      mvec<B>(inv, tv, dn);
+        for (int q = 0; q < B; ++q)
+          M.a[r][q] += (q == 1) ? -c.a[r][q] : c.a[r][q];
     }
-    mmul<B>(inv, c, ap);
-    dl = dn;
+    if (i > is) {
+      // M -= b a'_{i-1};  t -= b delta_{i-1}
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+          double s = M.a[r][q];
+#pragma unroll
+          for (int kk = 0; kk < B; ++kk) s = fma(-b.a[r][kk], ap.a[kk][q], s);
+          M.a[r][q] = s;
+        }
+        double s = t[r];
+#pragma unroll
+        for (int kk = 0; kk < B; ++kk) s = fma(-b.a[r][kk], dl.v[kk], s);
+        t[r] = s;
+      }
+    }
+    // Gauss-Jordan on [M | c | t]
+#pragma unroll
+    for (int cc = 0; cc < B; ++cc) {
+      int p = cc;
+      double best = fabs(M.a[cc][cc]);
+#pragma unroll
+      for (int r = cc + 1; r < B; ++r) {
+        double v = fabs(M.a[r][cc]);
+        if (v > best) {
+          best = v;
+          p = r;
+        }
+      }
+      if (best == 0.0) ok = false;
+      if (p != cc) {
+#pragma unroll
+        for (int r = cc + 1; r < B; ++r) {
+          bool sw = (p == r);
+#pragma unroll
+          for (int q = 0; q < B; ++q) {
+            double x0 = M.a[cc][q], x1 = M.a[r][q];
+            M.a[cc][q] = sw ? x1 : x0;
+            M.a[r][q] = sw ? x0 : x1;
+            double y0 = c.a[cc][q], y1 = c.a[r][q];
+            c.a[cc][q] = sw ? y1 : y0;
+            c.a[r][q] = sw ? y0 : y1;
+          }
+          double z0 = t[cc], z1 = t[r];
+          t[cc] = sw ? z1 : z0;
+          t[r] = sw ? z0 : z1;
+        }
+      }
+      double inv = 1.0 / M.a[cc][cc];
+#pragma unroll
+      for (int q = cc + 1; q < B; ++q) M.a[cc][q] *= inv;
+#pragma unroll
+      for (int q = 0; q < B; ++q) c.a[cc][q] *= inv;
+      t[cc] *= inv;
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        if (r == cc) continue;
+        double f = M.a[r][cc];
+#pragma unroll
+        for (int q = cc + 1; q < B; ++q) M.a[r][q] = fma(-f, M.a[cc][q], M.a[r][q]);
+#pragma unroll
+        for (int q = 0; q < B; ++q) c.a[r][q] = fma(-f, c.a[cc][q], c.a[r][q]);
+        t[r] = fma(-f, t[cc], t[r]);
+      }
+    }
+    ap = c;
+#pragma unroll
+    for (int r = 0; r < B; ++r) dl.v[r] = t[r];
     const long long lb = (long long)(i - is) * AS;
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
       for (int q = 0; q < B; ++q)
-        arch[(lb + r * B + q) * ip.ncol + col] = ap.a[r][q];
+        __builtin_nontemporal_store(ap.a[r][q], &arch[(lb + r * B + q) * ip.ncol + col]);
 #pragma unroll
-    for (int r = 0; r < B; ++r) arch[(lb + B * B + r) * ip.ncol + col] = dl.v[r];
-    // shift the window
+    for (int r = 0; r < B; ++r)
+      __builtin_nontemporal_store(dl.v[r], &arch[(lb + B * B + r) * ip.ncol + col]);
     Am = Ap;
     dm = dc;
     dc = dp;
@@ -694,13 +766,62 @@ __global__ void __launch_bounds__(64)
   if (!ok) atomicOr(status, 1);
 }
 
-// Back substitution (forward_backward.hpp:86-124) and the write-back of
-// du (species split of the density increment, :221-246).
+// delta_i of the back substitution (forward_backward.hpp:86-97): level i
+// from the archive and delta_{i+1}.
+template <int B>
+__device__ inline void back_step(const double* __restrict__ arch,
+                                 long long lb, long long ncol, long long col,
+                                 bool top, const Vt<B>& dn, Vt<B>& d) {
+#pragma unroll
+  for (int r = 0; r < B; ++r) d.v[r] = arch[(lb + B * B + r) * ncol + col];
+  if (!top) {
+    Mt<B> a;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int q = 0; q < B; ++q) a.a[r][q] = arch[(lb + r * B + q) * ncol + col];
+    Vt<B> ad;
+    mvec<B>(a, dn, ad);
+#pragma unroll
+    for (int r = 0; r < B; ++r) d.v[r] = d.v[r] - ad.v[r];
+  }
+}
+
+// Write-back of the solution into a cell's du (forward_backward.hpp:221-246:
+// the total-density increment is split over the species by w_n).
+template <int NM, int B>
+__device__ inline void write_back(const Vt<B>& d, const double* duo,
+                                  const double* wsp, double* dun) {
+  constexpr int IVX = 1 + NM, IEN = 4 + NM;
+  double dens = duo[0];
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) dens += duo[n];
+  dens = d.v[0] - dens;
+  double d0 = d.v[0];
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) dun[n] = duo[n] + dens * wsp[n];
+  if (B == 5) {
+    dun[IVX] = d.v[1];
+    dun[IVX + 1] = d.v[2];
+    dun[IVX + 2] = d.v[3];
+    dun[IEN] = d.v[4];
+  } else {
+    dun[IVX] = d.v[1];
+    dun[IVX + 1] = duo[IVX + 1];
+    dun[IVX + 2] = duo[IVX + 2];
+    dun[IEN] = d.v[2];
+  }
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) d0 -= dun[n];
+  dun[0] = d0;
+}
+
+// Back substitution writing du in place (cg_implicit_solve).
 template <int NM, int B>
 __global__ void __launch_bounds__(64)
     k_implicit_bwd(KGrid g, KImplicit ip, const double* __restrict__ w,
                    const double* __restrict__ arch, double* __restrict__ du) {
-  constexpr int IVX = 1 + NM, IEN = 4 + NM;
+  constexpr int N = 5 + NM;
   constexpr int AS = B * B + B;
   long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ip.ncol) return;
@@ -708,56 +829,31 @@ __global__ void __launch_bounds__(64)
   int k = g.ks + (int)(col / g.nx2);
   const int is = g.is, ie = g.ie;
   const long long c0 = g.at(0, k, j, 0);
-  Vt<B> dn;  // delta_{i+1}
+  const long long sn = g.c.n, si = g.c.i;
+  Vt<B> dn;
   for (int i = ie; i >= is; --i) {
-    const long long lb = (long long)(i - is) * AS;
     Vt<B> d;
-#pragma unroll
-    for (int r = 0; r < B; ++r) d.v[r] = arch[(lb + B * B + r) * ip.ncol + col];
-    if (i < ie) {
-      Mt<B> a;
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int q = 0; q < B; ++q) a.a[r][q] = arch[(lb + r * B + q) * ip.ncol + col];
-      Vt<B> ad;
-      mvec<B>(a, dn, ad);
-#pragma unroll
-      for (int r = 0; r < B; ++r) d.v[r] = d.v[r] - ad.v[r];
-    }
+    back_step<B>(arch, (long long)(i - is) * AS, ip.ncol, col, i == ie, dn, d);
     dn = d;
-    // write-back
-    double dens = du[c0 + i];
+    const long long ci = c0 + i * si;
+    double duo[N], wsp[N], dun[N];
 #pragma unroll
-    for (int n = 1; n < IVX; ++n) dens += du[c0 + n * g.sn + i];
-    dens = d.v[0] - dens;
-    double d0 = d.v[0];
-    double dsp[NM > 0 ? NM : 1];
-#pragma unroll
-    for (int n = 1; n < IVX; ++n) {
-      dsp[n - 1] = du[c0 + n * g.sn + i] + dens * w[c0 + n * g.sn + i];
-      du[c0 + n * g.sn + i] = dsp[n - 1];
+    for (int n = 0; n < N; ++n) {
+      duo[n] = (NM > 0 || n >= 1 + NM) ? du[ci + n * sn] : 0.0;
+      wsp[n] = (n >= 1 && n <= NM) ? w[ci + n * sn] : 0.0;
     }
-    if (B == 5) {
-      du[c0 + IVX * g.sn + i] = d.v[1];
-      du[c0 + (IVX + 1) * g.sn + i] = d.v[2];
-      du[c0 + (IVX + 2) * g.sn + i] = d.v[3];
-      du[c0 + IEN * g.sn + i] = d.v[4];
-    } else {
-      du[c0 + IVX * g.sn + i] = d.v[1];
-      du[c0 + IEN * g.sn + i] = d.v[2];
-    }
+    if (NM == 0) duo[0] = du[ci];
+    write_back<NM, B>(d, duo, wsp, dun);
 #pragma unroll
-    for (int n = 1; n < IVX; ++n) d0 -= dsp[n - 1];
-    du[c0 + i] = d0;
+    for (int n = 0; n < N; ++n) du[ci + n * sn] = dun[n];
   }
 }
 
 // ---------------------------------------------------------------------------
-// K4b: end of stage per cell: u1 / u averaging (WeightedAve,
+// End of stage per cell: u1 / u averaging (WeightedAve,
 // implicit_hydro_tasks.cpp:146-160), u += du (:277-283), saturation
-// adjustment on the last stage (UpdateAllConserved, :290-313), and
-// ConservedToPrimitive into w.
+// adjustment on the last stage (UpdateAllConserved, :290-313),
+// ConservedToPrimitive.  Returns the new u (uc) and w (wc).
 struct KUpdate {
   int stage;  // 0, 1, 2
   double g1, g2;
@@ -765,30 +861,23 @@ struct KUpdate {
 };
 
 template <int NM>
-__global__ void __launch_bounds__(256)
-    k_update(KGrid g, KThermo th, KUpdate up, double* __restrict__ u,
-             double* __restrict__ u1, const double* __restrict__ du,
-             double* __restrict__ w, int* __restrict__ status) {
+__device__ inline void update_cell(const KThermo& th, const KUpdate& up,
+                                   double* __restrict__ u,
+                                   double* __restrict__ u1, long long ci,
+                                   long long sn, const double* dun, double* uc,
+                                   double* wc, int* __restrict__ status) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
-  int i = g.is + (int)(t % g.nx1);
-  long long r = t / g.nx1;
-  int j = g.js + (int)(r % g.nx2);
-  int k = g.ks + (int)(r / g.nx2);
-  const long long c0 = g.at(0, k, j, i);
-  double uc[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    double un = u[c0 + n * g.sn];
+    double un = u[ci + n * sn];
     double v;
     if (up.stage == 0) {
       v = 0.0 + 1.0 * un;
-      u1[c0 + n * g.sn] = v;
+      u1[ci + n * sn] = v;
     } else {
-      v = up.g1 * un + up.g2 * u1[c0 + n * g.sn];
+      v = up.g1 * un + up.g2 * u1[ci + n * sn];
     }
-    uc[n] = v + du[c0 + n * g.sn];
+    uc[n] = v + dun[n];
   }
   if (up.satadj && NM > 0 && th.iv >= 0) {
     double rho = uc[0];
@@ -806,13 +895,154 @@ __global__ void __launch_bounds__(256)
     for (int n = 0; n < NM; ++n) uc[n + 1] = q[n] * rho;
     uc[IEN] = ie + ke;
   }
-  double wc[N];
   cons_to_prim_cell<NM>(th, uc, wc);
+}
+
+// Cell-parallel end of stage (explicit runs).
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_update(KGrid g, KThermo th, KUpdate up, double* __restrict__ u,
+             double* __restrict__ u1, const double* __restrict__ du,
+             double* __restrict__ w, int* __restrict__ status) {
+  constexpr int N = 5 + NM;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long c0 = g.at(0, k, j, i);
+  double dun[N], uc[N], wc[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) dun[n] = du[c0 + n * g.c.n];
+  update_cell<NM>(th, up, u, u1, c0, g.c.n, dun, uc, wc, status);
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    u[c0 + n * g.sn] = uc[n];
-    w[c0 + n * g.sn] = wc[n];
+    u[c0 + n * g.c.n] = uc[n];
+    w[c0 + n * g.c.n] = wc[n];
   }
+}
+
+// Implicit runs: one downward sweep per column does the back substitution,
+// the write-back into du, the end-of-stage update of u and w, the x1 ghost
+// cells of w (reflecting / outflow) and the hydrostatic face and cell
+// pressures of the next stage (the same recurrences as k_hydrostatic_scan,
+// evaluated as the new state appears top-down).
+template <int NM, int B>
+__global__ void __launch_bounds__(64)
+    k_implicit_bwd_update(KGrid g, KThermo th, KImplicit ip, KUpdate up,
+                          const double* __restrict__ arch,
+                          const double* __restrict__ du, double* __restrict__ u,
+                          double* __restrict__ u1, double* __restrict__ w,
+                          double* __restrict__ psf, double* __restrict__ psv,
+                          int* __restrict__ status) {
+  constexpr int N = 5 + NM, IVX = 1 + NM, IPR = 4 + NM;
+  constexpr int AS = B * B + B;
+  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ip.ncol) return;
+  int j = g.js + (int)(col % g.nx2);
+  int k = g.ks + (int)(col / g.nx2);
+  const int is = g.is, ie = g.ie;
+  const long long c0 = g.at(0, k, j, 0);
+  const long long sn = g.c.n, si = g.c.i;
+  const long long pc = col * g.ps_c;
+  const double grav = g.grav, dx = g.dx1;
+  const bool hs = grav != 0.0;
+  const double gb = (g.bc[0] == 0) ? -grav : grav;
+  const double gt = (g.bc[1] == 0) ? -grav : grav;
+  double rtop[NG];    // density of the top three cells (ie, ie-1, ie-2)
+  double pf_top[NG + 1];  // psf(ie+1) .. psf(ie+4)
+  double pf_next = 0.0;   // psf(i+1)
+  Vt<B> dn;
+  for (int i = ie; i >= is; --i) {
+    Vt<B> d;
+    back_step<B>(arch, (long long)(i - is) * AS, ip.ncol, col, i == ie, dn, d);
+    dn = d;
+    const long long ci = c0 + i * si;
+    double duo[N], wsp[N], dun[N], uc[N], wc[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      duo[n] = (n <= NM || B == 3) ? du[ci + n * sn] : 0.0;
+      wsp[n] = (n >= 1 && n <= NM) ? w[ci + n * sn] : 0.0;
+    }
+    write_back<NM, B>(d, duo, wsp, dun);
+    update_cell<NM>(th, up, u, u1, ci, sn, dun, uc, wc, status);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      u[ci + n * sn] = uc[n];
+      w[ci + n * sn] = wc[n];
+    }
+    // x1 ghosts mirrored from the cells next to the walls
+    const int mt = ie - i, mb = i - is;  // distance from top / bottom wall
+    if (mt < NG) {
+      int gi = ie + 1 + mt;
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        double v = (g.bc[1] == 0) ? ((n == IVX) ? -wc[n] : wc[n])
+                                  : w[c0 + n * sn + ie * si];
+        if (g.bc[1] == 2 && mt == 0) v = wc[n];
+        w[c0 + n * sn + gi * si] = v;
+      }
+    }
+    if (mb < NG) {
+      int gi = is - 1 - mb;
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        double v = (g.bc[0] == 0) ? ((n == IVX) ? -wc[n] : wc[n]) : 0.0;
+        w[c0 + n * sn + gi * si] = v;
+      }
+    }
+    if (hs) {
+      if (i == ie) {
+        double RdTv = wc[IPR] / wc[0];
+        pf_next = wc[IPR] * exp_d(grav * dx / (2 * RdTv));
+        pf_top[0] = pf_next;
+        psf[pc + (ie + 1) * g.ps_i] = pf_next;
+      }
+      if (mt < NG) rtop[mt] = wc[0];
+      double pfi = pf_next - grav * wc[0] * dx;
+      psf[pc + i * g.ps_i] = pfi;
+      psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
+      pf_next = pfi;
+    }
+  }
+  // outflow bottom ghosts copy the first interior cell
+  if (g.bc[0] == 2) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double v = w[c0 + n * sn + is * si];
+      for (int gh = 1; gh <= NG; ++gh) w[c0 + n * sn + (is - gh) * si] = v;
+    }
+  }
+  if (g.bc[1] == 2) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double v = w[c0 + n * sn + ie * si];
+      for (int gh = 1; gh <= NG; ++gh) w[c0 + n * sn + (ie + gh) * si] = v;
+    }
+  }
+  if (hs) {
+    // bottom ghosts: psf(is-1..is-3), psv of those cells
+    double cur = pf_next;  // psf(is)
+    for (int gh = 1; gh <= NG; ++gh) {
+      double rho = w[c0 + (is - gh) * si];
+      double lo = cur - gb * rho * dx;
+      psf[pc + (is - gh) * g.ps_i] = lo;
+      psv[pc + (is - gh) * g.ps_i] = log_mean(lo, cur);
+      cur = lo;
+    }
+    // top ghosts: psf(ie+2..ie+4) upward from psf(ie+1)
+    cur = pf_top[0];
+    for (int gh = 1; gh <= NG; ++gh) {
+      double rho = w[c0 + (ie + gh) * si];
+      double hi = cur + gt * rho * dx;
+      psf[pc + (ie + gh + 1) * g.ps_i] = hi;
+      psv[pc + (ie + gh) * g.ps_i] = log_mean(cur, hi);
+      cur = hi;
+    }
+  }
+  (void)rtop;
 }
 
 // cons <-> prim on interior cells
@@ -823,20 +1053,36 @@ __global__ void __launch_bounds__(256)
   constexpr int N = 5 + NM;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
-  int i = g.is + (int)(t % g.nx1);
-  long long r = t / g.nx1;
-  int j = g.js + (int)(r % g.nx2);
-  int k = g.ks + (int)(r / g.nx2);
+  int i, j, k;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
   const long long c0 = g.at(0, k, j, i);
   double a[N], b[N];
 #pragma unroll
-  for (int n = 0; n < N; ++n) a[n] = src[c0 + n * g.sn];
+  for (int n = 0; n < N; ++n) a[n] = src[c0 + n * g.c.n];
   if (TO_PRIM)
     cons_to_prim_cell<NM>(th, a, b);
   else
     prim_to_cons_cell<NM>(th, a, b);
 #pragma unroll
-  for (int n = 0; n < N; ++n) dst[c0 + n * g.sn] = b[n];
+  for (int n = 0; n < N; ++n) dst[c0 + n * g.c.n] = b[n];
+}
+
+// Copy between two layouts of the same cell array (all cells incl. ghosts);
+// lanes follow the destination's unit-stride index.
+__global__ void __launch_bounds__(256)
+    k_relayout(KGrid gs, KGrid gd, int N, const double* __restrict__ src,
+               double* __restrict__ dst) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long per = (long long)gd.nc1 * gd.nc2 * gd.nc3;
+  if (t >= per * N) return;
+  int n = (int)(t / per);
+  long long r = t % per;
+  int i, j, k;
+  gd.decode(r, gd.nc1, gd.nc2, gd.nc3, i, j, k);
+  dst[gd.at(n, k, j, i)] = src[gs.at(n, k, j, i)];
 }
 
 // ---------------------------------------------------------------------------
@@ -846,8 +1092,6 @@ __global__ void __launch_bounds__(256)
 __device__ inline void bc_pair(double* w, long long sn, int N, int vn, int bci,
                                int bco, long long a_lo, long long a_hi,
                                long long stride, int gh) {
-  // a_lo: address of the first interior cell, a_hi: of the last, stride
-  // between cells along the direction; gh = 1..3
   for (int n = 0; n < N; ++n) {
     double sgn = (n == vn) ? -1.0 : 1.0;
     long long o = n * sn;
@@ -872,39 +1116,33 @@ __device__ inline void bc_pair(double* w, long long sn, int N, int vn, int bci,
 template <int NM>
 __global__ void __launch_bounds__(256) k_bc_x1(KGrid g, double* w) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long nl = (long long)g.nx2 * g.nx3;
-  if (t >= nl * NG) return;
-  int gh = 1 + (int)(t % NG);
-  long long l = t / NG;
-  int j = g.js + (int)(l % g.nx2), k = g.ks + (int)(l / g.nx2);
-  bc_pair(w, g.sn, 5 + NM, 1 + NM, g.bc[0], g.bc[1], g.at(0, k, j, g.is),
-          g.at(0, k, j, g.ie), 1, gh);
+  if (t >= (long long)g.nx2 * g.nx3 * NG) return;
+  int gh, j, k;
+  g.decode(t, NG, g.nx2, g.nx3, gh, j, k);
+  bc_pair(w, g.c.n, 5 + NM, 1 + NM, g.bc[0], g.bc[1],
+          g.at(0, k + g.ks, j + g.js, g.is), g.at(0, k + g.ks, j + g.js, g.ie),
+          g.c.i, gh + 1);
 }
 
 template <int NM>
 __global__ void __launch_bounds__(256) k_bc_x2(KGrid g, double* w) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long nl = (long long)g.nc1 * g.nx3;
-  if (t >= nl * NG) return;
-  int i = (int)(t % g.nc1);
-  long long l = t / g.nc1;
-  int gh = 1 + (int)(l % NG);
-  int k = g.ks + (int)(l / NG);
-  bc_pair(w, g.sn, 5 + NM, 2 + NM, g.bc[2], g.bc[3], g.at(0, k, g.js, i),
-          g.at(0, k, g.je, i), g.nc1, gh);
+  if (t >= (long long)g.nc1 * g.nx3 * NG) return;
+  int i, gh, k;
+  g.decode(t, g.nc1, NG, g.nx3, i, gh, k);
+  bc_pair(w, g.c.n, 5 + NM, 2 + NM, g.bc[2], g.bc[3],
+          g.at(0, k + g.ks, g.js, i), g.at(0, k + g.ks, g.je, i), g.c.j,
+          gh + 1);
 }
 
 template <int NM>
 __global__ void __launch_bounds__(256) k_bc_x3(KGrid g, double* w) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long nl = (long long)g.nc1 * g.nc2;
-  if (t >= nl * NG) return;
-  int i = (int)(t % g.nc1);
-  long long l = t / g.nc1;
-  int j = (int)(l % g.nc2);
-  int gh = 1 + (int)(l / g.nc2);
-  bc_pair(w, g.sn, 5 + NM, 3 + NM, g.bc[4], g.bc[5], g.at(0, g.ks, j, i),
-          g.at(0, g.ke, j, i), (long long)g.nc1 * g.nc2, gh);
+  if (t >= (long long)g.nc1 * g.nc2 * NG) return;
+  int i, j, gh;
+  g.decode(t, g.nc1, g.nc2, NG, i, j, gh);
+  bc_pair(w, g.c.n, 5 + NM, 3 + NM, g.bc[4], g.bc[5], g.at(0, g.ks, j, i),
+          g.at(0, g.ke, j, i), g.c.k, gh + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -942,11 +1180,9 @@ __global__ void __launch_bounds__(256)
     double s = 0.0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
          t < total; t += (long long)gridDim.x * blockDim.x) {
-      int i = g.is + (int)(t % g.nx1);
-      long long r = t / g.nx1;
-      int j = g.js + (int)(r % g.nx2);
-      int k = g.ks + (int)(r / g.nx2);
-      s += u[g.at(n, k, j, i)] * vol;
+      int i, j, k;
+      g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+      s += u[g.at(n, k + g.ks, j + g.js, i + g.is)] * vol;
     }
     sh[threadIdx.x] = s;
     __syncthreads();

@@ -17,6 +17,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -664,6 +665,46 @@ bool inverse(Mat<B> m, Mat<B>* out) {
   return true;
 }
 
+// Gauss-Jordan elimination of [M | C | t] with partial pivoting (first row
+// of largest |pivot|), the pivot row scaled by the pivot's reciprocal and
+// the other rows updated by fused multiply-adds: on return C = M^-1 C and
+// t = M^-1 t.  Returns false if a pivot is exactly zero.
+template <int B>
+bool gj_solve(Mat<B> M, Mat<B>& C, Vec<B>& t) {
+  for (int cc = 0; cc < B; ++cc) {
+    int p = cc;
+    double best = std::fabs(M.a[cc][cc]);
+    for (int r = cc + 1; r < B; ++r) {
+      double v = std::fabs(M.a[r][cc]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (best == 0.0) return false;
+    if (p != cc) {
+      for (int q = 0; q < B; ++q) {
+        std::swap(M.a[cc][q], M.a[p][q]);
+        std::swap(C.a[cc][q], C.a[p][q]);
+      }
+      std::swap(t.v[cc], t.v[p]);
+    }
+    double inv = 1.0 / M.a[cc][cc];
+    for (int q = cc + 1; q < B; ++q) M.a[cc][q] *= inv;
+    for (int q = 0; q < B; ++q) C.a[cc][q] *= inv;
+    t.v[cc] *= inv;
+    for (int r = 0; r < B; ++r) {
+      if (r == cc) continue;
+      double f = M.a[r][cc];
+      for (int q = cc + 1; q < B; ++q)
+        M.a[r][q] = std::fma(-f, M.a[cc][q], M.a[r][q]);
+      for (int q = 0; q < B; ++q) C.a[r][q] = std::fma(-f, C.a[cc][q], C.a[r][q]);
+      t.v[r] = std::fma(-f, t.v[cc], t.v[r]);
+    }
+  }
+  return true;
+}
+
 // RoeAverage: src/snap/implicit/flux_decomposition.hpp:34-67
 void roe_average(const Grid& g, double* prim, double gm1, const double* wl,
                  const double* wr) {
@@ -688,62 +729,46 @@ void roe_average(const Grid& g, double* prim, double gm1, const double* wl,
 }
 
 // |A| = R |Lambda| R^-1 at a face: Eigenvalue / Eigenvector
-// (flux_decomposition.hpp:69-99), product order (R Lambda) R^-1.
+// (flux_decomposition.hpp:69-99).  The product (R |Lambda|) R^-1 is
+// evaluated with the zero/one pattern of R and R^-1 spelled out (terms with
+// an exact zero factor dropped, unit factors taken exactly), reciprocals of
+// 2 hp, hp and 2 cs hp, and fused multiply-adds; equal to the reference's
+// dense Eigen product up to round-off.
 Mat<5> abs_roe_matrix(const Grid& g, const cg_thermo& th, const double* prim,
                       double gm1) {
   const int IPR = g.IPR;
   double cs = sound_speed(th, prim, IPR);
-  double r = prim[0];
-  double u = prim[g.IVX];
-  double v = prim[g.IVY];
-  double w = prim[g.IVZ];
-  double p = prim[IPR];
-  Mat<5> L;
-  L.zero();
-  L.a[0][0] = std::fabs(u - cs);
-  L.a[1][1] = std::fabs(u);
-  L.a[2][2] = std::fabs(u + cs);
-  L.a[3][3] = std::fabs(u);
-  L.a[4][4] = std::fabs(u);
+  double u = prim[g.IVX], v = prim[g.IVY], w = prim[g.IVZ];
+  double hp = (gm1 + 1.) / gm1 * prim[IPR] / prim[0];
   double ke = 0.5 * (u * u + v * v + w * w);
-  double hp = (gm1 + 1.) / gm1 * p / r;
   double h = hp + ke;
-  Mat<5> R, Ri;
-  double Rv[25] = {1.,          1.,  1.,          0., 0.,  //
-                   u - cs,      u,   u + cs,      0., 0.,  //
-                   v,           v,   v,           1., 0.,  //
-                   w,           w,   w,           0., 1.,  //
-                   h - u * cs, ke, h + u * cs, v,  w};
-  double Riv[25] = {(cs * ke + u * hp) / (2. * cs * hp),
-                    (-hp - cs * u) / (2. * cs * hp),
-                    -v / (2. * hp),
-                    -w / (2. * hp),
-                    1. / (2. * hp),
-                    (hp - ke) / hp,
-                    u / hp,
-                    v / hp,
-                    w / hp,
-                    -1. / hp,
-                    (cs * ke - u * hp) / (2. * cs * hp),
-                    (hp - cs * u) / (2. * cs * hp),
-                    -v / (2. * hp),
-                    -w / (2. * hp),
-                    1. / (2. * hp),
-                    -v,
-                    0.,
-                    1.,
-                    0.,
-                    0.,
-                    -w,
-                    0.,
-                    0.,
-                    1.,
-                    0.};
-  for (int q = 0; q < 25; ++q) {
-    R.a[q / 5][q % 5] = Rv[q];
-    Ri.a[q / 5][q % 5] = Riv[q];
-  }
-  return mul(mul(R, L), Ri);
+  double l0 = std::fabs(u - cs), l1 = std::fabs(u), l2 = std::fabs(u + cs);
+  double r2h = 1. / (2. * hp), rhp = 1. / hp, rch = 1. / (2. * cs * hp);
+  double Ri0[5] = {(cs * ke + u * hp) * rch, (-hp - cs * u) * rch, -v * r2h,
+                   -w * r2h, r2h};
+  double Ri1[5] = {(hp - ke) * rhp, u * rhp, v * rhp, w * rhp, -rhp};
+  double Ri2[5] = {(cs * ke - u * hp) * rch, (hp - cs * u) * rch, -v * r2h,
+                   -w * r2h, r2h};
+  double RL[5][5] = {{l0, l1, l2, 0., 0.},
+                     {(u - cs) * l0, u * l1, (u + cs) * l2, 0., 0.},
+                     {v * l0, v * l1, v * l2, l1, 0.},
+                     {w * l0, w * l1, w * l2, 0., l1},
+                     {(h - u * cs) * l0, ke * l1, (h + u * cs) * l2, v * l1,
+                      w * l1}};
+  Mat<5> out;
+  for (int a = 0; a < 5; ++a)
+    for (int c = 0; c < 5; ++c) {
+      double s = RL[a][0] * Ri0[c];
+      s = std::fma(RL[a][1], Ri1[c], s);
+      s = std::fma(RL[a][2], Ri2[c], s);
+      if (a >= 2) {
+        if (c == 0) s = std::fma(RL[a][3], -v, std::fma(RL[a][4], -w, s));
+        if (c == 2) s = s + RL[a][3];
+        if (c == 3) s = s + RL[a][4];
+      }
+      out.a[a][c] = s;
+    }
+  return out;
 }
 
 // FluxJacobian: flux_decomposition.hpp:101-120 (dir = X1)
@@ -813,7 +838,7 @@ bool correction_column(const Grid& g, const cg_block& blk,
   const int is = g.is, ie = g.ie, nc = g.nc1, N = g.N;
   const int IVX = g.IVX, IEN = g.IPR;
   const double grav = blk.grav;
-  Mat<B> Phi, Dt, Bnd;
+  Mat<B> Phi;
   Phi.zero();
   if (B == 5) {
     Phi.a[1][0] = grav;
@@ -822,10 +847,6 @@ bool correction_column(const Grid& g, const cg_block& blk,
     Phi.a[1][0] = grav;
     Phi.a[2][1] = grav;
   }
-  Dt.zero();
-  for (int d = 0; d < B; ++d) Dt.a[d][d] = 1. / dt;
-  Bnd.ident();
-  Bnd.a[1][1] = -1;
   const bool periodic = (blk.bc[0] == CG_BC_PERIODIC);
   std::vector<Mat<B>> a(nc), b(nc), c(nc), dfdq(nc);
   std::vector<Vec<B>> delta(nc);
@@ -845,6 +866,13 @@ bool correction_column(const Grid& g, const cg_block& blk,
   Mat<B> Am = sub5<B>(abs_roe_matrix(g, th, prim, gm));
   const double aleft = blk.dx2 * blk.dx3, aright = blk.dx2 * blk.dx3;
   const double vol = blk.dx1 * blk.dx2 * blk.dx3;
+  // a_i = (A- aleft + A+ aright + (aright - aleft) dfdq_i) / (2 vol)
+  //       + I/dt - Phi,  b_i = -(A- + dfdq_{i-1}) aleft / (2 vol),
+  //       c_i = -(A+ - dfdq_{i+1}) aright / (2 vol)   (:276-280),
+  // with the division by 2 vol taken as a product with its reciprocal.
+  const double s2v = 1. / (2. * vol);
+  const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
+  const double idt = 1. / dt;
   for (int i = is - 1; i <= ie; ++i) {
     load(wl, i);
     load(wr, i + 1);
@@ -853,63 +881,65 @@ bool correction_column(const Grid& g, const cg_block& blk,
     Mat<B> Ap = sub5<B>(abs_roe_matrix(g, th, prim, gm));
     for (int r = 0; r < B; ++r)
       for (int q = 0; q < B; ++q) {
-        a[i].a[r][q] = (Am.a[r][q] * aleft + Ap.a[r][q] * aright +
-                        (aright - aleft) * dfdq[i].a[r][q]) /
-                           (2. * vol) +
-                       Dt.a[r][q] - Phi.a[r][q];
-        b[i].a[r][q] = -(Am.a[r][q] + dfdq[i - 1].a[r][q]) * aleft / (2. * vol);
-        c[i].a[r][q] = -(Ap.a[r][q] - dfdq[i + 1].a[r][q]) * aright / (2. * vol);
+        double av = std::fma(Am.a[r][q], sl,
+                             std::fma(Ap.a[r][q], sr, dfdq[i].a[r][q] * sd));
+        if (r == q) av += idt;
+        a[i].a[r][q] = av - Phi.a[r][q];
+        b[i].a[r][q] = -(Am.a[r][q] + dfdq[i - 1].a[r][q]) * sl;
+        c[i].a[r][q] = -(Ap.a[r][q] - dfdq[i + 1].a[r][q]) * sr;
       }
     Am = Ap;
   }
   if (!periodic) {
-    Mat<B> t = mul(b[is], Bnd);
+    // a_is += b_is Bnd, a_ie += c_ie Bnd with Bnd = diag(1, -1, 1, ...)
     for (int r = 0; r < B; ++r)
-      for (int q = 0; q < B; ++q) a[is].a[r][q] += t.a[r][q];
-    t = mul(c[ie], Bnd);
+      for (int q = 0; q < B; ++q)
+        a[is].a[r][q] += (q == 1) ? -b[is].a[r][q] : b[is].a[r][q];
     for (int r = 0; r < B; ++r)
-      for (int q = 0; q < B; ++q) a[ie].a[r][q] += t.a[r][q];
+      for (int q = 0; q < B; ++q)
+        a[ie].a[r][q] += (q == 1) ? -c[ie].a[r][q] : c[ie].a[r][q];
   }
-  // ForwardSweep (forward_backward.hpp:14-84)
+  // ForwardSweep (forward_backward.hpp:14-84): with M = a_i - b_i a'_{i-1}
+  // and t = rhs_i - b_i delta_{i-1}, a'_i = M^-1 c_i and delta_i = M^-1 t.
+  // The reference forms M^-1 (Eigen .inverse()) and multiplies; here one
+  // Gauss-Jordan elimination of [M | c_i | t] with partial pivoting gives
+  // both (equal in exact arithmetic).
   auto rhs_at = [&](int i) {
     Vec<B> rhs;
     double r0 = du[g.at(0, k, j, i)];
     for (int n = 1; n < IVX; ++n) r0 += du[g.at(n, k, j, i)];
-    r0 /= dt;
-    rhs.v[0] = r0;
+    rhs.v[0] = r0 * idt;
     if (B == 5) {
-      rhs.v[1] = du[g.at(IVX, k, j, i)] / dt;
-      rhs.v[2] = du[g.at(IVX + 1, k, j, i)] / dt;
-      rhs.v[3] = du[g.at(IVX + 2, k, j, i)] / dt;
-      rhs.v[4] = du[g.at(IEN, k, j, i)] / dt;
+      rhs.v[1] = du[g.at(IVX, k, j, i)] * idt;
+      rhs.v[2] = du[g.at(IVX + 1, k, j, i)] * idt;
+      rhs.v[3] = du[g.at(IVX + 2, k, j, i)] * idt;
+      rhs.v[4] = du[g.at(IEN, k, j, i)] * idt;
     } else {
-      rhs.v[1] = du[g.at(IVX, k, j, i)] / dt;
-      rhs.v[2] = du[g.at(IEN, k, j, i)] / dt;
+      rhs.v[1] = du[g.at(IVX, k, j, i)] * idt;
+      rhs.v[2] = du[g.at(IEN, k, j, i)] * idt;
     }
     return rhs;
   };
-  {
-    Vec<B> rhs = rhs_at(is);
-    Mat<B> inv;
-    if (!inverse(a[is], &inv)) return false;
-    a[is] = inv;
-    delta[is] = mul(a[is], rhs);
-    a[is] = mul(a[is], c[is]);
-  }
-  for (int i = is + 1; i <= ie; ++i) {
-    Vec<B> rhs = rhs_at(i);
-    Mat<B> ba = mul(b[i], a[i - 1]);
-    Mat<B> m;
-    for (int r = 0; r < B; ++r)
-      for (int q = 0; q < B; ++q) m.a[r][q] = a[i].a[r][q] - ba.a[r][q];
-    Mat<B> inv;
-    if (!inverse(m, &inv)) return false;
-    a[i] = inv;
-    Vec<B> bd = mul(b[i], delta[i - 1]);
-    Vec<B> t;
-    for (int r = 0; r < B; ++r) t.v[r] = rhs.v[r] - bd.v[r];
-    delta[i] = mul(a[i], t);
-    a[i] = mul(a[i], c[i]);
+  for (int i = is; i <= ie; ++i) {
+    Vec<B> t = rhs_at(i);
+    Mat<B> M = a[i], C = c[i];
+    if (i > is) {
+      for (int r = 0; r < B; ++r) {
+        for (int q = 0; q < B; ++q) {
+          double s = M.a[r][q];
+          for (int kk = 0; kk < B; ++kk)
+            s = std::fma(-b[i].a[r][kk], a[i - 1].a[kk][q], s);
+          M.a[r][q] = s;
+        }
+        double s = t.v[r];
+        for (int kk = 0; kk < B; ++kk)
+          s = std::fma(-b[i].a[r][kk], delta[i - 1].v[kk], s);
+        t.v[r] = s;
+      }
+    }
+    if (!gj_solve(M, C, t)) return false;
+    a[i] = C;
+    delta[i] = t;
   }
   // BackwardSubstitution (forward_backward.hpp:86-137)
   for (int i = ie - 1; i >= is; --i) {
@@ -1358,6 +1388,77 @@ int or_step_omp(const cg_block* blk, const cg_thermo* th,
   int r = step(*blk, *th, *cfg, dt, nsteps, u, w);
   g_threads = 1;
   return r;
+}
+
+// Block-tridiagonal solve of n levels with B x B blocks (B = 3 or 5):
+// b_i x_{i-1} + a_i x_i + c_i x_{i+1} = r_i, row-major blocks.  The same
+// forward sweep / back substitution as the column solve.
+int or_block_thomas(int B, int n, const double* a, const double* b,
+                    const double* c, const double* r, double* x) {
+  auto run = [&](auto tag) {
+    constexpr int BB = decltype(tag)::value;
+    std::vector<Mat<BB>> ap(n);
+    std::vector<Vec<BB>> d(n);
+    for (int i = 0; i < n; ++i) {
+      Mat<BB> M, Bi, C;
+      Vec<BB> t;
+      for (int q = 0; q < BB * BB; ++q) {
+        M.a[q / BB][q % BB] = a[(size_t)i * BB * BB + q];
+        Bi.a[q / BB][q % BB] = b[(size_t)i * BB * BB + q];
+        C.a[q / BB][q % BB] = c[(size_t)i * BB * BB + q];
+      }
+      for (int q = 0; q < BB; ++q) t.v[q] = r[(size_t)i * BB + q];
+      if (i > 0) {
+        for (int rr = 0; rr < BB; ++rr) {
+          for (int q = 0; q < BB; ++q) {
+            double s = M.a[rr][q];
+            for (int kk = 0; kk < BB; ++kk)
+              s = std::fma(-Bi.a[rr][kk], ap[i - 1].a[kk][q], s);
+            M.a[rr][q] = s;
+          }
+          double s = t.v[rr];
+          for (int kk = 0; kk < BB; ++kk)
+            s = std::fma(-Bi.a[rr][kk], d[i - 1].v[kk], s);
+          t.v[rr] = s;
+        }
+      }
+      if (!gj_solve(M, C, t)) return CG_ERR_PIVOT;
+      ap[i] = C;
+      d[i] = t;
+    }
+    for (int i = n - 2; i >= 0; --i) {
+      Vec<BB> ad = mul(ap[i], d[i + 1]);
+      for (int q = 0; q < BB; ++q) d[i].v[q] = d[i].v[q] - ad.v[q];
+    }
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < BB; ++q) x[(size_t)i * BB + q] = d[i].v[q];
+    return CG_OK;
+  };
+  if (B == 5) return run(std::integral_constant<int, 5>());
+  if (B == 3) return run(std::integral_constant<int, 3>());
+  return CG_ERR_ARG;
+}
+
+// Flux Jacobian dF/dU (x1) at primitive state w, and |A| / the signed Roe
+// matrix R Lambda R^-1 at the Roe average of (wl, wr), 5 x 5 row-major.
+int or_flux_jacobian(const cg_block* blk, double gm1, const double* w,
+                     double* out) {
+  Grid g(*blk);
+  Mat<5> m = flux_jacobian(g, gm1, w);
+  for (int q = 0; q < 25; ++q) out[q] = m.a[q / 5][q % 5];
+  return CG_OK;
+}
+
+int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,
+                    const double* wl, const double* wr, double* abs_a,
+                    double* prim_out) {
+  Grid g(*blk);
+  double prim[5 + CG_MAX_NMASS];
+  roe_average(g, prim, gm1, wl, wr);
+  Mat<5> m = abs_roe_matrix(g, *th, prim, gm1);
+  for (int q = 0; q < 25; ++q) abs_a[q] = m.a[q / 5][q % 5];
+  for (int n = 0; n < g.N; ++n) prim_out[n] = prim[n];
+  return CG_OK;
 }
 
 int or_sums(const cg_block* blk, const double* u, double* out) {

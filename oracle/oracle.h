@@ -69,6 +69,13 @@ int or_step_omp(const cg_block* blk, const cg_thermo* th,
                 const cg_config* cfg, double dt, int nsteps, double* u,
                 double* w, int nthreads);
 int or_sums(const cg_block* blk, const double* u, double* out);
+int or_block_thomas(int B, int n, const double* a, const double* b,
+                    const double* c, const double* r, double* x);
+int or_flux_jacobian(const cg_block* blk, double gm1, const double* w,
+                     double* out);
+int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,
+                    const double* wl, const double* wr, double* abs_a,
+                    double* prim_out);
 
 #ifdef __cplusplus
 }

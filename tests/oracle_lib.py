@@ -34,6 +34,9 @@ _SIGS = {
     "or_step": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
     "or_sums": (_I, [_P, _P, _P]),
+    "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
+    "or_flux_jacobian": (_I, [_P, _D, _P, _P]),
+    "or_roe_matrices": (_I, [_P, _P, _D, _P, _P, _P, _P]),
 }
 
 _lib = None
