@@ -16,34 +16,12 @@
 #include "kernels.hpp"
 #include "cg_timer.hpp"
 #include "flux32.hpp"
+#include "capi_internal.hpp"
 
 using namespace cg;
-
-struct cg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int* d_status = nullptr;  // device error flags (bit 0 pivot, bit 1 Newton)
-  char err[512] = {0};
-};
+using namespace cg_host;
 
 namespace {
-
-int set_err(cg_ctx* ctx, int code, const char* fmt, ...) {
-  if (ctx) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
-    va_end(ap);
-  }
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                    \
-  do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return set_err(ctx, CG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
 
 int check_block(cg_ctx* ctx, const cg_block* b, const cg_thermo* th) {
   if (!b || !th) return set_err(ctx, CG_ERR_ARG, "null block/thermo");
@@ -123,37 +101,6 @@ KGrid make_grid(const cg_block& b, int lvl) {
 size_t psf_doubles(const KGrid& g) {
   return (size_t)(g.nc1 + 1) * g.nx2 * g.nx3;
 }
-
-KThermo make_thermo(const cg_thermo& t) {
-  KThermo k;
-  memset(&k, 0, sizeof(k));
-  k.nmass = t.nmass;
-  k.nvapor = t.nvapor;
-  k.gammad = t.gammad;
-  k.rd = t.rd;
-  for (int n = 0; n < 3; ++n) {
-    k.cv_ratio_m1[n] = t.cv_ratio_m1[n];
-    k.inv_mu_ratio_m1[n] = t.inv_mu_ratio_m1[n];
-  }
-  k.iv = (t.ivap > 0) ? t.ivap - 1 : -1;
-  k.ic = (t.icld > 0) ? t.icld - 1 : -1;
-  k.cvd = t.rd / (t.gammad - 1.);
-  if (k.iv >= 0 && k.ic >= 0) {
-    k.cvv = k.cvd * (1. + t.cv_ratio_m1[k.iv]);
-    k.cvc = k.cvd * (1. + t.cv_ratio_m1[k.ic]);
-    k.rv = t.rd * (1. + t.inv_mu_ratio_m1[k.iv]);
-  }
-  k.L = t.latent;
-  k.t3 = t.t3;
-  k.p3 = t.p3;
-  k.beta = t.beta;
-  k.delta = t.delta;
-  k.tol = t.tol;
-  k.max_iter = t.max_iter;
-  return k;
-}
-
-inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
 
@@ -357,21 +304,6 @@ int run_one_step(cg_run* r, double dt) {
   for (int s = 0; s < 3; ++s) {
     int e = (r->N == 5) ? run_stage<0>(r, dt, s) : run_stage<3>(r, dt, s);
     if (e) return e;
-  }
-  return CG_OK;
-}
-
-int check_status(cg_ctx* ctx) {
-  int st = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&st, ctx->d_status, sizeof(int),
-                             hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  if (st) {
-    HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if (st & 1) return set_err(ctx, CG_ERR_PIVOT, "zero pivot in implicit solve");
-    if (st & 2)
-      return set_err(ctx, CG_ERR_CONVERGE, "saturation Newton did not converge");
   }
   return CG_OK;
 }
@@ -588,22 +520,6 @@ int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w) {
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
-}
-
-int cg_saturation_adjust(cg_ctx* ctx, const cg_thermo* th, int ncell,
-                         const double* rho, double* ie, double* y,
-                         const void* mlp_params, int* iters) {
-  if (!ctx || !th) return CG_ERR_ARG;
-  if (th->nmass != 3 || th->ivap <= 0 || th->icld <= 0)
-    return set_err(ctx, CG_ERR_ARG, "saturation adjustment needs nmass 3 and a vapour/cloud pair");
-  if (ncell <= 0) return CG_OK;
-  KThermo kt = make_thermo(*th);
-  const double* tguess = (const double*)mlp_params;  // see cg_mlp_* entry points
-  k_satadj<3><<<nblk(ncell, 256), 256, 0, ctx->stream>>>(kt, ncell, rho, ie, y, tguess, iters,
-                                                       ctx->d_status);
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  return check_status(ctx);
 }
 
 // ---------------------------------------------------------------------------

@@ -347,7 +347,7 @@ CG_HD double qsat_d(const KThermo& s, double T, double rho) {
 // all-vapour temperature.
 template <int NM>
 CG_HD int adjust_cell(const KThermo& th, double rho, double* ie, double* q,
-                      double tguess) {
+                      double tguess, double* tout = nullptr) {
   double cvd = th.cvd;
   double qd = 1.;
 #pragma unroll
@@ -398,6 +398,7 @@ CG_HD int adjust_cell(const KThermo& th, double rho, double* ie, double* q,
     if (n == th.iv) q[n] = qt - qc;
     if (n == th.ic) q[n] = qc;
   }
+  if (tout) *tout = T;
   return it;
 }
 

@@ -1072,7 +1072,7 @@ __global__ void __launch_bounds__(256)
 
 // Copy between two layouts of the same cell array (all cells incl. ghosts);
 // lanes follow the destination's unit-stride index.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
     k_relayout(KGrid gs, KGrid gd, int N, const double* __restrict__ src,
                double* __restrict__ dst) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -1,0 +1,692 @@
+// mlp.hpp -- the saturation-guess network on gfx950 matrix cores.
+//
+// A 4 -> 64 -> 64 -> 64 -> 3 perceptron with tanh-GELU maps the state of a
+// cell before the saturation adjustment to (adjusted T, vapour and condensate
+// fractions); its temperature is the Newton start of the guided adjustment.
+// Weights and layer inputs are bf16, products accumulate in fp32 on MFMA
+// (v_mfma_f32_16x16x32_bf16), everything elementwise is fp32.
+//
+// Wave-level formulation.  One wavefront carries 64 samples as 4 sample tiles
+// of 16.  Layers are evaluated transposed, Z^T = W H^T, so that W is the A
+// operand and the activations are the B operand:
+//   A (16 x 32):  lane l holds A[l % 16][8 (l / 16) + v],  v = 0..7
+//   B (32 x 16):  lane l holds B[8 (l / 16) + v][l % 16]
+//   D (16 x 16):  lane l holds D[4 (l / 16) + v][l % 16],  v = 0..3
+// The D tile of output features 16t..16t+15 therefore gives lane l the
+// features 16t + 4 (l/16) + v of sample l % 16.  A k-step of 32 input
+// features takes its 8 B slots from two such tiles: slot v < 4 is feature
+// 32kk + 4g + v, slot v >= 4 is feature 32kk + 16 + 4g + (v - 4), g = l / 16.
+// The weight tiles are packed in the same slot order, so one layer's output
+// feeds the next layer's B operand without any data exchange.
+//
+// Tile buffer (bf16, 64 lanes x 8 values per tile, A-operand order):
+//   0..3    W1 (out tile ot, one k-step, inputs 4..31 zero)
+//   4..11   W2 (2 ot + kk)        12..19  W3 (2 ot + kk)
+//   20..21  W4 (kk; rows 3..15 zero)
+//   22..29  W2^T (2 it + kk)      30..37  W3^T (2 it + kk)
+//   38..41  W4^T (it; k slots 3..31 zero)
+// The transposed tiles serve the backward pass (dH = W^T dZ).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cg_physics.hpp"
+
+namespace cg {
+namespace mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NPARAM = 8835;
+constexpr int TILE = 512;  // bf16 values per tile
+constexpr int TF1 = 0, TF2 = 4, TF3 = 12, TF4 = 20, TB2 = 22, TB3 = 30,
+              TB4 = 38, NTILE = 42;
+constexpr int CHUNK = 256;  // samples per weight-gradient partial
+
+template <int L> struct Lay;
+template <> struct Lay<0> { enum { IN = 4, OUT = 64, W = 0, B = 256, H = 0, DZ = 0 }; };
+template <> struct Lay<1> { enum { IN = 64, OUT = 64, W = 320, B = 4416, H = 4, DZ = 64 }; };
+template <> struct Lay<2> { enum { IN = 64, OUT = 64, W = 4480, B = 8576, H = 68, DZ = 128 }; };
+template <> struct Lay<3> { enum { IN = 64, OUT = 3, W = 8640, B = 8832, H = 132, DZ = 192 }; };
+// hbuf holds the layer inputs feature-major: H0 (4 rows), H1, H2, H3 (64 rows
+// each) = 196 rows of nbp bf16.  dzbuf holds dZ1, dZ2, dZ3 (64 rows), dY (3
+// rows) = 195 rows of nbp floats.
+constexpr int HROWS = 196, DZROWS = 195;
+
+// ---------------------------------------------------------------------------
+// Elementwise routines: the same algorithms as the oracle (bf16 round to
+// nearest even; Cody-Waite exp with the 0.693359375 - 2.12194440e-4 split of
+// ln2 and a Taylor series through r^6; tanh-form GELU), single precision,
+// no contraction.
+CG_HD uint16_t bf16_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+CG_HD float bf16_float(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+CG_HD float exp_f(float x) {
+  if (x != x) return x;
+  if (x > 88.0f) return __builtin_huge_valf();
+  if (x < -87.0f) return 0.0f;
+  float kf = floorf(x * 1.44269504f + 0.5f);
+  float r = (x - kf * 0.693359375f) - kf * -2.12194440e-4f;
+  float p = 1.38888889e-3f;
+  p = p * r + 8.33333333e-3f;
+  p = p * r + 4.16666667e-2f;
+  p = p * r + 1.66666667e-1f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return ldexpf(p, (int)kf);
+}
+
+CG_HD float gelu_tanh(float z) {
+  float z3 = (z * z) * z;
+  float u = 0.797884583f * (z + 0.044715f * z3);
+  float e = exp_f(2.0f * u);
+  return 1.0f - 2.0f / (e + 1.0f);
+}
+
+CG_HD float gelu(float z) {
+  float t = gelu_tanh(z);
+  return (0.5f * z) * (1.0f + t);
+}
+
+CG_HD float gelu_grad(float z) {
+  float t = gelu_tanh(z);
+  float dudz = 0.797884583f * (1.0f + 0.134145f * (z * z));
+  return 0.5f * (1.0f + t) + (0.5f * z) * ((1.0f - t * t) * dudz);
+}
+
+CG_HD uint32_t pack2(uint16_t lo, uint16_t hi) {
+  return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+// Slot v (0..7) of lane group g inside a 32-wide k-step -> feature offset.
+CG_HD int kslot(int g, int v) {
+  return (v < 4) ? 4 * g + v : 16 + 4 * g + (v - 4);
+}
+
+// Value of slot v of lane `lane` in tile t, from the fp32 parameters.
+CG_HD float tile_value(const float* P, int t, int lane, int v) {
+  const int g = lane >> 4, r = lane & 15, ks = kslot(g, v);
+  if (t < TF2) {  // W1: row 16 ot + r, input ks (< 4)
+    int row = 16 * t + r;
+    return ks < 4 ? P[Lay<0>::W + row * 4 + ks] : 0.f;
+  }
+  if (t < TF4) {  // W2, W3
+    const int base = (t < TF3) ? Lay<1>::W : Lay<2>::W;
+    const int q = t - ((t < TF3) ? TF2 : TF3);
+    const int row = 16 * (q >> 1) + r, col = 32 * (q & 1) + ks;
+    return P[base + row * 64 + col];
+  }
+  if (t < TB2) {  // W4: row r (< 3)
+    const int col = 32 * (t - TF4) + ks;
+    return r < 3 ? P[Lay<3>::W + r * 64 + col] : 0.f;
+  }
+  if (t < TB4) {  // W2^T, W3^T: row = input feature, k = output feature
+    const int base = (t < TB3) ? Lay<1>::W : Lay<2>::W;
+    const int q = t - ((t < TB3) ? TB2 : TB3);
+    const int row = 16 * (q >> 1) + r, col = 32 * (q & 1) + ks;
+    return P[base + col * 64 + row];
+  }
+  {  // W4^T: row = input feature 16 it + r, k = output (< 3)
+    const int row = 16 * (t - TB4) + r;
+    return ks < 3 ? P[Lay<3>::W + ks * 64 + row] : 0.f;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ tiles) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NTILE * TILE) return;
+  int t = e / TILE, lane = (e % TILE) / 8, v = e % 8;
+  tiles[e] = bf16_bits(tile_value(P, t, lane, v));
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level pieces.  Every function below must be called by all 64 lanes
+// of a wavefront (MFMA reads every lane).
+__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+      __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+      0);
+}
+
+__device__ __forceinline__ u32x4 ldtile(const uint16_t* __restrict__ tiles, int t,
+                               int lane) {
+  return *(reinterpret_cast<const u32x4*>(tiles + (size_t)t * TILE) + lane);
+}
+
+// bf16 pairs of 16 D tiles: [feature tile][sample tile][pair]
+struct Hid {
+  uint32_t h[4][4][2];
+};
+
+__device__ __forceinline__ u32x4 bop(const Hid& H, int kk, int s) {
+  return u32x4{H.h[2 * kk][s][0], H.h[2 * kk][s][1], H.h[2 * kk + 1][s][0],
+               H.h[2 * kk + 1][s][1]};
+}
+
+__device__ __forceinline__ f32x4 bias4(const float* __restrict__ b, int row0, int nrow) {
+  f32x4 c;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) c[v] = (row0 + v < nrow) ? b[row0 + v] : 0.f;
+  return c;
+}
+
+// B operands of layer 1: sample tile s takes the inputs of lanes 16 s + r.
+__device__ __forceinline__ void input_bop(const float xin[4], u32x4 b[4]) {
+  const int lane = threadIdx.x & 63, r = lane & 15;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    float a0 = __shfl(xin[0], 16 * s + r), a1 = __shfl(xin[1], 16 * s + r);
+    float a2 = __shfl(xin[2], 16 * s + r), a3 = __shfl(xin[3], 16 * s + r);
+    b[s] = (lane < 16) ? u32x4{pack2(bf16_bits(a0), bf16_bits(a1)),
+                               pack2(bf16_bits(a2), bf16_bits(a3)), 0u, 0u}
+                       : u32x4{0u, 0u, 0u, 0u};
+  }
+}
+
+__device__ __forceinline__ void layer1(const uint16_t* __restrict__ tiles,
+                              const float* __restrict__ P, const u32x4 b[4],
+                              f32x4 acc[4][4]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+#pragma unroll
+  for (int ot = 0; ot < 4; ++ot) {
+    u32x4 a = ldtile(tiles, TF1 + ot, lane);
+    f32x4 c0 = bias4(P + Lay<0>::B, 16 * ot + 4 * g, 64);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc[ot][s] = mfma(a, b[s], c0);
+  }
+}
+
+template <int TF, int L>
+__device__ __forceinline__ void layer64(const uint16_t* __restrict__ tiles,
+                               const float* __restrict__ P, const Hid& H,
+                               f32x4 acc[4][4]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+#pragma unroll
+  for (int ot = 0; ot < 4; ++ot) {
+    u32x4 a0 = ldtile(tiles, TF + 2 * ot, lane);
+    u32x4 a1 = ldtile(tiles, TF + 2 * ot + 1, lane);
+    f32x4 c0 = bias4(P + Lay<L>::B, 16 * ot + 4 * g, 64);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      acc[ot][s] = mfma(a1, bop(H, 1, s), mfma(a0, bop(H, 0, s), c0));
+  }
+}
+
+__device__ __forceinline__ void layer_out(const uint16_t* __restrict__ tiles,
+                                 const float* __restrict__ P, const Hid& H,
+                                 f32x4 out[4]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  u32x4 a0 = ldtile(tiles, TF4, lane), a1 = ldtile(tiles, TF4 + 1, lane);
+  f32x4 c0 = bias4(P + Lay<3>::B, 4 * g, 3);
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    out[s] = mfma(a1, bop(H, 1, s), mfma(a0, bop(H, 0, s), c0));
+}
+
+__device__ __forceinline__ void activate(const f32x4 acc[4][4], Hid& H) {
+#pragma unroll
+  for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        H.h[ot][s][p] = pack2(bf16_bits(gelu(acc[ot][s][2 * p])),
+                              bf16_bits(gelu(acc[ot][s][2 * p + 1])));
+}
+
+// Output of sample (this lane) from the output tiles: lane l of sample tile
+// l / 16 sits in lane l % 16 (rows 0..2) of that tile.
+__device__ __forceinline__ void own_output(const f32x4 out[4], float y[3]) {
+  const int lane = threadIdx.x & 63, r = lane & 15, so = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      float t = __shfl(out[s][v], r);
+      if (s == so) y[v] = t;
+    }
+}
+
+// Forward pass of the wave's 64 samples; xin / y are this lane's sample.
+__device__ __forceinline__ void forward_wave(const uint16_t* __restrict__ tiles,
+                                    const float* __restrict__ P,
+                                    const float xin[4], float y[3]) {
+  u32x4 b1[4];
+  input_bop(xin, b1);
+  f32x4 acc[4][4];
+  Hid H;
+  layer1(tiles, P, b1, acc);
+  activate(acc, H);
+  layer64<TF2, 1>(tiles, P, H, acc);
+  activate(acc, H);
+  layer64<TF3, 2>(tiles, P, H, acc);
+  activate(acc, H);
+  f32x4 out[4];
+  layer_out(tiles, P, H, out);
+  own_output(out, y);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_infer(const uint16_t* __restrict__ tiles, const float* __restrict__ P,
+                long long n, const float* __restrict__ x, float* __restrict__ y) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = c < n;
+  float xin[4] = {0.f, 0.f, 0.f, 0.f}, yo[3];
+  if (valid)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) xin[f] = x[c * 4 + f];
+  forward_wave(tiles, P, xin, yo);
+  if (valid)
+#pragma unroll
+    for (int v = 0; v < 3; ++v) y[c * 3 + v] = yo[v];
+}
+
+// ---------------------------------------------------------------------------
+// Training.  Batch position pos < nb maps to sample pos (random = 0) or to a
+// splitmix64 draw over [0, npairs) (random = 1); positions nb..nbp-1 are
+// padding with zero loss gradient.
+CG_HD uint64_t sm64_next(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ long long sample_of(long long pos, long long nb,
+                                      long long npairs, uint64_t seed,
+                                      int random) {
+  if (pos >= nb) return -1;
+  if (!random) return pos;
+  uint64_t st = seed + (uint64_t)pos * 0x9E3779B97F4A7C15ull;
+  return (long long)(sm64_next(st) % (uint64_t)npairs);
+}
+
+// Forward with saved pre-activations and feature-major layer inputs, then the
+// backward data pass dY -> dZ3 -> dZ2 -> dZ1 (dZ stored feature-major, fp32).
+// grid = nbp / 256 blocks of 256 (4 waves x 64 samples).
+__global__ void __launch_bounds__(256)
+    k_mlp_fb(const uint16_t* __restrict__ tiles, const float* __restrict__ P,
+             long long nb, long long nbp, long long npairs,
+             const float* __restrict__ X, const float* __restrict__ T,
+             uint64_t seed, int random, float scale, float* __restrict__ zbuf,
+             uint16_t* __restrict__ hbuf, float* __restrict__ dzbuf,
+             float* __restrict__ lossp) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long base = wave * 64;
+  long long idx = sample_of(base + lane, nb, npairs, seed, random);
+  float xin[4] = {0.f, 0.f, 0.f, 0.f};
+  if (idx >= 0)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) xin[f] = X[idx * 4 + f];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) hbuf[f * nbp + base + lane] = bf16_bits(xin[f]);
+  u32x4 b1[4];
+  input_bop(xin, b1);
+  f32x4 acc[4][4];
+  Hid H;
+  float* zw = zbuf + (size_t)wave * 3 * 4096;
+  auto save = [&](int l, int hrow) {
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          zw[(((l * 4 + ot) * 4 + s) * 4 + v) * 64 + lane] = acc[ot][s][v];
+          const int f = 16 * ot + 4 * g + v;
+          const uint32_t hp = H.h[ot][s][v >> 1];
+          hbuf[(hrow + f) * nbp + base + 16 * s + r] =
+              (uint16_t)((v & 1) ? (hp >> 16) : (hp & 0xffffu));
+        }
+  };
+  layer1(tiles, P, b1, acc);
+  activate(acc, H);
+  save(0, Lay<1>::H);
+  layer64<TF2, 1>(tiles, P, H, acc);
+  activate(acc, H);
+  save(1, Lay<2>::H);
+  layer64<TF3, 2>(tiles, P, H, acc);
+  activate(acc, H);
+  save(2, Lay<3>::H);
+  f32x4 out[4];
+  layer_out(tiles, P, H, out);
+
+  // output gradient dY = (y - t) * 2 / (3 nb)
+  float sse = 0.f;
+  u32x4 dyb[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const long long ps = base + 16 * s + r;
+    const long long is = sample_of(ps, nb, npairs, seed, random);
+    float d[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float dv = 0.f;
+      if (g == 0 && v < 3 && is >= 0) {
+        float e = out[s][v] - T[is * 3 + v];
+        sse += e * e;
+        dv = e * scale;
+      }
+      d[v] = dv;
+      if (g == 0 && v < 3) dzbuf[(Lay<3>::DZ + v) * nbp + ps] = dv;
+    }
+    dyb[s] = u32x4{pack2(bf16_bits(d[0]), bf16_bits(d[1])),
+                   pack2(bf16_bits(d[2]), bf16_bits(d[3])), 0u, 0u};
+  }
+  // dH3 = W4^T dY, dZ3 = dH3 gelu'(Z3)
+  f32x4 dh[4][4];
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    u32x4 a = ldtile(tiles, TB4 + it, lane);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) dh[it][s] = mfma(a, dyb[s], zero);
+  }
+  Hid DZ;
+  auto grad_act = [&](int l, int dzrow) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        float dz[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float z = zw[(((l * 4 + it) * 4 + s) * 4 + v) * 64 + lane];
+          dz[v] = dh[it][s][v] * gelu_grad(z);
+          dzbuf[(dzrow + 16 * it + 4 * g + v) * nbp + base + 16 * s + r] = dz[v];
+        }
+        DZ.h[it][s][0] = pack2(bf16_bits(dz[0]), bf16_bits(dz[1]));
+        DZ.h[it][s][1] = pack2(bf16_bits(dz[2]), bf16_bits(dz[3]));
+      }
+  };
+  auto back64 = [&](int tb) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      u32x4 a0 = ldtile(tiles, tb + 2 * it, lane);
+      u32x4 a1 = ldtile(tiles, tb + 2 * it + 1, lane);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        dh[it][s] = mfma(a1, bop(DZ, 1, s), mfma(a0, bop(DZ, 0, s), zero));
+    }
+  };
+  grad_act(2, Lay<2>::DZ);  // dZ3
+  back64(TB3);              // dH2 = W3^T dZ3
+  grad_act(1, Lay<1>::DZ);  // dZ2
+  back64(TB2);              // dH1 = W2^T dZ2
+  grad_act(0, Lay<0>::DZ);  // dZ1
+
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sse += __shfl_xor(sse, o);
+  if (lane == 0) lossp[wave] = sse;
+}
+
+// Weight and bias gradient partials of layer L over one chunk of CHUNK
+// samples: dW = sum_s bf16(dZ) H^T on MFMA with the samples as the k
+// dimension, db = sum_s dZ.  part is (nchunk, NPARAM).  One wave per chunk.
+template <int L>
+__global__ void __launch_bounds__(64)
+    k_mlp_dw(const uint16_t* __restrict__ hbuf, const float* __restrict__ dzbuf,
+             long long nbp, float* __restrict__ part) {
+  constexpr int IN = Lay<L>::IN, OUT = Lay<L>::OUT;
+  constexpr int NOT = (OUT + 15) / 16, NIT = (IN + 15) / 16;
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const long long c = blockIdx.x;
+  f32x4 acc[NOT][NIT];
+  float dbs[NOT];
+#pragma unroll
+  for (int ot = 0; ot < NOT; ++ot) {
+    dbs[ot] = 0.f;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) acc[ot][it] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int ks = 0; ks < CHUNK / 32; ++ks) {
+    const long long p0 = c * CHUNK + 32 * ks + 8 * g;
+    u32x4 A[NOT], B[NIT];
+#pragma unroll
+    for (int ot = 0; ot < NOT; ++ot) {
+      const int o = 16 * ot + r;
+      if (o < OUT) {
+        const f32x4* src =
+            reinterpret_cast<const f32x4*>(dzbuf + (Lay<L>::DZ + o) * nbp + p0);
+        f32x4 lo = src[0], hi = src[1];
+        float sacc = dbs[ot];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sacc += lo[v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sacc += hi[v];
+        dbs[ot] = sacc;
+        A[ot] = u32x4{pack2(bf16_bits(lo[0]), bf16_bits(lo[1])),
+                      pack2(bf16_bits(lo[2]), bf16_bits(lo[3])),
+                      pack2(bf16_bits(hi[0]), bf16_bits(hi[1])),
+                      pack2(bf16_bits(hi[2]), bf16_bits(hi[3]))};
+      } else {
+        A[ot] = u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int i = 16 * it + r;
+      B[it] = (i < IN) ? *reinterpret_cast<const u32x4*>(
+                             hbuf + (Lay<L>::H + i) * nbp + p0)
+                       : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int ot = 0; ot < NOT; ++ot)
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) acc[ot][it] = mfma(A[ot], B[it], acc[ot][it]);
+  }
+  float* pc = part + c * NPARAM;
+#pragma unroll
+  for (int ot = 0; ot < NOT; ++ot)
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = 16 * ot + 4 * g + v, i = 16 * it + r;
+        if (o < OUT && i < IN) pc[Lay<L>::W + o * IN + i] = acc[ot][it][v];
+      }
+#pragma unroll
+  for (int ot = 0; ot < NOT; ++ot) {
+    float s = dbs[ot];
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    if (g == 0 && 16 * ot + r < OUT) pc[Lay<L>::B + 16 * ot + r] = s;
+  }
+}
+
+// grad = sum of the chunk partials in chunk order; loss = sum of the wave
+// partials times inv_n3 (written by thread 0).
+__global__ void __launch_bounds__(256)
+    k_mlp_reduce(const float* __restrict__ part, int nchunk,
+                 const float* __restrict__ lossp, int nwave, float inv_n3,
+                 float* __restrict__ grad, float* __restrict__ loss) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NPARAM) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * NPARAM + p];
+    grad[p] = s;
+  }
+  if (p == 0) {
+    float s = 0.f;
+    for (int w = 0; w < nwave; ++w) s += lossp[w];
+    *loss = s * inv_n3;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master parameters; c1 = 1/(1-b1^t),
+// c2 = 1/(1-b2^t).
+__global__ void __launch_bounds__(256)
+    k_mlp_adam(float* __restrict__ P, float* __restrict__ m,
+               float* __restrict__ v, const float* __restrict__ grad, float lr,
+               float b1, float b2, float eps, float c1, float c2) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  float gp = grad[p];
+  float mp = b1 * m[p] + (1.f - b1) * gp;
+  float vp = b2 * v[p] + (1.f - b2) * (gp * gp);
+  m[p] = mp;
+  v[p] = vp;
+  P[p] = P[p] - lr * (mp * c1) / (sqrtf(vp * c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Network inputs of a cell (the same definition as the oracle): the
+// all-vapour temperature T_u (the plain Newton start), p_u = rho T_u (R_d q_d
+// + R_v q_t), total water q_t and the remaining mass fraction q_p:
+//   x = ((T_u - 260) / 40, ln(p_u / 1e5), 40 q_t, 40 q_p).
+struct CellFeat {
+  double cfix, qt, qp, em, tu;
+};
+
+template <int NM>
+CG_HD CellFeat cell_feat(const KThermo& th, double rho, double ie,
+                                const double* q, float* x) {
+  CellFeat f;
+  double qd = 1.;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) qd -= q[n];
+  f.cfix = th.cvd * qd;
+  f.qp = 0.;
+  double qiv = 0., qic = 0.;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) {
+    if (n == th.iv) qiv = q[n];
+    if (n == th.ic) qic = q[n];
+    if (n == th.iv || n == th.ic) continue;
+    f.cfix += th.cvd * (1. + th.cv_ratio_m1[n]) * q[n];
+    f.qp += q[n];
+  }
+  f.qt = qiv + qic;
+  f.em = ie / rho - th.L * qic;
+  f.tu = f.em / (f.cfix + th.cvv * f.qt);
+  double pu = rho * f.tu * (th.rd * qd + th.rv * f.qt);
+  x[0] = (float)((f.tu - 260.) / 40.);
+  x[1] = (float)log_d(pu / 1e5);
+  x[2] = (float)(40. * f.qt);
+  x[3] = (float)(40. * f.qp);
+  return f;
+}
+
+CG_HD double sm64_unit(uint64_t& st) {
+  return (double)(sm64_next(st) >> 11) * 0x1.0p-53;
+}
+
+// Training pairs (the oracle's or_mlp_pairs): base cell b = floor(u0 nbase),
+// total water scaled by 0.8 + 1.7 u1 into vapour, all-vapour temperature
+// shifted by 6 u2 - 3 K at fixed density, targets from the plain Newton
+// adjustment.  One thread per pair; y is (NM, nbase).
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_mlp_pairs(KThermo th, int nbase, const double* __restrict__ rho,
+                const double* __restrict__ ie, const double* __restrict__ y,
+                long long npairs, uint64_t seed, float* __restrict__ x,
+                float* __restrict__ t, int* __restrict__ iters,
+                int* __restrict__ status) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  uint64_t st = seed + 4ull * (uint64_t)p * 0x9E3779B97F4A7C15ull;
+  double u0 = sm64_unit(st), u1 = sm64_unit(st), u2 = sm64_unit(st);
+  int b = (int)(u0 * nbase);
+  if (b >= nbase) b = nbase - 1;
+  double q[NM];
+#pragma unroll
+  for (int n = 0; n < NM; ++n) q[n] = y[(long long)n * nbase + b];
+  float xb[4];
+  CellFeat f0 = cell_feat<NM>(th, rho[b], ie[b], q, xb);
+  double fac = 0.8 + 1.7 * u1;
+  double tn = f0.tu + (6. * u2 - 3.);
+#pragma unroll
+  for (int n = 0; n < NM; ++n) {
+    if (n == th.iv) q[n] = f0.qt * fac;
+    if (n == th.ic) q[n] = 0.;
+  }
+  double qd = 1.;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) qd -= q[n];
+  double cfix = th.cvd * qd, qv = 0.;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) {
+    if (n == th.iv) qv = q[n];
+    if (n == th.iv || n == th.ic) continue;
+    cfix += th.cvd * (1. + th.cv_ratio_m1[n]) * q[n];
+  }
+  double e = rho[b] * ((cfix + th.cvv * qv) * tn);
+  CellFeat f = cell_feat<NM>(th, rho[b], e, q, x + 4 * p);
+  double T = 0.;
+  int it = adjust_cell<NM>(th, rho[b], &e, q, -1.0, &T);
+  if (it < 0) atomicOr(status, 2);
+  double qvo = 0., qco = 0.;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) {
+    if (n == th.iv) qvo = q[n];
+    if (n == th.ic) qco = q[n];
+  }
+  t[3 * p + 0] = (float)((T - f.tu) / 10.);
+  t[3 * p + 1] = (float)(f.qt > 0. ? qvo / f.qt : 1.);
+  t[3 * p + 2] = (float)(f.qt > 0. ? qco / f.qt : 0.);
+  if (iters) iters[p] = it;
+}
+
+// Guided saturation adjustment on flat arrays: a wave whose cells include a
+// saturated one evaluates the network for its 64 cells and starts Newton at
+// T_u + 10 y0 (kept in [T_u, T_u + 100]); the Newton iteration, tolerance
+// and result are those of the plain adjustment.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_satadj_mlp(KThermo th, long long ncell, const double* __restrict__ rho,
+                 double* __restrict__ ie, double* __restrict__ y,
+                 const uint16_t* __restrict__ tiles,
+                 const float* __restrict__ P, int* __restrict__ iters,
+                 int* __restrict__ status) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = c < ncell;
+  double q[NM];
+  double r = 1.0, e = 1.0;
+  if (valid) {
+#pragma unroll
+    for (int n = 0; n < NM; ++n) q[n] = y[n * ncell + c];
+    r = rho[c];
+    e = ie[c];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NM; ++n) q[n] = 0.;
+  }
+  float xin[4];
+  CellFeat f = cell_feat<NM>(th, r, e, q, xin);
+  const bool sat = valid && f.qt > qsat_d(th, f.tu, r);
+  if (!valid)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xin[k] = 0.f;
+  double tg = -1.0;
+  if (__any(sat)) {
+    float yo[3];
+    forward_wave(tiles, P, xin, yo);
+    tg = f.tu + 10.0 * (double)yo[0];
+    if (!(tg >= f.tu)) tg = f.tu;
+    if (tg > f.tu + 100.0) tg = f.tu + 100.0;
+  }
+  if (!valid) return;
+  int it = adjust_cell<NM>(th, r, &e, q, sat ? tg : -1.0);
+  if (it < 0) atomicOr(status, 2);
+  ie[c] = e;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) y[n * ncell + c] = q[n];
+  if (iters) iters[c] = it;
+}
+
+}  // namespace mlp
+}  // namespace cg

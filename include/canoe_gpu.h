@@ -157,16 +157,66 @@ int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
  * outflow), as the upstream BoundaryValues do after CONS2PRIM. */
 int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w);
 
+/* ---- saturation-guess network -------------------------------------------- *
+ * No counterpart in the reference (its adjustment, kintera ThermoY::forward
+ * called at src/tasklist/implicit_hydro_tasks.cpp:308, always starts from the
+ * all-vapour state).  A 4 -> 64 -> 64 -> 64 -> 3 perceptron with tanh-GELU,
+ * bf16 weights and activations with fp32 accumulation on the matrix cores,
+ * maps a cell's inputs
+ *   x = ((T_u - 260)/40, ln(p_u/1e5), 40 q_t, 40 q_p)
+ * (T_u the all-vapour temperature, p_u = rho T_u (R_d q_d + R_v q_t), q_t
+ * total water, q_p the remaining species) to
+ *   y = ((T - T_u)/10, q_v/q_t, q_c/q_t)
+ * of the adjusted state; T_u + 10 y0 starts the Newton iteration of the
+ * guided adjustment.  Parameters: CG_MLP_NPARAM fp32 values, layout W1 (64x4),
+ * b1 (64), W2 (64x64), b2, W3 (64x64), b3, W4 (3x64), b4 (3), W row-major
+ * (out, in).  A network belongs to one context. */
+#define CG_MLP_NPARAM 8835
+typedef struct cg_mlp cg_mlp;
+typedef struct cg_mlp_train_cfg {
+  int batch;           /* samples per Adam step (<= 2^20) */
+  int steps;           /* Adam steps */
+  float lr, lr_final;  /* cosine schedule from lr to lr_final */
+  unsigned long long seed;  /* batch draws */
+} cg_mlp_train_cfg;
+/* Glorot-uniform weights from seed, zero biases. */
+int cg_mlp_create(cg_ctx* ctx, unsigned long long seed, cg_mlp** out);
+int cg_mlp_destroy(cg_mlp* net);
+/* Host fp32 parameter vectors; set resets the Adam moments. */
+int cg_mlp_set_params(cg_mlp* net, const float* host_params);
+int cg_mlp_get_params(cg_mlp* net, float* host_params);
+/* y (n, 3) = net(x (n, 4)), device arrays. */
+int cg_mlp_forward(cg_mlp* net, long long n, const float* x, float* y);
+/* Mean squared error over the n pairs (x, t) and its gradient (device,
+ * CG_MLP_NPARAM floats, nullable); loss_host nullable. */
+int cg_mlp_loss_grad(cg_mlp* net, long long n, const float* x, const float* t,
+                     float* grad, double* loss_host);
+/* cfg->steps Adam steps on batches drawn from the npairs device pairs;
+ * loss_host (nullable) receives cfg->steps batch losses. */
+int cg_mlp_train(cg_mlp* net, const cg_mlp_train_cfg* cfg, long long npairs,
+                 const float* x, const float* t, float* loss_host);
+/* Training pairs from nbase device states (rho, ie, y laid out as in
+ * cg_saturation_adjust): pair p takes base cell floor(u0 nbase), scales its
+ * total water by 0.8 + 1.7 u1 into vapour, shifts its all-vapour temperature
+ * by 6 u2 - 3 K at fixed density (u0..u2 from a splitmix64 stream of seed
+ * and p) and records the plain adjustment of that state.  x (npairs, 4),
+ * t (npairs, 3), iters (nullable) the plain Newton counts. */
+int cg_mlp_make_pairs(cg_ctx* ctx, const cg_thermo* th, int nbase,
+                      const double* rho, const double* ie, const double* y,
+                      long long npairs, unsigned long long seed, float* x,
+                      float* t, int* iters);
+
 /* ---- saturation adjustment ---------------------------------------------- *
  * Replaces kintera pthermo->forward(rho, ie, yfrac)
  * (src/tasklist/implicit_hydro_tasks.cpp:300-312) on ncell cells: rho total
  * density, ie internal energy density (in/out), y (nmass, ncell) mass
- * fractions (in/out).  Newton on T to th->tol.  If mlp_params is non-NULL the
- * Newton iteration starts from the network's guess (see cg_mlp_*).  iters
- * (nullable) receives the Newton iteration count per cell. */
+ * fractions (in/out).  Newton on T to th->tol.  With a network (net non-NULL)
+ * a wavefront holding a saturated cell starts Newton from the network's
+ * guess; iteration and tolerance are unchanged.  iters (nullable) receives
+ * the Newton iteration count per cell (0 for sub-saturated cells). */
 int cg_saturation_adjust(cg_ctx* ctx, const cg_thermo* th, int ncell,
                          const double* rho, double* ie, double* y,
-                         const void* mlp_params, int* iters);
+                         const cg_mlp* net, int* iters);
 
 /* ---- whole step on a device-resident block ------------------------------ *
  * A run owns u, w, u1, du and the scratch of one MeshBlock.  One step is

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -1132,7 +1133,8 @@ double qsat(const SatParams& s, double T, double rho) {
 
 // one cell; q (nmass) in/out, returns iterations or -1 if not converged
 int adjust_cell(const cg_thermo& th, const SatParams& s, double rho,
-                double* ie, double* q, const double* tguess) {
+                double* ie, double* q, const double* tguess,
+                double* tout = nullptr) {
   double cvd = s.cvd;
   double qd = 1.;
   for (int n = 0; n < th.nmass; ++n) qd -= q[n];
@@ -1172,6 +1174,7 @@ int adjust_cell(const cg_thermo& th, const SatParams& s, double rho,
   *ie = *ie + rho * s.L * (qc - q[s.ic]);
   q[s.iv] = qt - qc;
   q[s.ic] = qc;
+  if (tout) *tout = T;
   return it;
 }
 
@@ -1189,6 +1192,99 @@ int saturation_adjust(const cg_thermo& th, int ncell, const double* rho,
     if (it < 0) err = CG_ERR_CONVERGE;
     for (int n = 0; n < th.nmass; ++n) y[(size_t)n * ncell + c] = q[n];
     if (iters) iters[c] = it;
+  }
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// Training pairs of the saturation-guess network (no reference counterpart:
+// the learned initial guess is this project's addition to the adjustment).
+//
+// Network inputs of a cell (rho, ie, q): the all-vapour temperature T_u (the
+// plain Newton start), p_u = rho T_u (R_d q_d + R_v q_t), the total water
+// q_t = q_v + q_c and the remaining (precipitating) mass fraction q_p:
+//   x = ((T_u - 260) / 40, ln(p_u / 1e5), 40 q_t, 40 q_p).
+// Targets: ((T - T_u) / 10, q_v / q_t, q_c / q_t) of the adjusted state.
+struct CellFeat {
+  double cfix, qt, qp, em, tu;
+};
+
+CellFeat cell_feat(const cg_thermo& th, const SatParams& s, double rho,
+                   double ie, const double* q, float* x) {
+  CellFeat f;
+  double qd = 1.;
+  for (int n = 0; n < th.nmass; ++n) qd -= q[n];
+  f.cfix = s.cvd * qd;
+  f.qp = 0.;
+  for (int n = 0; n < th.nmass; ++n) {
+    if (n == s.iv || n == s.ic) continue;
+    f.cfix += s.cvd * (1. + th.cv_ratio_m1[n]) * q[n];
+    f.qp += q[n];
+  }
+  f.qt = q[s.iv] + q[s.ic];
+  f.em = ie / rho - s.L * q[s.ic];
+  f.tu = f.em / (f.cfix + s.cvv * f.qt);
+  double pu = rho * f.tu * (th.rd * qd + s.rv * f.qt);
+  x[0] = (float)((f.tu - 260.) / 40.);
+  x[1] = (float)cg_log(pu / 1e5);
+  x[2] = (float)(40. * f.qt);
+  x[3] = (float)(40. * f.qp);
+  return f;
+}
+
+// splitmix64 (Steele, Lea & Flood 2014): stream of pair p starts at
+// seed + 4 p * 0x9E3779B97F4A7C15; uniform doubles from the top 53 bits.
+inline uint64_t sm64_next(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline double sm64_unit(uint64_t& st) {
+  return (double)(sm64_next(st) >> 11) * 0x1.0p-53;
+}
+
+// Pair p: a base cell b = floor(u0 nbase) of the given states, its total
+// water scaled by f = 0.8 + 1.7 u1 and put into vapour, its all-vapour
+// temperature shifted by 6 u2 - 3 K at fixed density; the plain Newton
+// adjustment of that state gives the targets.
+int mlp_pairs(const cg_thermo& th, int nbase, const double* rho,
+              const double* ie, const double* y, int npairs, uint64_t seed,
+              float* x, float* t, int* iters) {
+  if (th.ivap <= 0 || th.icld <= 0 || th.nmass > CG_MAX_NMASS) return CG_ERR_ARG;
+  SatParams s = sat_params(th);
+  int err = CG_OK;
+  for (int p = 0; p < npairs; ++p) {
+    uint64_t st = seed + 4ull * (uint64_t)p * 0x9E3779B97F4A7C15ull;
+    double u0 = sm64_unit(st), u1 = sm64_unit(st), u2 = sm64_unit(st);
+    int b = (int)(u0 * nbase);
+    if (b >= nbase) b = nbase - 1;
+    double q[CG_MAX_NMASS];
+    for (int n = 0; n < th.nmass; ++n) q[n] = y[(size_t)n * nbase + b];
+    float xb[4];
+    CellFeat f0 = cell_feat(th, s, rho[b], ie[b], q, xb);
+    double fac = 0.8 + 1.7 * u1;
+    double tn = f0.tu + (6. * u2 - 3.);
+    q[s.iv] = f0.qt * fac;
+    q[s.ic] = 0.;
+    double qd = 1.;
+    for (int n = 0; n < th.nmass; ++n) qd -= q[n];
+    double cfix = s.cvd * qd;
+    for (int n = 0; n < th.nmass; ++n) {
+      if (n == s.iv || n == s.ic) continue;
+      cfix += s.cvd * (1. + th.cv_ratio_m1[n]) * q[n];
+    }
+    double e = rho[b] * ((cfix + s.cvv * q[s.iv]) * tn);
+    CellFeat f = cell_feat(th, s, rho[b], e, q, x + 4 * (size_t)p);
+    double T = 0.;
+    int it = adjust_cell(th, s, rho[b], &e, q, nullptr, &T);
+    if (it < 0) err = CG_ERR_CONVERGE;
+    double qt = f.qt;
+    t[3 * (size_t)p + 0] = (float)((T - f.tu) / 10.);
+    t[3 * (size_t)p + 1] = (float)(qt > 0. ? q[s.iv] / qt : 1.);
+    t[3 * (size_t)p + 2] = (float)(qt > 0. ? q[s.ic] / qt : 0.);
+    if (iters) iters[p] = it;
   }
   return err;
 }
@@ -1369,6 +1465,25 @@ int or_saturation_adjust(const cg_thermo* th, int ncell, const double* rho,
                          double* ie, double* y, const double* t_guess,
                          int* iters) {
   return saturation_adjust(*th, ncell, rho, ie, y, t_guess, iters);
+}
+
+int or_mlp_pairs(const cg_thermo* th, int nbase, const double* rho,
+                 const double* ie, const double* y, int npairs,
+                 unsigned long long seed, float* x, float* t, int* iters) {
+  return mlp_pairs(*th, nbase, rho, ie, y, npairs, seed, x, t, iters);
+}
+
+int or_mlp_features(const cg_thermo* th, int ncell, const double* rho,
+                    const double* ie, const double* y, float* x, double* tu) {
+  if (th->ivap <= 0 || th->icld <= 0 || th->nmass > CG_MAX_NMASS) return CG_ERR_ARG;
+  SatParams s = sat_params(*th);
+  for (int c = 0; c < ncell; ++c) {
+    double q[CG_MAX_NMASS];
+    for (int n = 0; n < th->nmass; ++n) q[n] = y[(size_t)n * ncell + c];
+    CellFeat f = cell_feat(*th, s, rho[c], ie[c], q, x + 4 * (size_t)c);
+    if (tu) tu[c] = f.tu;
+  }
+  return CG_OK;
 }
 
 int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,

@@ -77,6 +77,26 @@ int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,
                     const double* wl, const double* wr, double* abs_a,
                     double* prim_out);
 
+/* Saturation-guess network (oracle_mlp.cpp).  Parameters: or_mlp_nparam()
+ * floats, layout W1 (64x4), b1, W2 (64x64), b2, W3 (64x64), b3, W4 (3x64),
+ * b4.  emulate = 1 rounds as the device does (bf16 operands, fp32 sums). */
+int or_mlp_nparam(void);
+float or_mlp_gelu(float z);
+float or_mlp_gelu_grad(float z);
+float or_mlp_exp(float x);
+unsigned short or_mlp_bf16(float x);
+int or_mlp_forward(const float* params, int n, const float* x, double* y,
+                   int emulate);
+int or_mlp_loss_grad(const float* params, int n, const float* x,
+                     const float* t, double* grad, double* loss, int emulate);
+/* Training pairs from base states (rho, ie, y as in or_saturation_adjust) and
+ * network inputs of cells; x is (n, 4), t is (n, 3), tu the all-vapour T. */
+int or_mlp_pairs(const cg_thermo* th, int nbase, const double* rho,
+                 const double* ie, const double* y, int npairs,
+                 unsigned long long seed, float* x, float* t, int* iters);
+int or_mlp_features(const cg_thermo* th, int ncell, const double* rho,
+                    const double* ie, const double* y, float* x, double* tu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,5 @@
-// Oracle math routines -- TEST INFRASTRUCTURE ONLY (see oracle/README.md).
+// Oracle math routines -- TEST INFRASTRUCTURE ONLY (the checker, never the
+// product path).
 //
 // exp and log written from their textbook definitions so that the CPU oracle
 // and the device code (which carries its own copy of the same algorithm)
@@ -9,7 +10,7 @@
 //        by ldexp (exact).
 //   log: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
 //        s = (m-1)/(m+1), odd series through s^23; e ln2 added in two parts.
-// Accuracy is checked against glibc to <= 2 ulp in tests/test_oracle_math.py.
+// Accuracy is checked against numpy to <= 2 ulp in tests/test_oracle_pins.py.
 #pragma once
 #include <cmath>
 

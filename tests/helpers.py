@@ -48,10 +48,10 @@ def perturbed_state(config, dims, seed=7, vel=2.0, dp=5e-3, cloud=True):
 
 
 def cfl_dt(blk, th, cfg, w, cfl=0.9):
-    """Time step of the patched Hydro::NewBlockTimeStep
-    (patches/21.new_blockdt.patch:31-45): the implicit x1 direction drops
-    the sound speed.  Sound speed with the dry gamma (upper bound for moist
-    runs)."""
+    """Time step from the signal speeds of the patched
+    Hydro::NewBlockTimeStep (patches/21.new_blockdt.patch:31-45: the
+    implicit x1 direction drops the sound speed), summed over directions as
+    the unsplit update requires.  Sound speed with the dry gamma."""
     sk, sj, si = blk.interior()
     ivx = 1 + blk.nmass
     rho = w[0][sk, sj, si]
@@ -64,5 +64,5 @@ def cfl_dt(blk, th, cfg, w, cfl=0.9):
             continue
         v = np.abs(w[ivx + d][sk, sj, si])
         s = v if (d == 0 and cfg.implicit_flag) else v + cs
-        inv = np.maximum(inv, s / dxs[d]) if np.ndim(inv) else s / dxs[d]
+        inv = inv + s / dxs[d]
     return float(cfl / np.max(inv))

@@ -37,6 +37,16 @@ _SIGS = {
     "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
     "or_flux_jacobian": (_I, [_P, _D, _P, _P]),
     "or_roe_matrices": (_I, [_P, _P, _D, _P, _P, _P, _P]),
+    "or_mlp_nparam": (_I, []),
+    "or_mlp_gelu": (ctypes.c_float, [ctypes.c_float]),
+    "or_mlp_gelu_grad": (ctypes.c_float, [ctypes.c_float]),
+    "or_mlp_exp": (ctypes.c_float, [ctypes.c_float]),
+    "or_mlp_bf16": (ctypes.c_ushort, [ctypes.c_float]),
+    "or_mlp_forward": (_I, [_P, _I, _P, _P, _I]),
+    "or_mlp_loss_grad": (_I, [_P, _I, _P, _P, _P, _P, _I]),
+    "or_mlp_pairs": (_I, [_P, _I, _P, _P, _P, _I, ctypes.c_ulonglong, _P,
+                          _P, _P]),
+    "or_mlp_features": (_I, [_P, _I, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -155,3 +165,52 @@ def saturation_adjust(th, rho, ie, y, tguess=None):
     r = lib().or_saturation_adjust(ref(th), n, ptr(rho), ptr(ie), ptr(y), tg,
                                    ptr(it))
     return ie, y, it, r
+
+
+# ------------------------------------------------------- saturation guess
+def mlp_forward(params, x, emulate=1):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((x.shape[0], 3), np.float64)
+    r = lib().or_mlp_forward(ptr(p), x.shape[0], ptr(x), ptr(y), emulate)
+    assert r == 0
+    return y
+
+
+def mlp_loss_grad(params, x, t, emulate=1):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.zeros(p.size, np.float64)
+    loss = ctypes.c_double()
+    r = lib().or_mlp_loss_grad(ptr(p), x.shape[0], ptr(x), ptr(t), ptr(g),
+                               ctypes.cast(ctypes.byref(loss), ctypes.c_void_p),
+                               emulate)
+    assert r == 0
+    return g, loss.value
+
+
+def mlp_features(th, rho, ie, y):
+    rho = np.ascontiguousarray(rho, np.float64)
+    ie = np.ascontiguousarray(ie, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    n = rho.shape[0]
+    x = np.zeros((n, 4), np.float32)
+    tu = np.zeros(n, np.float64)
+    r = lib().or_mlp_features(ref(th), n, ptr(rho), ptr(ie), ptr(y), ptr(x),
+                              ptr(tu))
+    assert r == 0
+    return x, tu
+
+
+def mlp_pairs(th, rho, ie, y, npairs, seed):
+    rho = np.ascontiguousarray(rho, np.float64)
+    ie = np.ascontiguousarray(ie, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    x = np.zeros((npairs, 4), np.float32)
+    t = np.zeros((npairs, 3), np.float32)
+    it = np.zeros(npairs, np.int32)
+    r = lib().or_mlp_pairs(ref(th), rho.shape[0], ptr(rho), ptr(ie), ptr(y),
+                           npairs, seed, ptr(x), ptr(t), ptr(it))
+    assert r == 0
+    return x, t, it

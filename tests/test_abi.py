@@ -1,0 +1,35 @@
+"""The C-ABI library loads and exports every function include/canoe_gpu.h
+declares (CPU only: no compute call is made)."""
+import ctypes
+import os
+import re
+
+import canoe_amd as ca
+from canoe_amd import mlp as cm
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
+    __file__))), "include", "canoe_gpu.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_function_is_exported():
+    names = declared()
+    assert len(names) >= 30
+    L = ctypes.CDLL(ca.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    bound = set(ca._SIGS) | set(cm._SIGS)
+    missing = [n for n in declared() if n not in bound]
+    assert not missing, missing
+
+
+def test_version_string_names_the_arch():
+    assert b"gfx950" in ca.lib().cg_version()
