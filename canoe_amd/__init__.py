@@ -146,6 +146,16 @@ _SIGS = {
     "cg_run_kernel_stats": (_I, [_P, _I, ctypes.POINTER(_D),
                                  ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_char_p)]),
+    "cg_batch_create": (_I, [_P, _I, _P, _P, _P, ctypes.POINTER(_P)]),
+    "cg_run_nblocks": (_I, [_P, ctypes.POINTER(_I)]),
+    "cg_run_block_set_prim_host": (_I, [_P, _I, _P]),
+    "cg_run_block_get_cons_host": (_I, [_P, _I, _P]),
+    "cg_run_block_get_prim_host": (_I, [_P, _I, _P]),
+    "cg_run_block_sums": (_I, [_P, _I, _P]),
+    "cg_run_get_tune": (_I, [_P, _P]),
+    "cg_run_set_tune": (_I, [_P, _P]),
+    "cg_run_autotune": (_I, [_P, _D, _P]),
+    "cg_run_set_mlp": (_I, [_P, _P]),
     "cg_version": (ctypes.c_char_p, []),
     "cg_apply_bc_host": (_I, [_P, _P]),
     "cg_pgen_setup": (_I, [_I, _I, _I, _I, _P, _P, _P]),
@@ -180,7 +190,7 @@ def _ref(s):
 
 # ---------------------------------------------------------------------------
 def pgen_setup(config, nx1=0, nx2=0, nx3=0):
-    """(Block, Thermo, Config) of configuration `config` (1, 2 or 4)."""
+    """(Block, Thermo, Config) of configuration `config` (1, 2, 4 or 5)."""
     b, t, c = Block(), Thermo(), Config()
     r = lib().cg_pgen_setup(config, nx1, nx2, nx3, _ref(b), _ref(t), _ref(c))
     if r:
@@ -196,6 +206,23 @@ def pgen_init(config, seed, blk, th, cfg):
     if r:
         raise CanoeGpuError(r, "cg_pgen_init(%d)" % config)
     return w
+
+
+def batch_blocks(seed, nblocks=32, heights=(40, 64, 96, 128, 200, 256),
+                 footprints=(32, 64, 96, 128)):
+    """Seeded blocks of the uneven-column batch: heights and footprint edges
+    drawn from the given sets, each block with its own seeded profile.
+    Returns [(Block, Thermo, Config, w0)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(nblocks):
+        nx1 = int(rng.choice(heights))
+        nx2 = int(rng.choice(footprints))
+        nx3 = int(rng.choice(footprints))
+        blk, th, cfg = pgen_setup(5, nx1, nx2, nx3)
+        w = pgen_init(5, int(seed) * 1000 + b, blk, th, cfg)
+        out.append((blk, th, cfg, w))
+    return out
 
 
 def apply_bc_host(blk, w):
@@ -335,39 +362,79 @@ class ImplicitSolver:
         return out
 
 
-class Run:
-    """A device-resident MeshBlock advanced by full SSPRK(3,3) steps."""
+class Tune(ctypes.Structure):
+    """cg_tune: launch shape of a run."""
+    _fields_ = [("implicit_wpb", ctypes.c_int), ("implicit_grid", ctypes.c_int),
+                ("flux_block", ctypes.c_int), ("step_ms", ctypes.c_double)]
 
-    def __init__(self, ctx, blk, th, cfg):
-        self.ctx, self.blk, self.th, self.cfg = ctx, blk, th, cfg
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Run:
+    """Device-resident MeshBlocks advanced by full SSPRK(3,3) steps: one
+    block (blk) or a batch of independent blocks (blocks = [Block, ...]);
+    methods taking `b` address block b of a batch."""
+
+    def __init__(self, ctx, blk, th, cfg, blocks=None):
+        self.ctx, self.th, self.cfg = ctx, th, cfg
+        self.blocks = list(blocks) if blocks is not None else [blk]
+        self.blk = self.blocks[0]
         h = ctypes.c_void_p()
-        ctx._check(lib().cg_run_create(ctx.h, _ref(blk), _ref(th), _ref(cfg),
-                                       ctypes.byref(h)), "cg_run_create")
+        arr = (Block * len(self.blocks))(*self.blocks)
+        ctx._check(lib().cg_batch_create(
+            ctx.h, len(self.blocks), ctypes.cast(arr, ctypes.c_void_p),
+            _ref(th), _ref(cfg), ctypes.byref(h)), "cg_batch_create")
         self.h = h
 
-    def set_prim(self, w):
+    def set_prim(self, w, b=0):
         w = np.ascontiguousarray(w, np.float64)
-        assert w.shape == self.blk.shape()
-        self.ctx._check(lib().cg_run_set_prim_host(self.h, _ptr(w)), "set_prim")
+        assert w.shape == self.blocks[b].shape()
+        self.ctx._check(lib().cg_run_block_set_prim_host(self.h, b, _ptr(w)),
+                        "set_prim")
 
-    def cons(self):
-        u = np.zeros(self.blk.shape())
-        self.ctx._check(lib().cg_run_get_cons_host(self.h, _ptr(u)), "get_cons")
+    def cons(self, b=0):
+        u = np.zeros(self.blocks[b].shape())
+        self.ctx._check(lib().cg_run_block_get_cons_host(self.h, b, _ptr(u)),
+                        "get_cons")
         return u
 
-    def prim(self):
-        w = np.zeros(self.blk.shape())
-        self.ctx._check(lib().cg_run_get_prim_host(self.h, _ptr(w)), "get_prim")
+    def prim(self, b=0):
+        w = np.zeros(self.blocks[b].shape())
+        self.ctx._check(lib().cg_run_block_get_prim_host(self.h, b, _ptr(w)),
+                        "get_prim")
         return w
 
     def step(self, dt, nsteps=1, graph=True):
         self.ctx._check(lib().cg_run_step(self.h, dt, nsteps, int(graph)),
                         "cg_run_step")
 
-    def sums(self):
-        out = np.zeros(self.blk.N)
-        self.ctx._check(lib().cg_run_sums(self.h, _ptr(out)), "cg_run_sums")
+    def sums(self, b=0):
+        out = np.zeros(self.blocks[b].N)
+        self.ctx._check(lib().cg_run_block_sums(self.h, b, _ptr(out)),
+                        "cg_run_sums")
         return out
+
+    def set_mlp(self, net):
+        """Network of the guided saturation adjustment (config satadj =
+        SATADJ_MLP); None restores the plain Newton adjustment."""
+        self._mlp = net
+        self.ctx._check(lib().cg_run_set_mlp(
+            self.h, net.h if net is not None else None), "set_mlp")
+
+    def tune(self):
+        t = Tune()
+        self.ctx._check(lib().cg_run_get_tune(self.h, _ref(t)), "get_tune")
+        return t
+
+    def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256):
+        t = Tune(implicit_wpb, implicit_grid, flux_block, 0.0)
+        self.ctx._check(lib().cg_run_set_tune(self.h, _ref(t)), "set_tune")
+
+    def autotune(self, dt):
+        t = Tune()
+        self.ctx._check(lib().cg_run_autotune(self.h, dt, _ref(t)), "autotune")
+        return t
 
     def set_timing(self, on):
         self.ctx._check(lib().cg_run_set_timing(self.h, int(on)), "timing")

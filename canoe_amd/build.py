@@ -31,7 +31,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-SOURCES_HIP = ["capi.hip", "mlp.hip"]
+SOURCES_HIP = ["capi.hip", "run.hip", "mlp.hip"]
 SOURCES_CPP = ["pgen.cpp"]
 
 

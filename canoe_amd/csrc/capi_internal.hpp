@@ -87,4 +87,9 @@ inline int check_status(cg_ctx* ctx) {
   return CG_OK;
 }
 
+// Guided saturation adjustment of a run's resident interior cells (mlp.hip).
+int launch_satadj_run_mlp(int nmass, const cg::KGrid& g, const cg::KThermo& th,
+                          const cg_mlp* net, double* u, double* w, int* status,
+                          hipStream_t s);
+
 }  // namespace cg_host

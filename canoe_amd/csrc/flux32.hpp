@@ -353,7 +353,7 @@ template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* ff,
                      double* psf, double* psv, const double* pref, double* du,
-                     hipStream_t s, KTimer* tm, bool scan) {
+                     hipStream_t s, KTimer* tm, bool scan, int bs = 256) {
   const long long ncol = (long long)g.nx2 * g.nx3;
   const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   const int decomp = g.grav != 0.0;
@@ -364,35 +364,35 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     TIMED(tm, KID_SCAN,
           (k_hydrostatic_scan<NM><<<(unsigned)((ncol + 255) / 256), 256, 0, s>>>(
               g, th, w, psf, psv)));
-  const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + 255) / 256);
+  const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + bs - 1) / bs);
   if (cfg.order_v == 5)
-    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, 256, 0, s>>>(
+    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, bs, 0, s>>>(
                                g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
   else
-    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, 256, 0, s>>>(
+    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, bs, 0, s>>>(
                                g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
   if (g.f2) {
     const unsigned b2 =
-        (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + 255) / 256);
+        (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + bs - 1) / bs);
     if (cfg.order_h == 5)
-      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 5, 1><<<b2, 256, 0, s>>>(
+      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 5, 1><<<b2, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f2)));
     else
-      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 2, 1><<<b2, 256, 0, s>>>(
+      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 2, 1><<<b2, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f2)));
   }
   if (g.f3) {
     const unsigned b3 =
-        (unsigned)(((long long)g.nx1 * g.nx2 * (g.nx3 + 1) + 255) / 256);
+        (unsigned)(((long long)g.nx1 * g.nx2 * (g.nx3 + 1) + bs - 1) / bs);
     if (cfg.order_h == 5)
-      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 5, 2><<<b3, 256, 0, s>>>(
+      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 5, 2><<<b3, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f3)));
     else
-      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, 256, 0, s>>>(
+      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f3)));
   }
   TIMED(tm, KID_DIVERGENCE,
-        (k_divergence32<NM><<<(unsigned)((ncell + 255) / 256), 256, 0, s>>>(
+        (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
             g, th, src, f1, f2, f3, psf, decomp, w, du)));
 }
 

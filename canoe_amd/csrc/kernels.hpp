@@ -584,17 +584,17 @@ __device__ inline void abs_roe_fma(const KThermo& th, const double* prim,
 // M = a_i - b_i a'_{i-1} come from one Gauss-Jordan elimination of
 // [M | c_i | rhs] with partial pivoting (the reference forms M^-1 and then
 // multiplies; the two agree to round-off).
+// One column: c0 = offset of (n = 0, k, j, i = 0) in w / du; the archive of
+// level lvl, entry e goes to arch[(lvl (B*B+B) + e) as + ao].  Returns false
+// on a zero pivot.
 template <int NM, int B>
-__global__ void __launch_bounds__(64)
-    k_implicit_fwd(KGrid g, KThermo th, KImplicit ip,
-                   const double* __restrict__ w, const double* __restrict__ du,
-                   double* __restrict__ arch, int* __restrict__ status) {
+__device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
+                                           double dt, const double* __restrict__ w,
+                                           const double* __restrict__ du,
+                                           long long c0, double* __restrict__ arch,
+                                           long long as, long long ao) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   constexpr int AS = B * B + B;
-  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ip.ncol) return;
-  int j = g.js + (int)(col % g.nx2);
-  int k = g.ks + (int)(col / g.nx2);
   const int is = g.is, ie = g.ie;
   const double grav = g.grav;
   const double gm1 = th.gammad - 1.;
@@ -603,8 +603,7 @@ __global__ void __launch_bounds__(64)
   const double vol = g.dx1 * g.dx2 * g.dx3;
   const double s2v = 1. / (2. * vol);
   const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
-  const double idt = 1. / ip.dt;
-  const long long c0 = g.at(0, k, j, 0);
+  const double idt = 1. / dt;
   const long long sn = g.c.n, si = g.c.i;
   auto load = [&](double* dst, int i) {
 #pragma unroll
@@ -750,10 +749,10 @@ __global__ void __launch_bounds__(64)
     for (int r = 0; r < B; ++r)
 #pragma unroll
       for (int q = 0; q < B; ++q)
-        __builtin_nontemporal_store(ap.a[r][q], &arch[(lb + r * B + q) * ip.ncol + col]);
+        __builtin_nontemporal_store(ap.a[r][q], &arch[(lb + r * B + q) * as + ao]);
 #pragma unroll
     for (int r = 0; r < B; ++r)
-      __builtin_nontemporal_store(dl.v[r], &arch[(lb + B * B + r) * ip.ncol + col]);
+      __builtin_nontemporal_store(dl.v[r], &arch[(lb + B * B + r) * as + ao]);
     Am = Ap;
     dm = dc;
     dc = dp;
@@ -763,23 +762,36 @@ __global__ void __launch_bounds__(64)
       wc[n] = wp[n];
     }
   }
-  if (!ok) atomicOr(status, 1);
+  return ok;
+}
+
+template <int NM, int B>
+__global__ void __launch_bounds__(64)
+    k_implicit_fwd(KGrid g, KThermo th, KImplicit ip,
+                   const double* __restrict__ w, const double* __restrict__ du,
+                   double* __restrict__ arch, int* __restrict__ status) {
+  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ip.ncol) return;
+  int j = g.js + (int)(col % g.nx2);
+  int k = g.ks + (int)(col / g.nx2);
+  if (!fwd_column<NM, B>(g, th, ip.dt, w, du, g.at(0, k, j, 0), arch, ip.ncol, col))
+    atomicOr(status, 1);
 }
 
 // delta_i of the back substitution (forward_backward.hpp:86-97): level i
 // from the archive and delta_{i+1}.
 template <int B>
 __device__ inline void back_step(const double* __restrict__ arch,
-                                 long long lb, long long ncol, long long col,
+                                 long long lb, long long as, long long ao,
                                  bool top, const Vt<B>& dn, Vt<B>& d) {
 #pragma unroll
-  for (int r = 0; r < B; ++r) d.v[r] = arch[(lb + B * B + r) * ncol + col];
+  for (int r = 0; r < B; ++r) d.v[r] = arch[(lb + B * B + r) * as + ao];
   if (!top) {
     Mt<B> a;
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int q = 0; q < B; ++q) a.a[r][q] = arch[(lb + r * B + q) * ncol + col];
+      for (int q = 0; q < B; ++q) a.a[r][q] = arch[(lb + r * B + q) * as + ao];
     Vt<B> ad;
     mvec<B>(a, dn, ad);
 #pragma unroll
@@ -929,24 +941,19 @@ __global__ void __launch_bounds__(256)
 // cells of w (reflecting / outflow) and the hydrostatic face and cell
 // pressures of the next stage (the same recurrences as k_hydrostatic_scan,
 // evaluated as the new state appears top-down).
+// One column: c0 as in fwd_column, pc = offset of the column in psf / psv.
 template <int NM, int B>
-__global__ void __launch_bounds__(64)
-    k_implicit_bwd_update(KGrid g, KThermo th, KImplicit ip, KUpdate up,
-                          const double* __restrict__ arch,
-                          const double* __restrict__ du, double* __restrict__ u,
-                          double* __restrict__ u1, double* __restrict__ w,
-                          double* __restrict__ psf, double* __restrict__ psv,
-                          int* __restrict__ status) {
+__device__ __forceinline__ void bwd_update_column(
+    const KGrid& g, const KThermo& th, const KUpdate& up,
+    const double* __restrict__ arch, long long as, long long ao,
+    const double* __restrict__ du, double* __restrict__ u,
+    double* __restrict__ u1, double* __restrict__ w, double* __restrict__ psf,
+    double* __restrict__ psv, long long c0, long long pc,
+    int* __restrict__ status) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IPR = 4 + NM;
   constexpr int AS = B * B + B;
-  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ip.ncol) return;
-  int j = g.js + (int)(col % g.nx2);
-  int k = g.ks + (int)(col / g.nx2);
   const int is = g.is, ie = g.ie;
-  const long long c0 = g.at(0, k, j, 0);
   const long long sn = g.c.n, si = g.c.i;
-  const long long pc = col * g.ps_c;
   const double grav = g.grav, dx = g.dx1;
   const bool hs = grav != 0.0;
   const double gb = (g.bc[0] == 0) ? -grav : grav;
@@ -957,7 +964,7 @@ __global__ void __launch_bounds__(64)
   Vt<B> dn;
   for (int i = ie; i >= is; --i) {
     Vt<B> d;
-    back_step<B>(arch, (long long)(i - is) * AS, ip.ncol, col, i == ie, dn, d);
+    back_step<B>(arch, (long long)(i - is) * AS, as, ao, i == ie, dn, d);
     dn = d;
     const long long ci = c0 + i * si;
     double duo[N], wsp[N], dun[N], uc[N], wc[N];
@@ -1043,6 +1050,90 @@ __global__ void __launch_bounds__(64)
     }
   }
   (void)rtop;
+}
+
+template <int NM, int B>
+__global__ void __launch_bounds__(64)
+    k_implicit_bwd_update(KGrid g, KThermo th, KImplicit ip, KUpdate up,
+                          const double* __restrict__ arch,
+                          const double* __restrict__ du, double* __restrict__ u,
+                          double* __restrict__ u1, double* __restrict__ w,
+                          double* __restrict__ psf, double* __restrict__ psv,
+                          int* __restrict__ status) {
+  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ip.ncol) return;
+  int j = g.js + (int)(col % g.nx2);
+  int k = g.ks + (int)(col / g.nx2);
+  bwd_update_column<NM, B>(g, th, up, arch, ip.ncol, col, du, u, u1, w, psf, psv,
+                           g.at(0, k, j, 0), col * g.ps_c, status);
+}
+
+// ---------------------------------------------------------------------------
+// Batched column solve: every column of several blocks (of any heights) in
+// one launch.  Columns are taken in groups of 64 of one block (one group per
+// wavefront); a workgroup holds WPB wavefronts and walks the group table
+// with a stride of the whole grid, so a grid smaller than the table makes
+// the launch persistent.  The host sorts the groups by column height, tallest
+// first.
+struct KBlockPtr {
+  KGrid g;
+  double *u, *w, *u1, *du, *psf, *psv;
+};
+
+// Group of up to 64 columns col0 .. col0+63 of block b.  Its archive is
+// lane-contiguous from aoff: level lvl, entry e of lane l at
+// aoff + (lvl (B*B+B) + e) 64 + l.
+struct KGroup {
+  int b, col0;
+  long long aoff;
+};
+
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_fwd_batch(const KBlockPtr* __restrict__ bp,
+                         const KGroup* __restrict__ grp, int ngroups,
+                         KThermo th, double dt, double* __restrict__ arch,
+                         int* __restrict__ status) {
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int q = blockIdx.x * WPB + wv; q < ngroups; q += gridDim.x * WPB) {
+    const int b = __builtin_amdgcn_readfirstlane(grp[q].b);
+    const int col0 = __builtin_amdgcn_readfirstlane(grp[q].col0);
+    const long long aoff = grp[q].aoff;
+    const KGrid g = bp[b].g;
+    const long long col = (long long)col0 + lane;
+    if (col < (long long)g.nx2 * g.nx3) {
+      const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
+      if (!fwd_column<NM, B>(g, th, dt, bp[b].w, bp[b].du, g.at(0, k, j, 0),
+                             arch + aoff, 64, lane))
+        atomicOr(status, 1);
+    }
+  }
+}
+
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_bwd_update_batch(const KBlockPtr* __restrict__ bp,
+                                const KGroup* __restrict__ grp, int ngroups,
+                                KThermo th, KUpdate up,
+                                const double* __restrict__ arch,
+                                int* __restrict__ status) {
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int q = blockIdx.x * WPB + wv; q < ngroups; q += gridDim.x * WPB) {
+    const int b = __builtin_amdgcn_readfirstlane(grp[q].b);
+    const int col0 = __builtin_amdgcn_readfirstlane(grp[q].col0);
+    const long long aoff = grp[q].aoff;
+    const KGrid g = bp[b].g;
+    const long long col = (long long)col0 + lane;
+    if (col < (long long)g.nx2 * g.nx3) {
+      const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
+      const KBlockPtr& p = bp[b];
+      bwd_update_column<NM, B>(g, th, up, arch + aoff, 64, lane, p.du, p.u,
+                               p.u1, p.w, p.psf, p.psv, g.at(0, k, j, 0),
+                               col * g.ps_c, status);
+    }
+  }
 }
 
 // cons <-> prim on interior cells

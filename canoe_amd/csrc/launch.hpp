@@ -1,0 +1,184 @@
+// launch.hpp -- host-side launch helpers shared by the stateless entry points
+// (capi.hip) and the resident runs (run.hip): argument checks, the grid of a
+// block in either layout, buffer sizes and the kernel launch sequences of the
+// flux sweep, the implicit column solve and the ghost-cell fill.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "capi_internal.hpp"
+#include "cg_timer.hpp"
+#include "flux32.hpp"
+#include "kernels.hpp"
+
+namespace cg_host {
+using namespace cg;
+
+inline int check_block(cg_ctx* ctx, const cg_block* b, const cg_thermo* th) {
+  if (!b || !th) return set_err(ctx, CG_ERR_ARG, "null block/thermo");
+  if (b->nx1 < 1 || b->nx2 < 1 || b->nx3 < 1)
+    return set_err(ctx, CG_ERR_ARG, "bad block size");
+  if (b->nmass != th->nmass)
+    return set_err(ctx, CG_ERR_ARG, "block nmass %d != thermo nmass %d",
+                   b->nmass, th->nmass);
+  if (b->nmass != 0 && b->nmass != 3)
+    return set_err(ctx, CG_ERR_ARG, "nmass must be 0 or 3 (got %d)", b->nmass);
+  if (b->nx3 > 1 && b->nx2 == 1)
+    return set_err(ctx, CG_ERR_ARG, "nx3 > 1 requires nx2 > 1");
+  if (b->dx1 <= 0 || b->dx2 <= 0 || b->dx3 <= 0)
+    return set_err(ctx, CG_ERR_ARG, "non-positive spacing");
+  for (int s = 0; s < 6; ++s)
+    if (b->bc[s] < 0 || b->bc[s] > 2)
+      return set_err(ctx, CG_ERR_ARG, "bad boundary flag");
+  return CG_OK;
+}
+
+// Grid of a block in the AthenaArray layout (lvl = 0, the C-ABI's caller
+// arrays) or in the level-major layout of a run's resident state (lvl = 1).
+inline KGrid make_grid(const cg_block& b, int lvl) {
+  KGrid g;
+  g.nx1 = b.nx1;
+  g.nx2 = b.nx2;
+  g.nx3 = b.nx3;
+  g.f2 = b.nx2 > 1;
+  g.f3 = b.nx3 > 1;
+  g.nc1 = b.nx1 + 2 * NG;
+  g.nc2 = g.f2 ? b.nx2 + 2 * NG : 1;
+  g.nc3 = g.f3 ? b.nx3 + 2 * NG : 1;
+  g.is = NG;
+  g.ie = NG + b.nx1 - 1;
+  g.js = g.f2 ? NG : 0;
+  g.je = g.js + b.nx2 - 1;
+  g.ks = g.f3 ? NG : 0;
+  g.ke = g.ks + b.nx3 - 1;
+  g.lvl = lvl;
+  const long long c1 = g.nc1, c2 = g.nc2, c3 = g.nc3;
+  g.sn = c1 * c2 * c3;
+  long long ncol = (long long)b.nx2 * b.nx3;
+  // extents of cells (e1, e2, e3) -> strides
+  auto mk = [&](long long e1, long long e2, long long e3) {
+    Strides st;
+    if (lvl) {
+      st.j = 1;
+      st.k = e2;
+      st.i = e2 * e3;
+    } else {
+      st.i = 1;
+      st.j = e1;
+      st.k = e1 * e2;
+    }
+    st.n = e1 * e2 * e3;
+    return st;
+  };
+  g.c = mk(c1, c2, c3);
+  g.f[0] = mk(c1 + 1, c2, c3);
+  g.f[1] = mk(c1, c2 + 1, c3);
+  g.f[2] = mk(c1, c2, c3 + 1);
+  if (lvl) {
+    g.ps_i = ncol;
+    g.ps_c = 1;
+  } else {
+    g.ps_i = 1;
+    g.ps_c = c1 + 1;
+  }
+  g.dx1 = b.dx1;
+  g.dx2 = b.dx2;
+  g.dx3 = b.dx3;
+  g.grav = b.grav;
+  for (int s = 0; s < 6; ++s) g.bc[s] = b.bc[s];
+  return g;
+}
+
+inline size_t psf_doubles(const KGrid& g) {
+  return (size_t)(g.nc1 + 1) * g.nx2 * g.nx3;
+}
+
+
+inline size_t cell_doubles(const KGrid& g, int N) { return (size_t)N * g.sn; }
+inline size_t f1_doubles(const KGrid& g, int N) {
+  return (size_t)N * g.nc3 * g.nc2 * (g.nc1 + 1);
+}
+inline size_t f2_doubles(const KGrid& g, int N) {
+  return (size_t)N * g.nc3 * (g.nc2 + 1) * g.nc1;
+}
+inline size_t f3_doubles(const KGrid& g, int N) {
+  return (size_t)N * (g.nc3 + 1) * g.nc2 * g.nc1;
+}
+
+// ---- launch helpers (templated on the species count) ----
+template <int NM>
+void launch_fluxes(const KGrid& g, const KThermo& th, const cg_config& cfg,
+                   const double* w, double* psf, double* psv, double* f1,
+                   double* f2, double* f3, hipStream_t s, KTimer* tm = nullptr,
+                   bool scan = true, int bs = 256) {
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const int decomp = g.grav != 0.0;
+  if (decomp && scan)
+    TIMED(tm, KID_SCAN,
+          (k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, th, w, psf, psv)));
+  const long long n1 = ncol * (g.nx1 + 1);
+  if (cfg.order_v == 5)
+    TIMED(tm, KID_FLUX_X1,
+          (k_flux_x1<NM, 5><<<nblk(n1, bs), bs, 0, s>>>(g, th, cfg.rsolver, decomp,
+                                                           w, psf, psv, f1)));
+  else
+    TIMED(tm, KID_FLUX_X1,
+          (k_flux_x1<NM, 2><<<nblk(n1, bs), bs, 0, s>>>(g, th, cfg.rsolver, decomp,
+                                                           w, psf, psv, f1)));
+  if (g.f2) {
+    long long n2 = (long long)g.nx1 * (g.nx2 + 1) * g.nx3;
+    if (cfg.order_h == 5)
+      TIMED(tm, KID_FLUX_X2,
+            (k_flux_h<NM, 5, 1><<<nblk(n2, bs), bs, 0, s>>>(g, th, cfg.rsolver, w, f2)));
+    else
+      TIMED(tm, KID_FLUX_X2,
+            (k_flux_h<NM, 2, 1><<<nblk(n2, bs), bs, 0, s>>>(g, th, cfg.rsolver, w, f2)));
+  }
+  if (g.f3) {
+    long long n3 = (long long)g.nx1 * g.nx2 * (g.nx3 + 1);
+    if (cfg.order_h == 5)
+      TIMED(tm, KID_FLUX_X3,
+            (k_flux_h<NM, 5, 2><<<nblk(n3, bs), bs, 0, s>>>(g, th, cfg.rsolver, w, f3)));
+    else
+      TIMED(tm, KID_FLUX_X3,
+            (k_flux_h<NM, 2, 2><<<nblk(n3, bs), bs, 0, s>>>(g, th, cfg.rsolver, w, f3)));
+  }
+}
+
+template <int NM>
+void launch_implicit(const KGrid& g, const KThermo& th, double dt, int flag,
+                     double* du, const double* w, double* arch, int* status,
+                     hipStream_t s, KTimer* tm = nullptr) {
+  KImplicit ip;
+  ip.dt = dt;
+  ip.ncol = (long long)g.nx2 * g.nx3;
+  unsigned nb = nblk(ip.ncol, 64);
+  if (flag & 8) {
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 5><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd<NM, 5><<<nb, 64, 0, s>>>(g, ip, w, arch, du)));
+  } else {
+    TIMED(tm, KID_IMPLICIT_FWD,
+          (k_implicit_fwd<NM, 3><<<nb, 64, 0, s>>>(g, th, ip, w, du, arch, status)));
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd<NM, 3><<<nb, 64, 0, s>>>(g, ip, w, arch, du)));
+  }
+}
+
+template <int NM>
+void launch_bc(const KGrid& g, double* w, hipStream_t s, KTimer* tm = nullptr) {
+  TIMED(tm, KID_BC_X1,
+        (k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, s>>>(g, w)));
+  if (g.f2)
+    TIMED(tm, KID_BC_X2,
+          (k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, s>>>(g, w)));
+  if (g.f3)
+    TIMED(tm, KID_BC_X3,
+          (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, s>>>(g, w)));
+}
+
+inline size_t arch_doubles(const KGrid& g) {
+  return (size_t)g.nx1 * 30 * (size_t)g.nx2 * g.nx3;
+}
+
+}  // namespace cg_host

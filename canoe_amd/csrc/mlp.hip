@@ -120,6 +120,20 @@ void init_params(uint64_t seed, std::vector<float>& p) {
 
 }  // namespace
 
+namespace cg_host {
+
+int launch_satadj_run_mlp(int nmass, const KGrid& g, const KThermo& th,
+                          const cg_mlp* net, double* u, double* w, int* status,
+                          hipStream_t s) {
+  if (nmass != 3) return CG_ERR_ARG;
+  const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+  mlp::k_satadj_run_mlp<3><<<nblk(ncell, 256), 256, 0, s>>>(g, th, net->tiles, net->P, u,
+                                                            w, status);
+  return CG_OK;
+}
+
+}  // namespace cg_host
+
 extern "C" {
 
 int cg_mlp_create(cg_ctx* ctx, unsigned long long seed, cg_mlp** out) {

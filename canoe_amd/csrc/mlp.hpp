@@ -688,5 +688,73 @@ __global__ void __launch_bounds__(256)
   if (iters) iters[c] = it;
 }
 
+// Guided adjustment of a run's resident state at the end of the last stage
+// (UpdateAllConserved, src/tasklist/implicit_hydro_tasks.cpp:290-313, with
+// the network's Newton start): one lane per interior cell, lanes along the
+// unit-stride index; u (conserved) is adjusted and w recomputed.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_satadj_run_mlp(KGrid g, KThermo th, const uint16_t* __restrict__ tiles,
+                     const float* __restrict__ P, double* __restrict__ u,
+                     double* __restrict__ w, int* __restrict__ status) {
+  constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < (long long)g.nx1 * g.nx2 * g.nx3;
+  long long c0 = 0;
+  double uc[N];
+  if (valid) {
+    int i, j, k;
+    g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+    c0 = g.at(0, k + g.ks, j + g.js, i + g.is);
+#pragma unroll
+    for (int n = 0; n < N; ++n) uc[n] = u[c0 + n * g.c.n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < N; ++n) uc[n] = (n == 0 || n == IEN) ? 1.0 : 0.0;
+  }
+  double rho = uc[0];
+#pragma unroll
+  for (int n = 1; n <= NM; ++n) rho += uc[n];
+  const double m1 = uc[IVX], m2 = uc[IVX + 1], m3 = uc[IVX + 2];
+  const double ke = 0.5 * (m1 * (m1 / rho) + m2 * (m2 / rho) + m3 * (m3 / rho));
+  double ie = uc[IEN] - ke;
+  double q[NM];
+#pragma unroll
+  for (int n = 0; n < NM; ++n) q[n] = uc[n + 1] / rho;
+  float xin[4];
+  CellFeat f = cell_feat<NM>(th, rho, ie, q, xin);
+  const bool sat = valid && f.qt > qsat_d(th, f.tu, rho);
+  if (!valid)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xin[k] = 0.f;
+  double tg = -1.0;
+  if (__any(sat)) {
+    float yo[3];
+    forward_wave(tiles, P, xin, yo);
+    tg = f.tu + 10.0 * (double)yo[0];
+    if (!(tg >= f.tu)) tg = f.tu;
+    if (tg > f.tu + 100.0) tg = f.tu + 100.0;
+  }
+  if (!valid) return;
+  double qic = 0.0;
+#pragma unroll
+  for (int n = 0; n < NM; ++n)
+    if (n == th.ic) qic = q[n];
+  // a sub-saturated cell without condensate is left as it is
+  if (!sat && qic == 0.0) return;
+  int it = adjust_cell<NM>(th, rho, &ie, q, sat ? tg : -1.0);
+  if (it < 0) atomicOr(status, 2);
+#pragma unroll
+  for (int n = 0; n < NM; ++n) uc[n + 1] = q[n] * rho;
+  uc[IEN] = ie + ke;
+  double wc[N];
+  cons_to_prim_cell<NM>(th, uc, wc);
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    u[c0 + n * g.c.n] = uc[n];
+    w[c0 + n * g.c.n] = wc[n];
+  }
+}
+
 }  // namespace mlp
 }  // namespace cg

@@ -89,10 +89,12 @@ void moist_thermo(cg_thermo* th) {
   th->max_iter = 100;
 }
 
-// Hydrostatic profile of the stratified boxes: T falls 6.5 K/km from 300 K
-// to a 220 K isothermal cap; p from the analytic integral.
-void box_profile(double z, double g, double rd, double* T, double* p) {
-  const double Ts = 300.0, Tt = 220.0, lapse = 6.5e-3, ps = 1e5;
+// Hydrostatic profile of the stratified boxes: T falls `lapse` K/m from Ts
+// to an isothermal cap Tt (config 2 / 4: 6.5 K/km, 300 K, 220 K); p from the
+// analytic integral.
+void box_profile(double z, double g, double rd, double* T, double* p,
+                 double Ts = 300.0, double lapse = 6.5e-3, double Tt = 220.0) {
+  const double ps = 1e5;
   const double zt = (Ts - Tt) / lapse;
   if (z <= zt) {
     *T = Ts - lapse * z;
@@ -162,8 +164,10 @@ int cg_apply_bc_host(const cg_block* blk, double* w) {
 }
 
 // config: 1 = 2D dry density current (straka), 2 = 3D dry stratified box,
-// 4 = moist box.  nx* = 0 selects the configuration's own size; other sizes
-// keep the grid spacing (the domain shrinks or grows).
+// 4 = moist box, 5 = one block of the uneven-column batch (the dry box's
+// numerics; its profile is drawn from the seed at cg_pgen_init).  nx* = 0
+// selects the configuration's own size; other sizes keep the grid spacing
+// (the domain shrinks or grows).
 int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
                   cg_thermo* th, cg_config* cfg) {
   if (!blk || !th || !cfg) return CG_ERR_ARG;
@@ -190,7 +194,7 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
     cfg->implicit_flag = 0;
     cfg->visc = 75.0;
     cfg->p0 = 1e5;
-  } else if (config == 2 || config == 4) {
+  } else if (config == 2 || config == 4 || config == 5) {
     blk->nx1 = nx1 > 0 ? nx1 : 128;
     blk->nx2 = nx2 > 0 ? nx2 : 256;
     blk->nx3 = nx3 > 0 ? nx3 : 256;
@@ -200,10 +204,10 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
     blk->grav = -9.8;
     blk->bc[0] = blk->bc[1] = CG_BC_REFLECT;
     blk->bc[2] = blk->bc[3] = blk->bc[4] = blk->bc[5] = CG_BC_PERIODIC;
-    if (config == 2)
-      dry_thermo(th, 1.4, 287.0);
-    else
+    if (config == 4)
       moist_thermo(th);
+    else
+      dry_thermo(th, 1.4, 287.0);
     cfg->order_h = 5;
     cfg->order_v = 2;
     cfg->rsolver = CG_RS_HLLC;
@@ -247,15 +251,23 @@ int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
           w[d.at(d.IPR, k, j, i)] = p;
           w[d.at(0, k, j, i)] = p / (Rd * temp);
         }
-  } else if (config == 2 || config == 4) {
+  } else if (config == 2 || config == 4 || config == 5) {
     const double Rd = th->rd, p0 = cfg->p0 > 0 ? cfg->p0 : 1e5;
     const double chi = (th->gammad - 1.0) / th->gammad;
+    // config 5: surface temperature 250..320 K, lapse rate 4..8 K/km,
+    // isothermal above 200 K
+    double Ts = 300.0, lapse = 6.5e-3, Tt = 220.0;
+    if (config == 5) {
+      Ts = 250.0 + 70.0 * urand(seed, 0xFFFFFFFF00000001ull);
+      lapse = 4e-3 + 4e-3 * urand(seed, 0xFFFFFFFF00000002ull);
+      Tt = 200.0;
+    }
     for (int k = d.ks; k <= ke; ++k)
       for (int j = d.js; j <= je; ++j)
         for (int i = d.is; i <= ie; ++i) {
           double z = blk->x1min + (i - d.is + 0.5) * blk->dx1;
           double T, p;
-          box_profile(z, g, Rd, &T, &p);
+          box_profile(z, g, Rd, &T, &p, Ts, lapse, Tt);
           if (z < 1000.0) {
             uint64_t idx = ((uint64_t)(k - d.ks) * blk->nx2 + (j - d.js)) *
                                blk->nx1 + (i - d.is);

@@ -1,0 +1,687 @@
+// run.hip -- resident runs: one MeshBlock (cg_run_create) or a batch of
+// independent MeshBlocks of any sizes (cg_batch_create) kept on the device
+// and advanced by whole SSPRK(3,3) steps of ImplicitHydroTasks
+// (src/tasklist/implicit_hydro_tasks.cpp:134-313,
+//  src/integrator/integrators.cpp:14-44), replayed from one hipGraph.
+//
+// Per stage: each block's flux sweep and flux divergence (per-block launches;
+// the mixed form computes the faces in FP32), then ONE batched launch of the
+// implicit forward sweep over every column of every block and ONE batched
+// launch of the downward sweep (back substitution + stage update + x1 ghosts
+// + next stage's hydrostatic pressures), then each block's x2 / x3 ghosts.
+// Launch shapes (waves per workgroup and grid of the batched column solve,
+// workgroup size of the face kernels) come from cg_run_autotune.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/canoe_gpu.h"
+#include "launch.hpp"
+
+using namespace cg;
+using namespace cg_host;
+
+namespace {
+
+struct RunBlock {
+  cg_block blk;
+  KGrid g;   // level-major resident state
+  KGrid ga;  // AthenaArray layout of host transfers
+  double *u = nullptr, *w = nullptr, *u1 = nullptr, *du = nullptr;
+  double *f1 = nullptr, *f2 = nullptr, *f3 = nullptr;
+  double *psf = nullptr, *psv = nullptr;
+  float* ff = nullptr;     // FP32 face fluxes of the mixed form
+  double* pref = nullptr;  // per-level reference pressure of the mixed form
+};
+
+}  // namespace
+
+struct cg_run {
+  cg_ctx* ctx = nullptr;
+  cg_thermo th;
+  cg_config cfg;
+  KThermo kt;
+  int N = 5;
+  int AS = 30;  // archive doubles per level (B*B + B)
+  std::vector<RunBlock> blocks;
+  double* arch = nullptr;
+  KBlockPtr* d_bp = nullptr;
+  KGroup* d_grp = nullptr;
+  int ngroups = 0;
+  cg_tune tune;
+  int ncu = 256;
+  double* part = nullptr;
+  const cg_mlp* mlp = nullptr;  // network of the guided adjustment
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double graph_dt = 0.0;
+  KTimer tm;
+};
+
+namespace {
+
+void drop_graph(cg_run* r) {
+  if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  r->gexec = nullptr;
+  r->graph = nullptr;
+}
+
+void free_block(RunBlock& b) {
+  double* ps[] = {b.u, b.w, b.u1, b.du, b.f1, b.f2, b.f3, b.psf, b.psv, b.pref};
+  for (double* p : ps)
+    if (p) (void)hipFree(p);
+  if (b.ff) (void)hipFree(b.ff);
+  b = RunBlock();
+}
+
+unsigned implicit_grid(const cg_run* r) {
+  const int wpb = r->tune.implicit_wpb;
+  long long full = (r->ngroups + wpb - 1) / wpb;
+  long long g = r->tune.implicit_grid > 0 ? r->tune.implicit_grid : full;
+  return (unsigned)std::max(1LL, std::min(g, full));
+}
+
+template <int NM, int B, int WPB>
+void launch_col_batch(cg_run* r, double dt, const KUpdate& up, KTimer* tm) {
+  hipStream_t s = r->ctx->stream;
+  const unsigned gr = implicit_grid(r);
+  TIMED(tm, KID_IMPLICIT_FWD,
+        (k_implicit_fwd_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+            r->d_bp, r->d_grp, r->ngroups, r->kt, dt, r->arch, r->ctx->d_status)));
+  TIMED(tm, KID_IMPLICIT_BWD,
+        (k_implicit_bwd_update_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+            r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
+}
+
+template <int NM>
+void launch_columns(cg_run* r, double dt, const KUpdate& up, KTimer* tm) {
+  if (r->cfg.implicit_flag & 8) {
+    if (r->tune.implicit_wpb == 4)
+      launch_col_batch<NM, 5, 4>(r, dt, up, tm);
+    else if (r->tune.implicit_wpb == 2)
+      launch_col_batch<NM, 5, 2>(r, dt, up, tm);
+    else
+      launch_col_batch<NM, 5, 1>(r, dt, up, tm);
+  } else {
+    launch_col_batch<NM, 3, 1>(r, dt, up, tm);
+  }
+}
+
+template <int NM>
+int run_stage(cg_run* r, double dt, int s) {
+  const double sbeta[3] = {1.0, 0.25, 2. / 3.};
+  const double sg1[3] = {0.0, 0.25, 2. / 3.};
+  const double sg2[3] = {1.0, 0.75, 1. / 3.};
+  hipStream_t st = r->ctx->stream;
+  const double wght = sbeta[s] * dt;
+  KSource src;
+  src.wght = wght;
+  src.visc = r->cfg.visc;
+  src.p0 = r->cfg.p0;
+  src.cp = r->th.gammad / (r->th.gammad - 1.) * r->th.rd;
+  src.chi = (r->th.gammad - 1.) / r->th.gammad;
+  src.heat_flux = r->cfg.heat_flux;
+  KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  const bool imp = r->cfg.implicit_flag != 0;
+  const int bs = r->tune.flux_block;
+  KUpdate up;
+  up.stage = s;
+  up.g1 = sg1[s];
+  up.g2 = sg2[s];
+  // the guided adjustment (network set, satadj = MLP) runs as its own pass
+  // after the update; otherwise the plain Newton adjustment is fused into it
+  const bool guided = (s == 2) && r->cfg.satadj == CG_SATADJ_MLP && r->mlp;
+  up.satadj = (s == 2) && (r->cfg.satadj != CG_SATADJ_OFF) && !guided;
+  for (RunBlock& b : r->blocks) {
+    const KGrid& g = b.g;
+    const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+    // implicit runs get psf / psv from the previous downward sweep
+    if (r->cfg.precision == CG_PREC_FP32) {
+      launch_fluxes32<NM>(g, r->kt, r->cfg, src, b.w, b.ff, b.psf, b.psv, b.pref,
+                          b.du, st, tm, !imp, bs);
+    } else {
+      launch_fluxes<NM>(g, r->kt, r->cfg, b.w, b.psf, b.psv, b.f1, b.f2, b.f3, st,
+                        tm, !imp, bs);
+      TIMED(tm, KID_DIVERGENCE,
+            (k_divergence<NM><<<nblk(ncell, bs), bs, 0, st>>>(
+                g, r->kt, src, b.f1, b.f2, b.f3, b.w, b.du)));
+    }
+  }
+  if (imp) {
+    launch_columns<NM>(r, wght, up, tm);
+    for (RunBlock& b : r->blocks) {
+      const KGrid& g = b.g;
+      if (guided) {
+        // adjusted cells change w: x1 ghosts and hydrostatic pressures again
+        TIMED(tm, KID_SATADJ,
+              (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
+        TIMED(tm, KID_BC_X1,
+              (k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, st>>>(g, b.w)));
+        if (g.grav != 0.0)
+          TIMED(tm, KID_SCAN,
+                (k_hydrostatic_scan<NM><<<nblk((long long)g.nx2 * g.nx3, 256), 256, 0, st>>>(
+                    g, r->kt, b.w, b.psf, b.psv)));
+      }
+      if (g.f2)
+        TIMED(tm, KID_BC_X2,
+              (k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, st>>>(
+                  g, b.w)));
+      if (g.f3)
+        TIMED(tm, KID_BC_X3,
+              (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, st>>>(
+                  g, b.w)));
+    }
+  } else {
+    for (RunBlock& b : r->blocks) {
+      const KGrid& g = b.g;
+      const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+      TIMED(tm, KID_UPDATE,
+            (k_update<NM><<<nblk(ncell, 256), 256, 0, st>>>(
+                g, r->kt, up, b.u, b.u1, b.du, b.w, r->ctx->d_status)));
+      if (guided)
+        TIMED(tm, KID_SATADJ,
+              (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
+      launch_bc<NM>(g, b.w, st, tm);
+    }
+  }
+  return CG_OK;
+}
+
+int run_one_step(cg_run* r, double dt) {
+  for (int s = 0; s < 3; ++s) {
+    int e = (r->N == 5) ? run_stage<0>(r, dt, s) : run_stage<3>(r, dt, s);
+    if (e) return e;
+  }
+  return CG_OK;
+}
+
+int check_config(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                 const cg_config* cfg) {
+  int e = check_block(ctx, blk, th);
+  if (e) return e;
+  if ((cfg->order_h != 2 && cfg->order_h != 5) ||
+      (cfg->order_v != 2 && cfg->order_v != 5))
+    return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
+  if (cfg->implicit_flag != 0 && cfg->implicit_flag != 1 && cfg->implicit_flag != 9)
+    return set_err(ctx, CG_ERR_ARG, "implicit_flag must be 0, 1 or 9");
+  if (cfg->implicit_flag && (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC))
+    return set_err(ctx, CG_ERR_ARG, "periodic x1 implicit solve not supported");
+  if (cfg->satadj != CG_SATADJ_OFF && (th->nmass != 3 || th->ivap <= 0 || th->icld <= 0))
+    return set_err(ctx, CG_ERR_ARG, "saturation adjustment needs a vapour/cloud pair");
+  if (cfg->precision != CG_PREC_FP64 && cfg->precision != CG_PREC_FP32)
+    return set_err(ctx, CG_ERR_ARG, "bad precision");
+  return CG_OK;
+}
+
+bool alloc_block(RunBlock& b, int N, const cg_config& cfg) {
+  auto alloc = [&](double** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(double) * (n > 0 ? n : 1)) == hipSuccess;
+  };
+  const KGrid& g = b.g;
+  size_t nc = cell_doubles(g, N);
+  bool ok = alloc(&b.u, nc) && alloc(&b.w, nc) && alloc(&b.u1, nc) &&
+            alloc(&b.du, nc) && alloc(&b.psf, psf_doubles(g)) &&
+            alloc(&b.psv, psf_doubles(g));
+  if (ok && cfg.precision == CG_PREC_FP64)
+    ok = alloc(&b.f1, f1_doubles(g, N)) && alloc(&b.f2, f2_doubles(g, N)) &&
+         alloc(&b.f3, f3_doubles(g, N));
+  if (ok && cfg.precision == CG_PREC_FP32)
+    ok = hipMalloc((void**)&b.ff, sizeof(float) * (f1_doubles(g, N) + f2_doubles(g, N) +
+                                                   f3_doubles(g, N))) == hipSuccess &&
+         alloc(&b.pref, g.nc1);
+  if (ok)
+    ok = hipMemset(b.u, 0, sizeof(double) * nc) == hipSuccess &&
+         hipMemset(b.w, 0, sizeof(double) * nc) == hipSuccess &&
+         hipMemset(b.u1, 0, sizeof(double) * nc) == hipSuccess &&
+         hipMemset(b.du, 0, sizeof(double) * nc) == hipSuccess &&
+         hipMemset(b.psf, 0, sizeof(double) * psf_doubles(g)) == hipSuccess &&
+         hipMemset(b.psv, 0, sizeof(double) * psf_doubles(g)) == hipSuccess;
+  return ok;
+}
+
+// Column groups of 64, blocks taken tallest first; archive offsets.
+void build_groups(const std::vector<RunBlock>& blocks, int AS,
+                  std::vector<KGroup>& groups, size_t& arch_n) {
+  std::vector<int> order(blocks.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return blocks[a].g.nx1 > blocks[b].g.nx1;
+  });
+  groups.clear();
+  arch_n = 0;
+  for (int b : order) {
+    const KGrid& g = blocks[b].g;
+    const long long ncol = (long long)g.nx2 * g.nx3;
+    for (long long c0 = 0; c0 < ncol; c0 += 64) {
+      KGroup q;
+      q.b = b;
+      q.col0 = (int)c0;
+      q.aoff = (long long)arch_n;
+      groups.push_back(q);
+      arch_n += (size_t)g.nx1 * AS * 64;
+    }
+  }
+}
+
+int run_download(cg_run* r, const RunBlock& b, const double* src, double* host) {
+  cg_ctx* ctx = r->ctx;
+  size_t nc = cell_doubles(b.g, r->N);
+  double* tmp = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&tmp, sizeof(double) * nc, ctx->stream));
+  k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(b.g, b.ga, r->N, src,
+                                                               tmp);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(host, tmp, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipFreeAsync(tmp, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
+template <int NM>
+void prime_block(cg_run* r, RunBlock& b) {
+  hipStream_t s = r->ctx->stream;
+  const KGrid& g = b.g;
+  const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  k_eos<NM, 0><<<nblk(ncell, 256), 256, 0, s>>>(g, r->kt, b.w, b.u);
+  launch_bc<NM>(g, b.w, s);
+  if (b.pref) k_level_mean_p<NM><<<g.nc1, 256, 0, s>>>(g, b.w, b.pref);
+  if (g.grav != 0.0)
+    k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, r->kt, b.w, b.psf, b.psv);
+}
+
+int capture(cg_run* r, double dt) {
+  cg_ctx* ctx = r->ctx;
+  drop_graph(r);
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int e = run_one_step(r, dt);
+  hipGraph_t gr = nullptr;
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
+  if (e) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return e;
+  }
+  if (ce != hipSuccess)
+    return set_err(ctx, CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+  r->graph = gr;
+  HIPCHK(ctx, hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0));
+  r->graph_dt = dt;
+  return CG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
+                    const cg_thermo* th, const cg_config* cfg, cg_run** out) {
+  if (!ctx || !cfg || !out || !blks || !th) return CG_ERR_ARG;
+  *out = nullptr;
+  if (nblocks < 1) return set_err(ctx, CG_ERR_ARG, "nblocks must be >= 1");
+  for (int b = 0; b < nblocks; ++b) {
+    int e = check_config(ctx, &blks[b], th, cfg);
+    if (e) return e;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  cg_run* r = new cg_run();
+  r->ctx = ctx;
+  r->th = *th;
+  r->cfg = *cfg;
+  r->kt = make_thermo(*th);
+  r->N = 5 + th->nmass;
+  r->AS = (cfg->implicit_flag & 8) ? 30 : 12;
+  r->tune.implicit_wpb = 1;
+  r->tune.implicit_grid = 0;
+  r->tune.flux_block = 256;
+  r->tune.step_ms = 0.0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+    r->ncu = prop.multiProcessorCount;
+  r->blocks.resize(nblocks);
+  bool ok = true;
+  for (int b = 0; b < nblocks && ok; ++b) {
+    RunBlock& rb = r->blocks[b];
+    rb.blk = blks[b];
+    rb.g = make_grid(blks[b], 1);
+    rb.ga = make_grid(blks[b], 0);
+    ok = alloc_block(rb, r->N, *cfg);
+  }
+  ok = ok && hipMalloc((void**)&r->part, sizeof(double) * 1024 * 8) == hipSuccess;
+  if (ok && cfg->implicit_flag) {
+    std::vector<KGroup> groups;
+    size_t arch_n = 0;
+    build_groups(r->blocks, r->AS, groups, arch_n);
+    std::vector<KBlockPtr> bp(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      const RunBlock& rb = r->blocks[b];
+      bp[b].g = rb.g;
+      bp[b].u = rb.u;
+      bp[b].w = rb.w;
+      bp[b].u1 = rb.u1;
+      bp[b].du = rb.du;
+      bp[b].psf = rb.psf;
+      bp[b].psv = rb.psv;
+    }
+    r->ngroups = (int)groups.size();
+    ok = hipMalloc((void**)&r->arch, sizeof(double) * std::max<size_t>(arch_n, 1)) ==
+             hipSuccess &&
+         hipMalloc((void**)&r->d_bp, sizeof(KBlockPtr) * nblocks) == hipSuccess &&
+         hipMalloc((void**)&r->d_grp, sizeof(KGroup) * std::max(r->ngroups, 1)) ==
+             hipSuccess &&
+         hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(r->d_grp, groups.data(), sizeof(KGroup) * groups.size(),
+                   hipMemcpyHostToDevice) == hipSuccess;
+  }
+  r->tm.s = ctx->stream;
+  if (ok) ok = hipEventCreate(&r->tm.e0) == hipSuccess && hipEventCreate(&r->tm.e1) == hipSuccess;
+  if (!ok) {
+    cg_run_destroy(r);
+    return set_err(ctx, CG_ERR_HIP, "allocation failed");
+  }
+  *out = r;
+  return CG_OK;
+}
+
+int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                  const cg_config* cfg, cg_run** out) {
+  if (!blk) return CG_ERR_ARG;
+  return cg_batch_create(ctx, 1, blk, th, cfg, out);
+}
+
+int cg_run_destroy(cg_run* r) {
+  if (!r) return CG_ERR_ARG;
+  (void)hipStreamSynchronize(r->ctx->stream);
+  drop_graph(r);
+  for (RunBlock& b : r->blocks) free_block(b);
+  if (r->arch) (void)hipFree(r->arch);
+  if (r->d_bp) (void)hipFree(r->d_bp);
+  if (r->d_grp) (void)hipFree(r->d_grp);
+  if (r->part) (void)hipFree(r->part);
+  if (r->tm.e0) (void)hipEventDestroy(r->tm.e0);
+  if (r->tm.e1) (void)hipEventDestroy(r->tm.e1);
+  delete r;
+  return CG_OK;
+}
+
+int cg_run_nblocks(cg_run* r, int* n) {
+  if (!r || !n) return CG_ERR_ARG;
+  *n = (int)r->blocks.size();
+  return CG_OK;
+}
+
+int cg_run_block_set_prim_host(cg_run* r, int b, const double* w_host) {
+  if (!r || !w_host || b < 0 || b >= (int)r->blocks.size()) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  RunBlock& rb = r->blocks[b];
+  size_t nc = cell_doubles(rb.g, r->N);
+  HIPCHK(ctx, hipMemcpyAsync(rb.du, w_host, sizeof(double) * nc, hipMemcpyHostToDevice,
+                             ctx->stream));
+  k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(rb.ga, rb.g, r->N, rb.du,
+                                                               rb.w);
+  if (r->N == 5)
+    prime_block<0>(r, rb);
+  else
+    prime_block<3>(r, rb);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
+int cg_run_block_get_cons_host(cg_run* r, int b, double* u_host) {
+  if (!r || !u_host || b < 0 || b >= (int)r->blocks.size()) return CG_ERR_ARG;
+  return run_download(r, r->blocks[b], r->blocks[b].u, u_host);
+}
+
+int cg_run_block_get_prim_host(cg_run* r, int b, double* w_host) {
+  if (!r || !w_host || b < 0 || b >= (int)r->blocks.size()) return CG_ERR_ARG;
+  return run_download(r, r->blocks[b], r->blocks[b].w, w_host);
+}
+
+int cg_run_set_prim_host(cg_run* r, const double* w_host) {
+  return cg_run_block_set_prim_host(r, 0, w_host);
+}
+
+int cg_run_get_cons_host(cg_run* r, double* u_host) {
+  return cg_run_block_get_cons_host(r, 0, u_host);
+}
+
+int cg_run_get_prim_host(cg_run* r, double* w_host) {
+  return cg_run_block_get_prim_host(r, 0, w_host);
+}
+
+int cg_run_device_state(cg_run* r, double** u, double** w) {
+  if (!r) return CG_ERR_ARG;
+  if (u) *u = r->blocks[0].u;
+  if (w) *w = r->blocks[0].w;
+  return CG_OK;
+}
+
+int cg_run_set_timing(cg_run* r, int on) {
+  if (!r) return CG_ERR_ARG;
+  r->tm.on = on != 0;
+  for (int q = 0; q < KID_COUNT; ++q) {
+    r->tm.ms[q] = 0.0;
+    r->tm.cnt[q] = 0;
+  }
+  return CG_OK;
+}
+
+int cg_run_kernel_stats(cg_run* r, int kernel_id, double* total_ms,
+                        long long* launches, const char** name) {
+  if (!r || kernel_id < 0 || kernel_id >= KID_COUNT) return CG_ERR_ARG;
+  if (total_ms) *total_ms = r->tm.ms[kernel_id];
+  if (launches) *launches = r->tm.cnt[kernel_id];
+  if (name) *name = kKernelNames[kernel_id];
+  return CG_OK;
+}
+
+int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
+  if (!r) return CG_ERR_ARG;
+  if (!(dt > 0.0) || nsteps < 0) return set_err(r->ctx, CG_ERR_ARG, "bad dt/nsteps");
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (use_graph && !r->tm.on) {
+    if (!r->gexec || r->graph_dt != dt) {
+      int e = capture(r, dt);
+      if (e) return e;
+    }
+    for (int n = 0; n < nsteps; ++n) HIPCHK(ctx, hipGraphLaunch(r->gexec, ctx->stream));
+  } else {
+    for (int n = 0; n < nsteps; ++n) {
+      int e = run_one_step(r, dt);
+      if (e) return e;
+    }
+  }
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return check_status(ctx);
+}
+
+int cg_run_block_sums(cg_run* r, int b, double* out_host) {
+  if (!r || !out_host || b < 0 || b >= (int)r->blocks.size()) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  const RunBlock& rb = r->blocks[b];
+  const int nb = 1024;
+  if (r->N == 5)
+    k_sums<0><<<nb, 256, 0, ctx->stream>>>(rb.g, rb.u, r->part);
+  else
+    k_sums<3><<<nb, 256, 0, ctx->stream>>>(rb.g, rb.u, r->part);
+  HIPCHK(ctx, hipGetLastError());
+  std::vector<double> h((size_t)r->N * nb);
+  HIPCHK(ctx, hipMemcpyAsync(h.data(), r->part, sizeof(double) * h.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int n = 0; n < r->N; ++n) {
+    double s = 0.0;
+    for (int q = 0; q < nb; ++q) s += h[(size_t)n * nb + q];
+    out_host[n] = s;
+  }
+  return CG_OK;
+}
+
+int cg_run_sums(cg_run* r, double* out_host) { return cg_run_block_sums(r, 0, out_host); }
+
+int cg_run_set_mlp(cg_run* r, const cg_mlp* net) {
+  if (!r) return CG_ERR_ARG;
+  if (net && r->cfg.satadj != CG_SATADJ_MLP)
+    return set_err(r->ctx, CG_ERR_ARG, "the run's config does not use the guided adjustment");
+  r->mlp = net;
+  drop_graph(r);
+  return CG_OK;
+}
+
+int cg_run_get_tune(cg_run* r, cg_tune* out) {
+  if (!r || !out) return CG_ERR_ARG;
+  *out = r->tune;
+  return CG_OK;
+}
+
+int cg_run_set_tune(cg_run* r, const cg_tune* t) {
+  if (!r || !t) return CG_ERR_ARG;
+  if ((t->implicit_wpb != 1 && t->implicit_wpb != 2 && t->implicit_wpb != 4) ||
+      t->implicit_grid < 0 ||
+      (t->flux_block != 64 && t->flux_block != 128 && t->flux_block != 256))
+    return set_err(r->ctx, CG_ERR_ARG,
+                   "implicit_wpb in {1,2,4}, implicit_grid >= 0, flux_block in {64,128,256}");
+  if (!(r->cfg.implicit_flag & 8) && t->implicit_wpb != 1)
+    return set_err(r->ctx, CG_ERR_ARG, "3x3 solve runs with implicit_wpb 1");
+  r->tune = *t;
+  drop_graph(r);
+  return CG_OK;
+}
+
+// Times one graph-replayed step for each candidate launch shape, restoring
+// the state after every trial, and keeps the fastest: first the batched
+// column solve (waves per workgroup x grid: one workgroup per group set, or
+// a persistent grid of 1, 2 or 4 workgroups per compute unit), then the
+// workgroup size of the face kernels.
+int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
+  if (!r || !(dt > 0.0)) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  if (r->tm.on) return set_err(ctx, CG_ERR_ARG, "switch timing off before autotune");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // save the state a step reads
+  struct Saved {
+    double *u, *w, *psf, *psv;
+  };
+  std::vector<Saved> sv(r->blocks.size(), Saved{nullptr, nullptr, nullptr, nullptr});
+  auto release = [&]() {
+    for (Saved& s : sv) {
+      double* ps[] = {s.u, s.w, s.psf, s.psv};
+      for (double* p : ps)
+        if (p) (void)hipFree(p);
+    }
+  };
+  auto copy_all = [&](bool save) -> hipError_t {
+    for (size_t b = 0; b < r->blocks.size(); ++b) {
+      RunBlock& rb = r->blocks[b];
+      size_t nc = sizeof(double) * cell_doubles(rb.g, r->N);
+      size_t np = sizeof(double) * psf_doubles(rb.g);
+      double* dev[] = {rb.u, rb.w, rb.psf, rb.psv};
+      double* bak[] = {sv[b].u, sv[b].w, sv[b].psf, sv[b].psv};
+      size_t sz[] = {nc, nc, np, np};
+      for (int q = 0; q < 4; ++q) {
+        hipError_t e = save ? hipMemcpyAsync(bak[q], dev[q], sz[q], hipMemcpyDeviceToDevice,
+                                             ctx->stream)
+                            : hipMemcpyAsync(dev[q], bak[q], sz[q], hipMemcpyDeviceToDevice,
+                                             ctx->stream);
+        if (e != hipSuccess) return e;
+      }
+    }
+    return hipStreamSynchronize(ctx->stream);
+  };
+  for (size_t b = 0; b < r->blocks.size(); ++b) {
+    RunBlock& rb = r->blocks[b];
+    size_t nc = sizeof(double) * cell_doubles(rb.g, r->N);
+    size_t np = sizeof(double) * psf_doubles(rb.g);
+    if (hipMalloc((void**)&sv[b].u, nc) != hipSuccess ||
+        hipMalloc((void**)&sv[b].w, nc) != hipSuccess ||
+        hipMalloc((void**)&sv[b].psf, np) != hipSuccess ||
+        hipMalloc((void**)&sv[b].psv, np) != hipSuccess) {
+      release();
+      return set_err(ctx, CG_ERR_HIP, "autotune: no memory for the state copy");
+    }
+  }
+  if (copy_all(true) != hipSuccess) {
+    release();
+    return set_err(ctx, CG_ERR_HIP, "autotune: state copy failed");
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    release();
+    return set_err(ctx, CG_ERR_HIP, "autotune: events");
+  }
+  int err = CG_OK;
+  auto trial = [&](const cg_tune& t, double* ms) -> int {
+    r->tune = t;
+    int e = capture(r, dt);
+    if (e) return e;
+    if (hipGraphLaunch(r->gexec, ctx->stream) != hipSuccess) return CG_ERR_HIP;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return CG_ERR_HIP;
+    const int reps = 2;
+    for (int q = 0; q < reps; ++q)
+      if (hipGraphLaunch(r->gexec, ctx->stream) != hipSuccess) return CG_ERR_HIP;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess) return CG_ERR_HIP;
+    if (hipEventSynchronize(e1) != hipSuccess) return CG_ERR_HIP;
+    float f = 0.f;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    *ms = f / reps;
+    e = check_status(ctx);
+    if (e) return e;
+    return copy_all(false) == hipSuccess ? CG_OK : CG_ERR_HIP;
+  };
+  cg_tune best = r->tune;
+  double best_ms = 1e300;
+  std::vector<cg_tune> cands;
+  const int wpbs[3] = {1, 2, 4};
+  for (int wi = 0; wi < 3; ++wi) {
+    int wpb = wpbs[wi];
+    if (!(r->cfg.implicit_flag & 8) && wpb != 1) continue;
+    const int grids[4] = {0, r->ncu * 4 / wpb, r->ncu * 2 / wpb, r->ncu / wpb};
+    for (int gi = 0; gi < (r->cfg.implicit_flag ? 4 : 1); ++gi) {
+      cg_tune t = best;
+      t.implicit_wpb = wpb;
+      t.implicit_grid = grids[gi];
+      t.flux_block = 256;
+      cands.push_back(t);
+    }
+  }
+  for (const cg_tune& t : cands) {
+    double ms = 0.0;
+    err = trial(t, &ms);
+    if (err) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  const int fbs[2] = {128, 64};
+  for (int fi = 0; fi < 2 && !err; ++fi) {
+    cg_tune t = best;
+    t.flux_block = fbs[fi];
+    double ms = 0.0;
+    err = trial(t, &ms);
+    if (err) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!err && copy_all(false) != hipSuccess) err = CG_ERR_HIP;
+  release();
+  drop_graph(r);
+  best.step_ms = best_ms;
+  r->tune = best;
+  if (err) return err == CG_ERR_HIP ? set_err(ctx, CG_ERR_HIP, "autotune trial failed") : err;
+  if (out) *out = best;
+  return CG_OK;
+}
+
+}  // extern "C"

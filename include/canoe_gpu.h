@@ -243,10 +243,47 @@ int cg_run_sums(cg_run* run, double* out_host);
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of
  * launches and the kernel name of kernel id 0..CG_NUM_KERNEL_IDS-1 (ids
  * with no launches report 0).  cg_run_set_timing resets the counters. */
-#define CG_NUM_KERNEL_IDS 13
+#define CG_NUM_KERNEL_IDS 14
 int cg_run_set_timing(cg_run* run, int on);
 int cg_run_kernel_stats(cg_run* run, int kernel_id, double* total_ms,
                         long long* launches, const char** name);
+
+/* ---- batches of MeshBlocks ---------------------------------------------- *
+ * A batch is a run over nblocks independent blocks of any sizes (each with
+ * its own boundary flags), sharing thermo and config.  Replaces the loop
+ * over the MeshBlocks of a rank in Mesh::UpdateOneStep / the task list
+ * (src/tasklist/implicit_hydro_tasks.cpp:134-313, one ImplicitSolver per
+ * block): every column of every block is solved in one batched launch per
+ * sweep.  cg_run_create is a batch of one block; the cg_run_* calls without
+ * a block index act on block 0 (cg_run_step on all blocks). */
+int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
+                    const cg_thermo* th, const cg_config* cfg, cg_run** out);
+int cg_run_nblocks(cg_run* run, int* nblocks);
+int cg_run_block_set_prim_host(cg_run* run, int block, const double* w_host);
+int cg_run_block_get_cons_host(cg_run* run, int block, double* u_host);
+int cg_run_block_get_prim_host(cg_run* run, int block, double* w_host);
+int cg_run_block_sums(cg_run* run, int block, double* out_host);
+
+/* Launch shape of a run: waves per workgroup (1, 2, 4) and grid (0 = one
+ * workgroup per set of column groups, else a persistent grid of that many
+ * workgroups) of the batched column solve, workgroup size (64, 128, 256) of
+ * the face and divergence kernels; step_ms is the step time measured by the
+ * autotuner.  No counterpart in the reference (CPU loop nests). */
+typedef struct cg_tune {
+  int implicit_wpb;
+  int implicit_grid;
+  int flux_block;
+  double step_ms;
+} cg_tune;
+/* Network of a run whose config has satadj = CG_SATADJ_MLP (NULL: plain
+ * Newton).  The guided adjustment then runs as its own pass at the end of
+ * the last stage.  No counterpart in the reference. */
+int cg_run_set_mlp(cg_run* run, const cg_mlp* net);
+int cg_run_get_tune(cg_run* run, cg_tune* out);
+int cg_run_set_tune(cg_run* run, const cg_tune* tune);
+/* Times one graph-replayed step of size dt per candidate shape (the state is
+ * restored after every trial) and keeps the fastest. */
+int cg_run_autotune(cg_run* run, double dt, cg_tune* out);
 
 /* Library build string (arch, flags). */
 const char* cg_version(void);

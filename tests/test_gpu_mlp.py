@@ -153,3 +153,49 @@ def test_train_and_guided_adjustment(ctx):
         net.close()
         for d in (dx, dt):
             d.free()
+
+
+def test_run_guided_adjustment_matches_newton(ctx):
+    """A moist run with the guided adjustment (satadj = MLP) reproduces the
+    plain Newton run and the oracle to the Newton tolerance."""
+    from helpers import cfl_dt, perturbed_state, relerr
+    blk, th, cfg, w = perturbed_state(4, (24, 8, 8), vel=1.0, dp=1e-3,
+                                      cloud=True)
+    sk, sj, si = blk.interior()
+    # supersaturate the lower levels so that the adjustment condenses
+    w[1][sk, sj, si][..., :12] *= 1.6
+    w = ca.apply_bc_host(blk, w)
+    dt = cfl_dt(blk, th, cfg, w, 0.6)
+    u0 = orl.prim_to_cons(blk, th, w)
+    ths, rho, ie, q = moist_states(dims=(24, 8, 8), seed=3)
+    dx, dt_, _ = cm.make_pairs(ctx, th, rho, ie, q, 1 << 16, seed=5,
+                               device=True)
+    net = cm.Mlp(ctx, seed=6)
+    try:
+        net.train(dx, dt_, steps=2000, batch=4096)
+        res = {}
+        for mode in (ca.SATADJ_NEWTON, ca.SATADJ_MLP):
+            cfg.satadj = mode
+            run = ca.Run(ctx, blk, th, cfg)
+            try:
+                if mode == ca.SATADJ_MLP:
+                    run.set_mlp(net)
+                run.set_prim(w)
+                run.step(dt, 2)
+                res[mode] = run.cons()
+            finally:
+                run.close()
+        cfg.satadj = ca.SATADJ_NEWTON
+        uo, _ = orl.step(blk, th, cfg, dt, 2, u0, w)
+        a = res[ca.SATADJ_MLP][:, sk, sj, si]
+        b = res[ca.SATADJ_NEWTON][:, sk, sj, si]
+        np.testing.assert_array_equal(b, uo[:, sk, sj, si])
+        e = relerr(a, b)
+        print("guided vs Newton run relerr %.3e" % e)
+        assert e < 1e-11
+        # the adjustment did condense
+        assert np.max(b[2]) > 0
+    finally:
+        net.close()
+        dx.free()
+        dt_.free()
