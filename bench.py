@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the canoe dynamical-core step on one MI355X.
 
-Workload: the 3D dry stratified convection box, 256 x 256 x 128 cells
-(x1 = vertical, 128 levels), WENO5 horizontal / PLM vertical reconstruction,
-HLLC fluxes, 5x5 block-tridiagonal implicit x1 solve, bottom heating,
-SSPRK(3,3).  One step = one full RK3 time step of every cell; the metric is
-cell-updates/s (the reference's zone-cycles/s, src/mesh_destroy.cpp:47-53).
+Headline workload (`value`): the 3D dry stratified convection box,
+256 x 256 x 128 cells (x1 = vertical, 128 levels), WENO5 horizontal / PLM
+vertical reconstruction, HLLC fluxes, 5x5 block-tridiagonal implicit x1
+solve, bottom heating, SSPRK(3,3), in the form chosen by accuracy: FP32
+reconstruction + Riemann fluxes with FP64 state and implicit solve when its
+100-step relative L2 error against the plain FP64 form is <= 1e-5, else the
+plain FP64 form.  One step = one full RK3 time step of every cell; the
+metric is cell-updates/s (the reference's zone-cycles/s,
+src/mesh_destroy.cpp:47-53).
 
-Timed region: K steps replayed from the captured hipGraph, state resident in
-HBM, bracketed by stream synchronisation.  A second, event-timed pass of the
-same steps (no graph) gives the per-kernel durations used for the roofline.
-The CPU baseline is the project's FP64 oracle timed on this host on a
-bounded sample of the same workload (same column height, smaller footprint).
+Also reported (not `value`): the moist box (the dry box + vapour, cloud,
+precipitation, saturation adjustment) as plain FP64 with Newton from the
+all-vapour state vs FP32 fluxes with the network-guided Newton start, with
+the network trained on this GPU from pairs of the plain adjustment; and the
+uneven-column batch (32 MeshBlocks, heights 40..256, footprints 32..128)
+with the autotuned launch shape.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Timed region: K steps, state resident in HBM, bracketed by stream
+synchronisation.  A second, event-timed pass of the same steps gives the
+per-kernel durations used for the roofline.  The CPU baseline is the
+project's FP64 oracle timed on this host on a bounded sample of the
+headline workload (same column height, smaller footprint).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
 """
 import argparse
 import json
@@ -27,11 +38,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import canoe_amd as ca  # noqa: E402
+from canoe_amd import mlp as cm  # noqa: E402
 
 METRIC = ("cell-updates/s/GPU, mixed vs plain FP64 at ≤1e-5 rel L2; "
           "% HBM roofline")
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+MLP_FLOP = 2 * (4 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # per sample forward
+USE_GRAPH = True
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
 def cfl_dt(blk, th, cfg, w, cfl=0.9):
@@ -52,58 +71,87 @@ def cfl_dt(blk, th, cfg, w, cfl=0.9):
     return float(cfl / max(float(np.max(inv)), 1e-30))
 
 
-def time_gpu(ctx, blk, th, cfg, w0, dt, steps, warmup):
-    run = ca.Run(ctx, blk, th, cfg)
-    run.set_prim(w0)
-    run.step(dt, warmup, graph=True)
-    t0 = time.perf_counter()
-    run.step(dt, steps, graph=True)       # returns after stream sync
-    t1 = time.perf_counter()
-    ms = 1e3 * (t1 - t0) / steps
-    # event-timed breakdown of the same steps (no graph)
-    run.set_prim(w0)
-    run.set_timing(True)
-    nt = max(2, min(steps, 5))
-    run.step(dt, nt, graph=False)
-    stats = {k: (v[0] / v[1], v[1] / nt) for k, v in run.kernel_stats().items()}
-    run.set_timing(False)
-    run.close()
-    return ms, stats
+def make_run(ctx, blocks, th, cfg, w0s, net=None, tune=None):
+    run = ca.Run(ctx, None, th, cfg, blocks=blocks)
+    if net is not None:
+        run.set_mlp(net)
+    if tune is not None:
+        run.set_tune(tune.implicit_wpb, tune.implicit_grid, tune.flux_block)
+    for b, w in enumerate(w0s):
+        run.set_prim(w, b)
+    return run
 
 
-def accuracy(ctx, blk, th, cfg, w0, dt, nsteps):
-    """Relative L2 error of the FP32-flux form against the FP64 form after
-    nsteps steps from the same state (the FP64 device path reproduces the
-    CPU oracle bit for bit; tests/test_gpu_parity.py).  Per conserved
-    variable group: density (and species), momentum, energy."""
-    out = {}
-    us = {}
-    for prec in (ca.PREC_FP64, ca.PREC_FP32):
-        cfg.precision = prec
-        run = ca.Run(ctx, blk, th, cfg)
-        run.set_prim(w0)
-        run.step(dt, nsteps, graph=True)
-        us[prec] = run.cons()
+def time_form(ctx, blocks, th, cfg, w0s, dt, steps, warmup, net=None,
+              tune=None, autotune=False):
+    """ms per step (graph replay, or direct launches when graphs are off),
+    per-kernel event-timed averages, and the launch shape used."""
+    run = make_run(ctx, blocks, th, cfg, w0s, net, tune)
+    try:
+        if autotune:
+            run.autotune(dt)
+            for b, w in enumerate(w0s):
+                run.set_prim(w, b)
+        t = run.tune()
+        run.step(dt, warmup, graph=USE_GRAPH)
+        t0 = time.perf_counter()
+        run.step(dt, steps, graph=USE_GRAPH)       # returns after stream sync
+        t1 = time.perf_counter()
+        ms = 1e3 * (t1 - t0) / steps
+        for b, w in enumerate(w0s):
+            run.set_prim(w, b)
+        run.set_timing(True)
+        nt = max(2, min(steps, 5))
+        run.step(dt, nt, graph=False)
+        stats = {k: (v[0] / v[1], v[1] / nt)
+                 for k, v in run.kernel_stats().items()}
+        run.set_timing(False)
+        return ms, stats, t.as_dict()
+    finally:
         run.close()
-    sk, sj, si = blk.interior()
-    a = us[ca.PREC_FP32][:, sk, sj, si]
-    b = us[ca.PREC_FP64][:, sk, sj, si]
-    ivx = 1 + blk.nmass
-    groups = {"density": list(range(0, ivx)),
-              "momentum": [ivx, ivx + 1, ivx + 2], "energy": [ivx + 3]}
+
+
+def final_state(ctx, blocks, th, cfg, w0s, dt, nsteps, net=None, tune=None):
+    run = make_run(ctx, blocks, th, cfg, w0s, net, tune)
+    try:
+        run.step(dt, nsteps, graph=USE_GRAPH)
+        return [run.cons(b) for b in range(len(blocks))]
+    finally:
+        run.close()
+
+
+def errors(blocks, ua, ub):
+    """Relative L2 and max errors of ua against ub over the interior cells of
+    all blocks, per conserved group: density, each tracer, momentum,
+    energy."""
+    nm = blocks[0].nmass
+    ivx = 1 + nm
+    groups = {"density": [0]}
+    for n in range(1, ivx):
+        groups["tracer%d" % n] = [n]
+    groups["momentum"] = [ivx, ivx + 1, ivx + 2]
+    groups["energy"] = [ivx + 3]
+    out = {}
     mx = 0.0
     for name, idx in groups.items():
-        num = np.sqrt(np.sum((a[idx] - b[idx]) ** 2))
-        den = np.sqrt(np.sum(b[idx] ** 2))
-        e = float(num / den) if den > 0 else float(num)
+        num = den = 0.0
+        dmax = bmax = 0.0
+        for blk, a, b in zip(blocks, ua, ub):
+            sk, sj, si = blk.interior()
+            x = a[idx][:, sk, sj, si]
+            y = b[idx][:, sk, sj, si]
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(y ** 2))
+            dmax = max(dmax, float(np.max(np.abs(x - y))))
+            bmax = max(bmax, float(np.max(np.abs(y))))
+        e = float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
         if not np.isfinite(e):
             e = float("inf")
-        m = float(np.max(np.abs(a[idx] - b[idx])) / max(np.max(np.abs(b[idx])),
-                                                         1e-300))
+        m = dmax / bmax if bmax > 0 else dmax
         out[name] = {"rel_l2": e, "rel_max": m}
-        mx = e if not e <= mx else mx
+        if name in ("density", "momentum", "energy"):
+            mx = e if not e <= mx else mx
     out["max_rel_l2"] = mx
-    out["steps"] = nsteps
     return out
 
 
@@ -127,14 +175,14 @@ OPS = {
                         "k_flux_h<x3>", "k_divergence", "k_flux32",
                         "k_mirror32"],
     "implicit_solve": ["k_implicit_fwd", "k_implicit_bwd"],
-    "update": ["k_update", "k_bc_x1", "k_bc_x2", "k_bc_x3"],
+    "update": ["k_update", "k_bc_x1", "k_bc_x2", "k_bc_x3",
+               "k_satadj_run_mlp"],
 }
 
 
 def roofline(blk, cfg, stats):
     """Per-operation achieved HBM bandwidth from the live per-kernel
-    averages; the dominant operation (largest time per step) is returned
-    first."""
+    averages; the dominant operation is the largest time per step."""
     alg = algorithmic(blk, cfg)
     ops = {}
     for op, names in OPS.items():
@@ -160,7 +208,7 @@ def traffic_from_profiles(op):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE, calibrated against a copy kernel), if a summary is
     committed under profiles/."""
-    p = os.path.join(ROOT, "profiles", "traffic_box256_fp64.json")
+    p = os.path.join(ROOT, "profiles", "traffic_box256.json")
     if not os.path.exists(p):
         return None
     try:
@@ -190,8 +238,224 @@ def cpu_baseline(seconds, threads):
             break
     return {"value": round(cells * n / el, 1), "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
-            "sample": "%d steps of 128x32x32 cells (config-2 physics, %d "
+            "sample": "%d steps of 128x32x32 cells (dry-box physics, %d "
                       "columns of 128 levels) in %.1f s" % (n, 32 * 32, el)}
+
+
+# ---------------------------------------------------------------------------
+def bench_box(ctx, nx, steps, warmup, acc_steps):
+    blk, th, cfg = ca.pgen_setup(2, *nx)
+    w0 = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0)
+    cells = blk.ncells()
+    log("dry box %dx%dx%d, dt=%.4f s" % (blk.nx1, blk.nx2, blk.nx3, dt))
+    forms = {}
+    for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
+        cfg.precision = prec
+        ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
+                                    warmup)
+        dom, ops = roofline(blk, cfg, stats)
+        forms[name] = {"ms_per_step": round(ms, 4),
+                       "cell_updates_per_s": cells / (ms * 1e-3),
+                       "dominant": dom, "ops": ops,
+                       "kernels_avg_ms": {k: round(v[0], 4)
+                                          for k, v in stats.items()}}
+        log("%s: %.3f ms/step, %.3e cell-updates/s"
+            % (name, ms, cells / (ms * 1e-3)))
+    us = {}
+    for prec in (ca.PREC_FP64, ca.PREC_FP32):
+        cfg.precision = prec
+        us[prec] = final_state(ctx, [blk], th, cfg, [w0], dt, acc_steps)
+    acc = errors([blk], us[ca.PREC_FP32], us[ca.PREC_FP64])
+    acc["steps"] = acc_steps
+    forms["mixed"]["error_vs_fp64"] = acc
+    forms["mixed"]["speedup_vs_plain"] = round(
+        forms["plain_fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
+    log("mixed vs FP64 after %d steps: max rel L2 %.3e"
+        % (acc_steps, acc["max_rel_l2"]))
+    return blk, cfg, dt, cells, forms, acc
+
+
+def moist_states(blk, th, w):
+    """(rho, ie, q) of the interior cells of a moist primitive state: ie is
+    the sensible internal energy density p / (gamma(q) - 1) = rho c_v T."""
+    sk, sj, si = blk.interior()
+    ivx = 1 + blk.nmass
+    rho = np.ascontiguousarray(w[0][sk, sj, si].ravel())
+    q = np.ascontiguousarray(np.stack([w[n][sk, sj, si].ravel()
+                                       for n in (1, 2, 3)]))
+    p = w[ivx + 3][sk, sj, si].ravel()
+    cvd = th.rd / (th.gammad - 1)
+    rv = th.rd * (1 + th.inv_mu_ratio_m1[0])
+    qd = 1 - q.sum(0)
+    R = th.rd * qd + rv * q[0]
+    cv = cvd * qd + sum(cvd * (1 + th.cv_ratio_m1[n]) * q[n] for n in range(3))
+    return rho, np.ascontiguousarray(cv * p / R), q
+
+
+def supersaturated(th, rho, ie, q, n, seed):
+    """n states drawn from the given cells, water scaled by 1.1..2.2 into
+    vapour and temperature shifted by -2..2 K (most become saturated)."""
+    rng = np.random.default_rng(seed)
+    b = rng.integers(0, rho.size, n)
+    cvd = th.rd / (th.gammad - 1)
+    cvv = cvd * (1 + th.cv_ratio_m1[0])
+    cvp = cvd * (1 + th.cv_ratio_m1[2])
+    qb = q[:, b].copy()
+    qd = 1 - qb.sum(0)
+    cvb = cvd * qd + cvv * qb[0] + cvd * (1 + th.cv_ratio_m1[1]) * qb[1] \
+        + cvp * qb[2]
+    T = ie[b] / (rho[b] * cvb) + rng.uniform(-2, 2, n)
+    qn = qb.copy()
+    qn[0] = (qb[0] + qb[1]) * rng.uniform(1.1, 2.2, n)
+    qn[1] = 0.0
+    cvn = cvd * (1 - qn.sum(0)) + cvv * qn[0] + cvp * qn[2]
+    r = rho[b].copy()
+    return r, r * cvn * T, np.ascontiguousarray(qn)
+
+
+def bench_moist(ctx, nx, steps, warmup, acc_steps, npairs, train_steps):
+    blk, th, cfg = ca.pgen_setup(4, *nx)
+    w0 = ca.pgen_init(4, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0)
+    cells = blk.ncells()
+    log("moist box %dx%dx%d, dt=%.4f s" % (blk.nx1, blk.nx2, blk.nx3, dt))
+    out = {"workload": "moist box %dx%dx%d: dry box + vapour/cloud/precip, "
+                       "saturation adjustment (Newton to 1e-10 K)"
+                       % (blk.nx2, blk.nx3, blk.nx1), "cells": cells, "dt": dt}
+    # training pairs from the state of a moist run (the plain adjustment
+    # labels them); the network is trained on this GPU
+    cfg.precision = ca.PREC_FP64
+    cfg.satadj = ca.SATADJ_NEWTON
+    run = make_run(ctx, [blk], th, cfg, [w0])
+    run.step(dt, 10, graph=USE_GRAPH)
+    wrun = run.prim()
+    run.close()
+    rho, ie, q = moist_states(blk, th, wrun)
+    t0 = time.perf_counter()
+    dx, dtt, _ = cm.make_pairs(ctx, th, rho, ie, q, npairs, seed=5,
+                               device=True)
+    t1 = time.perf_counter()
+    net = cm.Mlp(ctx, seed=7)
+    losses = net.train(dx, dtt, steps=train_steps, batch=8192, lr=3e-3,
+                       lr_final=1e-5, seed=9)
+    t2 = time.perf_counter()
+    xh, th_, _ = cm.make_pairs(ctx, th, rho, ie, q, 1 << 18, seed=77)
+    yh = net.forward(xh)
+    sat = th_[:, 2] > 0
+    rms_k = float(np.sqrt(np.mean((yh[sat, 0] - th_[sat, 0]) ** 2)) * 10)
+    # inference throughput on the matrix cores: forward of 8M samples
+    nbig = 1 << 23
+    xb = ctx.array(np.tile(xh, (nbig // xh.shape[0], 1)), dtype=np.float32)
+    yb = ctx.array(shape=(nbig, 3), dtype=np.float32)
+    L = cm._lib()
+    ctx._check(L.cg_mlp_forward(net.h, nbig, xb.ptr, yb.ptr), "forward")
+    reps = 5
+    t3 = time.perf_counter()
+    for _ in range(reps):
+        ctx._check(L.cg_mlp_forward(net.h, nbig, xb.ptr, yb.ptr), "forward")
+    t4 = time.perf_counter()
+    xb.free()
+    yb.free()
+    tf = MLP_FLOP * nbig * reps / (t4 - t3) / 1e12
+    # guided vs plain adjustment on held-out supersaturated states
+    rr, en, qn = supersaturated(th, rho, ie, q, 1 << 20, seed=3)
+    t5 = time.perf_counter()
+    ie_p, y_p, it_p = cm.saturation_adjust(ctx, th, rr, en, qn)
+    t6 = time.perf_counter()
+    ie_g, y_g, it_g = cm.saturation_adjust(ctx, th, rr, en, qn, net)
+    t7 = time.perf_counter()
+    s = it_p > 0
+    out["mlp"] = {
+        "net": "4-64-64-64-3 tanh-GELU, bf16 MFMA (fp32 accumulate)",
+        "pairs": npairs, "pair_gen_s": round(t1 - t0, 3),
+        "train_steps": train_steps, "batch": 8192,
+        "train_s": round(t2 - t1, 3),
+        "loss_first": float(np.mean(losses[:20])),
+        "loss_last": float(np.mean(losses[-100:])),
+        "heldout_T_rms_K": round(rms_k, 4),
+        "newton_iters_plain": round(float(it_p[s].mean()), 3),
+        "newton_iters_guided": round(float(it_g[s].mean()), 3),
+        "saturated_cells": int(s.sum()),
+        "adjust_call_s_plain": round(t6 - t5, 4),
+        "adjust_call_s_guided": round(t7 - t6, 4),
+        "max_rel_diff_ie": float(np.max(np.abs(ie_g - ie_p) / np.abs(ie_p))),
+        "forward_tflops": round(tf, 3),
+        "forward_mfma_frac": round(tf / BF16_PEAK_TFLOPS, 5),
+    }
+    log("mlp: %s" % out["mlp"])
+    forms = {}
+    for name, prec, mode, n in (("plain_fp64_newton", ca.PREC_FP64,
+                                 ca.SATADJ_NEWTON, None),
+                                ("mixed_guided", ca.PREC_FP32, ca.SATADJ_MLP,
+                                 net)):
+        cfg.precision = prec
+        cfg.satadj = mode
+        ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
+                                    warmup, net=n)
+        dom, ops = roofline(blk, cfg, stats)
+        forms[name] = {"ms_per_step": round(ms, 4),
+                       "cell_updates_per_s": cells / (ms * 1e-3),
+                       "dominant": dom, "ops": ops,
+                       "kernels_avg_ms": {k: round(v[0], 4)
+                                          for k, v in stats.items()}}
+        log("moist %s: %.3f ms/step" % (name, ms))
+    us = {}
+    for name, prec, mode, n in (("plain", ca.PREC_FP64, ca.SATADJ_NEWTON,
+                                 None),
+                                ("chosen", ca.PREC_FP32, ca.SATADJ_MLP, net)):
+        cfg.precision = prec
+        cfg.satadj = mode
+        us[name] = final_state(ctx, [blk], th, cfg, [w0], dt, acc_steps, net=n)
+    acc = errors([blk], us["chosen"], us["plain"])
+    acc["steps"] = acc_steps
+    forms["mixed_guided"]["error_vs_fp64"] = acc
+    forms["mixed_guided"]["speedup_vs_plain"] = round(
+        forms["plain_fp64_newton"]["ms_per_step"]
+        / forms["mixed_guided"]["ms_per_step"], 3)
+    out["forms"] = forms
+    net.close()
+    dx.free()
+    dtt.free()
+    cfg.satadj = ca.SATADJ_NEWTON
+    return out
+
+
+def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
+    bl = ca.batch_blocks(seed, nblocks)
+    blocks = [b[0] for b in bl]
+    th, cfg = bl[0][1], bl[0][2]
+    w0s = [b[3] for b in bl]
+    dt = min(cfl_dt(b[0], b[1], b[2], b[3]) for b in bl)
+    cells = sum(b.ncells() for b in blocks)
+    cols = sum(b.nx2 * b.nx3 for b in blocks)
+    log("batch of %d blocks, %d cells, %d columns, dt=%.4f"
+        % (nblocks, cells, cols, dt))
+    out = {"workload": "uneven-column batch: %d blocks, heights %s, "
+                       "footprints %s, seed %d"
+                       % (nblocks, sorted(set(b.nx1 for b in blocks)),
+                          sorted(set(b.nx2 for b in blocks)), seed),
+           "cells": cells, "columns": cols, "dt": dt}
+    forms = {}
+    for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
+        cfg.precision = prec
+        ms, stats, tune = time_form(ctx, blocks, th, cfg, w0s, dt, steps,
+                                    warmup, autotune=True)
+        forms[name] = {"ms_per_step": round(ms, 4),
+                       "cell_updates_per_s": cells / (ms * 1e-3),
+                       "tune": tune,
+                       "kernels_avg_ms": {k: round(v[0], 4)
+                                          for k, v in stats.items()}}
+        log("batch %s: %.3f ms/step, tune %s" % (name, ms, tune))
+    us = {}
+    for prec in (ca.PREC_FP64, ca.PREC_FP32):
+        cfg.precision = prec
+        us[prec] = final_state(ctx, blocks, th, cfg, w0s, dt, acc_steps)
+    acc = errors(blocks, us[ca.PREC_FP32], us[ca.PREC_FP64])
+    acc["steps"] = acc_steps
+    forms["mixed"]["error_vs_fp64"] = acc
+    out["forms"] = forms
+    return out
 
 
 def main():
@@ -201,43 +465,32 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline workload only (no moist box, no batch)")
     ap.add_argument("--acc-steps", type=int, default=100)
+    ap.add_argument("--pairs", type=int, default=10_000_000)
+    ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--nx", type=int, nargs=3, default=[128, 256, 256],
                     help="nx1 (vertical) nx2 nx3")
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(columns would shard by horizontal MeshBlocks; see "
               "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    global USE_GRAPH
+    USE_GRAPH = not args.no_graph
 
-    blk, th, cfg = ca.pgen_setup(2, *args.nx)
-    w0 = ca.pgen_init(2, 1, blk, th, cfg)
-    dt = cfl_dt(blk, th, cfg, w0)
-    cells = blk.ncells()
-    print("# config-2 box %dx%dx%d, dt=%.4f s" % (blk.nx1, blk.nx2, blk.nx3,
-                                                  dt), file=sys.stderr)
     ctx = ca.Context(0)
-    forms = {}
-    for name, prec in (("plain_fp64", ca.PREC_FP64),
-                       ("mixed", ca.PREC_FP32)):
-        cfg.precision = prec
-        ms, stats = time_gpu(ctx, blk, th, cfg, w0, dt, args.steps,
-                             args.warmup)
-        dom, ops = roofline(blk, cfg, stats)
-        forms[name] = {"ms_per_step": round(ms, 4),
-                       "cell_updates_per_s": cells / (ms * 1e-3),
-                       "dominant": dom, "ops": ops,
-                       "kernels_avg_ms": {k: round(v[0], 4)
-                                          for k, v in stats.items()}}
-        print("# %s: %.3f ms/step, %.3e cell-updates/s" %
-              (name, ms, cells / (ms * 1e-3)), file=sys.stderr)
-    acc = accuracy(ctx, blk, th, cfg, w0, dt, args.acc_steps)
-    forms["mixed"]["error_vs_fp64"] = acc
-    forms["mixed"]["speedup_vs_plain"] = round(
-        forms["plain_fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
-    print("# mixed vs FP64 after %d steps: %s" % (args.acc_steps, acc),
-          file=sys.stderr)
+    blk, cfg, dt, cells, forms, acc = bench_box(ctx, args.nx, args.steps,
+                                                args.warmup, args.acc_steps)
+    moist = batch = None
+    if not args.quick:
+        moist = bench_moist(ctx, args.nx, args.steps, args.warmup,
+                            args.acc_steps, args.pairs, args.train_steps)
+        batch = bench_batch(ctx, 2024, 32, max(3, args.steps // 4),
+                            args.warmup, max(10, args.acc_steps // 5))
     ctx.close()
 
     ok = acc["max_rel_l2"] <= 1e-5
@@ -274,12 +527,14 @@ def main():
                                "WENO5-h/PLM-v, HLLC, 5x5 implicit x1, "
                                "SSPRK(3,3), bottom heating"
                                % (blk.nx2, blk.nx3, blk.nx1),
-                   "cells": cells, "dt": dt,
+                   "cells": cells, "dt": dt, "graph": USE_GRAPH,
                    "form": "plain FP64" if chosen is forms["plain_fp64"]
                    else "FP32 fluxes + FP64 state"},
         "roofline": rf,
         "cpu_baseline": cpu,
         "forms": forms,
+        "moist_box": moist,
+        "batch": batch,
     }
     print(json.dumps(out))
 

@@ -22,6 +22,42 @@
 
 namespace cg {
 
+// Single-precision division and square root of the mixed form on the
+// hardware reciprocal / square-root instructions (1 ulp) instead of the
+// correctly rounded sequences; the FP32 faces are an approximation to the
+// FP64 ones anyway, checked against them by relative error.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * frcp(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Upper-face WENO5 value (Jiang & Shu 1996, eps = 1e-6) of the cell c of the
+// window (a, b, c, d, e); the lower face is weno5_up_f(e, d, c, b, a).
+__device__ __forceinline__ float weno5_up_f(float a, float b, float c, float d,
+                                            float e) {
+  const float eps = 1.0e-6f;
+  float s0 = a - 2.f * b + c, t0 = a - 4.f * b + 3.f * c;
+  float s1 = b - 2.f * c + d, t1 = b - d;
+  float s2 = c - 2.f * d + e, t2 = 3.f * c - 4.f * d + e;
+  float b0 = (13.f / 12.f) * (s0 * s0) + 0.25f * (t0 * t0);
+  float b1 = (13.f / 12.f) * (s1 * s1) + 0.25f * (t1 * t1);
+  float b2 = (13.f / 12.f) * (s2 * s2) + 0.25f * (t2 * t2);
+  float e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  float a0 = 0.1f * frcp(e0 * e0), a1 = 0.6f * frcp(e1 * e1),
+        a2 = 0.3f * frcp(e2 * e2);
+  float p0 = (2.f * a - 7.f * b + 11.f * c) * (1.f / 6.f);
+  float p1 = (-b + 5.f * c + 2.f * d) * (1.f / 6.f);
+  float p2 = (2.f * c + 5.f * d - e) * (1.f / 6.f);
+  return (a0 * p0 + a1 * p1 + a2 * p2) * frcp(a0 + a1 + a2);
+}
+
+// PLM (van Leer) upper face of cell b of (a, b, c); lower face plm_up_f(c, b, a).
+__device__ __forceinline__ float plm_up_f(float a, float b, float c) {
+  float dql = b - a, dqr = c - b;
+  float dq2 = dql * dqr;
+  float dqm = dq2 > 0.f ? 2.f * dq2 * frcp(dql + dqr) : 0.f;
+  return b + 0.5f * dqm;
+}
+
 // get_gamma in float (src/interface/eos.hpp:47-67)
 template <int NM>
 __device__ inline float gamma_f(const KThermo& th, const float* u) {
@@ -38,7 +74,7 @@ __device__ inline float gamma_f(const KThermo& th, const float* u) {
       fsig += u[n] * (float)th.cv_ratio_m1[n - 1];
       feps -= u[n];
     }
-  return 1.f + ((float)th.gammad - 1.f) * feps / fsig;
+  return 1.f + ((float)th.gammad - 1.f) * fdiv(feps, fsig);
 }
 
 // LMARS (src/snap/riemann/lmars.cpp:19-85) in perturbation form: wl[IPR],
@@ -51,21 +87,21 @@ __device__ inline void lmars_p(const KThermo& th, const float* wli,
   constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
   constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
   float pl = wli[IPR] + pb, pr = wri[IPR] + pb;
-  float kappal = 1.f / (gamma_f<NM>(th, wli) - 1.f);
-  float kappar = 1.f / (gamma_f<NM>(th, wri) - 1.f);
-  float hl = pl / wli[IDN] * (kappal + 1.f) +
+  float kappal = frcp(gamma_f<NM>(th, wli) - 1.f);
+  float kappar = frcp(gamma_f<NM>(th, wri) - 1.f);
+  float hl = fdiv(pl, wli[IDN]) * (kappal + 1.f) +
              0.5f * (wli[ivx] * wli[ivx] + wli[ivy] * wli[ivy] +
                      wli[ivz] * wli[ivz]);
-  float hr = pr / wri[IDN] * (kappar + 1.f) +
+  float hr = fdiv(pr, wri[IDN]) * (kappar + 1.f) +
              0.5f * (wri[ivx] * wri[ivx] + wri[ivy] * wri[ivy] +
                      wri[ivz] * wri[ivz]);
   float rhobar = 0.5f * (wli[IDN] + wri[IDN]);
-  float cbar = sqrtf(0.5f * (1.f + (1.f / kappar + 1.f / kappal) * 0.5f) *
-                     (pl + pr) / rhobar);
+  float cbar = fsqrt(fdiv(0.5f * (1.f + (frcp(kappar) + frcp(kappal)) * 0.5f) *
+                              (pl + pr), rhobar));
   float rc = rhobar * cbar;
   float pbar = 0.5f * (wli[IPR] + wri[IPR]) + 0.5f * rc * (wli[ivx] - wri[ivx]);
   float ubar = 0.5f * (wli[ivx] + wri[ivx]) +
-               0.5f / rc * (wli[IPR] - wri[IPR]);
+               0.5f * frcp(rc) * (wli[IPR] - wri[IPR]);
   float rdl = 1.f, rdr = 1.f;
 #pragma unroll
   for (int n = 1; n < IVX; ++n) {
@@ -116,9 +152,9 @@ __device__ inline void hllc_p(const KThermo& th, const float* wl,
   }
   float pl = wli[IPR] + pb, pr = wri[IPR] + pb;
   float gammal = gamma_f<NM>(th, wli), gammar = gamma_f<NM>(th, wri);
-  float kappal = 1.f / (gammal - 1.f), kappar = 1.f / (gammar - 1.f);
-  float cl = sqrtf(gammal * pl / wli[IDN]);
-  float cr = sqrtf(gammar * pr / wri[IDN]);
+  float kappal = frcp(gammal - 1.f), kappar = frcp(gammar - 1.f);
+  float cl = fsqrt(fdiv(gammal * pl, wli[IDN]));
+  float cr = fsqrt(fdiv(gammar * pr, wri[IDN]));
   float KE_l = 0.5f * wli[IDN] *
                (wli[IVX] * wli[IVX] + wli[IVY] * wli[IVY] + wli[IVZ] * wli[IVZ]);
   float KE_r = 0.5f * wri[IDN] *
@@ -129,11 +165,11 @@ __device__ inline void hllc_p(const KThermo& th, const float* wl,
   float ca = 0.5f * (cl + cr);
   float pmid = 0.5f * (pl + pr + (wli[IVX] - wri[IVX]) * rhoa * ca);
   float ql = (pmid <= pl) ? 1.f
-                          : sqrtf(1.f + (gammal + 1.f) / (2.f * gammal) *
-                                            (pmid / pl - 1.f));
+                          : fsqrt(1.f + fdiv(gammal + 1.f, 2.f * gammal) *
+                                            (fdiv(pmid, pl) - 1.f));
   float qr = (pmid <= pr) ? 1.f
-                          : sqrtf(1.f + (gammar + 1.f) / (2.f * gammar) *
-                                            (pmid / pr - 1.f));
+                          : fsqrt(1.f + fdiv(gammar + 1.f, 2.f * gammar) *
+                                            (fdiv(pmid, pr) - 1.f));
   float al = wli[IVX] - cl * ql;
   float ar = wri[IVX] + cr * qr;
   float bp = ar > 0.f ? ar : TINY;
@@ -144,8 +180,9 @@ __device__ inline void hllc_p(const KThermo& th, const float* wl,
   float tr = wri[IPR] + vxr * wri[IDN] * wri[IVX];
   float ml = wli[IDN] * vxl;
   float mr = -(wri[IDN] * vxr);
-  float am = (tl - tr) / (ml + mr);
-  float cp = (ml * tr + mr * tl) / (ml + mr);  // contact pressure - pb
+  const float imm = frcp(ml + mr);
+  float am = (tl - tr) * imm;
+  float cp = (ml * tr + mr * tl) * imm;  // contact pressure - pb
   cp = cp > -pb ? cp : -pb;
   vxl = wli[IVX] - bm;
   vxr = wri[IVX] - bp;
@@ -172,13 +209,15 @@ __device__ inline void hllc_p(const KThermo& th, const float* wl,
   fr[IEN] = er * vxr + pr * wri[IVX];
   float sl, sr, sm;
   if (am >= 0.f) {
-    sl = am / (am - bm);
+    const float r = frcp(am - bm);
+    sl = am * r;
     sr = 0.f;
-    sm = -bm / (am - bm);
+    sm = -bm * r;
   } else {
+    const float r = frcp(bp - am);
     sl = 0.f;
-    sr = -am / (bp - am);
-    sm = bp / (bp - am);
+    sr = -am * r;
+    sm = bp * r;
   }
 #pragma unroll
   for (int n = 0; n < IVX; ++n) flxi[n] = sl * fl[n] + sr * fr[n];
@@ -203,19 +242,16 @@ __device__ inline void riemann_p(const KThermo& th, int solver, const float* wl,
     lmars_p<NM, DIR>(th, wl, wr, pb, flx);
 }
 
+// Face states from the 6-cell window q[0..5] around the face between q[2]
+// and q[3]: the upper face of q[2] and the lower face of q[3].
 template <int ORDER>
-__device__ inline void recon_pair_f(const float* q, float* l, float* r) {
-  float qm, qp;
+__device__ __forceinline__ void recon_pair_f(const float* q, float* l, float* r) {
   if (ORDER == 5) {
-    weno5_cell<float>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
-    *l = qp;
-    weno5_cell<float>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
-    *r = qm;
+    *l = weno5_up_f(q[0], q[1], q[2], q[3], q[4]);
+    *r = weno5_up_f(q[5], q[4], q[3], q[2], q[1]);
   } else {
-    plm_cell<float>(q[1], q[2], q[3], &qm, &qp);
-    *l = qp;
-    plm_cell<float>(q[2], q[3], q[4], &qm, &qp);
-    *r = qm;
+    *l = plm_up_f(q[1], q[2], q[3]);
+    *r = plm_up_f(q[4], q[3], q[2]);
   }
 }
 
@@ -243,27 +279,27 @@ __global__ void __launch_bounds__(256)
     double wc[N];
 #pragma unroll
     for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i - 3 + m)];
-    if (decomp) {
-      rdv[m] = decomp_r<NM>(th, wc);
-      wc[0] = wc[IPR] / (wc[0] * rdv[m]);
-      wc[IPR] = wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i];
-    }
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
+    if (decomp) {
+      // T in float; the perturbation p - psv in FP64 before rounding
+      rdv[m] = decomp_r<NM>(th, wc);
+      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)rdv[m]);
+      q[IPR][m] = (float)(wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i]);
+    }
   }
   float wl[N], wr[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
   float pb = 0.f;
   if (decomp) {
-    double ps = psf[pc + i * g.ps_i];
-    pb = (float)ps;
-    float gam = (float)sqrt(rdv[3] * rdv[2]);
+    pb = (float)psf[pc + i * g.ps_i];
+    float gam = fsqrt((float)(rdv[3] * rdv[2]));
     // RestoreFromTemperature (:155-164): p = psf + p', clamped at psf
     if (wr[IPR] + pb < 0.f) wr[IPR] = 0.f;
     if (wl[IPR] + pb < 0.f) wl[IPR] = 0.f;
-    wr[0] = (float)((ps + (double)wr[IPR]) / ((double)wr[0] * gam));
-    wl[0] = (float)((ps + (double)wl[IPR]) / ((double)wl[0] * gam));
+    wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
+    wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
   }
   float fx[N];
   riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
@@ -327,6 +363,7 @@ __global__ void __launch_bounds__(256)
   k += g.ks;
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
   const double vol = g.dx1 * g.dx2 * g.dx3;
+  const double wv = src.wght / vol;
   double d[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
@@ -342,11 +379,109 @@ __global__ void __launch_bounds__(256)
     if (g.f3)
       dflx += A3 * (double)f3[g.f[2].at(n, k + 1, j, i)] -
               A3 * (double)f3[g.f[2].at(n, k, j, i)];
-    d[n] = 0.0 - src.wght * dflx / vol;
+    d[n] = 0.0 - wv * dflx;
   }
   add_sources_cell<NM>(g, th, src, w, k, j, i, d);
 #pragma unroll
   for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
+}
+
+// x1 faces marched up a column segment, fused with the flux divergence of
+// every cell of the segment: one lane per (column, segment), lanes along the
+// unit-stride index.  The face states come from a sliding 6-cell window
+// (the decomposition of k_flux32_x1: T and p - psv with FP64 differences);
+// each cell then takes its two x1 face fluxes from the march, its x2 / x3
+// face fluxes from f2 / f3, the hydrostatic term A1 (psf(i+1) - psf(i)) in
+// FP64, and the sources, and writes du once.
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_mix_x1_div(KGrid g, KThermo th, KSource src, int rsolver, int decomp,
+                 int seg, const double* __restrict__ w,
+                 const double* __restrict__ psf, const double* __restrict__ psv,
+                 const float* __restrict__ f2, const float* __restrict__ f3,
+                 double* __restrict__ du) {
+  constexpr int N = 5 + NM, IPR = 4 + NM, IVX = 1 + NM;
+  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const int nseg = (g.nx1 + seg - 1) / seg;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * nseg) return;
+  const long long col = t % ncol;
+  const int sg = (int)(t / ncol);
+  const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
+  const int c0 = g.is + sg * seg;
+  const int c1 = min(c0 + seg, g.ie + 1);  // cells c0 .. c1-1
+  const long long pc = g.pcol(k, j) * g.ps_c;
+  const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
+  const double wv = src.wght / (g.dx1 * g.dx2 * g.dx3);
+  float q[N][6];
+  float rdv[6];
+  auto load = [&](int m, int i) {
+    double wc[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i)];
+#pragma unroll
+    for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
+    if (decomp) {
+      double r = decomp_r<NM>(th, wc);
+      rdv[m] = (float)r;
+      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)r);
+      q[IPR][m] = (float)(wc[IPR] - psv[pc + i * g.ps_i]);
+    }
+  };
+  // face f (between cells f-1 and f) uses window cells f-3 .. f+2
+  auto face = [&](int f, float* fx) {
+    float wl[N], wr[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
+    float pb = 0.f;
+    if (decomp) {
+      pb = (float)psf[pc + f * g.ps_i];
+      float gam = fsqrt(rdv[3] * rdv[2]);
+      if (wr[IPR] + pb < 0.f) wr[IPR] = 0.f;
+      if (wl[IPR] + pb < 0.f) wl[IPR] = 0.f;
+      wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
+      wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
+    }
+    riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
+  };
+#pragma unroll
+  for (int m = LO; m <= HI; ++m) load(m, c0 - 3 + m);
+  float fm[N];
+  face(c0, fm);
+  double pfm = psf[pc + c0 * g.ps_i];
+  for (int i = c0; i < c1; ++i) {
+    // slide the window up one cell: cells i-2 .. i+3 for face i+1
+#pragma unroll
+    for (int m = LO; m < HI; ++m) {
+#pragma unroll
+      for (int n = 0; n < N; ++n) q[n][m] = q[n][m + 1];
+      rdv[m] = rdv[m + 1];
+    }
+    load(HI, i + 1 - 3 + HI);
+    float fp[N];
+    face(i + 1, fp);
+    const double pfp = psf[pc + (i + 1) * g.ps_i];
+    double d[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double dflx = A1 * ((double)fp[n] - (double)fm[n]);
+      if (n == IVX && decomp) dflx += A1 * (pfp - pfm);
+      if (g.f2)
+        dflx += A2 * ((double)f2[g.f[1].at(n, k, j + 1, i)] -
+                      (double)f2[g.f[1].at(n, k, j, i)]);
+      if (g.f3)
+        dflx += A3 * ((double)f3[g.f[2].at(n, k + 1, j, i)] -
+                      (double)f3[g.f[2].at(n, k, j, i)]);
+      d[n] = 0.0 - wv * dflx;
+    }
+    add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+#pragma unroll
+    for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
+#pragma unroll
+    for (int n = 0; n < N; ++n) fm[n] = fp[n];
+    pfm = pfp;
+  }
 }
 
 template <int NM>
@@ -364,13 +499,6 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     TIMED(tm, KID_SCAN,
           (k_hydrostatic_scan<NM><<<(unsigned)((ncol + 255) / 256), 256, 0, s>>>(
               g, th, w, psf, psv)));
-  const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + bs - 1) / bs);
-  if (cfg.order_v == 5)
-    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, bs, 0, s>>>(
-                               g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
-  else
-    TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, bs, 0, s>>>(
-                               g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
   if (g.f2) {
     const unsigned b2 =
         (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + bs - 1) / bs);
@@ -391,9 +519,20 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
       TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f3)));
   }
-  TIMED(tm, KID_DIVERGENCE,
-        (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
-            g, th, src, f1, f2, f3, psf, decomp, w, du)));
+  // x1 faces + flux divergence + sources, marched up column segments
+  const int seg = 16;
+  const long long nx1t = ncol * ((g.nx1 + seg - 1) / seg);
+  const unsigned b1 = (unsigned)((nx1t + bs - 1) / bs);
+  if (cfg.order_v == 5)
+    TIMED(tm, KID_FLUX_X1, (k_mix_x1_div<NM, 5><<<b1, bs, 0, s>>>(
+                               g, th, src, cfg.rsolver, decomp, seg, w, psf, psv,
+                               f2, f3, du)));
+  else
+    TIMED(tm, KID_FLUX_X1, (k_mix_x1_div<NM, 2><<<b1, bs, 0, s>>>(
+                               g, th, src, cfg.rsolver, decomp, seg, w, psf, psv,
+                               f2, f3, du)));
+  (void)f1;
+  (void)ncell;
 }
 
 // per-level reference pressure: horizontal mean of p over interior columns
