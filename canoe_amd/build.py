@@ -58,6 +58,13 @@ def build(verbose=True):
     run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH,
                                                 "-fPIC"])
     os.replace(tmp, LIB)
+    # C++ host driver over the C-ABI (the reference's main loop)
+    exe = os.path.join(LIBDIR, "canoe_step")
+    run(["g++", "-std=c++17", "-O2", "-Wall",
+         "-I" + os.path.join(ROOT, "include"),
+         os.path.join(CSRC, "canoe_step.cpp"), "-o", exe + ".tmp",
+         LIB, "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

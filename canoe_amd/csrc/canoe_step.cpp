@@ -1,0 +1,205 @@
+// canoe_step.cpp -- C++ host driver over the C-ABI (include/canoe_gpu.h).
+//
+// The main integration loop of the reference (main.cpp:114-152): cycles of
+// the task-list stages (here one SSPRK(3,3) step of the resident block,
+// replayed from its hipGraph), time += dt, a new time step, cycle
+// diagnostics; at the end the history totals and the zone-cycles report of
+// src/mesh_destroy.cpp:38-57.  The block and its initial state come from
+// the benchmark configurations (1: dry density current, 2: dry stratified
+// box, 4: moist box, 5: one seeded block of the uneven-column batch).
+//
+//   canoe_step <config> [--nx nx1 nx2 nx3] [--nlim N] [--tlim T] [--cfl C]
+//              [--dt-every K] [--fp32] [--no-graph] [--seed S] [--out K]
+//
+// The time step follows the patched Hydro::NewBlockTimeStep
+// (patches/21.new_blockdt.patch:31-45: the implicit x1 direction drops the
+// sound speed) with the signal speeds summed over directions; it is
+// recomputed from the device state every K cycles (--dt-every, default 10).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/canoe_gpu.h"
+
+extern "C" {
+int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
+                  cg_thermo* th, cg_config* cfg);
+int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
+                 const cg_thermo* th, const cg_config* cfg, double* w);
+}
+
+namespace {
+
+struct Dims {
+  int nc1, nc2, nc3, is, js, ks, N;
+  size_t sn;
+  explicit Dims(const cg_block& b) {
+    nc1 = b.nx1 + 2 * CG_NGHOST;
+    nc2 = b.nx2 > 1 ? b.nx2 + 2 * CG_NGHOST : 1;
+    nc3 = b.nx3 > 1 ? b.nx3 + 2 * CG_NGHOST : 1;
+    is = CG_NGHOST;
+    js = b.nx2 > 1 ? CG_NGHOST : 0;
+    ks = b.nx3 > 1 ? CG_NGHOST : 0;
+    N = 5 + b.nmass;
+    sn = (size_t)nc1 * nc2 * nc3;
+  }
+  size_t at(int n, int k, int j, int i) const {
+    return (size_t)n * sn + ((size_t)k * nc2 + j) * nc1 + i;
+  }
+};
+
+// CFL time step of a host primitive array (dry gamma for the sound speed).
+double cfl_dt(const cg_block& b, const cg_thermo& th, const cg_config& cfg,
+              const std::vector<double>& w, double cfl) {
+  Dims d(b);
+  const int ivx = 1 + b.nmass, ipr = ivx + 3;
+  const double dx[3] = {b.dx1, b.dx2, b.dx3};
+  const bool act[3] = {true, b.nx2 > 1, b.nx3 > 1};
+  double mx = 1e-30;
+  for (int k = d.ks; k < d.ks + b.nx3; ++k)
+    for (int j = d.js; j < d.js + b.nx2; ++j)
+      for (int i = d.is; i < d.is + b.nx1; ++i) {
+        double rho = w[d.at(0, k, j, i)];
+        double cs = std::sqrt(th.gammad * w[d.at(ipr, k, j, i)] / rho);
+        double s = 0.0;
+        for (int q = 0; q < 3; ++q) {
+          if (!act[q]) continue;
+          double v = std::fabs(w[d.at(ivx + q, k, j, i)]);
+          s += ((q == 0 && cfg.implicit_flag) ? v : v + cs) / dx[q];
+        }
+        mx = std::fmax(mx, s);
+      }
+  return cfl / mx;
+}
+
+void usage() {
+  std::fprintf(stderr,
+               "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
+               "[--nlim N] [--tlim T] [--cfl C] [--dt-every K] [--fp32] "
+               "[--no-graph] [--seed S] [--out K]\n");
+}
+
+int check(cg_ctx* ctx, int rc, const char* what) {
+  if (rc != CG_OK)
+    std::fprintf(stderr, "canoe_step: %s failed (%d): %s\n", what, rc,
+                 ctx ? cg_last_error(ctx) : "");
+  return rc;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    usage();
+    return 2;
+  }
+  int config = std::atoi(argv[1]);
+  int nx[3] = {0, 0, 0};
+  long nlim = 10, dt_every = 10, out_every = 1;
+  double tlim = 1e300, cfl = 0.9;
+  bool fp32 = false, graph = true;
+  uint64_t seed = 1;
+  for (int a = 2; a < argc; ++a) {
+    std::string s = argv[a];
+    auto need = [&](int n) {
+      if (a + n >= argc) {
+        usage();
+        std::exit(2);
+      }
+    };
+    if (s == "--nx") {
+      need(3);
+      for (int q = 0; q < 3; ++q) nx[q] = std::atoi(argv[++a]);
+    } else if (s == "--nlim") {
+      need(1);
+      nlim = std::atol(argv[++a]);
+    } else if (s == "--tlim") {
+      need(1);
+      tlim = std::atof(argv[++a]);
+    } else if (s == "--cfl") {
+      need(1);
+      cfl = std::atof(argv[++a]);
+    } else if (s == "--dt-every") {
+      need(1);
+      dt_every = std::atol(argv[++a]);
+    } else if (s == "--out") {
+      need(1);
+      out_every = std::atol(argv[++a]);
+    } else if (s == "--seed") {
+      need(1);
+      seed = std::strtoull(argv[++a], nullptr, 10);
+    } else if (s == "--fp32") {
+      fp32 = true;
+    } else if (s == "--no-graph") {
+      graph = false;
+    } else {
+      usage();
+      return 2;
+    }
+  }
+  cg_block blk;
+  cg_thermo th;
+  cg_config cfg;
+  if (cg_pgen_setup(config, nx[0], nx[1], nx[2], &blk, &th, &cfg) != CG_OK) {
+    std::fprintf(stderr, "canoe_step: unknown configuration %d\n", config);
+    return 2;
+  }
+  if (fp32) cfg.precision = CG_PREC_FP32;
+  Dims d(blk);
+  std::vector<double> w((size_t)d.N * d.sn);
+  if (cg_pgen_init(config, seed, &blk, &th, &cfg, w.data()) != CG_OK) return 1;
+
+  cg_ctx* ctx = nullptr;
+  int rc = cg_create(0, &ctx);
+  if (rc != CG_OK) {
+    std::fprintf(stderr, "canoe_step: no HIP device (%d)\n", rc);
+    return 1;
+  }
+  cg_run* run = nullptr;
+  if (check(ctx, cg_run_create(ctx, &blk, &th, &cfg, &run), "cg_run_create") ||
+      check(ctx, cg_run_set_prim_host(run, w.data()), "set_prim")) {
+    cg_destroy(ctx);
+    return 1;
+  }
+  const long long cells = (long long)blk.nx1 * blk.nx2 * blk.nx3;
+  std::printf("canoe_step: config %d, %d x %d x %d cells (x1 vertical), %s, %s\n",
+              config, blk.nx1, blk.nx2, blk.nx3,
+              fp32 ? "FP32 fluxes + FP64 state" : "FP64", cg_version());
+  double time = 0.0, dt = cfl_dt(blk, th, cfg, w, cfl);
+  long ncycle = 0;
+  std::vector<double> sums(d.N), sums0(d.N);
+  check(ctx, cg_run_sums(run, sums0.data()), "sums");
+  auto t0 = std::chrono::steady_clock::now();
+  int err = 0;
+  while (time < tlim && ncycle < nlim) {
+    if (ncycle % out_every == 0)
+      std::printf("cycle=%ld time=%.6e dt=%.6e\n", ncycle, time, dt);
+    double step = std::fmin(dt, tlim - time);
+    if ((err = check(ctx, cg_run_step(run, step, 1, graph ? 1 : 0), "cg_run_step")))
+      break;
+    ++ncycle;
+    time += step;
+    if (dt_every > 0 && ncycle % dt_every == 0) {
+      if ((err = check(ctx, cg_run_get_prim_host(run, w.data()), "get_prim"))) break;
+      dt = cfl_dt(blk, th, cfg, w, cfl);
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  double wall = std::chrono::duration<double>(t1 - t0).count();
+  check(ctx, cg_run_sums(run, sums.data()), "sums");
+  std::printf("cycle=%ld time=%.6e dt=%.6e\n", ncycle, time, dt);
+  std::printf("history totals (initial -> final):\n");
+  for (int n = 0; n < d.N; ++n)
+    std::printf("  var %d: %.16e -> %.16e\n", n, sums0[n], sums[n]);
+  const double zc = (double)cells * ncycle;
+  std::printf("zone-cycles = %.0f\n", zc);
+  std::printf("wall time = %.6f s\n", wall);
+  std::printf("zone-cycles/wall_second = %.6e\n", wall > 0 ? zc / wall : 0.0);
+  cg_run_destroy(run);
+  cg_destroy(ctx);
+  return err ? 1 : 0;
+}

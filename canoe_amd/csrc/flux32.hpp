@@ -50,6 +50,33 @@ __device__ __forceinline__ float weno5_up_f(float a, float b, float c, float d,
   return (a0 * p0 + a1 * p1 + a2 * p2) * frcp(a0 + a1 + a2);
 }
 
+// Both faces of cell c of the window (a, b, c, d, e) from one set of
+// smoothness indicators: qm = weno5_up_f(e, d, c, b, a), qp = weno5_up_f(a,
+// b, c, d, e).
+__device__ __forceinline__ void weno5_both_f(float a, float b, float c,
+                                             float d, float e, float* qm,
+                                             float* qp) {
+  const float eps = 1.0e-6f;
+  float s0 = a - 2.f * b + c, t0 = a - 4.f * b + 3.f * c;
+  float s1 = b - 2.f * c + d, t1 = b - d;
+  float s2 = c - 2.f * d + e, t2 = 3.f * c - 4.f * d + e;
+  float b0 = (13.f / 12.f) * (s0 * s0) + 0.25f * (t0 * t0);
+  float b1 = (13.f / 12.f) * (s1 * s1) + 0.25f * (t1 * t1);
+  float b2 = (13.f / 12.f) * (s2 * s2) + 0.25f * (t2 * t2);
+  float e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  float g0 = frcp(e0 * e0), g1 = frcp(e1 * e1), g2 = frcp(e2 * e2);
+  float p0 = (2.f * a - 7.f * b + 11.f * c) * (1.f / 6.f);
+  float p1 = (-b + 5.f * c + 2.f * d) * (1.f / 6.f);
+  float p2 = (2.f * c + 5.f * d - e) * (1.f / 6.f);
+  float m0 = (2.f * c + 5.f * b - a) * (1.f / 6.f);
+  float m1 = (-d + 5.f * c + 2.f * b) * (1.f / 6.f);
+  float m2 = (2.f * e - 7.f * d + 11.f * c) * (1.f / 6.f);
+  float a0 = 0.1f * g0, a1 = 0.6f * g1, a2 = 0.3f * g2;
+  *qp = (a0 * p0 + a1 * p1 + a2 * p2) * frcp(a0 + a1 + a2);
+  float c0 = 0.3f * g0, c1 = 0.6f * g1, c2 = 0.1f * g2;
+  *qm = (c0 * m0 + c1 * m1 + c2 * m2) * frcp(c0 + c1 + c2);
+}
+
 // PLM (van Leer) upper face of cell b of (a, b, c); lower face plm_up_f(c, b, a).
 __device__ __forceinline__ float plm_up_f(float a, float b, float c) {
   float dql = b - a, dqr = c - b;
@@ -386,6 +413,70 @@ __global__ void __launch_bounds__(256)
   for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 }
 
+// x3 faces marched along k over a segment of faces, one lane per (i, j,
+// segment), lanes along j: every cell's two face states come from one WENO
+// evaluation (weno5_both_f) on a sliding 5-cell window, and each face takes
+// the upper state of the cell below it (carried) and the lower state of the
+// cell above.  Same face fluxes as k_flux32_h<x3> up to float rounding.
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_mix_x3(KGrid g, KThermo th, int rsolver, int seg,
+             const double* __restrict__ w, const double* __restrict__ pref,
+             float* __restrict__ f3) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  const int nf = g.nx3 + 1;
+  const int nseg = (nf + seg - 1) / seg;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx1 * g.nx2 * nseg) return;
+  const int j = g.js + (int)(t % g.nx2);
+  const long long r = t / g.nx2;
+  const int i = g.is + (int)(r % g.nx1);
+  const int sg = (int)(r / g.nx1);
+  const int f0 = g.ks + sg * seg, f1 = min(f0 + seg, g.ks + nf);  // faces f0..f1-1
+  const double prd = pref[i];
+  float q[N][5];
+  auto load = [&](int m, int k) {
+    const long long c = g.at(0, k, j, i);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double v = w[c + n * g.c.n];
+      if (n == IPR) v -= prd;
+      q[n][m] = (float)v;
+    }
+  };
+  auto cell = [&](float* qm, float* qp) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      if (ORDER == 5) {
+        weno5_both_f(q[n][0], q[n][1], q[n][2], q[n][3], q[n][4], &qm[n], &qp[n]);
+      } else {
+        qp[n] = plm_up_f(q[n][1], q[n][2], q[n][3]);
+        qm[n] = plm_up_f(q[n][3], q[n][2], q[n][1]);
+      }
+    }
+  };
+  // window of cell f0 - 1: cells f0 - 3 .. f0 + 1
+#pragma unroll
+  for (int m = 0; m < 5; ++m) load(m, f0 - 3 + m);
+  float qm[N], qpp[N];
+  cell(qm, qpp);
+  for (int f = f0; f < f1; ++f) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < N; ++n) q[n][m] = q[n][m + 1];
+    load(4, f + 2);
+    float qp[N];
+    cell(qm, qp);
+    float fx[N];
+    riemann_p<NM, 2>(th, rsolver, qpp, qm, (float)prd, fx);
+#pragma unroll
+    for (int n = 0; n < N; ++n) f3[g.f[2].at(n, f, j, i)] = fx[n];
+#pragma unroll
+    for (int n = 0; n < N; ++n) qpp[n] = qp[n];
+  }
+}
+
 // x1 faces marched up a column segment, fused with the flux divergence of
 // every cell of the segment: one lane per (column, segment), lanes along the
 // unit-stride index.  The face states come from a sliding 6-cell window
@@ -510,14 +601,16 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                                  g, th, cfg.rsolver, w, pref, f2)));
   }
   if (g.f3) {
-    const unsigned b3 =
-        (unsigned)(((long long)g.nx1 * g.nx2 * (g.nx3 + 1) + bs - 1) / bs);
+    // x3 faces marched along k over segments of 16 faces
+    const int seg3 = 16;
+    const long long n3 = (long long)g.nx1 * g.nx2 * ((g.nx3 + 1 + seg3 - 1) / seg3);
+    const unsigned b3 = (unsigned)((n3 + bs - 1) / bs);
     if (cfg.order_h == 5)
-      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 5, 2><<<b3, bs, 0, s>>>(
-                                 g, th, cfg.rsolver, w, pref, f3)));
+      TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 5><<<b3, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, seg3, w, pref, f3)));
     else
-      TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, bs, 0, s>>>(
-                                 g, th, cfg.rsolver, w, pref, f3)));
+      TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 2><<<b3, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, seg3, w, pref, f3)));
   }
   // x1 faces + flux divergence + sources, marched up column segments
   const int seg = 16;
