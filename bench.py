@@ -76,7 +76,8 @@ def make_run(ctx, blocks, th, cfg, w0s, net=None, tune=None):
     if net is not None:
         run.set_mlp(net)
     if tune is not None:
-        run.set_tune(tune.implicit_wpb, tune.implicit_grid, tune.flux_block)
+        run.set_tune(tune.implicit_wpb, tune.implicit_grid, tune.flux_block,
+                     tune.face_variant, tune.march_seg)
     for b, w in enumerate(w0s):
         run.set_prim(w, b)
     return run

@@ -365,7 +365,8 @@ class ImplicitSolver:
 class Tune(ctypes.Structure):
     """cg_tune: launch shape of a run."""
     _fields_ = [("implicit_wpb", ctypes.c_int), ("implicit_grid", ctypes.c_int),
-                ("flux_block", ctypes.c_int), ("step_ms", ctypes.c_double)]
+                ("flux_block", ctypes.c_int), ("face_variant", ctypes.c_int),
+                ("march_seg", ctypes.c_int), ("step_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -427,8 +428,10 @@ class Run:
         self.ctx._check(lib().cg_run_get_tune(self.h, _ref(t)), "get_tune")
         return t
 
-    def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256):
-        t = Tune(implicit_wpb, implicit_grid, flux_block, 0.0)
+    def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256,
+                 face_variant=1, march_seg=16):
+        t = Tune(implicit_wpb, implicit_grid, flux_block, face_variant,
+                 march_seg, 0.0)
         self.ctx._check(lib().cg_run_set_tune(self.h, _ref(t)), "set_tune")
 
     def autotune(self, dt):

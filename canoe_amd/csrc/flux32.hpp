@@ -413,6 +413,65 @@ __global__ void __launch_bounds__(256)
   for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 }
 
+// x2 faces from one WENO evaluation per cell: lane l of a wavefront holds
+// cell cb + l of an x2 row and gets its 5-cell window from lanes l-2 .. l+2
+// by cross-lane shuffles; the face between lanes l-1 and l takes the upper
+// state of lane l-1 and the lower state of lane l.  Lanes 3..61 produce the
+// faces cb+3 .. cb+61 (59 per wavefront); wavefronts of a row overlap by 5
+// cells.
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_mix_x2(KGrid g, KThermo th, int rsolver, const double* __restrict__ w,
+             const double* __restrict__ pref, float* __restrict__ f2) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  constexpr int FPW = 59;  // faces per wavefront
+  const int nf = g.nx2 + 1;
+  const int wpr = (nf + FPW - 1) / FPW;  // wavefronts per (i, k) row
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long nwave = (long long)g.nx1 * g.nx3 * wpr;
+  if (wave >= nwave) return;  // whole wavefronts leave together
+  const int wr = (int)(wave % wpr);
+  const long long row = wave / wpr;
+  const int i = g.is + (int)(row % g.nx1);
+  const int k = g.ks + (int)(row / g.nx1);
+  const int cb = g.js - 3 + FPW * wr;
+  const int c = cb + lane;
+  const int cc = min(max(c, 0), g.nc2 - 1);  // lanes past the row load a copy
+  const double prd = pref[i];
+  const long long base = g.at(0, k, cc, i);
+  float v[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    double x = w[base + n * g.c.n];
+    if (n == IPR) x -= prd;
+    v[n] = (float)x;
+  }
+  float qm[N], qp[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    float a = __shfl(v[n], lane - 2 < 0 ? 0 : lane - 2);
+    float b = __shfl(v[n], lane - 1 < 0 ? 0 : lane - 1);
+    float d = __shfl(v[n], lane + 1 > 63 ? 63 : lane + 1);
+    float e = __shfl(v[n], lane + 2 > 63 ? 63 : lane + 2);
+    if (ORDER == 5) {
+      weno5_both_f(a, b, v[n], d, e, &qm[n], &qp[n]);
+    } else {
+      qp[n] = plm_up_f(b, v[n], d);
+      qm[n] = plm_up_f(d, v[n], b);
+    }
+  }
+  float ql[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) ql[n] = __shfl(qp[n], lane - 1 < 0 ? 0 : lane - 1);
+  const int f = c;  // face between cells c-1 and c
+  if (lane < 3 || lane > 61 || f > g.je + 1) return;
+  float fx[N];
+  riemann_p<NM, 1>(th, rsolver, ql, qm, (float)prd, fx);
+#pragma unroll
+  for (int n = 0; n < N; ++n) f2[g.f[1].at(n, k, f, i)] = fx[n];
+}
+
 // x3 faces marched along k over a segment of faces, one lane per (i, j,
 // segment), lanes along j: every cell's two face states come from one WENO
 // evaluation (weno5_both_f) on a sliding 5-cell window, and each face takes
@@ -575,14 +634,21 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Face kernels of the mixed form.  variant 1: x2 faces by cross-lane WENO
+// (k_mix_x2), x3 faces marched over segments of `seg` faces (k_mix_x3),
+// x1 faces marched up column segments of `seg` cells fused with the
+// divergence (k_mix_x1_div).  variant 0: one lane per face in every
+// direction (k_flux32_h, k_flux32_x1) and a separate divergence kernel.
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* ff,
                      double* psf, double* psv, const double* pref, double* du,
-                     hipStream_t s, KTimer* tm, bool scan, int bs = 256) {
+                     hipStream_t s, KTimer* tm, bool scan, int bs = 256,
+                     int variant = 1, int seg = 16) {
   const long long ncol = (long long)g.nx2 * g.nx3;
   const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   const int decomp = g.grav != 0.0;
+  const bool w5h = cfg.order_h == 5, w5v = cfg.order_v == 5;
   float* f1 = ff;
   float* f2 = f1 + (size_t)(5 + NM) * g.f[0].n;
   float* f3 = f2 + (size_t)(5 + NM) * g.f[1].n;
@@ -590,33 +656,63 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     TIMED(tm, KID_SCAN,
           (k_hydrostatic_scan<NM><<<(unsigned)((ncol + 255) / 256), 256, 0, s>>>(
               g, th, w, psf, psv)));
+  if (variant == 0) {
+    const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + bs - 1) / bs);
+    if (w5v)
+      TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+    else
+      TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+    if (g.f2) {
+      const unsigned b2 =
+          (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + bs - 1) / bs);
+      if (w5h)
+        TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 5, 1><<<b2, bs, 0, s>>>(
+                                   g, th, cfg.rsolver, w, pref, f2)));
+      else
+        TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 2, 1><<<b2, bs, 0, s>>>(
+                                   g, th, cfg.rsolver, w, pref, f2)));
+    }
+    if (g.f3) {
+      const unsigned b3 =
+          (unsigned)(((long long)g.nx1 * g.nx2 * (g.nx3 + 1) + bs - 1) / bs);
+      if (w5h)
+        TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 5, 2><<<b3, bs, 0, s>>>(
+                                   g, th, cfg.rsolver, w, pref, f3)));
+      else
+        TIMED(tm, KID_FLUX_X3, (k_flux32_h<NM, 2, 2><<<b3, bs, 0, s>>>(
+                                   g, th, cfg.rsolver, w, pref, f3)));
+    }
+    TIMED(tm, KID_DIVERGENCE,
+          (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
+              g, th, src, f1, f2, f3, psf, decomp, w, du)));
+    return;
+  }
   if (g.f2) {
-    const unsigned b2 =
-        (unsigned)(((long long)g.nx1 * (g.nx2 + 1) * g.nx3 + bs - 1) / bs);
-    if (cfg.order_h == 5)
-      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 5, 1><<<b2, bs, 0, s>>>(
+    // one wavefront per 59 faces of an x2 row
+    const long long nw = (long long)g.nx1 * g.nx3 * ((g.nx2 + 1 + 58) / 59);
+    const unsigned b2 = (unsigned)((nw * 64 + bs - 1) / bs);
+    if (w5h)
+      TIMED(tm, KID_FLUX_X2, (k_mix_x2<NM, 5><<<b2, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f2)));
     else
-      TIMED(tm, KID_FLUX_X2, (k_flux32_h<NM, 2, 1><<<b2, bs, 0, s>>>(
+      TIMED(tm, KID_FLUX_X2, (k_mix_x2<NM, 2><<<b2, bs, 0, s>>>(
                                  g, th, cfg.rsolver, w, pref, f2)));
   }
   if (g.f3) {
-    // x3 faces marched along k over segments of 16 faces
-    const int seg3 = 16;
-    const long long n3 = (long long)g.nx1 * g.nx2 * ((g.nx3 + 1 + seg3 - 1) / seg3);
+    const long long n3 = (long long)g.nx1 * g.nx2 * ((g.nx3 + 1 + seg - 1) / seg);
     const unsigned b3 = (unsigned)((n3 + bs - 1) / bs);
-    if (cfg.order_h == 5)
+    if (w5h)
       TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 5><<<b3, bs, 0, s>>>(
-                                 g, th, cfg.rsolver, seg3, w, pref, f3)));
+                                 g, th, cfg.rsolver, seg, w, pref, f3)));
     else
       TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 2><<<b3, bs, 0, s>>>(
-                                 g, th, cfg.rsolver, seg3, w, pref, f3)));
+                                 g, th, cfg.rsolver, seg, w, pref, f3)));
   }
-  // x1 faces + flux divergence + sources, marched up column segments
-  const int seg = 16;
   const long long nx1t = ncol * ((g.nx1 + seg - 1) / seg);
   const unsigned b1 = (unsigned)((nx1t + bs - 1) / bs);
-  if (cfg.order_v == 5)
+  if (w5v)
     TIMED(tm, KID_FLUX_X1, (k_mix_x1_div<NM, 5><<<b1, bs, 0, s>>>(
                                g, th, src, cfg.rsolver, decomp, seg, w, psf, psv,
                                f2, f3, du)));
@@ -624,8 +720,6 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     TIMED(tm, KID_FLUX_X1, (k_mix_x1_div<NM, 2><<<b1, bs, 0, s>>>(
                                g, th, src, cfg.rsolver, decomp, seg, w, psf, psv,
                                f2, f3, du)));
-  (void)f1;
-  (void)ncell;
 }
 
 // per-level reference pressure: horizontal mean of p over interior columns

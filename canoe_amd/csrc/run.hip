@@ -141,7 +141,8 @@ int run_stage(cg_run* r, double dt, int s) {
     // implicit runs get psf / psv from the previous downward sweep
     if (r->cfg.precision == CG_PREC_FP32) {
       launch_fluxes32<NM>(g, r->kt, r->cfg, src, b.w, b.ff, b.psf, b.psv, b.pref,
-                          b.du, st, tm, !imp, bs);
+                          b.du, st, tm, !imp, bs, r->tune.face_variant,
+                          r->tune.march_seg);
     } else {
       launch_fluxes<NM>(g, r->kt, r->cfg, b.w, b.psf, b.psv, b.f1, b.f2, b.f3, st,
                         tm, !imp, bs);
@@ -337,6 +338,8 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   r->tune.implicit_wpb = 1;
   r->tune.implicit_grid = 0;
   r->tune.flux_block = 256;
+  r->tune.face_variant = 1;
+  r->tune.march_seg = 16;
   r->tune.step_ms = 0.0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -545,9 +548,12 @@ int cg_run_set_tune(cg_run* r, const cg_tune* t) {
   if (!r || !t) return CG_ERR_ARG;
   if ((t->implicit_wpb != 1 && t->implicit_wpb != 2 && t->implicit_wpb != 4) ||
       t->implicit_grid < 0 ||
-      (t->flux_block != 64 && t->flux_block != 128 && t->flux_block != 256))
+      (t->flux_block != 64 && t->flux_block != 128 && t->flux_block != 256) ||
+      (t->face_variant != 0 && t->face_variant != 1) || t->march_seg < 1 ||
+      t->march_seg > 4096)
     return set_err(r->ctx, CG_ERR_ARG,
-                   "implicit_wpb in {1,2,4}, implicit_grid >= 0, flux_block in {64,128,256}");
+                   "implicit_wpb in {1,2,4}, implicit_grid >= 0, flux_block in "
+                   "{64,128,256}, face_variant in {0,1}, march_seg in [1,4096]");
   if (!(r->cfg.implicit_flag & 8) && t->implicit_wpb != 1)
     return set_err(r->ctx, CG_ERR_ARG, "3x3 solve runs with implicit_wpb 1");
   r->tune = *t;
@@ -670,6 +676,25 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     if (ms < best_ms) {
       best_ms = ms;
       best = t;
+    }
+  }
+  // face kernels of the mixed form: per-face lanes, or the cell-centric
+  // kernels with march segments of 8, 16, 32 cells
+  if (r->cfg.precision == CG_PREC_FP32) {
+    const int vs[4][2] = {{0, 16}, {1, 8}, {1, 16}, {1, 32}};
+    for (int vi = 0; vi < 4 && !err; ++vi) {
+      cg_tune t = best;
+      t.face_variant = vs[vi][0];
+      t.march_seg = vs[vi][1];
+      if (t.face_variant == best.face_variant && t.march_seg == best.march_seg)
+        continue;
+      double ms = 0.0;
+      err = trial(t, &ms);
+      if (err) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
     }
   }
   (void)hipEventDestroy(e0);

@@ -267,12 +267,17 @@ int cg_run_block_sums(cg_run* run, int block, double* out_host);
 /* Launch shape of a run: waves per workgroup (1, 2, 4) and grid (0 = one
  * workgroup per set of column groups, else a persistent grid of that many
  * workgroups) of the batched column solve, workgroup size (64, 128, 256) of
- * the face and divergence kernels; step_ms is the step time measured by the
+ * the face and divergence kernels, and for the FP32-flux form the face
+ * kernel variant (0: one lane per face; 1: one WENO evaluation per cell,
+ * x2 across lanes, x3 and x1 marched over segments of march_seg cells, x1
+ * fused with the divergence); step_ms is the step time measured by the
  * autotuner.  No counterpart in the reference (CPU loop nests). */
 typedef struct cg_tune {
   int implicit_wpb;
   int implicit_grid;
   int flux_block;
+  int face_variant;
+  int march_seg;
   double step_ms;
 } cg_tune;
 /* Network of a run whose config has satadj = CG_SATADJ_MLP (NULL: plain

@@ -76,15 +76,21 @@ def test_batch_equals_single_block_runs(ctx):
         np.testing.assert_array_equal(ub[i], us)
 
 
-def test_batch_mixed_close_to_fp64(ctx):
+@pytest.mark.parametrize("variant", [(1, 256, 0, 16), (1, 256, 1, 16),
+                                     (1, 128, 1, 3), (1, 64, 1, 40)])
+def test_batch_mixed_close_to_fp64(ctx, variant):
+    """Both face-kernel variants of the FP32-flux form, several march
+    segments (shorter than, and longer than, the columns)."""
     bl = _blocks(seed=11)
     dt = min(cfl_dt(b[0], b[1], b[2], b[3], 0.6) for b in bl)
     u64 = _run_batch(ctx, bl, dt, 5)
-    u32 = _run_batch(ctx, bl, dt, 5, prec=ca.PREC_FP32)
+    wpb, fb, fv, seg = variant
+    u32 = _run_batch(ctx, bl, dt, 5, prec=ca.PREC_FP32,
+                     tune=(wpb, 0, fb, fv, seg))
     for i, b in enumerate(bl):
         sk, sj, si = b[0].interior()
         e = relerr(u32[i][:, sk, sj, si], u64[i][:, sk, sj, si])
-        _say("  block %d mixed vs fp64 relerr %.3e" % (i, e))
+        _say("  block %d mixed %s vs fp64 relerr %.3e" % (i, variant, e))
         assert e < 1e-5
 
 
@@ -101,6 +107,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape(ctx):
         t = run.autotune(dt)
         _say("  autotune: %s" % t.as_dict())
         assert t.implicit_wpb in (1, 2, 4) and t.flux_block in (64, 128, 256)
+        assert t.face_variant in (0, 1) and t.march_seg >= 1
         assert t.step_ms > 0
         for i in range(len(bl)):
             np.testing.assert_array_equal(run.cons(i), u0[i])

@@ -119,3 +119,26 @@ def test_step(ctx, case, graph):
     ew = relerr(wg[:, sk, sj, si], wo[:, sk, sj, si])
     _say("  relerr u %.3e w %.3e" % (eu, ew))
     assert eu <= TOL and ew <= TOL
+
+
+def test_graph_equals_direct_full_box(ctx):
+    """The full 256x256x128 box: two graph-replayed steps equal two directly
+    launched steps bit for bit, also for a second run in the same process
+    (whose buffers reuse freed device memory)."""
+    blk, th, cfg = ca.pgen_setup(2)
+    w = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w, 0.9)
+    sk, sj, si = blk.interior()
+    out = []
+    for graph in (False, True, True):
+        _say("full box graph=%s" % graph)
+        run = ca.Run(ctx, blk, th, cfg)
+        try:
+            run.set_prim(w)
+            run.step(dt, 2, graph=graph)
+            out.append(run.cons()[:, sk, sj, si])
+        finally:
+            run.close()
+    assert np.all(np.isfinite(out[0]))
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
