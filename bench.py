@@ -251,22 +251,25 @@ def bench_box(ctx, nx, steps, warmup, acc_steps):
     cells = blk.ncells()
     log("dry box %dx%dx%d, dt=%.4f s" % (blk.nx1, blk.nx2, blk.nx3, dt))
     forms = {}
+    tunes = {}
     for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
         cfg.precision = prec
         ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
-                                    warmup)
+                                    warmup, autotune=True)
         dom, ops = roofline(blk, cfg, stats)
         forms[name] = {"ms_per_step": round(ms, 4),
                        "cell_updates_per_s": cells / (ms * 1e-3),
-                       "dominant": dom, "ops": ops,
+                       "dominant": dom, "ops": ops, "tune": tune,
                        "kernels_avg_ms": {k: round(v[0], 4)
                                           for k, v in stats.items()}}
-        log("%s: %.3f ms/step, %.3e cell-updates/s"
-            % (name, ms, cells / (ms * 1e-3)))
+        tunes[name] = tune
+        log("%s: %.3f ms/step, %.3e cell-updates/s, tune %s"
+            % (name, ms, cells / (ms * 1e-3), tune))
     us = {}
-    for prec in (ca.PREC_FP64, ca.PREC_FP32):
+    for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
         cfg.precision = prec
-        us[prec] = final_state(ctx, [blk], th, cfg, [w0], dt, acc_steps)
+        us[prec] = final_state(ctx, [blk], th, cfg, [w0], dt, acc_steps,
+                               tune=ca.Tune(**tunes[name]))
     acc = errors([blk], us[ca.PREC_FP32], us[ca.PREC_FP64])
     acc["steps"] = acc_steps
     forms["mixed"]["error_vs_fp64"] = acc
@@ -393,14 +396,14 @@ def bench_moist(ctx, nx, steps, warmup, acc_steps, npairs, train_steps):
         cfg.precision = prec
         cfg.satadj = mode
         ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
-                                    warmup, net=n)
+                                    warmup, net=n, autotune=True)
         dom, ops = roofline(blk, cfg, stats)
         forms[name] = {"ms_per_step": round(ms, 4),
                        "cell_updates_per_s": cells / (ms * 1e-3),
-                       "dominant": dom, "ops": ops,
+                       "dominant": dom, "ops": ops, "tune": tune,
                        "kernels_avg_ms": {k: round(v[0], 4)
                                           for k, v in stats.items()}}
-        log("moist %s: %.3f ms/step" % (name, ms))
+        log("moist %s: %.3f ms/step, tune %s" % (name, ms, tune))
     us = {}
     for name, prec, mode, n in (("plain", ca.PREC_FP64, ca.SATADJ_NEWTON,
                                  None),
@@ -459,6 +462,35 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     return out
 
 
+def profile_run(nx, steps):
+    """A short, fixed run for rocprofv3: a streaming copy of 1 GiB (the
+    calibration of the HBM traffic counters: it reads and writes exactly
+    2^30 bytes) three times, then `steps` graph-replayed steps of the dry box
+    in the FP32-flux form with the default launch shape."""
+    ctx = ca.Context(0)
+    nbytes = 1 << 30
+    a = ctx.array(shape=(nbytes // 8,), dtype=np.float64)
+    b = ctx.array(shape=(nbytes // 8,), dtype=np.float64)
+    L = ca.lib()
+    for _ in range(3):
+        ctx._check(L.cg_memcpy_d2d(ctx.h, b.ptr, a.ptr, nbytes), "d2d")
+    a.free()
+    b.free()
+    blk, th, cfg = ca.pgen_setup(2, *nx)
+    w0 = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0)
+    cfg.precision = ca.PREC_FP32
+    run = make_run(ctx, [blk], th, cfg, [w0])
+    t0 = time.perf_counter()
+    run.step(dt, steps, graph=USE_GRAPH)
+    t1 = time.perf_counter()
+    run.close()
+    ctx.close()
+    print(json.dumps({"profile": "dry box %dx%dx%d FP32 fluxes" % tuple(nx),
+                      "steps": steps, "copy_bytes": nbytes,
+                      "ms_per_step": round(1e3 * (t1 - t0) / steps, 4)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,6 +506,9 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[128, 256, 256],
                     help="nx1 (vertical) nx2 nx3")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed run for rocprofv3 (copy calibration + "
+                         "a few steps of the dry box, FP32 fluxes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -482,6 +517,9 @@ def main():
         sys.exit(2)
     global USE_GRAPH
     USE_GRAPH = not args.no_graph
+    if args.profile:
+        profile_run(args.nx, max(1, min(args.steps, 5)))
+        return
 
     ctx = ca.Context(0)
     blk, cfg, dt, cells, forms, acc = bench_box(ctx, args.nx, args.steps,

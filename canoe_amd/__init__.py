@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 
 NGHOST = 3
 MAX_NMASS = 3
-NUM_KERNEL_IDS = 13
+NUM_KERNEL_IDS = 14
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV = range(9)
@@ -125,6 +125,7 @@ _SIGS = {
     "cg_free": (_I, [_P, _P]),
     "cg_memcpy_h2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cg_memcpy_d2h": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "cg_memcpy_d2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cg_riemann": (_I, [_P, _P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
                         ctypes.c_ssize_t]),
     "cg_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -429,7 +430,7 @@ class Run:
         return t
 
     def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256,
-                 face_variant=1, march_seg=16):
+                 face_variant=2, march_seg=16):
         t = Tune(implicit_wpb, implicit_grid, flux_block, face_variant,
                  march_seg, 0.0)
         self.ctx._check(lib().cg_run_set_tune(self.h, _ref(t)), "set_tune")

@@ -7,6 +7,7 @@
 // runs are in run.hip, the saturation network and adjustment in mlp.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +81,24 @@ int cg_memcpy_h2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
 }
+int cg_memcpy_d2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return CG_ERR_ARG;
+  if (bytes == 0) return CG_OK;
+  const uintptr_t a = (uintptr_t)dst | (uintptr_t)src;
+  const long long n16 = (a % 16 == 0) ? (long long)(bytes / 16) : 0;
+  if (n16 > 0) {
+    long long nb = std::min<long long>((n16 + 255) / 256, 256LL * 1024);
+    k_copy16<<<(unsigned)nb, 256, 0, ctx->stream>>>((const uint4*)src, (uint4*)dst, n16);
+  }
+  const long long done = n16 * 16, rest = (long long)bytes - done;
+  if (rest > 0)
+    k_copy1<<<nblk(rest, 256), 256, 0, ctx->stream>>>(
+        (const unsigned char*)src + done, (unsigned char*)dst + done, rest);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
 int cg_memcpy_d2h(cg_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx) return CG_ERR_ARG;
   HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -634,17 +634,19 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Face kernels of the mixed form.  variant 1: x2 faces by cross-lane WENO
-// (k_mix_x2), x3 faces marched over segments of `seg` faces (k_mix_x3),
-// x1 faces marched up column segments of `seg` cells fused with the
-// divergence (k_mix_x1_div).  variant 0: one lane per face in every
-// direction (k_flux32_h, k_flux32_x1) and a separate divergence kernel.
+// Face kernels of the mixed form.  variant 0: one lane per face in every
+// direction (k_flux32_h, k_flux32_x1) and a separate divergence kernel
+// (k_divergence32).  variants 1 and 2: x2 faces by cross-lane WENO
+// (k_mix_x2), x3 faces marched over segments of `seg` faces (k_mix_x3);
+// x1 faces then either marched up column segments of `seg` cells fused with
+// the divergence (k_mix_x1_div, variant 1) or one lane per face followed by
+// the divergence kernel (variant 2).
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* ff,
                      double* psf, double* psv, const double* pref, double* du,
                      hipStream_t s, KTimer* tm, bool scan, int bs = 256,
-                     int variant = 1, int seg = 16) {
+                     int variant = 2, int seg = 16) {
   const long long ncol = (long long)g.nx2 * g.nx3;
   const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   const int decomp = g.grav != 0.0;
@@ -709,6 +711,19 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     else
       TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 2><<<b3, bs, 0, s>>>(
                                  g, th, cfg.rsolver, seg, w, pref, f3)));
+  }
+  if (variant == 2) {
+    const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + bs - 1) / bs);
+    if (w5v)
+      TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+    else
+      TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 2><<<b1, bs, 0, s>>>(
+                                 g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
+    TIMED(tm, KID_DIVERGENCE,
+          (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
+              g, th, src, f1, f2, f3, psf, decomp, w, du)));
+    return;
   }
   const long long nx1t = ncol * ((g.nx1 + seg - 1) / seg);
   const unsigned b1 = (unsigned)((nx1t + bs - 1) / bs);

@@ -1136,6 +1136,22 @@ __global__ void __launch_bounds__(64 * WPB)
   }
 }
 
+// Streaming copy of n 16-byte words (cg_memcpy_d2d): grid-stride loop.
+static __global__ void __launch_bounds__(256)
+    k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
+             long long n) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
+static __global__ void __launch_bounds__(256)
+    k_copy1(const unsigned char* __restrict__ src,
+            unsigned char* __restrict__ dst, long long n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
 // cons <-> prim on interior cells
 template <int NM, int TO_PRIM>
 __global__ void __launch_bounds__(256)

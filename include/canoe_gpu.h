@@ -105,6 +105,9 @@ const char* cg_last_error(cg_ctx* ctx);
 int cg_malloc(cg_ctx* ctx, size_t bytes, void** ptr);
 int cg_free(cg_ctx* ctx, void* ptr);
 int cg_memcpy_h2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy through a streaming kernel (the copy used to
+ * calibrate HBM traffic counters). */
+int cg_memcpy_d2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cg_memcpy_d2h(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- Riemann plugin ----------------------------------------------------- *
@@ -268,10 +271,12 @@ int cg_run_block_sums(cg_run* run, int block, double* out_host);
  * workgroup per set of column groups, else a persistent grid of that many
  * workgroups) of the batched column solve, workgroup size (64, 128, 256) of
  * the face and divergence kernels, and for the FP32-flux form the face
- * kernel variant (0: one lane per face; 1: one WENO evaluation per cell,
- * x2 across lanes, x3 and x1 marched over segments of march_seg cells, x1
- * fused with the divergence); step_ms is the step time measured by the
- * autotuner.  No counterpart in the reference (CPU loop nests). */
+ * kernel variant (0: one lane per face in every direction; 1: one WENO
+ * evaluation per cell, x2 across lanes, x3 and x1 marched over segments of
+ * march_seg cells, x1 fused with the divergence; 2: as 1 for x2 / x3, one
+ * lane per x1 face and a divergence kernel); step_ms is the step time
+ * measured by the autotuner.  No counterpart in the reference (CPU loop
+ * nests). */
 typedef struct cg_tune {
   int implicit_wpb;
   int implicit_grid;

@@ -33,3 +33,12 @@ def test_python_binding_covers_the_header():
 
 def test_version_string_names_the_arch():
     assert b"gfx950" in ca.lib().cg_version()
+
+
+def test_python_constants_match_the_header():
+    src = open(HEADER).read()
+    nk = int(re.search(r"#define CG_NUM_KERNEL_IDS (\d+)", src).group(1))
+    npar = int(re.search(r"#define CG_MLP_NPARAM (\d+)", src).group(1))
+    assert ca.NUM_KERNEL_IDS == nk
+    assert cm.NPARAM == npar
+    assert ctypes.sizeof(ca.Tune) == 5 * 4 + 4 + 8  # 5 ints, padding, double
