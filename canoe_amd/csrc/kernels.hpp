@@ -620,8 +620,26 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
   Mt<B> ap;  // a'_{i-1}
   Vt<B> dl;  // delta_{i-1}
   bool ok = true;
+  // the next level's w and this level's du are loaded one level ahead, so
+  // that their latency overlaps the block algebra of the current level
+  double wn[N], dun[N];
+  auto load_du = [&](double* dst, int i) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) dst[n] = du[c0 + n * sn + i * si];
+  };
+  load(wn, is + 1);
+  load_du(dun, is);
   for (int i = is; i <= ie; ++i) {
-    load(wp, i + 1);
+    double dui[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      wp[n] = wn[n];
+      dui[n] = dun[n];
+    }
+    if (i < ie) {
+      load(wn, i + 2);
+      load_du(dun, i + 1);
+    }
     roe_average<NM>(prim, gm, wc, wp);
     abs_roe_fma<NM, B>(th, prim, gm, Ap);
     flux_jac<NM, B>(gm1, wp, dp);
@@ -629,19 +647,18 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
     double t[B];
     // rhs (forward_backward.hpp:30-36)
     {
-      const long long ci = c0 + i * si;
-      double r0 = du[ci];
+      double r0 = dui[0];
 #pragma unroll
-      for (int n = 1; n < IVX; ++n) r0 += du[ci + n * sn];
+      for (int n = 1; n < IVX; ++n) r0 += dui[n];
       t[0] = r0 * idt;
       if (B == 5) {
-        t[1] = du[ci + IVX * sn] * idt;
-        t[2] = du[ci + (IVX + 1) * sn] * idt;
-        t[3] = du[ci + (IVX + 2) * sn] * idt;
-        t[4] = du[ci + IEN * sn] * idt;
+        t[1] = dui[IVX] * idt;
+        t[2] = dui[IVX + 1] * idt;
+        t[3] = dui[IVX + 2] * idt;
+        t[4] = dui[IEN] * idt;
       } else {
-        t[1] = du[ci + IVX * sn] * idt;
-        t[2] = du[ci + IEN * sn] * idt;
+        t[1] = dui[IVX] * idt;
+        t[2] = dui[IEN] * idt;
       }
     }
 #pragma unroll
@@ -872,22 +889,24 @@ struct KUpdate {
   int satadj;
 };
 
+// uo / u1o: the cell's u and u1 (u1o unused on stage 0, which writes u1).
 template <int NM>
-__device__ inline void update_cell(const KThermo& th, const KUpdate& up,
-                                   double* __restrict__ u,
-                                   double* __restrict__ u1, long long ci,
-                                   long long sn, const double* dun, double* uc,
-                                   double* wc, int* __restrict__ status) {
+__device__ inline void update_cell_pre(const KThermo& th, const KUpdate& up,
+                                       const double* uo, const double* u1o,
+                                       double* __restrict__ u1, long long ci,
+                                       long long sn, const double* dun,
+                                       double* uc, double* wc,
+                                       int* __restrict__ status) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    double un = u[ci + n * sn];
+    double un = uo[n];
     double v;
     if (up.stage == 0) {
       v = 0.0 + 1.0 * un;
       u1[ci + n * sn] = v;
     } else {
-      v = up.g1 * un + up.g2 * u1[ci + n * sn];
+      v = up.g1 * un + up.g2 * u1o[n];
     }
     uc[n] = v + dun[n];
   }
@@ -908,6 +927,22 @@ __device__ inline void update_cell(const KThermo& th, const KUpdate& up,
     uc[IEN] = ie + ke;
   }
   cons_to_prim_cell<NM>(th, uc, wc);
+}
+
+template <int NM>
+__device__ inline void update_cell(const KThermo& th, const KUpdate& up,
+                                   double* __restrict__ u,
+                                   double* __restrict__ u1, long long ci,
+                                   long long sn, const double* dun, double* uc,
+                                   double* wc, int* __restrict__ status) {
+  constexpr int N = 5 + NM;
+  double uo[N], u1o[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    uo[n] = u[ci + n * sn];
+    u1o[n] = (up.stage == 0) ? 0.0 : u1[ci + n * sn];
+  }
+  update_cell_pre<NM>(th, up, uo, u1o, u1, ci, sn, dun, uc, wc, status);
 }
 
 // Cell-parallel end of stage (explicit runs).
@@ -961,20 +996,53 @@ __device__ __forceinline__ void bwd_update_column(
   double rtop[NG];    // density of the top three cells (ie, ie-1, ie-2)
   double pf_top[NG + 1];  // psf(ie+1) .. psf(ie+4)
   double pf_next = 0.0;   // psf(i+1)
-  Vt<B> dn;
-  for (int i = ie; i >= is; --i) {
-    Vt<B> d;
-    back_step<B>(arch, (long long)(i - is) * AS, as, ao, i == ie, dn, d);
-    dn = d;
+  // the archive and the state of the next level down are loaded one level
+  // ahead, so that their latency overlaps the current level's update
+  struct Level {
+    Mt<B> a;
+    Vt<B> dl;
+    double duo[N], wsp[N], uo[N], u1o[N];
+  };
+  auto fetch = [&](Level& L, int i) {
+    const long long lb = (long long)(i - is) * AS;
+#pragma unroll
+    for (int r = 0; r < B; ++r) L.dl.v[r] = arch[(lb + B * B + r) * as + ao];
+    if (i != ie) {
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) L.a.a[r][q] = arch[(lb + r * B + q) * as + ao];
+    }
     const long long ci = c0 + i * si;
-    double duo[N], wsp[N], dun[N], uc[N], wc[N];
 #pragma unroll
     for (int n = 0; n < N; ++n) {
-      duo[n] = (n <= NM || B == 3) ? du[ci + n * sn] : 0.0;
-      wsp[n] = (n >= 1 && n <= NM) ? w[ci + n * sn] : 0.0;
+      L.duo[n] = (n <= NM || B == 3) ? du[ci + n * sn] : 0.0;
+      L.wsp[n] = (n >= 1 && n <= NM) ? w[ci + n * sn] : 0.0;
+      L.uo[n] = u[ci + n * sn];
+      L.u1o[n] = (up.stage == 0) ? 0.0 : u1[ci + n * sn];
     }
-    write_back<NM, B>(d, duo, wsp, dun);
-    update_cell<NM>(th, up, u, u1, ci, sn, dun, uc, wc, status);
+  };
+  Vt<B> dn;
+  Level cur, nxt;
+  fetch(cur, ie);
+  for (int i = ie; i >= is; --i) {
+    if (i > is) fetch(nxt, i - 1);
+    // back substitution delta_i = d_i - a'_i delta_{i+1}
+    // (forward_backward.hpp:86-97)
+    Vt<B> d;
+    if (i == ie) {
+      d = cur.dl;
+    } else {
+      Vt<B> ad;
+      mvec<B>(cur.a, dn, ad);
+#pragma unroll
+      for (int r = 0; r < B; ++r) d.v[r] = cur.dl.v[r] - ad.v[r];
+    }
+    dn = d;
+    const long long ci = c0 + i * si;
+    double dun[N], uc[N], wc[N];
+    write_back<NM, B>(d, cur.duo, cur.wsp, dun);
+    update_cell_pre<NM>(th, up, cur.uo, cur.u1o, u1, ci, sn, dun, uc, wc, status);
 #pragma unroll
     for (int n = 0; n < N; ++n) {
       u[ci + n * sn] = uc[n];
@@ -1013,6 +1081,7 @@ __device__ __forceinline__ void bwd_update_column(
       psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
       pf_next = pfi;
     }
+    cur = nxt;
   }
   // outflow bottom ghosts copy the first interior cell
   if (g.bc[0] == 2) {
