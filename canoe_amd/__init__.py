@@ -430,7 +430,7 @@ class Run:
         return t
 
     def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256,
-                 face_variant=2, march_seg=16):
+                 face_variant=3, march_seg=16):
         t = Tune(implicit_wpb, implicit_grid, flux_block, face_variant,
                  march_seg, 0.0)
         self.ctx._check(lib().cg_run_set_tune(self.h, _ref(t)), "set_tune")

@@ -634,13 +634,217 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// All faces of a tile fused with the flux divergence (mixed form, 3D
+// blocks): a workgroup owns a TJ x TK tile of columns and marches up a
+// segment of levels.  Per level, the tile's plane with a 3-cell halo is
+// staged in LDS (float, p - pref(i)); the x2 and x3 face states come from
+// one WENO5/PLM evaluation per cell in LDS, the x2 / x3 Riemann fluxes go to
+// LDS, and each lane takes its cell's two x2 and two x3 face fluxes from
+// there.  The x1 faces are marched in registers as in k_mix_x1_div (a
+// sliding window of T, p - psv with FP64 differences).  Every cell's du is
+// written once: w is read from HBM about once, the face fluxes never leave
+// the chip.
+// The tile is FT_J x FT_K columns (32 x 8 or 16 x 16, both 256 lanes).
+template <int NM, int OH, int OV, int FT_J, int FT_K>
+__global__ void __launch_bounds__(FT_J * FT_K)
+    k_mix_fused(KGrid g, KThermo th, KSource src, int rsolver, int decomp,
+                int seg, const double* __restrict__ w,
+                const double* __restrict__ psf, const double* __restrict__ psv,
+                const double* __restrict__ pref, double* __restrict__ du) {
+  constexpr int FT_H = 3;
+  constexpr int FT_PJ = FT_J + 2 * FT_H, FT_PK = FT_K + 2 * FT_H;
+  constexpr int NT = FT_J * FT_K, NP = FT_PJ * FT_PK;
+  constexpr int N = 5 + NM, IPR = 4 + NM, IVX = 1 + NM;
+  constexpr int LO = (OV == 5) ? 0 : 1, HI = (OV == 5) ? 5 : 4;
+  constexpr int C2 = FT_J + 2, C3 = FT_K + 2;  // cells with face states
+  __shared__ float plane[N][FT_PK][FT_PJ];
+  __shared__ float q2m[N][FT_K][C2], q2p[N][FT_K][C2];
+  __shared__ float q3m[N][C3][FT_J], q3p[N][C3][FT_J];
+  __shared__ float fx2[N][FT_K][FT_J + 1];
+  __shared__ float fx3[N][FT_K + 1][FT_J];
+  const int nbj = (g.nx2 + FT_J - 1) / FT_J, nbk = (g.nx3 + FT_K - 1) / FT_K;
+  const int bid = blockIdx.x;
+  const int bj = bid % nbj, bk = (bid / nbj) % nbk, sg = bid / (nbj * nbk);
+  const int t = threadIdx.x;
+  const int tj = t % FT_J, tk = t / FT_J;
+  const int j0 = g.js + bj * FT_J, k0 = g.ks + bk * FT_K;
+  const int j = j0 + tj, k = k0 + tk;
+  const bool own = (j <= g.je) && (k <= g.ke);
+  const int c0 = g.is + sg * seg;
+  const int c1 = min(c0 + seg, g.ie + 1);
+  const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
+  const double wv = src.wght / (g.dx1 * g.dx2 * g.dx3);
+  const int jc = min(j, g.je), kc = min(k, g.ke);  // lanes past the block
+  const long long pc = g.pcol(kc, jc) * g.ps_c;
+  // x1 window (own column)
+  float q[N][6];
+  float rdv[6];
+  auto load1 = [&](int m, int i) {
+    double wc[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, kc, jc, i)];
+#pragma unroll
+    for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
+    if (decomp) {
+      double r = decomp_r<NM>(th, wc);
+      rdv[m] = (float)r;
+      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)r);
+      q[IPR][m] = (float)(wc[IPR] - psv[pc + i * g.ps_i]);
+    }
+  };
+  auto face1 = [&](int f, float* fx) {
+    float wl[N], wr[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) recon_pair_f<OV>(q[n], &wl[n], &wr[n]);
+    float pb = 0.f;
+    if (decomp) {
+      pb = (float)psf[pc + f * g.ps_i];
+      float gam = fsqrt(rdv[3] * rdv[2]);
+      if (wr[IPR] + pb < 0.f) wr[IPR] = 0.f;
+      if (wl[IPR] + pb < 0.f) wl[IPR] = 0.f;
+      wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
+      wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
+    }
+    riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
+  };
+#pragma unroll
+  for (int m = LO; m <= HI; ++m) load1(m, c0 - 3 + m);
+  float fm[N];
+  face1(c0, fm);
+  double pfm = psf[pc + c0 * g.ps_i];
+
+  auto recon = [&](float a, float b, float c, float d, float e, float* qm,
+                   float* qp) {
+    if (OH == 5) {
+      weno5_both_f(a, b, c, d, e, qm, qp);
+    } else {
+      *qp = plm_up_f(b, c, d);
+      *qm = plm_up_f(d, c, b);
+    }
+  };
+  for (int i = c0; i < c1; ++i) {
+    const double prd = pref[i];
+    // the own column's next x1 window cell: its loads are issued here so
+    // that they complete during the plane phases
+    const int inew = i + 1 - 3 + HI;
+    double wn[N], psvn = 0.0;
+#pragma unroll
+    for (int n = 0; n < N; ++n) wn[n] = w[g.at(n, kc, jc, inew)];
+    if (decomp) psvn = psv[pc + inew * g.ps_i];
+    const double pfp = psf[pc + (i + 1) * g.ps_i];
+    // (1) the level-i plane of the tile with its halo
+    for (int e = t; e < NP; e += NT) {
+      const int ej = e % FT_PJ, ek = e / FT_PJ;
+      const int gj = min(max(j0 - FT_H + ej, 0), g.nc2 - 1);
+      const int gk = min(max(k0 - FT_H + ek, 0), g.nc3 - 1);
+      const long long c = g.at(0, gk, gj, i);
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        double v = w[c + n * g.c.n];
+        if (n == IPR) v -= prd;
+        plane[n][ek][ej] = (float)v;
+      }
+    }
+    __syncthreads();
+    // (2) face states: x2 for cells j0-1 .. j0+FT_J of the interior rows,
+    //     x3 for cells k0-1 .. k0+FT_K of the interior columns
+    for (int e = t; e < FT_K * C2; e += NT) {
+      const int cj = e % C2, rk = e / C2;  // cell j0 - 1 + cj, row k0 + rk
+      const int pj = cj + FT_H - 1, pk = rk + FT_H;
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        float qm, qp;
+        recon(plane[n][pk][pj - 2], plane[n][pk][pj - 1], plane[n][pk][pj],
+              plane[n][pk][pj + 1], plane[n][pk][pj + 2], &qm, &qp);
+        q2m[n][rk][cj] = qm;
+        q2p[n][rk][cj] = qp;
+      }
+    }
+    for (int e = t; e < C3 * FT_J; e += NT) {
+      const int cj = e % FT_J, ck = e / FT_J;  // column j0 + cj, cell k0 - 1 + ck
+      const int pj = cj + FT_H, pk = ck + FT_H - 1;
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        float qm, qp;
+        recon(plane[n][pk - 2][pj], plane[n][pk - 1][pj], plane[n][pk][pj],
+              plane[n][pk + 1][pj], plane[n][pk + 2][pj], &qm, &qp);
+        q3m[n][ck][cj] = qm;
+        q3p[n][ck][cj] = qp;
+      }
+    }
+    __syncthreads();
+    // (3) x2 / x3 Riemann fluxes of the tile's faces
+    for (int e = t; e < FT_K * (FT_J + 1); e += NT) {
+      const int fj = e % (FT_J + 1), rk = e / (FT_J + 1);  // face j0 + fj
+      float wl[N], wr[N], fx[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        wl[n] = q2p[n][rk][fj];
+        wr[n] = q2m[n][rk][fj + 1];
+      }
+      riemann_p<NM, 1>(th, rsolver, wl, wr, (float)prd, fx);
+#pragma unroll
+      for (int n = 0; n < N; ++n) fx2[n][rk][fj] = fx[n];
+    }
+    for (int e = t; e < (FT_K + 1) * FT_J; e += NT) {
+      const int cj = e % FT_J, fk = e / FT_J;  // face k0 + fk
+      float wl[N], wr[N], fx[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        wl[n] = q3p[n][fk][cj];
+        wr[n] = q3m[n][fk + 1][cj];
+      }
+      riemann_p<NM, 2>(th, rsolver, wl, wr, (float)prd, fx);
+#pragma unroll
+      for (int n = 0; n < N; ++n) fx3[n][fk][cj] = fx[n];
+    }
+    __syncthreads();
+    // (4) x1 face i+1 in registers, the cell's divergence and sources
+#pragma unroll
+    for (int m = LO; m < HI; ++m) {
+#pragma unroll
+      for (int n = 0; n < N; ++n) q[n][m] = q[n][m + 1];
+      rdv[m] = rdv[m + 1];
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) q[n][HI] = (float)wn[n];
+    if (decomp) {
+      double r = decomp_r<NM>(th, wn);
+      rdv[HI] = (float)r;
+      q[0][HI] = fdiv((float)wn[IPR], (float)wn[0] * (float)r);
+      q[IPR][HI] = (float)(wn[IPR] - psvn);
+    }
+    float fp[N];
+    face1(i + 1, fp);
+    if (own) {
+      double d[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        double dflx = A1 * ((double)fp[n] - (double)fm[n]);
+        if (n == IVX && decomp) dflx += A1 * (pfp - pfm);
+        dflx += A2 * ((double)fx2[n][tk][tj + 1] - (double)fx2[n][tk][tj]);
+        dflx += A3 * ((double)fx3[n][tk + 1][tj] - (double)fx3[n][tk][tj]);
+        d[n] = 0.0 - wv * dflx;
+      }
+      add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+#pragma unroll
+      for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) fm[n] = fp[n];
+    pfm = pfp;
+  }
+}
+
 // Face kernels of the mixed form.  variant 0: one lane per face in every
 // direction (k_flux32_h, k_flux32_x1) and a separate divergence kernel
 // (k_divergence32).  variants 1 and 2: x2 faces by cross-lane WENO
 // (k_mix_x2), x3 faces marched over segments of `seg` faces (k_mix_x3);
 // x1 faces then either marched up column segments of `seg` cells fused with
 // the divergence (k_mix_x1_div, variant 1) or one lane per face followed by
-// the divergence kernel (variant 2).
+// the divergence kernel (variant 2).  variants 3 and 4 (3D blocks): every
+// face and the divergence in one tile kernel (k_mix_fused, 32 x 8 or 16 x
+// 16 columns); 2D blocks fall back to variant 2.
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* ff,
@@ -691,6 +895,28 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
               g, th, src, f1, f2, f3, psf, decomp, w, du)));
     return;
   }
+  if ((variant == 3 || variant == 4) && g.f2 && g.f3) {
+    const int tjs = (variant == 3) ? 32 : 16, tks = (variant == 3) ? 8 : 16;
+    const long long nb = (long long)((g.nx2 + tjs - 1) / tjs) *
+                         ((g.nx3 + tks - 1) / tks) * ((g.nx1 + seg - 1) / seg);
+#define CG_FUSED(OH, OV, TJ, TK)                                               \
+  TIMED(tm, KID_FLUX_X1, (k_mix_fused<NM, OH, OV, TJ, TK><<<(unsigned)nb, 256, 0, s>>>( \
+                             g, th, src, cfg.rsolver, decomp, seg, w, psf, psv, pref, du)))
+#define CG_FUSED_T(TJ, TK)                       \
+  do {                                           \
+    if (w5h && w5v) CG_FUSED(5, 5, TJ, TK);      \
+    else if (w5h) CG_FUSED(5, 2, TJ, TK);        \
+    else if (w5v) CG_FUSED(2, 5, TJ, TK);        \
+    else CG_FUSED(2, 2, TJ, TK);                 \
+  } while (0)
+    if (variant == 3)
+      CG_FUSED_T(32, 8);
+    else
+      CG_FUSED_T(16, 16);
+#undef CG_FUSED_T
+#undef CG_FUSED
+    return;
+  }
   if (g.f2) {
     // one wavefront per 59 faces of an x2 row
     const long long nw = (long long)g.nx1 * g.nx3 * ((g.nx2 + 1 + 58) / 59);
@@ -712,7 +938,7 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
       TIMED(tm, KID_FLUX_X3, (k_mix_x3<NM, 2><<<b3, bs, 0, s>>>(
                                  g, th, cfg.rsolver, seg, w, pref, f3)));
   }
-  if (variant == 2) {
+  if (variant >= 2) {
     const unsigned b1 = (unsigned)((ncol * (g.nx1 + 1) + bs - 1) / bs);
     if (w5v)
       TIMED(tm, KID_FLUX_X1, (k_flux32_x1<NM, 5><<<b1, bs, 0, s>>>(

@@ -338,7 +338,7 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   r->tune.implicit_wpb = 1;
   r->tune.implicit_grid = 0;
   r->tune.flux_block = 256;
-  r->tune.face_variant = 2;
+  r->tune.face_variant = 3;
   r->tune.march_seg = 16;
   r->tune.step_ms = 0.0;
   hipDeviceProp_t prop;
@@ -549,11 +549,11 @@ int cg_run_set_tune(cg_run* r, const cg_tune* t) {
   if ((t->implicit_wpb != 1 && t->implicit_wpb != 2 && t->implicit_wpb != 4) ||
       t->implicit_grid < 0 ||
       (t->flux_block != 64 && t->flux_block != 128 && t->flux_block != 256) ||
-      t->face_variant < 0 || t->face_variant > 2 || t->march_seg < 1 ||
+      t->face_variant < 0 || t->face_variant > 4 || t->march_seg < 1 ||
       t->march_seg > 4096)
     return set_err(r->ctx, CG_ERR_ARG,
                    "implicit_wpb in {1,2,4}, implicit_grid >= 0, flux_block in "
-                   "{64,128,256}, face_variant in {0,1,2}, march_seg in [1,4096]");
+                   "{64,128,256}, face_variant in 0..4, march_seg in [1,4096]");
   if (!(r->cfg.implicit_flag & 8) && t->implicit_wpb != 1)
     return set_err(r->ctx, CG_ERR_ARG, "3x3 solve runs with implicit_wpb 1");
   r->tune = *t;
@@ -681,9 +681,10 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
   // face kernels of the mixed form: per-face lanes, or the cell-centric
   // kernels with march segments of 8, 16, 32 cells
   if (r->cfg.precision == CG_PREC_FP32) {
-    const int vs[7][2] = {{0, 16}, {1, 8}, {1, 16}, {1, 32},
-                          {2, 8}, {2, 16}, {2, 32}};
-    for (int vi = 0; vi < 7 && !err; ++vi) {
+    const int vs[13][2] = {{0, 16}, {1, 8},  {1, 16}, {1, 32}, {2, 8},
+                           {2, 16}, {2, 32}, {3, 16}, {3, 32}, {3, 64},
+                           {4, 16}, {4, 32}, {4, 64}};
+    for (int vi = 0; vi < 13 && !err; ++vi) {
       cg_tune t = best;
       t.face_variant = vs[vi][0];
       t.march_seg = vs[vi][1];

@@ -274,7 +274,10 @@ int cg_run_block_sums(cg_run* run, int block, double* out_host);
  * kernel variant (0: one lane per face in every direction; 1: one WENO
  * evaluation per cell, x2 across lanes, x3 and x1 marched over segments of
  * march_seg cells, x1 fused with the divergence; 2: as 1 for x2 / x3, one
- * lane per x1 face and a divergence kernel); step_ms is the step time
+ * lane per x1 face and a divergence kernel; 3 / 4: all faces and the
+ * divergence in one LDS tile kernel of 32 x 8 / 16 x 16 columns marching up
+ * segments of march_seg levels, 3D blocks only, else as 2); step_ms is the
+ * step time
  * measured by the autotuner.  No counterpart in the reference (CPU loop
  * nests). */
 typedef struct cg_tune {

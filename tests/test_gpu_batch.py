@@ -78,7 +78,8 @@ def test_batch_equals_single_block_runs(ctx):
 
 @pytest.mark.parametrize("variant", [(1, 256, 0, 16), (1, 256, 1, 16),
                                      (1, 128, 1, 3), (1, 64, 1, 40),
-                                     (1, 256, 2, 5)])
+                                     (1, 256, 2, 5), (1, 256, 3, 7),
+                                     (1, 256, 3, 64), (1, 256, 4, 9)])
 def test_batch_mixed_close_to_fp64(ctx, variant):
     """Both face-kernel variants of the FP32-flux form, several march
     segments (shorter than, and longer than, the columns)."""
@@ -108,7 +109,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape(ctx):
         t = run.autotune(dt)
         _say("  autotune: %s" % t.as_dict())
         assert t.implicit_wpb in (1, 2, 4) and t.flux_block in (64, 128, 256)
-        assert t.face_variant in (0, 1, 2) and t.march_seg >= 1
+        assert t.face_variant in range(5) and t.march_seg >= 1
         assert t.step_ms > 0
         for i in range(len(bl)):
             np.testing.assert_array_equal(run.cons(i), u0[i])
