@@ -112,6 +112,33 @@ def time_form(ctx, blocks, th, cfg, w0s, dt, steps, warmup, net=None,
         run.close()
 
 
+def main_loop(ctx, blk, th, cfg, w0, cfl, cycles, warmup, tune=None):
+    """The device main loop (cg_run_advance): per cycle one step, the
+    NewBlockTimeStep reduction over the resident state and Mesh::NewTimeStep
+    on the host from one read-back double.  ms per cycle, the reduction
+    kernel's average duration, and the steps taken."""
+    run = make_run(ctx, [blk], th, cfg, [w0], tune=tune)
+    try:
+        t, dt, n = run.advance(cfl, nlim=warmup, graph=USE_GRAPH)
+        t0 = time.perf_counter()
+        t, dt, n = run.advance(cfl, nlim=warmup + cycles, time=t, dt=dt,
+                               ncycle=n, graph=USE_GRAPH)
+        t1 = time.perf_counter()
+        run.set_timing(True)
+        run.new_dt(cfl)
+        ms_dt = run.kernel_stats().get("k_new_dt", (0.0, 1))
+        run.set_timing(False)
+        return {"cfl": cfl, "cycles": cycles,
+                "ms_per_cycle": round(1e3 * (t1 - t0) / cycles, 4),
+                "k_new_dt_ms": round(ms_dt[0] / max(ms_dt[1], 1), 4),
+                "k_new_dt_GBs": round(blk.N * 8 * blk.ncells()
+                                      / (ms_dt[0] / max(ms_dt[1], 1) * 1e-3)
+                                      / 1e9, 1) if ms_dt[0] > 0 else None,
+                "last_dt": dt, "time": t}
+    finally:
+        run.close()
+
+
 def final_state(ctx, blocks, th, cfg, w0s, dt, nsteps, net=None, tune=None):
     run = make_run(ctx, blocks, th, cfg, w0s, net, tune)
     try:
@@ -277,6 +304,13 @@ def bench_box(ctx, nx, steps, warmup, acc_steps):
         forms["plain_fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
     log("mixed vs FP64 after %d steps: max rel L2 %.3e"
         % (acc_steps, acc["max_rel_l2"]))
+    # the same form driven by the device main loop: per-direction CFL 0.45
+    # gives the step of the summed-direction bound at 0.9 on this box
+    cfg.precision = ca.PREC_FP32
+    forms["mixed"]["main_loop"] = main_loop(
+        ctx, blk, th, cfg, w0, 0.45, steps, warmup,
+        tune=ca.Tune(**tunes["mixed"]))
+    log("main loop: %s" % forms["mixed"]["main_loop"])
     return blk, cfg, dt, cells, forms, acc
 
 

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 
 NGHOST = 3
 MAX_NMASS = 3
-NUM_KERNEL_IDS = 14
+NUM_KERNEL_IDS = 15
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV = range(9)
@@ -134,6 +134,7 @@ _SIGS = {
     "cg_cons_to_prim": (_I, [_P, _P, _P, _P, _P]),
     "cg_prim_to_cons": (_I, [_P, _P, _P, _P, _P]),
     "cg_apply_bc": (_I, [_P, _P, _P]),
+    "cg_new_block_dt": (_I, [_P, _P, _P, _P, _P, _D, ctypes.POINTER(_D)]),
     "cg_saturation_adjust": (_I, [_P, _P, _I, _P, _P, _P, _P, _P]),
     "cg_run_create": (_I, [_P, _P, _P, _P, ctypes.POINTER(_P)]),
     "cg_run_destroy": (_I, [_P]),
@@ -141,6 +142,10 @@ _SIGS = {
     "cg_run_get_cons_host": (_I, [_P, _P]),
     "cg_run_get_prim_host": (_I, [_P, _P]),
     "cg_run_step": (_I, [_P, _D, _I, _I]),
+    "cg_run_new_dt": (_I, [_P, _D, ctypes.POINTER(_D)]),
+    "cg_run_advance": (_I, [_P, _D, _D, ctypes.c_longlong, _I,
+                            ctypes.POINTER(_D), ctypes.POINTER(_D),
+                            ctypes.POINTER(ctypes.c_longlong)]),
     "cg_run_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "cg_run_sums": (_I, [_P, _P]),
     "cg_run_set_timing": (_I, [_P, _I]),
@@ -346,6 +351,18 @@ class Hydro:
             a.free()
         return out
 
+    def new_block_dt(self, w, cfl):
+        """Hydro::NewBlockTimeStep on the primitive array w (the patched
+        form of patches/21.new_blockdt.patch): cfl times the minimum
+        signal-speed step over interior cells."""
+        dw = self.ctx.array(w)
+        out = ctypes.c_double()
+        self.ctx._check(lib().cg_new_block_dt(
+            self.ctx.h, _ref(self.blk), _ref(self.th), _ref(self.cfg), dw.ptr,
+            cfl, ctypes.byref(out)), "cg_new_block_dt")
+        dw.free()
+        return out.value
+
 
 class ImplicitSolver:
     def __init__(self, ctx, blk, th, flag=9):
@@ -410,6 +427,26 @@ class Run:
     def step(self, dt, nsteps=1, graph=True):
         self.ctx._check(lib().cg_run_step(self.h, dt, nsteps, int(graph)),
                         "cg_run_step")
+
+    def new_dt(self, cfl):
+        """Minimum over the run's blocks of NewBlockTimeStep, reduced on the
+        device from the resident state."""
+        out = ctypes.c_double()
+        self.ctx._check(lib().cg_run_new_dt(self.h, cfl, ctypes.byref(out)),
+                        "cg_run_new_dt")
+        return out.value
+
+    def advance(self, cfl, tlim=float("inf"), nlim=-1, time=0.0, dt=0.0,
+                ncycle=0, graph=True):
+        """The main loop of main.cpp:114-152 on the device (cycles until
+        time >= tlim or ncycle >= nlim, each followed by Mesh::NewTimeStep).
+        Returns (time, dt, ncycle) to continue from."""
+        t, d, n = ctypes.c_double(time), ctypes.c_double(dt), \
+            ctypes.c_longlong(ncycle)
+        self.ctx._check(lib().cg_run_advance(
+            self.h, cfl, tlim, nlim, int(graph), ctypes.byref(t),
+            ctypes.byref(d), ctypes.byref(n)), "cg_run_advance")
+        return t.value, d.value, n.value
 
     def sums(self, b=0):
         out = np.zeros(self.blocks[b].N)

@@ -9,12 +9,14 @@
 // box, 4: moist box, 5: one seeded block of the uneven-column batch).
 //
 //   canoe_step <config> [--nx nx1 nx2 nx3] [--nlim N] [--tlim T] [--cfl C]
-//              [--dt-every K] [--fp32] [--no-graph] [--seed S] [--out K]
+//              [--fp32] [--no-graph] [--seed S] [--out K]
 //
-// The time step follows the patched Hydro::NewBlockTimeStep
-// (patches/21.new_blockdt.patch:31-45: the implicit x1 direction drops the
-// sound speed) with the signal speeds summed over directions; it is
-// recomputed from the device state every K cycles (--dt-every, default 10).
+// The cycles run on the device through cg_run_advance: each step is
+// followed by the patched Hydro::NewBlockTimeStep reduced over the resident
+// state (patches/21.new_blockdt.patch:16-45) and Mesh::NewTimeStep's limits;
+// the host sees one double per cycle and prints the cycle diagnostics every
+// K cycles (--out).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -52,34 +54,10 @@ struct Dims {
   }
 };
 
-// CFL time step of a host primitive array (dry gamma for the sound speed).
-double cfl_dt(const cg_block& b, const cg_thermo& th, const cg_config& cfg,
-              const std::vector<double>& w, double cfl) {
-  Dims d(b);
-  const int ivx = 1 + b.nmass, ipr = ivx + 3;
-  const double dx[3] = {b.dx1, b.dx2, b.dx3};
-  const bool act[3] = {true, b.nx2 > 1, b.nx3 > 1};
-  double mx = 1e-30;
-  for (int k = d.ks; k < d.ks + b.nx3; ++k)
-    for (int j = d.js; j < d.js + b.nx2; ++j)
-      for (int i = d.is; i < d.is + b.nx1; ++i) {
-        double rho = w[d.at(0, k, j, i)];
-        double cs = std::sqrt(th.gammad * w[d.at(ipr, k, j, i)] / rho);
-        double s = 0.0;
-        for (int q = 0; q < 3; ++q) {
-          if (!act[q]) continue;
-          double v = std::fabs(w[d.at(ivx + q, k, j, i)]);
-          s += ((q == 0 && cfg.implicit_flag) ? v : v + cs) / dx[q];
-        }
-        mx = std::fmax(mx, s);
-      }
-  return cfl / mx;
-}
-
 void usage() {
   std::fprintf(stderr,
                "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
-               "[--nlim N] [--tlim T] [--cfl C] [--dt-every K] [--fp32] "
+               "[--nlim N] [--tlim T] [--cfl C] [--fp32] "
                "[--no-graph] [--seed S] [--out K]\n");
 }
 
@@ -99,7 +77,7 @@ int main(int argc, char** argv) {
   }
   int config = std::atoi(argv[1]);
   int nx[3] = {0, 0, 0};
-  long nlim = 10, dt_every = 10, out_every = 1;
+  long long nlim = 10, out_every = 1;
   double tlim = 1e300, cfl = 0.9;
   bool fp32 = false, graph = true;
   uint64_t seed = 1;
@@ -116,19 +94,16 @@ int main(int argc, char** argv) {
       for (int q = 0; q < 3; ++q) nx[q] = std::atoi(argv[++a]);
     } else if (s == "--nlim") {
       need(1);
-      nlim = std::atol(argv[++a]);
+      nlim = std::atoll(argv[++a]);
     } else if (s == "--tlim") {
       need(1);
       tlim = std::atof(argv[++a]);
     } else if (s == "--cfl") {
       need(1);
       cfl = std::atof(argv[++a]);
-    } else if (s == "--dt-every") {
-      need(1);
-      dt_every = std::atol(argv[++a]);
     } else if (s == "--out") {
       need(1);
-      out_every = std::atol(argv[++a]);
+      out_every = std::max(1LL, std::atoll(argv[++a]));
     } else if (s == "--seed") {
       need(1);
       seed = std::strtoull(argv[++a], nullptr, 10);
@@ -169,29 +144,26 @@ int main(int argc, char** argv) {
   std::printf("canoe_step: config %d, %d x %d x %d cells (x1 vertical), %s, %s\n",
               config, blk.nx1, blk.nx2, blk.nx3,
               fp32 ? "FP32 fluxes + FP64 state" : "FP64", cg_version());
-  double time = 0.0, dt = cfl_dt(blk, th, cfg, w, cfl);
-  long ncycle = 0;
+  double time = 0.0, dt = 0.0;
+  long long ncycle = 0;
   std::vector<double> sums(d.N), sums0(d.N);
   check(ctx, cg_run_sums(run, sums0.data()), "sums");
+  // the first step: the blocks' NewBlockTimeStep, cut to end at tlim
+  int err = check(ctx, cg_run_new_dt(run, cfl, &dt), "cg_run_new_dt");
+  if (!err && tlim - time < dt) dt = tlim - time;
   auto t0 = std::chrono::steady_clock::now();
-  int err = 0;
-  while (time < tlim && ncycle < nlim) {
-    if (ncycle % out_every == 0)
-      std::printf("cycle=%ld time=%.6e dt=%.6e\n", ncycle, time, dt);
-    double step = std::fmin(dt, tlim - time);
-    if ((err = check(ctx, cg_run_step(run, step, 1, graph ? 1 : 0), "cg_run_step")))
+  while (!err && time < tlim && ncycle < nlim) {
+    std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, time, dt);
+    const long long upto = std::min(nlim, ncycle + out_every);
+    if ((err = check(ctx, cg_run_advance(run, cfl, tlim, upto, graph ? 1 : 0, &time,
+                                         &dt, &ncycle),
+                     "cg_run_advance")))
       break;
-    ++ncycle;
-    time += step;
-    if (dt_every > 0 && ncycle % dt_every == 0) {
-      if ((err = check(ctx, cg_run_get_prim_host(run, w.data()), "get_prim"))) break;
-      dt = cfl_dt(blk, th, cfg, w, cfl);
-    }
   }
   auto t1 = std::chrono::steady_clock::now();
   double wall = std::chrono::duration<double>(t1 - t0).count();
   check(ctx, cg_run_sums(run, sums.data()), "sums");
-  std::printf("cycle=%ld time=%.6e dt=%.6e\n", ncycle, time, dt);
+  std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, time, dt);
   std::printf("history totals (initial -> final):\n");
   for (int n = 0; n < d.N; ++n)
     std::printf("  var %d: %.16e -> %.16e\n", n, sums0[n], sums[n]);

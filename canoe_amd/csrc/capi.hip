@@ -236,6 +236,32 @@ int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   return CG_OK;
 }
 
+int cg_new_block_dt(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const cg_config* cfg, const double* w, double cfl,
+                    double* dt_out) {
+  if (!ctx || !cfg || !dt_out || !(cfl > 0.0)) return CG_ERR_ARG;
+  int e = check_block(ctx, blk, th);
+  if (e) return e;
+  KGrid g = make_grid(*blk, 0);
+  KThermo kt = make_thermo(*th);
+  constexpr int parts = 256;
+  double* part = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&part, sizeof(double) * (parts + 1), ctx->stream));
+  if (blk->nmass == 0)
+    k_new_dt<0><<<parts, 256, 0, ctx->stream>>>(nullptr, g, w, kt, cfl,
+                                                 cfg->implicit_flag, part);
+  else
+    k_new_dt<3><<<parts, 256, 0, ctx->stream>>>(nullptr, g, w, kt, cfl,
+                                                 cfg->implicit_flag, part);
+  k_new_dt_finish<<<1, 256, 0, ctx->stream>>>(part, parts, cfl, part + parts);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(dt_out, part + parts, sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipFreeAsync(part, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
 int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w) {
   if (!ctx || !blk) return CG_ERR_ARG;
   KGrid g = make_grid(*blk, 0);

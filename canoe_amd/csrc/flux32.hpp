@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256)
   k += g.ks;
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
   const double vol = g.dx1 * g.dx2 * g.dx3;
-  const double wv = src.wght / vol;
+  const double wv = src.weight() / vol;
   double d[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(256)
   const int c1 = min(c0 + seg, g.ie + 1);  // cells c0 .. c1-1
   const long long pc = g.pcol(k, j) * g.ps_c;
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
-  const double wv = src.wght / (g.dx1 * g.dx2 * g.dx3);
+  const double wv = src.weight() / (g.dx1 * g.dx2 * g.dx3);
   float q[N][6];
   float rdv[6];
   auto load = [&](int m, int i) {
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(FT_J * FT_K)
   const int c0 = g.is + sg * seg;
   const int c1 = min(c0 + seg, g.ie + 1);
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
-  const double wv = src.wght / (g.dx1 * g.dx2 * g.dx3);
+  const double wv = src.weight() / (g.dx1 * g.dx2 * g.dx3);
   const int jc = min(j, g.je), kc = min(k, g.ke);  // lanes past the block
   const long long pc = g.pcol(kc, jc) * g.ps_c;
   // x1 window (own column)

@@ -223,8 +223,10 @@ CG_HD double temperature_d(const KThermo& th, const double* wc) {
 }
 
 struct KSource {
-  double wght;  // beta dt
+  double wght;                  // beta dt, or beta when dtp is set
+  const double* dtp = nullptr;  // device-resident time step of a run
   double visc, p0, cp, chi, heat_flux;
+  __device__ double weight() const { return dtp ? wght * dtp[0] : wght; }
 };
 
 // Source terms of one interior cell added to d[] in HydroSourceTerms order:
@@ -239,12 +241,12 @@ __device__ inline void add_sources_cell(const KGrid& g, const KThermo& th,
   constexpr int N = 5 + NM, IVX = 1 + NM, IVY = IVX + 1, IEN = 4 + NM;
   const long long c0 = g.at(0, k, j, i);
   if (g.grav != 0.0) {
-    double s = src.wght * w[c0] * g.grav;
+    double s = src.weight() * w[c0] * g.grav;
     d[IVX] += s;
     d[IEN] += s * w[c0 + IVX * g.c.n];
   }
   if (src.visc != 0.0 && g.f2) {
-    const double K = src.visc, dx = g.dx1, dy = g.dx2, dt = src.wght;
+    const double K = src.visc, dx = g.dx1, dy = g.dx2, dt = src.weight();
     auto W = [&](int n, int jj, int ii) { return w[g.at(n, k, jj, ii)]; };
     auto theta = [&](int jj, int ii, double* tmp) {
       double wc[N];
@@ -269,7 +271,7 @@ __device__ inline void add_sources_cell(const KGrid& g, const KThermo& th,
               (tjp + tjm + tip + tim - 4. * thc);
   }
   if (src.heat_flux != 0.0 && i == g.is)
-    d[IEN] += src.wght * src.heat_flux / g.dx1;
+    d[IEN] += src.weight() * src.heat_flux / g.dx1;
 }
 
 // K4a: du = 0 - wght div F (Hydro::AddFluxDivergence) plus the sources.
@@ -289,6 +291,7 @@ __global__ void __launch_bounds__(256)
   k += g.ks;
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
   const double vol = g.dx1 * g.dx2 * g.dx3;
+  const double wght = src.weight();
   double d[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
@@ -297,7 +300,7 @@ __global__ void __launch_bounds__(256)
       dflx += A2 * f2[g.f[1].at(n, k, j + 1, i)] - A2 * f2[g.f[1].at(n, k, j, i)];
     if (g.f3)
       dflx += A3 * f3[g.f[2].at(n, k + 1, j, i)] - A3 * f3[g.f[2].at(n, k, j, i)];
-    d[n] = 0.0 - src.wght * dflx / vol;
+    d[n] = 0.0 - wght * dflx / vol;
   }
   add_sources_cell<NM>(g, th, src, w, k, j, i, d);
 #pragma unroll
@@ -1161,8 +1164,10 @@ template <int NM, int B, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
     k_implicit_fwd_batch(const KBlockPtr* __restrict__ bp,
                          const KGroup* __restrict__ grp, int ngroups,
-                         KThermo th, double dt, double* __restrict__ arch,
-                         int* __restrict__ status) {
+                         KThermo th, double dt, const double* __restrict__ dtp,
+                         double* __restrict__ arch, int* __restrict__ status) {
+  // dt is beta dt, or beta times the run's device time step dtp[0]
+  if (dtp) dt = dt * dtp[0];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int q = blockIdx.x * WPB + wv; q < ngroups; q += gridDim.x * WPB) {
@@ -1369,6 +1374,84 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) part[(long long)n * gridDim.x + blockIdx.x] = sh[0];
     __syncthreads();
   }
+}
+
+// Hydro::NewBlockTimeStep (patched by patches/21.new_blockdt.patch:16-45):
+// the step of one interior cell, min over the active directions of
+// dx / (|v| + c_s), or dx / (cfl |v|) in a direction solved implicitly
+// (x1 only when nx2 > 1).  The minimum over cells times cfl is the block's
+// step; minima are exact, so any reduction order gives the same double.
+template <int NM>
+__device__ inline double cell_new_dt(const KGrid& g, const KThermo& th,
+                                     const double* __restrict__ w, long long c0,
+                                     double cfl, int iflag) {
+  constexpr int N = 5 + NM, IVX = 1 + NM;
+  double wi[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) wi[n] = w[c0 + n * g.c.n];
+  const double cs = sound_speed<NM>(th, wi);
+  const double dxs[3] = {g.dx1, g.dx2, g.dx3};
+  const bool act[3] = {true, g.f2 != 0, g.f3 != 0};
+  const bool imp[3] = {(iflag & 1) && g.f2, (iflag & 2) != 0, (iflag & 4) != 0};
+  double m = __DBL_MAX__;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!act[d]) continue;
+    const double v = fabs(wi[IVX + d]);
+    const double dtd = imp[d] ? dxs[d] / (cfl * v) : dxs[d] / (v + cs);
+    m = fmin(m, dtd);
+  }
+  return m;
+}
+
+// Minimum of block-wide minima through LDS: one value per workgroup.
+__device__ inline double wg_min256(double m, double* sh) {
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Per-workgroup minima of the cell steps.  Blocks of a run come from the
+// table bp (blockIdx.y = block, part[blockIdx.y gridDim.x + blockIdx.x]);
+// with bp null, the single block (g0, w0) of a caller array.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_new_dt(const KBlockPtr* __restrict__ bp, KGrid g0,
+             const double* __restrict__ w0, KThermo th, double cfl, int iflag,
+             double* __restrict__ part) {
+  const KGrid g = bp ? bp[blockIdx.y].g : g0;
+  const double* w = bp ? bp[blockIdx.y].w : w0;
+  const long long total = (long long)g.nx1 * g.nx2 * g.nx3;
+  double m = __DBL_MAX__;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+    m = fmin(m, cell_new_dt<NM>(g, th, w, g.at(0, k + g.ks, j + g.js, i + g.is), cfl, iflag));
+  }
+  __shared__ double sh[256];
+  m = wg_min256(m, sh);
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = m;
+}
+
+// out[0] = cfl times the minimum of part[0 .. n-1] (one workgroup of 256).
+static __global__ void __launch_bounds__(256)
+    k_new_dt_finish(const double* __restrict__ part, int n, double cfl,
+                    double* __restrict__ out) {
+  __shared__ double sh[256];
+  double m = __DBL_MAX__;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmin(m, part[q]);
+  m = wg_min256(m, sh);
+  if (threadIdx.x == 0) out[0] = cfl * m;
+}
+
+// out[0] = v (a device scalar written in stream order).
+static __global__ void k_set_scalar(double* __restrict__ out, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = v;
 }
 
 }  // namespace cg

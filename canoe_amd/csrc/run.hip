@@ -56,7 +56,12 @@ struct cg_run {
   const cg_mlp* mlp = nullptr;  // network of the guided adjustment
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  double graph_dt = 0.0;
+  // d_dt[0]: the step every stage kernel reads (a graph replays with any
+  // step); d_dt[1]: the last NewBlockTimeStep minimum; h_dt: pinned copy.
+  double* d_dt = nullptr;
+  double* h_dt = nullptr;
+  double* dt_part = nullptr;  // per-workgroup minima, dt_parts per block
+  int dt_parts = 1;
   KTimer tm;
 };
 
@@ -84,41 +89,44 @@ unsigned implicit_grid(const cg_run* r) {
   return (unsigned)std::max(1LL, std::min(g, full));
 }
 
+// beta: the stage weight; the sweep's step is beta times the device step.
 template <int NM, int B, int WPB>
-void launch_col_batch(cg_run* r, double dt, const KUpdate& up, KTimer* tm) {
+void launch_col_batch(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   hipStream_t s = r->ctx->stream;
   const unsigned gr = implicit_grid(r);
   TIMED(tm, KID_IMPLICIT_FWD,
         (k_implicit_fwd_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
-            r->d_bp, r->d_grp, r->ngroups, r->kt, dt, r->arch, r->ctx->d_status)));
+            r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, r->arch,
+            r->ctx->d_status)));
   TIMED(tm, KID_IMPLICIT_BWD,
         (k_implicit_bwd_update_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
             r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
 }
 
 template <int NM>
-void launch_columns(cg_run* r, double dt, const KUpdate& up, KTimer* tm) {
+void launch_columns(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   if (r->cfg.implicit_flag & 8) {
     if (r->tune.implicit_wpb == 4)
-      launch_col_batch<NM, 5, 4>(r, dt, up, tm);
+      launch_col_batch<NM, 5, 4>(r, beta, up, tm);
     else if (r->tune.implicit_wpb == 2)
-      launch_col_batch<NM, 5, 2>(r, dt, up, tm);
+      launch_col_batch<NM, 5, 2>(r, beta, up, tm);
     else
-      launch_col_batch<NM, 5, 1>(r, dt, up, tm);
+      launch_col_batch<NM, 5, 1>(r, beta, up, tm);
   } else {
-    launch_col_batch<NM, 3, 1>(r, dt, up, tm);
+    launch_col_batch<NM, 3, 1>(r, beta, up, tm);
   }
 }
 
+// Stage s of a step whose size the kernels read from r->d_dt[0].
 template <int NM>
-int run_stage(cg_run* r, double dt, int s) {
+int run_stage(cg_run* r, int s) {
   const double sbeta[3] = {1.0, 0.25, 2. / 3.};
   const double sg1[3] = {0.0, 0.25, 2. / 3.};
   const double sg2[3] = {1.0, 0.75, 1. / 3.};
   hipStream_t st = r->ctx->stream;
-  const double wght = sbeta[s] * dt;
   KSource src;
-  src.wght = wght;
+  src.wght = sbeta[s];
+  src.dtp = r->d_dt;
   src.visc = r->cfg.visc;
   src.p0 = r->cfg.p0;
   src.cp = r->th.gammad / (r->th.gammad - 1.) * r->th.rd;
@@ -152,7 +160,7 @@ int run_stage(cg_run* r, double dt, int s) {
     }
   }
   if (imp) {
-    launch_columns<NM>(r, wght, up, tm);
+    launch_columns<NM>(r, sbeta[s], up, tm);
     for (RunBlock& b : r->blocks) {
       const KGrid& g = b.g;
       if (guided) {
@@ -191,9 +199,9 @@ int run_stage(cg_run* r, double dt, int s) {
   return CG_OK;
 }
 
-int run_one_step(cg_run* r, double dt) {
+int run_one_step(cg_run* r) {
   for (int s = 0; s < 3; ++s) {
-    int e = (r->N == 5) ? run_stage<0>(r, dt, s) : run_stage<3>(r, dt, s);
+    int e = (r->N == 5) ? run_stage<0>(r, s) : run_stage<3>(r, s);
     if (e) return e;
   }
   return CG_OK;
@@ -295,11 +303,11 @@ void prime_block(cg_run* r, RunBlock& b) {
     k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, r->kt, b.w, b.psf, b.psv);
 }
 
-int capture(cg_run* r, double dt) {
+int capture(cg_run* r) {
   cg_ctx* ctx = r->ctx;
   drop_graph(r);
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int e = run_one_step(r, dt);
+  int e = run_one_step(r);
   hipGraph_t gr = nullptr;
   hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
   if (e) {
@@ -310,7 +318,52 @@ int capture(cg_run* r, double dt) {
     return set_err(ctx, CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
   r->graph = gr;
   HIPCHK(ctx, hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0));
-  r->graph_dt = dt;
+  return CG_OK;
+}
+
+// Mesh::NewTimeStep (upstream athenapp mesh/mesh.cpp): at most twice the
+// old step, at most the blocks' minimum, cut to end exactly at tlim.
+double mesh_new_dt(double dt_old, double blocks_dt, double time, double tlim) {
+  double dt = std::fmin(2.0 * dt_old, blocks_dt);
+  if (time < tlim && tlim - time < dt) dt = tlim - time;
+  return dt;
+}
+
+// Enqueues the step size every stage kernel reads.
+int set_step(cg_run* r, double dt) {
+  k_set_scalar<<<1, 64, 0, r->ctx->stream>>>(r->d_dt, dt);
+  HIPCHK(r->ctx, hipGetLastError());
+  return CG_OK;
+}
+
+// Enqueues Hydro::NewBlockTimeStep over every block of the run (one launch
+// over the block table) and the minimum over blocks into d_dt[1].
+int enqueue_new_dt(cg_run* r, double cfl) {
+  hipStream_t s = r->ctx->stream;
+  const int nb = (int)r->blocks.size();
+  dim3 grid(r->dt_parts, nb);
+  KGrid g0 = r->blocks[0].g;
+  KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  if (r->N == 5)
+    TIMED(tm, KID_NEW_DT,
+          (k_new_dt<0><<<grid, 256, 0, s>>>(r->d_bp, g0, nullptr, r->kt, cfl,
+                                            r->cfg.implicit_flag, r->dt_part)));
+  else
+    TIMED(tm, KID_NEW_DT,
+          (k_new_dt<3><<<grid, 256, 0, s>>>(r->d_bp, g0, nullptr, r->kt, cfl,
+                                            r->cfg.implicit_flag, r->dt_part)));
+  k_new_dt_finish<<<1, 256, 0, s>>>(r->dt_part, r->dt_parts * nb, cfl, r->d_dt + 1);
+  HIPCHK(r->ctx, hipGetLastError());
+  return CG_OK;
+}
+
+// Reads d_dt[1] (and the status word) after the queued work.
+int read_new_dt(cg_run* r, double* out) {
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipMemcpyAsync(r->h_dt, r->d_dt + 1, sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = r->h_dt[0];
   return CG_OK;
 }
 
@@ -353,11 +406,19 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
     rb.ga = make_grid(blks[b], 0);
     ok = alloc_block(rb, r->N, *cfg);
   }
-  ok = ok && hipMalloc((void**)&r->part, sizeof(double) * 1024 * 8) == hipSuccess;
-  if (ok && cfg->implicit_flag) {
-    std::vector<KGroup> groups;
-    size_t arch_n = 0;
-    build_groups(r->blocks, r->AS, groups, arch_n);
+  // workgroups per block of the time-step reduction: about 16 cells per
+  // thread in the largest block, at most 1024
+  long long maxcells = 1;
+  for (const RunBlock& rb : r->blocks)
+    maxcells = std::max(maxcells, (long long)rb.g.nx1 * rb.g.nx2 * rb.g.nx3);
+  r->dt_parts = (int)std::min(1024LL, std::max(1LL, maxcells / (256 * 16)));
+  ok = ok && hipMalloc((void**)&r->part, sizeof(double) * 1024 * 8) == hipSuccess &&
+       hipMalloc((void**)&r->d_dt, sizeof(double) * 2) == hipSuccess &&
+       hipMemset(r->d_dt, 0, sizeof(double) * 2) == hipSuccess &&
+       hipHostMalloc((void**)&r->h_dt, sizeof(double) * 2, hipHostMallocDefault) ==
+           hipSuccess &&
+       hipMalloc((void**)&r->dt_part, sizeof(double) * r->dt_parts * nblocks) == hipSuccess;
+  if (ok) {
     std::vector<KBlockPtr> bp(nblocks);
     for (int b = 0; b < nblocks; ++b) {
       const RunBlock& rb = r->blocks[b];
@@ -369,14 +430,19 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
       bp[b].psf = rb.psf;
       bp[b].psv = rb.psv;
     }
+    ok = hipMalloc((void**)&r->d_bp, sizeof(KBlockPtr) * nblocks) == hipSuccess &&
+         hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
+                   hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok && cfg->implicit_flag) {
+    std::vector<KGroup> groups;
+    size_t arch_n = 0;
+    build_groups(r->blocks, r->AS, groups, arch_n);
     r->ngroups = (int)groups.size();
     ok = hipMalloc((void**)&r->arch, sizeof(double) * std::max<size_t>(arch_n, 1)) ==
              hipSuccess &&
-         hipMalloc((void**)&r->d_bp, sizeof(KBlockPtr) * nblocks) == hipSuccess &&
          hipMalloc((void**)&r->d_grp, sizeof(KGroup) * std::max(r->ngroups, 1)) ==
              hipSuccess &&
-         hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
-                   hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(r->d_grp, groups.data(), sizeof(KGroup) * groups.size(),
                    hipMemcpyHostToDevice) == hipSuccess;
   }
@@ -405,6 +471,9 @@ int cg_run_destroy(cg_run* r) {
   if (r->d_bp) (void)hipFree(r->d_bp);
   if (r->d_grp) (void)hipFree(r->d_grp);
   if (r->part) (void)hipFree(r->part);
+  if (r->d_dt) (void)hipFree(r->d_dt);
+  if (r->dt_part) (void)hipFree(r->dt_part);
+  if (r->h_dt) (void)hipHostFree(r->h_dt);
   if (r->tm.e0) (void)hipEventDestroy(r->tm.e0);
   if (r->tm.e1) (void)hipEventDestroy(r->tm.e1);
   delete r;
@@ -488,21 +557,55 @@ int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
   if (!(dt > 0.0) || nsteps < 0) return set_err(r->ctx, CG_ERR_ARG, "bad dt/nsteps");
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  int e = set_step(r, dt);
+  if (e) return e;
   if (use_graph && !r->tm.on) {
-    if (!r->gexec || r->graph_dt != dt) {
-      int e = capture(r, dt);
-      if (e) return e;
-    }
+    if (!r->gexec && (e = capture(r))) return e;
     for (int n = 0; n < nsteps; ++n) HIPCHK(ctx, hipGraphLaunch(r->gexec, ctx->stream));
   } else {
-    for (int n = 0; n < nsteps; ++n) {
-      int e = run_one_step(r, dt);
-      if (e) return e;
-    }
+    for (int n = 0; n < nsteps; ++n)
+      if ((e = run_one_step(r))) return e;
   }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_status(ctx);
+}
+
+int cg_run_new_dt(cg_run* r, double cfl, double* dt_out) {
+  if (!r || !dt_out || !(cfl > 0.0)) return CG_ERR_ARG;
+  HIPCHK(r->ctx, hipSetDevice(r->ctx->device));
+  int e = enqueue_new_dt(r, cfl);
+  if (!e) e = read_new_dt(r, dt_out);
+  return e;
+}
+
+int cg_run_advance(cg_run* r, double cfl, double tlim, long long nlim,
+                   int use_graph, double* time, double* dt, long long* ncycle) {
+  if (!r || !time || !dt || !ncycle || !(cfl > 0.0)) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int e = CG_OK;
+  double bdt = 0.0;
+  if (!(*dt > 0.0)) {  // first step: the blocks' minimum, cut at tlim
+    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt))) return e;
+    *dt = mesh_new_dt(__DBL_MAX__, bdt, *time, tlim);
+  }
+  if (use_graph && !r->tm.on && !r->gexec && (e = capture(r))) return e;
+  while (*time < tlim && (nlim < 0 || *ncycle < nlim)) {
+    if (!(*dt > 0.0)) return set_err(ctx, CG_ERR_ARG, "time step %g is not positive", *dt);
+    if ((e = set_step(r, *dt))) return e;
+    if (use_graph && !r->tm.on)
+      HIPCHK(ctx, hipGraphLaunch(r->gexec, ctx->stream));
+    else if ((e = run_one_step(r)))
+      return e;
+    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt)) ||
+        (e = check_status(ctx)))
+      return e;
+    *ncycle += 1;
+    *time += *dt;
+    *dt = mesh_new_dt(*dt, bdt, *time, tlim);
+  }
+  return CG_OK;
 }
 
 int cg_run_block_sums(cg_run* r, int b, double* out_host) {
@@ -622,10 +725,10 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     release();
     return set_err(ctx, CG_ERR_HIP, "autotune: events");
   }
-  int err = CG_OK;
+  int err = set_step(r, dt);
   auto trial = [&](const cg_tune& t, double* ms) -> int {
     r->tune = t;
-    int e = capture(r, dt);
+    int e = capture(r);
     if (e) return e;
     if (hipGraphLaunch(r->gexec, ctx->stream) != hipSuccess) return CG_ERR_HIP;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return CG_ERR_HIP;
@@ -658,6 +761,7 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     }
   }
   for (const cg_tune& t : cands) {
+    if (err) break;
     double ms = 0.0;
     err = trial(t, &ms);
     if (err) break;

@@ -160,6 +160,18 @@ int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
  * outflow), as the upstream BoundaryValues do after CONS2PRIM. */
 int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w);
 
+/* ---- time step ------------------------------------------------------------ *
+ * Replaces void Hydro::NewBlockTimeStep() (upstream athenapp
+ * hydro/new_blockdt.cpp as patched by patches/21.new_blockdt.patch:16-45)
+ * on a caller primitive array w: cfl times the minimum over interior cells
+ * and active directions of dx / (|v| + c_s) (moist sound speed,
+ * src/snap/eos/ideal_moist_hydro.cpp:87-91), or dx / (cfl |v|) in a
+ * direction cfg->implicit_flag solves implicitly (x1 only when nx2 > 1).
+ * dt_out: host double. */
+int cg_new_block_dt(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const cg_config* cfg, const double* w, double cfl,
+                    double* dt_out);
+
 /* ---- saturation-guess network -------------------------------------------- *
  * No counterpart in the reference (its adjustment, kintera ThermoY::forward
  * called at src/tasklist/implicit_hydro_tasks.cpp:308, always starts from the
@@ -234,8 +246,22 @@ int cg_run_destroy(cg_run* run);
 int cg_run_set_prim_host(cg_run* run, const double* w_host);
 int cg_run_get_cons_host(cg_run* run, double* u_host);
 int cg_run_get_prim_host(cg_run* run, double* w_host);
-/* nsteps steps of size dt (use_graph != 0: replay the captured graph). */
+/* nsteps steps of size dt (use_graph != 0: replay the captured graph).  The
+ * stage kernels read the step from device memory, so one captured graph
+ * serves every dt. */
 int cg_run_step(cg_run* run, double dt, int nsteps, int use_graph);
+/* The minimum of cg_new_block_dt over the run's blocks, reduced on the
+ * device from the resident state (Mesh::NewTimeStep's min over blocks). */
+int cg_run_new_dt(cg_run* run, double cfl, double* dt_out);
+/* The main integration loop of main.cpp:114-152 on the device: while
+ * *time < tlim and (nlim < 0 or *ncycle < nlim), one step of size *dt, then
+ * ++*ncycle, *time += *dt, and the next step as in Mesh::NewTimeStep
+ * (upstream athenapp): min(2 *dt, the blocks' NewBlockTimeStep), cut to end
+ * at tlim.  *dt <= 0 on entry starts from the blocks' step.  Per cycle the
+ * host reads back one double (and the status word); the state stays on the
+ * device. */
+int cg_run_advance(cg_run* run, double cfl, double tlim, long long nlim,
+                   int use_graph, double* time, double* dt, long long* ncycle);
 /* Device pointers of the run's state (for in-place use by a host code). */
 int cg_run_device_state(cg_run* run, double** u, double** w);
 /* FP64 global sums of the conserved variables over interior cells times the
@@ -246,7 +272,7 @@ int cg_run_sums(cg_run* run, double* out_host);
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of
  * launches and the kernel name of kernel id 0..CG_NUM_KERNEL_IDS-1 (ids
  * with no launches report 0).  cg_run_set_timing resets the counters. */
-#define CG_NUM_KERNEL_IDS 14
+#define CG_NUM_KERNEL_IDS 15
 int cg_run_set_timing(cg_run* run, int on);
 int cg_run_kernel_stats(cg_run* run, int kernel_id, double* total_ms,
                         long long* launches, const char** name);

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1378,6 +1379,50 @@ int step(const cg_block& blk, const cg_thermo& th, const cg_config& cfg,
   return CG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Hydro::NewBlockTimeStep (upstream athenapp hydro/new_blockdt.cpp as patched
+// by patches/21.new_blockdt.patch:16-45): per interior cell and active
+// direction d, dt_d = dx_d / (|v_d| + c_s) with the moist sound speed of
+// ideal_moist_hydro.cpp:87-91; a direction solved implicitly (implicit_flag
+// bit 0 for x1 when nx2 > 1, bit 1 for x2, bit 2 for x3) uses
+// dt_d = dx_d / (cfl |v_d|) instead (patch:31-45).  The block's step is cfl
+// times the minimum over cells and active directions (upstream multiplies
+// the hyperbolic minimum by cfl_number).  The `cspeed` floor of the patched
+// max(cspeed, |v| + c_s) belongs to the athenapp fork, which is not vendored:
+// taken as 0 (no configuration sets a floor).
+double new_block_dt(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                    const cg_config& cfg, const double* w, double cfl) {
+  double mind = std::numeric_limits<double>::max();
+  const double dx[3] = {blk.dx1, blk.dx2, blk.dx3};
+  const bool act[3] = {true, g.f2, g.f3};
+  const bool imp[3] = {(cfg.implicit_flag & 1) && g.f2,
+                       (cfg.implicit_flag & 2) != 0,
+                       (cfg.implicit_flag & 4) != 0};
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        double wi[CG_MAX_NMASS + 5];
+        for (int n = 0; n < g.N; ++n) wi[n] = w[g.at(n, k, j, i)];
+        const double cs = sound_speed(th, wi, g.IPR);
+        for (int d = 0; d < 3; ++d) {
+          if (!act[d]) continue;
+          const double v = std::fabs(wi[g.IVX + d]);
+          const double dtd = imp[d] ? dx[d] / (cfl * v) : dx[d] / (v + cs);
+          mind = std::fmin(mind, dtd);
+        }
+      }
+  return cfl * mind;
+}
+
+// Mesh::NewTimeStep (upstream athenapp mesh/mesh.cpp): the new step is the
+// minimum of twice the old step and every block's step, cut to end exactly at
+// tlim; the caller passes the blocks' minimum.
+double new_mesh_dt(double dt_old, double blocks_dt, double time, double tlim) {
+  double dt = std::fmin(2.0 * dt_old, blocks_dt);
+  if (time < tlim && tlim - time < dt) dt = tlim - time;
+  return dt;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -1574,6 +1619,17 @@ int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,
   for (int q = 0; q < 25; ++q) abs_a[q] = m.a[q / 5][q % 5];
   for (int n = 0; n < g.N; ++n) prim_out[n] = prim[n];
   return CG_OK;
+}
+
+double or_new_block_dt(const cg_block* blk, const cg_thermo* th,
+                       const cg_config* cfg, const double* w, double cfl) {
+  Grid g(*blk);
+  return new_block_dt(g, *blk, *th, *cfg, w, cfl);
+}
+
+double or_new_mesh_dt(double dt_old, double blocks_dt, double time,
+                      double tlim) {
+  return new_mesh_dt(dt_old, blocks_dt, time, tlim);
 }
 
 int or_sums(const cg_block* blk, const double* u, double* out) {

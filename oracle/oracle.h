@@ -68,6 +68,12 @@ int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
 int or_step_omp(const cg_block* blk, const cg_thermo* th,
                 const cg_config* cfg, double dt, int nsteps, double* u,
                 double* w, int nthreads);
+/* Hydro::NewBlockTimeStep of one block (cfl times the minimum signal-speed
+ * step over interior cells) and Mesh::NewTimeStep's limits. */
+double or_new_block_dt(const cg_block* blk, const cg_thermo* th,
+                       const cg_config* cfg, const double* w, double cfl);
+double or_new_mesh_dt(double dt_old, double blocks_dt, double time,
+                      double tlim);
 int or_sums(const cg_block* blk, const double* u, double* out);
 int or_block_thomas(int B, int n, const double* a, const double* b,
                     const double* c, const double* r, double* x);

@@ -34,6 +34,8 @@ _SIGS = {
     "or_step": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
     "or_sums": (_I, [_P, _P, _P]),
+    "or_new_block_dt": (_D, [_P, _P, _P, _P, _D]),
+    "or_new_mesh_dt": (_D, [_D, _D, _D, _D]),
     "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
     "or_flux_jacobian": (_I, [_P, _D, _P, _P]),
     "or_roe_matrices": (_I, [_P, _P, _D, _P, _P, _P, _P]),
@@ -152,6 +154,16 @@ def step(blk, th, cfg, dt, nsteps, u, w, threads=1):
                               ptr(w), threads)
     assert r == 0, r
     return u, w
+
+
+def new_block_dt(blk, th, cfg, w, cfl):
+    """Hydro::NewBlockTimeStep of the oracle (cfl times the minimum step)."""
+    w = np.ascontiguousarray(w, np.float64)
+    return lib().or_new_block_dt(ref(blk), ref(th), ref(cfg), ptr(w), cfl)
+
+
+def new_mesh_dt(dt_old, blocks_dt, time, tlim):
+    return lib().or_new_mesh_dt(dt_old, blocks_dt, time, tlim)
 
 
 def saturation_adjust(th, rho, ie, y, tguess=None):
