@@ -645,33 +645,46 @@ __global__ void __launch_bounds__(256)
 // written once: w is read from HBM about once, the face fluxes never leave
 // the chip.
 // The tile is FT_J x FT_K columns (32 x 8 or 16 x 16, both 256 lanes).
+
+// LDS of one tile of the fused face kernel.
+template <int NM, int FT_J, int FT_K>
+struct FusedLds {
+  static constexpr int H = 3, PJ = FT_J + 2 * H, PK = FT_K + 2 * H, N = 5 + NM;
+  static constexpr int C2 = FT_J + 2, C3 = FT_K + 2;  // cells with face states
+  float plane[N][PK][PJ];
+  float q2m[N][FT_K][C2], q2p[N][FT_K][C2];
+  float q3m[N][C3][FT_J], q3p[N][C3][FT_J];
+  float fx2[N][FT_K][FT_J + 1];
+  float fx3[N][FT_K + 1][FT_J];
+};
+
+// One tile: columns j0 .. j0+FT_J-1, k0 .. k0+FT_K-1 of block g, levels
+// c0 .. c1-1.
 template <int NM, int OH, int OV, int FT_J, int FT_K>
-__global__ void __launch_bounds__(FT_J * FT_K)
-    k_mix_fused(KGrid g, KThermo th, KSource src, int rsolver, int decomp,
-                int seg, const double* __restrict__ w,
-                const double* __restrict__ psf, const double* __restrict__ psv,
-                const double* __restrict__ pref, double* __restrict__ du) {
+__device__ __forceinline__ void mix_fused_tile(
+    const KGrid& g, const KThermo& th, const KSource& src, int rsolver, int j0,
+    int k0, int c0, int c1, const double* __restrict__ w,
+    const double* __restrict__ psf, const double* __restrict__ psv,
+    const double* __restrict__ pref, double* __restrict__ du,
+    FusedLds<NM, FT_J, FT_K>& L) {
   constexpr int FT_H = 3;
   constexpr int FT_PJ = FT_J + 2 * FT_H, FT_PK = FT_K + 2 * FT_H;
   constexpr int NT = FT_J * FT_K, NP = FT_PJ * FT_PK;
   constexpr int N = 5 + NM, IPR = 4 + NM, IVX = 1 + NM;
   constexpr int LO = (OV == 5) ? 0 : 1, HI = (OV == 5) ? 5 : 4;
   constexpr int C2 = FT_J + 2, C3 = FT_K + 2;  // cells with face states
-  __shared__ float plane[N][FT_PK][FT_PJ];
-  __shared__ float q2m[N][FT_K][C2], q2p[N][FT_K][C2];
-  __shared__ float q3m[N][C3][FT_J], q3p[N][C3][FT_J];
-  __shared__ float fx2[N][FT_K][FT_J + 1];
-  __shared__ float fx3[N][FT_K + 1][FT_J];
-  const int nbj = (g.nx2 + FT_J - 1) / FT_J, nbk = (g.nx3 + FT_K - 1) / FT_K;
-  const int bid = blockIdx.x;
-  const int bj = bid % nbj, bk = (bid / nbj) % nbk, sg = bid / (nbj * nbk);
+  auto& plane = L.plane;
+  auto& q2m = L.q2m;
+  auto& q2p = L.q2p;
+  auto& q3m = L.q3m;
+  auto& q3p = L.q3p;
+  auto& fx2 = L.fx2;
+  auto& fx3 = L.fx3;
+  const int decomp = g.grav != 0.0;
   const int t = threadIdx.x;
   const int tj = t % FT_J, tk = t / FT_J;
-  const int j0 = g.js + bj * FT_J, k0 = g.ks + bk * FT_K;
   const int j = j0 + tj, k = k0 + tk;
   const bool own = (j <= g.je) && (k <= g.ke);
-  const int c0 = g.is + sg * seg;
-  const int c1 = min(c0 + seg, g.ie + 1);
   const double A1 = g.dx2 * g.dx3, A2 = g.dx1 * g.dx3, A3 = g.dx1 * g.dx2;
   const double wv = src.weight() / (g.dx1 * g.dx2 * g.dx3);
   const int jc = min(j, g.je), kc = min(k, g.ke);  // lanes past the block
@@ -836,15 +849,39 @@ __global__ void __launch_bounds__(FT_J * FT_K)
   }
 }
 
+// Tile of a batch: block b, first column (j0, k0), levels c0 .. c1-1.
+struct KTile {
+  int b, j0, k0, c0, c1;
+};
+
+// All faces and the divergence of the mixed form, over the tiles of every 3D
+// block of a run in one launch: a workgroup owns a TJ x TK tile of columns
+// and marches up a segment of levels (mix_fused_tile).
+template <int NM, int OH, int OV, int FT_J, int FT_K>
+__global__ void __launch_bounds__(FT_J * FT_K)
+    k_mix_fused_batch(const KBlockPtr* __restrict__ bp,
+                      const KTile* __restrict__ tiles, KThermo th, KSource src,
+                      int rsolver) {
+  __shared__ FusedLds<NM, FT_J, FT_K> L;
+  const KTile tt = tiles[blockIdx.x];
+  const int b = __builtin_amdgcn_readfirstlane(tt.b);
+  const KBlockPtr& p = bp[b];
+  mix_fused_tile<NM, OH, OV, FT_J, FT_K>(
+      p.g, th, src, rsolver, __builtin_amdgcn_readfirstlane(tt.j0),
+      __builtin_amdgcn_readfirstlane(tt.k0), __builtin_amdgcn_readfirstlane(tt.c0),
+      __builtin_amdgcn_readfirstlane(tt.c1), p.w, p.psf, p.psv, p.pref, p.du, L);
+}
+
 // Face kernels of the mixed form.  variant 0: one lane per face in every
 // direction (k_flux32_h, k_flux32_x1) and a separate divergence kernel
 // (k_divergence32).  variants 1 and 2: x2 faces by cross-lane WENO
 // (k_mix_x2), x3 faces marched over segments of `seg` faces (k_mix_x3);
 // x1 faces then either marched up column segments of `seg` cells fused with
 // the divergence (k_mix_x1_div, variant 1) or one lane per face followed by
-// the divergence kernel (variant 2).  variants 3 and 4 (3D blocks): every
-// face and the divergence in one tile kernel (k_mix_fused, 32 x 8 or 16 x
-// 16 columns); 2D blocks fall back to variant 2.
+// the divergence kernel (variant 2).  Variants 3 and 4 (every face and the
+// divergence in one tile kernel of 32 x 8 or 16 x 16 columns) run over a
+// run's tile table (launch_fused_batch); here, for 2D blocks, they act as
+// variant 2.
 template <int NM>
 void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                      const KSource& src, const double* w, float* ff,
@@ -895,28 +932,6 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
               g, th, src, f1, f2, f3, psf, decomp, w, du)));
     return;
   }
-  if ((variant == 3 || variant == 4) && g.f2 && g.f3) {
-    const int tjs = (variant == 3) ? 32 : 16, tks = (variant == 3) ? 8 : 16;
-    const long long nb = (long long)((g.nx2 + tjs - 1) / tjs) *
-                         ((g.nx3 + tks - 1) / tks) * ((g.nx1 + seg - 1) / seg);
-#define CG_FUSED(OH, OV, TJ, TK)                                               \
-  TIMED(tm, KID_FLUX_X1, (k_mix_fused<NM, OH, OV, TJ, TK><<<(unsigned)nb, 256, 0, s>>>( \
-                             g, th, src, cfg.rsolver, decomp, seg, w, psf, psv, pref, du)))
-#define CG_FUSED_T(TJ, TK)                       \
-  do {                                           \
-    if (w5h && w5v) CG_FUSED(5, 5, TJ, TK);      \
-    else if (w5h) CG_FUSED(5, 2, TJ, TK);        \
-    else if (w5v) CG_FUSED(2, 5, TJ, TK);        \
-    else CG_FUSED(2, 2, TJ, TK);                 \
-  } while (0)
-    if (variant == 3)
-      CG_FUSED_T(32, 8);
-    else
-      CG_FUSED_T(16, 16);
-#undef CG_FUSED_T
-#undef CG_FUSED
-    return;
-  }
   if (g.f2) {
     // one wavefront per 59 faces of an x2 row
     const long long nw = (long long)g.nx1 * g.nx3 * ((g.nx2 + 1 + 58) / 59);
@@ -961,6 +976,32 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     TIMED(tm, KID_FLUX_X1, (k_mix_x1_div<NM, 2><<<b1, bs, 0, s>>>(
                                g, th, src, cfg.rsolver, decomp, seg, w, psf, psv,
                                f2, f3, du)));
+}
+
+// The fused tile kernel over a batch's tile table (face variants 3 / 4).
+template <int NM>
+void launch_fused_batch(const KBlockPtr* bp, const KTile* tiles, int ntiles,
+                        const KThermo& th, const cg_config& cfg,
+                        const KSource& src, int variant, hipStream_t s,
+                        KTimer* tm) {
+  const bool w5h = cfg.order_h == 5, w5v = cfg.order_v == 5;
+#define CG_FUSEDB(OH, OV, TJ, TK)                                                \
+  TIMED(tm, KID_FLUX_X1,                                                        \
+        (k_mix_fused_batch<NM, OH, OV, TJ, TK><<<(unsigned)ntiles, 256, 0, s>>>( \
+            bp, tiles, th, src, cfg.rsolver)))
+#define CG_FUSEDB_T(TJ, TK)                      \
+  do {                                           \
+    if (w5h && w5v) CG_FUSEDB(5, 5, TJ, TK);     \
+    else if (w5h) CG_FUSEDB(5, 2, TJ, TK);       \
+    else if (w5v) CG_FUSEDB(2, 5, TJ, TK);       \
+    else CG_FUSEDB(2, 2, TJ, TK);                \
+  } while (0)
+  if (variant == 3)
+    CG_FUSEDB_T(32, 8);
+  else
+    CG_FUSEDB_T(16, 16);
+#undef CG_FUSEDB_T
+#undef CG_FUSEDB
 }
 
 // per-level reference pressure: horizontal mean of p over interior columns

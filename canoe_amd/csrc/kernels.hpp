@@ -1150,6 +1150,7 @@ __global__ void __launch_bounds__(64)
 struct KBlockPtr {
   KGrid g;
   double *u, *w, *u1, *du, *psf, *psv;
+  const double* pref;  // per-level reference pressure (FP32-flux form)
 };
 
 // Group of up to 64 columns col0 .. col0+63 of block b.  Its archive is
@@ -1324,6 +1325,41 @@ __global__ void __launch_bounds__(256) k_bc_x3(KGrid g, double* w) {
   g.decode(t, g.nc1, g.nc2, NG, i, j, gh);
   bc_pair(w, g.c.n, 5 + NM, 3 + NM, g.bc[4], g.bc[5], g.at(0, g.ks, j, i),
           g.at(0, g.ke, j, i), g.c.k, gh + 1);
+}
+
+// x2 / x3 ghosts of every block of a run in one launch each (blockIdx.y =
+// block; blocks without the direction are skipped).
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_bc_x2_batch(const KBlockPtr* __restrict__ bp) {
+  const KGrid& g = bp[blockIdx.y].g;
+  if (!g.f2) return;
+  double* w = bp[blockIdx.y].w;
+  const long long n = (long long)g.nc1 * g.nx3 * NG;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, gh, k;
+    g.decode(t, g.nc1, NG, g.nx3, i, gh, k);
+    bc_pair(w, g.c.n, 5 + NM, 2 + NM, g.bc[2], g.bc[3],
+            g.at(0, k + g.ks, g.js, i), g.at(0, k + g.ks, g.je, i), g.c.j,
+            gh + 1);
+  }
+}
+
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_bc_x3_batch(const KBlockPtr* __restrict__ bp) {
+  const KGrid& g = bp[blockIdx.y].g;
+  if (!g.f3) return;
+  double* w = bp[blockIdx.y].w;
+  const long long n = (long long)g.nc1 * g.nc2 * NG;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, gh;
+    g.decode(t, g.nc1, g.nc2, NG, i, j, gh);
+    bc_pair(w, g.c.n, 5 + NM, 3 + NM, g.bc[4], g.bc[5], g.at(0, g.ks, j, i),
+            g.at(0, g.ke, j, i), g.c.k, gh + 1);
+  }
 }
 
 // ---------------------------------------------------------------------------

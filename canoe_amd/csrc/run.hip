@@ -62,6 +62,11 @@ struct cg_run {
   double* h_dt = nullptr;
   double* dt_part = nullptr;  // per-workgroup minima, dt_parts per block
   int dt_parts = 1;
+  // tiles of the fused face kernel over every 3D block (FP32-flux form,
+  // face variants 3 / 4), built for (tile_variant, tile_seg)
+  KTile* d_tiles = nullptr;
+  int ntiles = 0, tile_variant = -1, tile_seg = -1;
+  unsigned bc2_grid = 1, bc3_grid = 1;  // workgroups per block of the ghost kernels
   KTimer tm;
 };
 
@@ -80,6 +85,12 @@ void free_block(RunBlock& b) {
     if (p) (void)hipFree(p);
   if (b.ff) (void)hipFree(b.ff);
   b = RunBlock();
+}
+
+// Face variants 3 / 4 of the FP32-flux form: one tile launch per stage.
+bool fused_tiles(const cg_run* r) {
+  return r->cfg.precision == CG_PREC_FP32 &&
+         (r->tune.face_variant == 3 || r->tune.face_variant == 4);
 }
 
 unsigned implicit_grid(const cg_run* r) {
@@ -143,11 +154,19 @@ int run_stage(cg_run* r, int s) {
   // after the update; otherwise the plain Newton adjustment is fused into it
   const bool guided = (s == 2) && r->cfg.satadj == CG_SATADJ_MLP && r->mlp;
   up.satadj = (s == 2) && (r->cfg.satadj != CG_SATADJ_OFF) && !guided;
+  const bool tiled = fused_tiles(r) && r->ntiles > 0;
   for (RunBlock& b : r->blocks) {
     const KGrid& g = b.g;
     const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
     // implicit runs get psf / psv from the previous downward sweep
     if (r->cfg.precision == CG_PREC_FP32) {
+      if (tiled && g.f2 && g.f3) {  // faces in the batched tile launch below
+        if (!imp && g.grav != 0.0)
+          TIMED(tm, KID_SCAN,
+                (k_hydrostatic_scan<NM><<<nblk((long long)g.nx2 * g.nx3, 256), 256, 0, st>>>(
+                    g, r->kt, b.w, b.psf, b.psv)));
+        continue;
+      }
       launch_fluxes32<NM>(g, r->kt, r->cfg, src, b.w, b.ff, b.psf, b.psv, b.pref,
                           b.du, st, tm, !imp, bs, r->tune.face_variant,
                           r->tune.march_seg);
@@ -159,6 +178,9 @@ int run_stage(cg_run* r, int s) {
                 g, r->kt, src, b.f1, b.f2, b.f3, b.w, b.du)));
     }
   }
+  if (tiled)
+    launch_fused_batch<NM>(r->d_bp, r->d_tiles, r->ntiles, r->kt, r->cfg, src,
+                           r->tune.face_variant, st, tm);
   if (imp) {
     launch_columns<NM>(r, sbeta[s], up, tm);
     for (RunBlock& b : r->blocks) {
@@ -174,15 +196,13 @@ int run_stage(cg_run* r, int s) {
                 (k_hydrostatic_scan<NM><<<nblk((long long)g.nx2 * g.nx3, 256), 256, 0, st>>>(
                     g, r->kt, b.w, b.psf, b.psv)));
       }
-      if (g.f2)
-        TIMED(tm, KID_BC_X2,
-              (k_bc_x2<NM><<<nblk((long long)g.nc1 * g.nx3 * NG, 256), 256, 0, st>>>(
-                  g, b.w)));
-      if (g.f3)
-        TIMED(tm, KID_BC_X3,
-              (k_bc_x3<NM><<<nblk((long long)g.nc1 * g.nc2 * NG, 256), 256, 0, st>>>(
-                  g, b.w)));
     }
+    // x2 then x3 ghosts of every block (x3 copies the x2 ghosts' corners)
+    const unsigned nb = (unsigned)r->blocks.size();
+    TIMED(tm, KID_BC_X2,
+          (k_bc_x2_batch<NM><<<dim3(r->bc2_grid, nb), 256, 0, st>>>(r->d_bp)));
+    TIMED(tm, KID_BC_X3,
+          (k_bc_x3_batch<NM><<<dim3(r->bc3_grid, nb), 256, 0, st>>>(r->d_bp)));
   } else {
     for (RunBlock& b : r->blocks) {
       const KGrid& g = b.g;
@@ -303,9 +323,13 @@ void prime_block(cg_run* r, RunBlock& b) {
     k_hydrostatic_scan<NM><<<nblk(ncol, 256), 256, 0, s>>>(g, r->kt, b.w, b.psf, b.psv);
 }
 
+int ensure_tiles(cg_run* r);
+
 int capture(cg_run* r) {
   cg_ctx* ctx = r->ctx;
   drop_graph(r);
+  int te = ensure_tiles(r);
+  if (te) return te;
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int e = run_one_step(r);
   hipGraph_t gr = nullptr;
@@ -318,6 +342,40 @@ int capture(cg_run* r) {
     return set_err(ctx, CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
   r->graph = gr;
   HIPCHK(ctx, hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0));
+  return CG_OK;
+}
+
+// The tile table of the run's launch shape (host work and a synchronous
+// copy: called before capture and before direct steps, never while
+// capturing).  Tiles of the tallest blocks come first.
+int ensure_tiles(cg_run* r) {
+  if (!fused_tiles(r)) return CG_OK;
+  const int v = r->tune.face_variant, seg = r->tune.march_seg;
+  if (r->d_tiles && r->tile_variant == v && r->tile_seg == seg) return CG_OK;
+  const int tj = (v == 3) ? 32 : 16, tk = (v == 3) ? 8 : 16;
+  std::vector<int> order(r->blocks.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return r->blocks[a].g.nx1 > r->blocks[b].g.nx1;
+  });
+  std::vector<KTile> tiles;
+  for (int b : order) {
+    const KGrid& g = r->blocks[b].g;
+    if (!(g.f2 && g.f3)) continue;
+    for (int c0 = g.is; c0 <= g.ie; c0 += seg)
+      for (int k0 = g.ks; k0 <= g.ke; k0 += tk)
+        for (int j0 = g.js; j0 <= g.je; j0 += tj)
+          tiles.push_back(KTile{b, j0, k0, c0, std::min(c0 + seg, g.ie + 1)});
+  }
+  if (r->d_tiles) (void)hipFree(r->d_tiles);
+  r->d_tiles = nullptr;
+  r->ntiles = (int)tiles.size();
+  r->tile_variant = v;
+  r->tile_seg = seg;
+  if (tiles.empty()) return CG_OK;
+  HIPCHK(r->ctx, hipMalloc((void**)&r->d_tiles, sizeof(KTile) * tiles.size()));
+  HIPCHK(r->ctx, hipMemcpy(r->d_tiles, tiles.data(), sizeof(KTile) * tiles.size(),
+                           hipMemcpyHostToDevice));
   return CG_OK;
 }
 
@@ -429,6 +487,10 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
       bp[b].du = rb.du;
       bp[b].psf = rb.psf;
       bp[b].psv = rb.psv;
+      bp[b].pref = rb.pref;
+      const KGrid& g = rb.g;
+      r->bc2_grid = std::max(r->bc2_grid, nblk((long long)g.nc1 * g.nx3 * NG, 256));
+      r->bc3_grid = std::max(r->bc3_grid, nblk((long long)g.nc1 * g.nc2 * NG, 256));
     }
     ok = hipMalloc((void**)&r->d_bp, sizeof(KBlockPtr) * nblocks) == hipSuccess &&
          hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
@@ -473,6 +535,7 @@ int cg_run_destroy(cg_run* r) {
   if (r->part) (void)hipFree(r->part);
   if (r->d_dt) (void)hipFree(r->d_dt);
   if (r->dt_part) (void)hipFree(r->dt_part);
+  if (r->d_tiles) (void)hipFree(r->d_tiles);
   if (r->h_dt) (void)hipHostFree(r->h_dt);
   if (r->tm.e0) (void)hipEventDestroy(r->tm.e0);
   if (r->tm.e1) (void)hipEventDestroy(r->tm.e1);
@@ -558,6 +621,7 @@ int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int e = set_step(r, dt);
+  if (!e) e = ensure_tiles(r);
   if (e) return e;
   if (use_graph && !r->tm.on) {
     if (!r->gexec && (e = capture(r))) return e;
@@ -590,6 +654,7 @@ int cg_run_advance(cg_run* r, double cfl, double tlim, long long nlim,
     if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt))) return e;
     *dt = mesh_new_dt(__DBL_MAX__, bdt, *time, tlim);
   }
+  if ((e = ensure_tiles(r))) return e;
   if (use_graph && !r->tm.on && !r->gexec && (e = capture(r))) return e;
   while (*time < tlim && (nlim < 0 || *ncycle < nlim)) {
     if (!(*dt > 0.0)) return set_err(ctx, CG_ERR_ARG, "time step %g is not positive", *dt);

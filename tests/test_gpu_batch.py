@@ -120,3 +120,17 @@ def test_autotune_keeps_state_and_picks_a_valid_shape(ctx):
     ub = _run_batch(ctx, bl, dt, 2)
     for i in range(len(bl)):
         np.testing.assert_array_equal(ua[i], ub[i])
+
+
+@pytest.mark.parametrize("variant", [(3, 16), (4, 9)])
+def test_batch_tiles_equal_single_block_runs_mixed(ctx, variant):
+    """The FP32-flux form's tile kernel runs over one table of tiles of
+    every block: a batch reproduces each block's own run bit for bit."""
+    bl = _blocks(seed=17)
+    dt = min(cfl_dt(b[0], b[1], b[2], b[3], 0.6) for b in bl)
+    tune = (1, 0, 256) + variant
+    _say("batch tiles mixed variant=%s" % (variant,))
+    ub = _run_batch(ctx, bl, dt, 2, prec=ca.PREC_FP32, tune=tune)
+    for i, b in enumerate(bl):
+        us = _run_batch(ctx, [b], dt, 2, prec=ca.PREC_FP32, tune=tune)[0]
+        np.testing.assert_array_equal(ub[i], us)

@@ -18,7 +18,7 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                   ["2", "--nx", "32", "16", "16", "--fp32"]])
 def test_cpp_driver(args):
     print("canoe_step", " ".join(args), flush=True)
-    r = subprocess.run([EXE] + args + ["--nlim", "6", "--dt-every", "3"],
+    r = subprocess.run([EXE] + args + ["--nlim", "6", "--out", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = r.stdout
@@ -29,3 +29,14 @@ def test_cpp_driver(args):
     a, b = float(v0.group(1)), float(v0.group(2))
     assert abs(b - a) <= 1e-13 * abs(a)
     assert "zone-cycles/wall_second" in out
+
+
+def test_cpp_driver_stops_at_tlim():
+    """The device main loop cuts the last step to end at tlim."""
+    r = subprocess.run([EXE, "2", "--nx", "32", "16", "16", "--tlim", "3.0",
+                        "--cfl", "0.45", "--nlim", "1000", "--out", "5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    last = re.findall(r"cycle=(\d+) time=(\S+) dt=(\S+)", r.stdout)[-1]
+    assert abs(float(last[1]) - 3.0) <= 1e-12
+    assert int(last[0]) > 1
