@@ -869,7 +869,8 @@ __global__ void __launch_bounds__(FT_J * FT_K)
   mix_fused_tile<NM, OH, OV, FT_J, FT_K>(
       p.g, th, src, rsolver, __builtin_amdgcn_readfirstlane(tt.j0),
       __builtin_amdgcn_readfirstlane(tt.k0), __builtin_amdgcn_readfirstlane(tt.c0),
-      __builtin_amdgcn_readfirstlane(tt.c1), p.w, p.psf, p.psv, p.pref, p.du, L);
+      __builtin_amdgcn_readfirstlane(tt.c1), gptr(p.w), gptr(p.psf), gptr(p.psv),
+      gptr(p.pref), gptr(p.du), L);
 }
 
 // Face kernels of the mixed form.  variant 0: one lane per face in every

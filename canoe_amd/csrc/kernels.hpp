@@ -999,8 +999,8 @@ __device__ __forceinline__ void bwd_update_column(
   double rtop[NG];    // density of the top three cells (ie, ie-1, ie-2)
   double pf_top[NG + 1];  // psf(ie+1) .. psf(ie+4)
   double pf_next = 0.0;   // psf(i+1)
-  // the archive and the state of the next level down are loaded one level
-  // ahead, so that their latency overlaps the current level's update
+  // the archive and the state of the levels below are loaded two levels
+  // ahead, so that their latency overlaps the current levels' updates
   struct Level {
     Mt<B> a;
     Vt<B> dl;
@@ -1026,65 +1026,77 @@ __device__ __forceinline__ void bwd_update_column(
     }
   };
   Vt<B> dn;
-  Level cur, nxt;
-  fetch(cur, ie);
-  for (int i = ie; i >= is; --i) {
-    if (i > is) fetch(nxt, i - 1);
-    // back substitution delta_i = d_i - a'_i delta_{i+1}
-    // (forward_backward.hpp:86-97)
-    Vt<B> d;
-    if (i == ie) {
-      d = cur.dl;
-    } else {
-      Vt<B> ad;
-      mvec<B>(cur.a, dn, ad);
-#pragma unroll
-      for (int r = 0; r < B; ++r) d.v[r] = cur.dl.v[r] - ad.v[r];
-    }
-    dn = d;
-    const long long ci = c0 + i * si;
-    double dun[N], uc[N], wc[N];
-    write_back<NM, B>(d, cur.duo, cur.wsp, dun);
-    update_cell_pre<NM>(th, up, cur.uo, cur.u1o, u1, ci, sn, dun, uc, wc, status);
-#pragma unroll
-    for (int n = 0; n < N; ++n) {
-      u[ci + n * sn] = uc[n];
-      w[ci + n * sn] = wc[n];
-    }
-    // x1 ghosts mirrored from the cells next to the walls
-    const int mt = ie - i, mb = i - is;  // distance from top / bottom wall
-    if (mt < NG) {
-      int gi = ie + 1 + mt;
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        double v = (g.bc[1] == 0) ? ((n == IVX) ? -wc[n] : wc[n])
-                                  : w[c0 + n * sn + ie * si];
-        if (g.bc[1] == 2 && mt == 0) v = wc[n];
-        w[c0 + n * sn + gi * si] = v;
-      }
-    }
-    if (mb < NG) {
-      int gi = is - 1 - mb;
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        double v = (g.bc[0] == 0) ? ((n == IVX) ? -wc[n] : wc[n]) : 0.0;
-        w[c0 + n * sn + gi * si] = v;
-      }
-    }
-    if (hs) {
+  // level i of the sweep from the buffer cur (which then is free)
+  auto level = [&](Level& cur, int i) {
+      // back substitution delta_i = d_i - a'_i delta_{i+1}
+      // (forward_backward.hpp:86-97)
+      Vt<B> d;
       if (i == ie) {
-        double RdTv = wc[IPR] / wc[0];
-        pf_next = wc[IPR] * exp_d(grav * dx / (2 * RdTv));
-        pf_top[0] = pf_next;
-        psf[pc + (ie + 1) * g.ps_i] = pf_next;
+        d = cur.dl;
+      } else {
+        Vt<B> ad;
+        mvec<B>(cur.a, dn, ad);
+#pragma unroll
+        for (int r = 0; r < B; ++r) d.v[r] = cur.dl.v[r] - ad.v[r];
       }
-      if (mt < NG) rtop[mt] = wc[0];
-      double pfi = pf_next - grav * wc[0] * dx;
-      psf[pc + i * g.ps_i] = pfi;
-      psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
-      pf_next = pfi;
-    }
-    cur = nxt;
+      dn = d;
+      const long long ci = c0 + i * si;
+      double dun[N], uc[N], wc[N];
+      write_back<NM, B>(d, cur.duo, cur.wsp, dun);
+      update_cell_pre<NM>(th, up, cur.uo, cur.u1o, u1, ci, sn, dun, uc, wc, status);
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        u[ci + n * sn] = uc[n];
+        w[ci + n * sn] = wc[n];
+      }
+      // x1 ghosts mirrored from the cells next to the walls
+      const int mt = ie - i, mb = i - is;  // distance from top / bottom wall
+      if (mt < NG) {
+        int gi = ie + 1 + mt;
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          double v = (g.bc[1] == 0) ? ((n == IVX) ? -wc[n] : wc[n])
+                                    : w[c0 + n * sn + ie * si];
+          if (g.bc[1] == 2 && mt == 0) v = wc[n];
+          w[c0 + n * sn + gi * si] = v;
+        }
+      }
+      if (mb < NG) {
+        int gi = is - 1 - mb;
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          double v = (g.bc[0] == 0) ? ((n == IVX) ? -wc[n] : wc[n]) : 0.0;
+          w[c0 + n * sn + gi * si] = v;
+        }
+      }
+      if (hs) {
+        if (i == ie) {
+          double RdTv = wc[IPR] / wc[0];
+          pf_next = wc[IPR] * exp_d(grav * dx / (2 * RdTv));
+          pf_top[0] = pf_next;
+          psf[pc + (ie + 1) * g.ps_i] = pf_next;
+        }
+        if (mt < NG) rtop[mt] = wc[0];
+        double pfi = pf_next - grav * wc[0] * dx;
+        psf[pc + i * g.ps_i] = pfi;
+        psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
+        pf_next = pfi;
+      }
+  };
+  // three buffers: the archive and state of level i - 2 are requested
+  // while level i is processed
+  Level L0, L1, L2;
+  fetch(L0, ie);
+  if (ie - 1 >= is) fetch(L1, ie - 1);
+  for (int i = ie; i >= is; i -= 3) {
+    if (i - 2 >= is) fetch(L2, i - 2);
+    level(L0, i);
+    if (i - 1 < is) break;
+    if (i - 3 >= is) fetch(L0, i - 3);
+    level(L1, i - 1);
+    if (i - 2 < is) break;
+    if (i - 4 >= is) fetch(L1, i - 4);
+    level(L2, i - 2);
   }
   // outflow bottom ghosts copy the first interior cell
   if (g.bc[0] == 2) {
@@ -1153,6 +1165,15 @@ struct KBlockPtr {
   const double* pref;  // per-level reference pressure (FP32-flux form)
 };
 
+// A pointer read from a block table, marked as global memory: kernels then
+// use global (not flat) loads and stores, which neither wait on nor hold up
+// LDS and scalar-memory counters.
+template <class T>
+__device__ __forceinline__ T* gptr(T* p) {
+  typedef __attribute__((address_space(1))) T gT;
+  return (T*)(gT*)(uintptr_t)p;
+}
+
 // Group of up to 64 columns col0 .. col0+63 of block b.  Its archive is
 // lane-contiguous from aoff: level lvl, entry e of lane l at
 // aoff + (lvl (B*B+B) + e) 64 + l.
@@ -1179,8 +1200,8 @@ __global__ void __launch_bounds__(64 * WPB)
     const long long col = (long long)col0 + lane;
     if (col < (long long)g.nx2 * g.nx3) {
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
-      if (!fwd_column<NM, B>(g, th, dt, bp[b].w, bp[b].du, g.at(0, k, j, 0),
-                             arch + aoff, 64, lane))
+      if (!fwd_column<NM, B>(g, th, dt, gptr(bp[b].w), gptr(bp[b].du),
+                             g.at(0, k, j, 0), arch + aoff, 64, lane))
         atomicOr(status, 1);
     }
   }
@@ -1204,9 +1225,10 @@ __global__ void __launch_bounds__(64 * WPB)
     if (col < (long long)g.nx2 * g.nx3) {
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       const KBlockPtr& p = bp[b];
-      bwd_update_column<NM, B>(g, th, up, arch + aoff, 64, lane, p.du, p.u,
-                               p.u1, p.w, p.psf, p.psv, g.at(0, k, j, 0),
-                               col * g.ps_c, status);
+      bwd_update_column<NM, B>(g, th, up, arch + aoff, 64, lane, gptr(p.du),
+                               gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
+                               gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
+                               status);
     }
   }
 }
@@ -1334,7 +1356,7 @@ __global__ void __launch_bounds__(256)
     k_bc_x2_batch(const KBlockPtr* __restrict__ bp) {
   const KGrid& g = bp[blockIdx.y].g;
   if (!g.f2) return;
-  double* w = bp[blockIdx.y].w;
+  double* w = gptr(bp[blockIdx.y].w);
   const long long n = (long long)g.nc1 * g.nx3 * NG;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -1351,7 +1373,7 @@ __global__ void __launch_bounds__(256)
     k_bc_x3_batch(const KBlockPtr* __restrict__ bp) {
   const KGrid& g = bp[blockIdx.y].g;
   if (!g.f3) return;
-  double* w = bp[blockIdx.y].w;
+  double* w = gptr(bp[blockIdx.y].w);
   const long long n = (long long)g.nc1 * g.nc2 * NG;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -1460,7 +1482,7 @@ __global__ void __launch_bounds__(256)
              const double* __restrict__ w0, KThermo th, double cfl, int iflag,
              double* __restrict__ part) {
   const KGrid g = bp ? bp[blockIdx.y].g : g0;
-  const double* w = bp ? bp[blockIdx.y].w : w0;
+  const double* w = gptr(bp ? bp[blockIdx.y].w : w0);
   const long long total = (long long)g.nx1 * g.nx2 * g.nx3;
   double m = __DBL_MAX__;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
