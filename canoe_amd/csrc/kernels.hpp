@@ -1174,9 +1174,11 @@ __device__ __forceinline__ T* gptr(T* p) {
   return (T*)(gT*)(uintptr_t)p;
 }
 
-// Group of up to 64 columns col0 .. col0+63 of block b.  Its archive is
-// lane-contiguous from aoff: level lvl, entry e of lane l at
-// aoff + (lvl (B*B+B) + e) 64 + l.
+// Group of up to 64 columns col0 .. col0+63 of block b.  aoff is the start
+// of block b's archive, laid out level-major like the state: entry e of
+// level lvl of column c at aoff + (lvl (B*B+B) + e) nx2 nx3 + c.  A
+// wavefront's access of one entry is one 512-byte segment, and at any time
+// every wavefront works in the same few pages of the archive.
 struct KGroup {
   int b, col0;
   long long aoff;
@@ -1201,7 +1203,8 @@ __global__ void __launch_bounds__(64 * WPB)
     if (col < (long long)g.nx2 * g.nx3) {
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       if (!fwd_column<NM, B>(g, th, dt, gptr(bp[b].w), gptr(bp[b].du),
-                             g.at(0, k, j, 0), arch + aoff, 64, lane))
+                             g.at(0, k, j, 0), arch + aoff,
+                             (long long)g.nx2 * g.nx3, col))
         atomicOr(status, 1);
     }
   }
@@ -1225,7 +1228,8 @@ __global__ void __launch_bounds__(64 * WPB)
     if (col < (long long)g.nx2 * g.nx3) {
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       const KBlockPtr& p = bp[b];
-      bwd_update_column<NM, B>(g, th, up, arch + aoff, 64, lane, gptr(p.du),
+      bwd_update_column<NM, B>(g, th, up, arch + aoff, (long long)g.nx2 * g.nx3,
+                               col, gptr(p.du),
                                gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
                                gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
                                status);

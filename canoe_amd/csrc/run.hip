@@ -271,7 +271,8 @@ bool alloc_block(RunBlock& b, int N, const cg_config& cfg) {
   return ok;
 }
 
-// Column groups of 64, blocks taken tallest first; archive offsets.
+// Column groups of 64, blocks taken tallest first; each block's archive
+// (nx1 AS nx2 nx3 doubles, level-major) starts at its group's aoff.
 void build_groups(const std::vector<RunBlock>& blocks, int AS,
                   std::vector<KGroup>& groups, size_t& arch_n) {
   std::vector<int> order(blocks.size());
@@ -290,8 +291,8 @@ void build_groups(const std::vector<RunBlock>& blocks, int AS,
       q.col0 = (int)c0;
       q.aoff = (long long)arch_n;
       groups.push_back(q);
-      arch_n += (size_t)g.nx1 * AS * 64;
     }
+    arch_n += (size_t)g.nx1 * AS * ncol;
   }
 }
 
