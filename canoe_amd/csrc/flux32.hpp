@@ -26,6 +26,10 @@ namespace cg {
 // hardware reciprocal / square-root instructions (1 ulp) instead of the
 // correctly rounded sequences; the FP32 faces are an approximation to the
 // FP64 ones anyway, checked against them by relative error.
+// The FP32 face arithmetic (reconstruction, Riemann fluxes) lets the compiler
+// fuse multiplies and adds; FP64 code keeps -ffp-contract=off.
+#define CG_FP32_CONTRACT _Pragma("clang fp contract(fast)")
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * frcp(b); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -34,6 +38,7 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 // window (a, b, c, d, e); the lower face is weno5_up_f(e, d, c, b, a).
 __device__ __forceinline__ float weno5_up_f(float a, float b, float c, float d,
                                             float e) {
+  CG_FP32_CONTRACT
   const float eps = 1.0e-6f;
   float s0 = a - 2.f * b + c, t0 = a - 4.f * b + 3.f * c;
   float s1 = b - 2.f * c + d, t1 = b - d;
@@ -56,6 +61,7 @@ __device__ __forceinline__ float weno5_up_f(float a, float b, float c, float d,
 __device__ __forceinline__ void weno5_both_f(float a, float b, float c,
                                              float d, float e, float* qm,
                                              float* qp) {
+  CG_FP32_CONTRACT
   const float eps = 1.0e-6f;
   float s0 = a - 2.f * b + c, t0 = a - 4.f * b + 3.f * c;
   float s1 = b - 2.f * c + d, t1 = b - d;
@@ -79,6 +85,7 @@ __device__ __forceinline__ void weno5_both_f(float a, float b, float c,
 
 // PLM (van Leer) upper face of cell b of (a, b, c); lower face plm_up_f(c, b, a).
 __device__ __forceinline__ float plm_up_f(float a, float b, float c) {
+  CG_FP32_CONTRACT
   float dql = b - a, dqr = c - b;
   float dq2 = dql * dqr;
   float dqm = dq2 > 0.f ? 2.f * dq2 * frcp(dql + dqr) : 0.f;
@@ -88,6 +95,7 @@ __device__ __forceinline__ float plm_up_f(float a, float b, float c) {
 // get_gamma in float (src/interface/eos.hpp:47-67)
 template <int NM>
 __device__ inline float gamma_f(const KThermo& th, const float* u) {
+  CG_FP32_CONTRACT
   float fsig = 1.f, feps = 1.f;
 #pragma unroll
   for (int n = 1; n <= NM; ++n)
@@ -109,6 +117,7 @@ __device__ inline float gamma_f(const KThermo& th, const float* u) {
 template <int NM, int DIR>
 __device__ inline void lmars_p(const KThermo& th, const float* wli,
                                const float* wri, float pb, float* flx) {
+  CG_FP32_CONTRACT
   constexpr int IVX = 1 + NM, IPR = IVX + 3, IDN = 0, IEN = IPR;
   constexpr int ivx = IVX + DIR;
   constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
@@ -155,6 +164,7 @@ __device__ inline void lmars_p(const KThermo& th, const float* wli,
 template <int NM, int DIR>
 __device__ inline void hllc_p(const KThermo& th, const float* wl,
                               const float* wr, float pb, float* flx) {
+  CG_FP32_CONTRACT
   constexpr int IVX = 1 + NM, IVY = IVX + 1, IVZ = IVX + 2, IPR = IVX + 3;
   constexpr int IDN = 0, IEN = IPR, N = 5 + NM;
   constexpr int ivx = IVX + DIR;
