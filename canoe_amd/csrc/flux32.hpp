@@ -661,11 +661,13 @@ template <int NM, int FT_J, int FT_K>
 struct FusedLds {
   static constexpr int H = 3, PJ = FT_J + 2 * H, PK = FT_K + 2 * H, N = 5 + NM;
   static constexpr int C2 = FT_J + 2, C3 = FT_K + 2;  // cells with face states
+  // x2 items first (FT_K rows of C2 cells / FT_J + 1 faces), then x3 items
+  // (C3 cells / FT_K + 1 faces by FT_J columns)
+  static constexpr int NQ2 = FT_K * C2, NQ = NQ2 + C3 * FT_J;
+  static constexpr int NF2 = FT_K * (FT_J + 1), NF = NF2 + (FT_K + 1) * FT_J;
   float plane[N][PK][PJ];
-  float q2m[N][FT_K][C2], q2p[N][FT_K][C2];
-  float q3m[N][C3][FT_J], q3p[N][C3][FT_J];
-  float fx2[N][FT_K][FT_J + 1];
-  float fx3[N][FT_K + 1][FT_J];
+  float qm[N][NQ], qp[N][NQ];  // face states, velocities rotated
+  float fx[N][NF];             // face fluxes, momenta rotated
 };
 
 // One tile: columns j0 .. j0+FT_J-1, k0 .. k0+FT_K-1 of block g, levels
@@ -683,13 +685,9 @@ __device__ __forceinline__ void mix_fused_tile(
   constexpr int N = 5 + NM, IPR = 4 + NM, IVX = 1 + NM;
   constexpr int LO = (OV == 5) ? 0 : 1, HI = (OV == 5) ? 5 : 4;
   constexpr int C2 = FT_J + 2, C3 = FT_K + 2;  // cells with face states
+  using LDS = FusedLds<NM, FT_J, FT_K>;
+  constexpr int NQ2 = LDS::NQ2, NQ = LDS::NQ, NF2 = LDS::NF2, NF = LDS::NF;
   auto& plane = L.plane;
-  auto& q2m = L.q2m;
-  auto& q2p = L.q2p;
-  auto& q3m = L.q3m;
-  auto& q3p = L.q3p;
-  auto& fx2 = L.fx2;
-  auto& fx3 = L.fx3;
   const int decomp = g.grav != 0.0;
   const int t = threadIdx.x;
   const int tj = t % FT_J, tk = t / FT_J;
@@ -770,56 +768,45 @@ __device__ __forceinline__ void mix_fused_tile(
     }
     __syncthreads();
     // (2) face states: x2 for cells j0-1 .. j0+FT_J of the interior rows,
-    //     x3 for cells k0-1 .. k0+FT_K of the interior columns
-    for (int e = t; e < FT_K * C2; e += NT) {
-      const int cj = e % C2, rk = e / C2;  // cell j0 - 1 + cj, row k0 + rk
-      const int pj = cj + FT_H - 1, pk = rk + FT_H;
+    //     x3 for cells k0-1 .. k0+FT_K of the interior columns, as one
+    //     item list split evenly over the workgroup.  The velocity slots
+    //     are rotated so that the normal component comes first (x2: v2, v3,
+    //     v1; x3: v3, v1, v2), the order in which the Riemann solver of
+    //     that direction reads them: one code path serves both directions.
+    for (int e = t; e < NQ; e += NT) {
+      const bool x2 = e < NQ2;
+      const int e3 = e - NQ2;
+      const int pj = x2 ? (e % C2) + FT_H - 1 : (e3 % FT_J) + FT_H;
+      const int pk = x2 ? (e / C2) + FT_H : (e3 / FT_J) + FT_H - 1;
+      const int st = x2 ? 1 : FT_PJ;
+      const int rot = x2 ? 1 : 2;
+      const float* pc0 = &plane[0][pk][pj];
 #pragma unroll
       for (int n = 0; n < N; ++n) {
+        const int sv = (n >= IVX && n < IVX + 3) ? IVX + (n - IVX + rot) % 3 : n;
+        const float* pv = pc0 + sv * (FT_PK * FT_PJ);
         float qm, qp;
-        recon(plane[n][pk][pj - 2], plane[n][pk][pj - 1], plane[n][pk][pj],
-              plane[n][pk][pj + 1], plane[n][pk][pj + 2], &qm, &qp);
-        q2m[n][rk][cj] = qm;
-        q2p[n][rk][cj] = qp;
-      }
-    }
-    for (int e = t; e < C3 * FT_J; e += NT) {
-      const int cj = e % FT_J, ck = e / FT_J;  // column j0 + cj, cell k0 - 1 + ck
-      const int pj = cj + FT_H, pk = ck + FT_H - 1;
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        float qm, qp;
-        recon(plane[n][pk - 2][pj], plane[n][pk - 1][pj], plane[n][pk][pj],
-              plane[n][pk + 1][pj], plane[n][pk + 2][pj], &qm, &qp);
-        q3m[n][ck][cj] = qm;
-        q3p[n][ck][cj] = qp;
+        recon(pv[-2 * st], pv[-st], pv[0], pv[st], pv[2 * st], &qm, &qp);
+        L.qm[n][e] = qm;
+        L.qp[n][e] = qp;
       }
     }
     __syncthreads();
-    // (3) x2 / x3 Riemann fluxes of the tile's faces
-    for (int e = t; e < FT_K * (FT_J + 1); e += NT) {
-      const int fj = e % (FT_J + 1), rk = e / (FT_J + 1);  // face j0 + fj
+    // (3) x2 / x3 Riemann fluxes of the tile's faces, one item list
+    for (int e = t; e < NF; e += NT) {
+      const bool x2 = e < NF2;
+      const int e3 = e - NF2;
+      const int lq = x2 ? (e / (FT_J + 1)) * C2 + e % (FT_J + 1) : NQ2 + e3;
+      const int rq = lq + (x2 ? 1 : FT_J);
       float wl[N], wr[N], fx[N];
 #pragma unroll
       for (int n = 0; n < N; ++n) {
-        wl[n] = q2p[n][rk][fj];
-        wr[n] = q2m[n][rk][fj + 1];
+        wl[n] = L.qp[n][lq];
+        wr[n] = L.qm[n][rq];
       }
-      riemann_p<NM, 1>(th, rsolver, wl, wr, (float)prd, fx);
+      riemann_p<NM, 0>(th, rsolver, wl, wr, (float)prd, fx);
 #pragma unroll
-      for (int n = 0; n < N; ++n) fx2[n][rk][fj] = fx[n];
-    }
-    for (int e = t; e < (FT_K + 1) * FT_J; e += NT) {
-      const int cj = e % FT_J, fk = e / FT_J;  // face k0 + fk
-      float wl[N], wr[N], fx[N];
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        wl[n] = q3p[n][fk][cj];
-        wr[n] = q3m[n][fk + 1][cj];
-      }
-      riemann_p<NM, 2>(th, rsolver, wl, wr, (float)prd, fx);
-#pragma unroll
-      for (int n = 0; n < N; ++n) fx3[n][fk][cj] = fx[n];
+      for (int n = 0; n < N; ++n) L.fx[n][e] = fx[n];
     }
     __syncthreads();
     // (4) x1 face i+1 in registers, the cell's divergence and sources
@@ -845,8 +832,12 @@ __device__ __forceinline__ void mix_fused_tile(
       for (int n = 0; n < N; ++n) {
         double dflx = A1 * ((double)fp[n] - (double)fm[n]);
         if (n == IVX && decomp) dflx += A1 * (pfp - pfm);
-        dflx += A2 * ((double)fx2[n][tk][tj + 1] - (double)fx2[n][tk][tj]);
-        dflx += A3 * ((double)fx3[n][tk + 1][tj] - (double)fx3[n][tk][tj]);
+        // un-rotate: global component IVX + c sits in slot IVX + (c - rot) % 3
+        const int s2 = (n >= IVX && n < IVX + 3) ? IVX + (n - IVX + 2) % 3 : n;
+        const int s3 = (n >= IVX && n < IVX + 3) ? IVX + (n - IVX + 1) % 3 : n;
+        const int f2 = tk * (FT_J + 1) + tj, f3 = NF2 + tk * FT_J + tj;
+        dflx += A2 * ((double)L.fx[s2][f2 + 1] - (double)L.fx[s2][f2]);
+        dflx += A3 * ((double)L.fx[s3][f3 + FT_J] - (double)L.fx[s3][f3]);
         d[n] = 0.0 - wv * dflx;
       }
       add_sources_cell<NM>(g, th, src, w, k, j, i, d);
