@@ -77,7 +77,7 @@ def make_run(ctx, blocks, th, cfg, w0s, net=None, tune=None):
         run.set_mlp(net)
     if tune is not None:
         run.set_tune(tune.implicit_wpb, tune.implicit_grid, tune.flux_block,
-                     tune.face_variant, tune.march_seg)
+                     tune.face_variant, tune.march_seg, tune.implicit_fused)
     for b, w in enumerate(w0s):
         run.set_prim(w, b)
     return run
@@ -201,8 +201,8 @@ def algorithmic(blk, cfg):
 OPS = {
     "flux_divergence": ["k_hydrostatic_scan", "k_flux_x1", "k_flux_h<x2>",
                         "k_flux_h<x3>", "k_divergence", "k_flux32",
-                        "k_mirror32"],
-    "implicit_solve": ["k_implicit_fwd", "k_implicit_bwd"],
+                        "k_mirror32", "k_mix_fused"],
+    "implicit_solve": ["k_implicit_fwd", "k_implicit_bwd", "k_implicit_fused"],
     "update": ["k_update", "k_bc_x1", "k_bc_x2", "k_bc_x3",
                "k_satadj_run_mlp"],
 }
@@ -496,11 +496,17 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     return out
 
 
+PROFILE_TUNE = ca.Tune(4, 256, 256, 3, 16, 1, 0.0)  # shape the autotuner picks
+
+
 def profile_run(nx, steps):
     """A short, fixed run for rocprofv3: a streaming copy of 1 GiB (the
     calibration of the HBM traffic counters: it reads and writes exactly
     2^30 bytes) three times, then `steps` graph-replayed steps of the dry box
-    in the FP32-flux form with the default launch shape."""
+    in the FP32-flux form with the launch shape the autotuner picks on one
+    MI355X (4 waves per workgroup, one workgroup per compute unit for the
+    column solve; 32 x 8 face tiles, 16-level segments), each step followed
+    by the NewBlockTimeStep reduction."""
     ctx = ca.Context(0)
     nbytes = 1 << 30
     a = ctx.array(shape=(nbytes // 8,), dtype=np.float64)
@@ -514,9 +520,11 @@ def profile_run(nx, steps):
     w0 = ca.pgen_init(2, 1, blk, th, cfg)
     dt = cfl_dt(blk, th, cfg, w0)
     cfg.precision = ca.PREC_FP32
-    run = make_run(ctx, [blk], th, cfg, [w0])
+    run = make_run(ctx, [blk], th, cfg, [w0], tune=PROFILE_TUNE)
     t0 = time.perf_counter()
-    run.step(dt, steps, graph=USE_GRAPH)
+    for _ in range(steps):
+        run.step(dt, 1, graph=USE_GRAPH)
+        run.new_dt(0.45)
     t1 = time.perf_counter()
     run.close()
     ctx.close()

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 
 NGHOST = 3
 MAX_NMASS = 3
-NUM_KERNEL_IDS = 15
+NUM_KERNEL_IDS = 17
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV = range(9)
@@ -384,7 +384,8 @@ class Tune(ctypes.Structure):
     """cg_tune: launch shape of a run."""
     _fields_ = [("implicit_wpb", ctypes.c_int), ("implicit_grid", ctypes.c_int),
                 ("flux_block", ctypes.c_int), ("face_variant", ctypes.c_int),
-                ("march_seg", ctypes.c_int), ("step_ms", ctypes.c_double)]
+                ("march_seg", ctypes.c_int), ("implicit_fused", ctypes.c_int),
+                ("step_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -467,9 +468,9 @@ class Run:
         return t
 
     def set_tune(self, implicit_wpb=1, implicit_grid=0, flux_block=256,
-                 face_variant=3, march_seg=16):
+                 face_variant=3, march_seg=16, implicit_fused=1):
         t = Tune(implicit_wpb, implicit_grid, flux_block, face_variant,
-                 march_seg, 0.0)
+                 march_seg, implicit_fused, 0.0)
         self.ctx._check(lib().cg_run_set_tune(self.h, _ref(t)), "set_tune")
 
     def autotune(self, dt):

@@ -988,7 +988,7 @@ void launch_fused_batch(const KBlockPtr* bp, const KTile* tiles, int ntiles,
                         KTimer* tm) {
   const bool w5h = cfg.order_h == 5, w5v = cfg.order_v == 5;
 #define CG_FUSEDB(OH, OV, TJ, TK)                                                \
-  TIMED(tm, KID_FLUX_X1,                                                        \
+  TIMED(tm, KID_MIX_FUSED,                                                      \
         (k_mix_fused_batch<NM, OH, OV, TJ, TK><<<(unsigned)ntiles, 256, 0, s>>>( \
             bp, tiles, th, src, cfg.rsolver)))
 #define CG_FUSEDB_T(TJ, TK)                      \

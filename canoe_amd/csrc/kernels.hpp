@@ -1237,6 +1237,41 @@ __global__ void __launch_bounds__(64 * WPB)
   }
 }
 
+// Forward sweep and downward sweep of each group back to back in one wave:
+// the columns of a group depend on no other group within a stage, and the
+// top levels' archive, written last by the forward sweep, is read first by
+// the downward sweep while still in the caches.
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_fused_batch(const KBlockPtr* __restrict__ bp,
+                           const KGroup* __restrict__ grp, int ngroups,
+                           KThermo th, double dt, const double* __restrict__ dtp,
+                           KUpdate up, double* __restrict__ arch,
+                           int* __restrict__ status) {
+  if (dtp) dt = dt * dtp[0];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int q = blockIdx.x * WPB + wv; q < ngroups; q += gridDim.x * WPB) {
+    const int b = __builtin_amdgcn_readfirstlane(grp[q].b);
+    const int col0 = __builtin_amdgcn_readfirstlane(grp[q].col0);
+    const long long aoff = grp[q].aoff;
+    const KGrid g = bp[b].g;
+    const long long col = (long long)col0 + lane;
+    if (col < (long long)g.nx2 * g.nx3) {
+      const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
+      const KBlockPtr& p = bp[b];
+      const long long ncol = (long long)g.nx2 * g.nx3;
+      if (!fwd_column<NM, B>(g, th, dt, gptr(p.w), gptr(p.du), g.at(0, k, j, 0),
+                             arch + aoff, ncol, col))
+        atomicOr(status, 1);
+      bwd_update_column<NM, B>(g, th, up, arch + aoff, ncol, col, gptr(p.du),
+                               gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
+                               gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
+                               status);
+    }
+  }
+}
+
 // Streaming copy of n 16-byte words (cg_memcpy_d2d): grid-stride loop.
 static __global__ void __launch_bounds__(256)
     k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,

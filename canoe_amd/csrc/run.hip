@@ -105,6 +105,13 @@ template <int NM, int B, int WPB>
 void launch_col_batch(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   hipStream_t s = r->ctx->stream;
   const unsigned gr = implicit_grid(r);
+  if (r->tune.implicit_fused) {
+    TIMED(tm, KID_IMPLICIT_FUSED,
+          (k_implicit_fused_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+              r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, up, r->arch,
+              r->ctx->d_status)));
+    return;
+  }
   TIMED(tm, KID_IMPLICIT_FWD,
         (k_implicit_fwd_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
             r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, r->arch,
@@ -452,6 +459,7 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   r->tune.flux_block = 256;
   r->tune.face_variant = 3;
   r->tune.march_seg = 16;
+  r->tune.implicit_fused = 1;
   r->tune.step_ms = 0.0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -719,10 +727,11 @@ int cg_run_set_tune(cg_run* r, const cg_tune* t) {
       t->implicit_grid < 0 ||
       (t->flux_block != 64 && t->flux_block != 128 && t->flux_block != 256) ||
       t->face_variant < 0 || t->face_variant > 4 || t->march_seg < 1 ||
-      t->march_seg > 4096)
+      t->march_seg > 4096 || (t->implicit_fused != 0 && t->implicit_fused != 1))
     return set_err(r->ctx, CG_ERR_ARG,
                    "implicit_wpb in {1,2,4}, implicit_grid >= 0, flux_block in "
-                   "{64,128,256}, face_variant in 0..4, march_seg in [1,4096]");
+                   "{64,128,256}, face_variant in 0..4, march_seg in [1,4096], "
+                   "implicit_fused in {0,1}");
   if (!(r->cfg.implicit_flag & 8) && t->implicit_wpb != 1)
     return set_err(r->ctx, CG_ERR_ARG, "3x3 solve runs with implicit_wpb 1");
   r->tune = *t;
@@ -733,8 +742,9 @@ int cg_run_set_tune(cg_run* r, const cg_tune* t) {
 // Times one graph-replayed step for each candidate launch shape, restoring
 // the state after every trial, and keeps the fastest: first the batched
 // column solve (waves per workgroup x grid: one workgroup per group set, or
-// a persistent grid of 1, 2 or 4 workgroups per compute unit), then the
-// workgroup size of the face kernels.
+// a persistent grid of 1, 2 or 4 workgroups per compute unit; sweeps fused
+// per group or in two launches), then the workgroup size of the face
+// kernels.
 int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
   if (!r || !(dt > 0.0)) return CG_ERR_ARG;
   cg_ctx* ctx = r->ctx;
@@ -818,13 +828,15 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     int wpb = wpbs[wi];
     if (!(r->cfg.implicit_flag & 8) && wpb != 1) continue;
     const int grids[4] = {0, r->ncu * 4 / wpb, r->ncu * 2 / wpb, r->ncu / wpb};
-    for (int gi = 0; gi < (r->cfg.implicit_flag ? 4 : 1); ++gi) {
-      cg_tune t = best;
-      t.implicit_wpb = wpb;
-      t.implicit_grid = grids[gi];
-      t.flux_block = 256;
-      cands.push_back(t);
-    }
+    for (int gi = 0; gi < (r->cfg.implicit_flag ? 4 : 1); ++gi)
+      for (int fu = 1; fu >= (r->cfg.implicit_flag ? 0 : 1); --fu) {
+        cg_tune t = best;
+        t.implicit_wpb = wpb;
+        t.implicit_grid = grids[gi];
+        t.implicit_fused = fu;
+        t.flux_block = 256;
+        cands.push_back(t);
+      }
   }
   for (const cg_tune& t : cands) {
     if (err) break;

@@ -272,7 +272,7 @@ int cg_run_sums(cg_run* run, double* out_host);
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of
  * launches and the kernel name of kernel id 0..CG_NUM_KERNEL_IDS-1 (ids
  * with no launches report 0).  cg_run_set_timing resets the counters. */
-#define CG_NUM_KERNEL_IDS 15
+#define CG_NUM_KERNEL_IDS 17
 int cg_run_set_timing(cg_run* run, int on);
 int cg_run_kernel_stats(cg_run* run, int kernel_id, double* total_ms,
                         long long* launches, const char** name);
@@ -295,7 +295,10 @@ int cg_run_block_sums(cg_run* run, int block, double* out_host);
 
 /* Launch shape of a run: waves per workgroup (1, 2, 4) and grid (0 = one
  * workgroup per set of column groups, else a persistent grid of that many
- * workgroups) of the batched column solve, workgroup size (64, 128, 256) of
+ * workgroups) of the batched column solve, whether each wave runs a group's
+ * forward and downward sweeps back to back in one launch (implicit_fused =
+ * 1) or all forward sweeps then all downward sweeps (0), workgroup size
+ * (64, 128, 256) of
  * the face and divergence kernels, and for the FP32-flux form the face
  * kernel variant (0: one lane per face in every direction; 1: one WENO
  * evaluation per cell, x2 across lanes, x3 and x1 marched over segments of
@@ -312,6 +315,7 @@ typedef struct cg_tune {
   int flux_block;
   int face_variant;
   int march_seg;
+  int implicit_fused;
   double step_ms;
 } cg_tune;
 /* Network of a run whose config has satadj = CG_SATADJ_MLP (NULL: plain

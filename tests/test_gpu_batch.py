@@ -54,7 +54,8 @@ def _oracle(bl, dt, nsteps):
     return out
 
 
-@pytest.mark.parametrize("tune", [None, (1, 3, 64), (2, 0, 128), (4, 1, 256)])
+@pytest.mark.parametrize("tune", [None, (1, 3, 64), (2, 0, 128), (4, 1, 256),
+                                  (1, 3, 64, 3, 16, 0), (4, 0, 256, 3, 16, 0)])
 def test_batch_fp64_matches_oracle_per_block(ctx, tune):
     bl = _blocks()
     dt = min(cfl_dt(b[0], b[1], b[2], b[3], 0.6) for b in bl)
@@ -110,6 +111,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape(ctx):
         _say("  autotune: %s" % t.as_dict())
         assert t.implicit_wpb in (1, 2, 4) and t.flux_block in (64, 128, 256)
         assert t.face_variant in range(5) and t.march_seg >= 1
+        assert t.implicit_fused in (0, 1)
         assert t.step_ms > 0
         for i in range(len(bl)):
             np.testing.assert_array_equal(run.cons(i), u0[i])
