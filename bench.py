@@ -232,13 +232,16 @@ def roofline(blk, cfg, stats):
     return dom, ops
 
 
-def traffic_from_profiles(op):
+def traffic_from_profiles(op, tune):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE, calibrated against a copy kernel), if a summary is
-    committed under profiles/."""
+    committed under profiles/; the column solve's figure for the sweeps
+    fused per group or in two launches, as the autotuner chose."""
     p = os.path.join(ROOT, "profiles", "traffic_box256.json")
     if not os.path.exists(p):
         return None
+    if op == "implicit_solve" and not tune.get("implicit_fused", 1):
+        op = "implicit_solve_two_launches"
     try:
         d = json.load(open(p))
         return d.get(op)
@@ -496,17 +499,19 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     return out
 
 
-PROFILE_TUNE = ca.Tune(4, 256, 256, 3, 16, 1, 0.0)  # shape the autotuner picks
+# the launch shapes the autotuner picks between on one MI355X: column solve
+# with 4 waves per workgroup on one workgroup per compute unit, sweeps fused
+# per group or in two launches; 32 x 8 face tiles marching 16 levels
+PROFILE_TUNES = (ca.Tune(4, 256, 256, 3, 16, 1, 0.0),
+                 ca.Tune(4, 256, 256, 3, 16, 0, 0.0))
 
 
 def profile_run(nx, steps):
     """A short, fixed run for rocprofv3: a streaming copy of 1 GiB (the
     calibration of the HBM traffic counters: it reads and writes exactly
-    2^30 bytes) three times, then `steps` graph-replayed steps of the dry box
-    in the FP32-flux form with the launch shape the autotuner picks on one
-    MI355X (4 waves per workgroup, one workgroup per compute unit for the
-    column solve; 32 x 8 face tiles, 16-level segments), each step followed
-    by the NewBlockTimeStep reduction."""
+    2^30 bytes) three times, then, for each launch shape of PROFILE_TUNES,
+    `steps` graph-replayed steps of the dry box in the FP32-flux form, each
+    followed by the NewBlockTimeStep reduction."""
     ctx = ca.Context(0)
     nbytes = 1 << 30
     a = ctx.array(shape=(nbytes // 8,), dtype=np.float64)
@@ -520,17 +525,21 @@ def profile_run(nx, steps):
     w0 = ca.pgen_init(2, 1, blk, th, cfg)
     dt = cfl_dt(blk, th, cfg, w0)
     cfg.precision = ca.PREC_FP32
-    run = make_run(ctx, [blk], th, cfg, [w0], tune=PROFILE_TUNE)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        run.step(dt, 1, graph=USE_GRAPH)
-        run.new_dt(0.45)
-    t1 = time.perf_counter()
-    run.close()
+    ms = []
+    for tune in PROFILE_TUNES:
+        run = make_run(ctx, [blk], th, cfg, [w0], tune=tune)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.step(dt, 1, graph=USE_GRAPH)
+            run.new_dt(0.45)
+        t1 = time.perf_counter()
+        run.close()
+        ms.append(round(1e3 * (t1 - t0) / steps, 4))
     ctx.close()
     print(json.dumps({"profile": "dry box %dx%dx%d FP32 fluxes" % tuple(nx),
                       "steps": steps, "copy_bytes": nbytes,
-                      "ms_per_step": round(1e3 * (t1 - t0) / steps, 4)}))
+                      "tunes": [t.as_dict() for t in PROFILE_TUNES],
+                      "ms_per_step": ms}))
 
 
 def main():
@@ -581,7 +590,7 @@ def main():
     rf = {"bound": "hbm", "op": dom, "kernels": op["kernels"],
           "achieved": op["achieved_GBs"], "peak": HBM_PEAK_GBS,
           "unit": "GB/s", "frac": op["frac"],
-          "traffic": traffic_from_profiles(dom)}
+          "traffic": traffic_from_profiles(dom, chosen["tune"])}
 
     cpu = None
     if not args.no_cpu:
