@@ -842,7 +842,7 @@ __device__ __forceinline__ void mix_fused_tile(
       }
       add_sources_cell<NM>(g, th, src, w, k, j, i, d);
 #pragma unroll
-      for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
+      for (int n = 0; n < N; ++n) __builtin_nontemporal_store(d[n], &du[g.at(n, k, j, i)]);
     }
 #pragma unroll
     for (int n = 0; n < N; ++n) fm[n] = fp[n];

@@ -907,7 +907,7 @@ __device__ inline void update_cell_pre(const KThermo& th, const KUpdate& up,
     double v;
     if (up.stage == 0) {
       v = 0.0 + 1.0 * un;
-      u1[ci + n * sn] = v;
+      __builtin_nontemporal_store(v, &u1[ci + n * sn]);
     } else {
       v = up.g1 * un + up.g2 * u1o[n];
     }
@@ -1046,8 +1046,9 @@ __device__ __forceinline__ void bwd_update_column(
       update_cell_pre<NM>(th, up, cur.uo, cur.u1o, u1, ci, sn, dun, uc, wc, status);
 #pragma unroll
       for (int n = 0; n < N; ++n) {
-        u[ci + n * sn] = uc[n];
-        w[ci + n * sn] = wc[n];
+        // streamed: read again only by the next stage's kernels
+        __builtin_nontemporal_store(uc[n], &u[ci + n * sn]);
+        __builtin_nontemporal_store(wc[n], &w[ci + n * sn]);
       }
       // x1 ghosts mirrored from the cells next to the walls
       const int mt = ie - i, mb = i - is;  // distance from top / bottom wall
