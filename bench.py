@@ -214,6 +214,10 @@ def roofline(blk, cfg, stats):
     alg = algorithmic(blk, cfg)
     ops = {}
     for op, names in OPS.items():
+        # implicit runs update the state inside the downward sweep: the
+        # ghost kernels left in "update" move no algorithmic bytes
+        if op == "update" and "k_update" not in stats:
+            continue
         per_launch_ms = sum(stats[n][0] for n in names if n in stats)
         per_step_ms = sum(stats[n][0] * stats[n][1] for n in names
                           if n in stats)
