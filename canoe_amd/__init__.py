@@ -148,6 +148,8 @@ _SIGS = {
                             ctypes.POINTER(ctypes.c_longlong)]),
     "cg_run_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "cg_run_sums": (_I, [_P, _P]),
+    "cg_run_history": (_I, [_P, _P]),
+    "cg_run_debug": (_I, [_P, _P, _P, _P, _P]),
     "cg_run_set_timing": (_I, [_P, _I]),
     "cg_run_kernel_stats": (_I, [_P, _I, ctypes.POINTER(_D),
                                  ctypes.POINTER(ctypes.c_longlong),
@@ -454,6 +456,21 @@ class Run:
         self.ctx._check(lib().cg_run_block_sums(self.h, b, _ptr(out)),
                         "cg_run_sums")
         return out
+
+    def history(self):
+        """Athena++ default history sums over all blocks (8 + nmass)."""
+        out = np.zeros(8 + self.th.nmass)
+        self.ctx._check(lib().cg_run_history(self.h, _ptr(out)), "history")
+        return out
+
+    def debug(self):
+        """DebugOutput: (vmin, vmax, xmin (N, 3), xmax (N, 3)) of w."""
+        N = self.blocks[0].N
+        vmin, vmax = np.zeros(N), np.zeros(N)
+        xmin, xmax = np.zeros((N, 3)), np.zeros((N, 3))
+        self.ctx._check(lib().cg_run_debug(self.h, _ptr(vmin), _ptr(vmax),
+                                           _ptr(xmin), _ptr(xmax)), "debug")
+        return vmin, vmax, xmin, xmax
 
     def set_mlp(self, net):
         """Network of the guided saturation adjustment (config satadj =

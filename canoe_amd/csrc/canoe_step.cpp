@@ -9,13 +9,17 @@
 // box, 4: moist box, 5: one seeded block of the uneven-column batch).
 //
 //   canoe_step <config> [--nx nx1 nx2 nx3] [--nlim N] [--tlim T] [--cfl C]
-//              [--fp32] [--no-graph] [--seed S] [--out K]
+//              [--fp32] [--no-graph] [--seed S] [--out K] [--hst] [--dbg]
+//              [--basename NAME]
 //
 // The cycles run on the device through cg_run_advance: each step is
 // followed by the patched Hydro::NewBlockTimeStep reduced over the resident
 // state (patches/21.new_blockdt.patch:16-45) and Mesh::NewTimeStep's limits;
 // the host sees one double per cycle and prints the cycle diagnostics every
-// K cycles (--out).
+// K cycles (--out).  At the same points --hst appends Athena++'s history
+// columns to NAME.hst and --dbg the per-variable extremes of DebugOutput
+// (src/outputs/debug.cpp:137-182) to NAME.dbg, in the reference's text
+// layouts with its %15.12e data format.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -58,7 +62,8 @@ void usage() {
   std::fprintf(stderr,
                "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
                "[--nlim N] [--tlim T] [--cfl C] [--fp32] "
-               "[--no-graph] [--seed S] [--out K]\n");
+               "[--no-graph] [--seed S] [--out K] [--hst] [--dbg] "
+               "[--basename NAME]\n");
 }
 
 int check(cg_ctx* ctx, int rc, const char* what) {
@@ -79,7 +84,8 @@ int main(int argc, char** argv) {
   int nx[3] = {0, 0, 0};
   long long nlim = 10, out_every = 1;
   double tlim = 1e300, cfl = 0.9;
-  bool fp32 = false, graph = true;
+  bool fp32 = false, graph = true, hst = false, dbg = false;
+  std::string base = "canoe";
   uint64_t seed = 1;
   for (int a = 2; a < argc; ++a) {
     std::string s = argv[a];
@@ -107,6 +113,13 @@ int main(int argc, char** argv) {
     } else if (s == "--seed") {
       need(1);
       seed = std::strtoull(argv[++a], nullptr, 10);
+    } else if (s == "--hst") {
+      hst = true;
+    } else if (s == "--dbg") {
+      dbg = true;
+    } else if (s == "--basename") {
+      need(1);
+      base = argv[++a];
     } else if (s == "--fp32") {
       fp32 = true;
     } else if (s == "--no-graph") {
@@ -151,9 +164,62 @@ int main(int argc, char** argv) {
   // the first step: the blocks' NewBlockTimeStep, cut to end at tlim
   int err = check(ctx, cg_run_new_dt(run, cfl, &dt), "cg_run_new_dt");
   if (!err && tlim - time < dt) dt = tlim - time;
+  FILE* fh = hst ? std::fopen((base + ".hst").c_str(), "w") : nullptr;
+  FILE* fd = dbg ? std::fopen((base + ".dbg").c_str(), "w") : nullptr;
+  if ((hst && !fh) || (dbg && !fd)) {
+    std::fprintf(stderr, "canoe_step: cannot open the output files\n");
+    return 1;
+  }
+  const char* fmt = " %15.12e";
+  if (fh) {
+    const char* cols[] = {"time", "dt", "mass", "1-mom", "2-mom", "3-mom",
+                          "1-KE", "2-KE", "3-KE", "tot-E"};
+    std::fprintf(fh, "# Athena++ history data\n#");
+    int c = 1;
+    for (const char* nm : cols) std::fprintf(fh, " [%d]=%-8s", c++, nm);
+    for (int n = 1; n <= blk.nmass; ++n) std::fprintf(fh, " [%d]=%d-mass  ", c++, n);
+    std::fprintf(fh, "\n");
+  }
+  if (fd) {
+    std::fprintf(fd, "# Athena++ debug data\n# ");
+    const char* cols[] = {"time", "id", "dmin", "dmax", "x1min", "x1max",
+                          "x2min", "x2max", "x3min", "x3max"};
+    int c = 1;
+    for (const char* nm : cols) std::fprintf(fd, "[%d]=%-8s", c++, nm);
+    std::fprintf(fd, "\n");
+  }
+  std::vector<double> hv(8 + blk.nmass), vmin(d.N), vmax(d.N), xmin(3 * d.N),
+      xmax(3 * d.N);
+  auto outputs = [&]() {
+    if (fh && !check(ctx, cg_run_history(run, hv.data()), "cg_run_history")) {
+      std::fprintf(fh, fmt, time);
+      std::fprintf(fh, fmt, dt);
+      for (double v : hv) std::fprintf(fh, fmt, v);
+      std::fprintf(fh, "\n");
+    }
+    if (fd && !check(ctx, cg_run_debug(run, vmin.data(), vmax.data(), xmin.data(),
+                                       xmax.data()),
+                     "cg_run_debug")) {
+      for (int n = 0; n < d.N; ++n) {
+        if (n == 0)
+          std::fprintf(fd, fmt, time);
+        else
+          std::fprintf(fd, "                ");
+        std::fprintf(fd, " --  %2d  -- ", n);
+        std::fprintf(fd, fmt, vmin[n]);
+        std::fprintf(fd, fmt, vmax[n]);
+        for (int q = 0; q < 3; ++q) {
+          std::fprintf(fd, fmt, xmin[3 * n + q]);
+          std::fprintf(fd, fmt, xmax[3 * n + q]);
+        }
+        std::fprintf(fd, "\n");
+      }
+    }
+  };
   auto t0 = std::chrono::steady_clock::now();
   while (!err && time < tlim && ncycle < nlim) {
     std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, time, dt);
+    outputs();
     const long long upto = std::min(nlim, ncycle + out_every);
     if ((err = check(ctx, cg_run_advance(run, cfl, tlim, upto, graph ? 1 : 0, &time,
                                          &dt, &ncycle),
@@ -161,6 +227,9 @@ int main(int argc, char** argv) {
       break;
   }
   auto t1 = std::chrono::steady_clock::now();
+  if (!err) outputs();
+  if (fh) std::fclose(fh);
+  if (fd) std::fclose(fd);
   double wall = std::chrono::duration<double>(t1 - t0).count();
   check(ctx, cg_run_sums(run, sums.data()), "sums");
   std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, time, dt);

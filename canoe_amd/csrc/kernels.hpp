@@ -1547,6 +1547,136 @@ static __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) out[0] = cfl * m;
 }
 
+// Athena++'s default history sums of every block of a run (blockIdx.y =
+// block): volume-weighted u(IDN), momenta, kinetic energies 0.5 m^2 / u(IDN),
+// u(IEN), then the species densities; 8 + NM partials per workgroup at
+// part[((blockIdx.y gridDim.x + blockIdx.x) (8 + NM) + q], summed on the host
+// in a fixed order.
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_history(const KBlockPtr* __restrict__ bp, double* __restrict__ part) {
+  constexpr int NQ = 8 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  __shared__ double sh[256];
+  const KGrid& g = bp[blockIdx.y].g;
+  const double* u = gptr(bp[blockIdx.y].u);
+  const long long total = (long long)g.nx1 * g.nx2 * g.nx3;
+  const double vol = g.dx1 * g.dx2 * g.dx3;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+    const long long c0 = g.at(0, k + g.ks, j + g.js, i + g.is);
+    const double d = u[c0];
+    const double m1 = u[c0 + IVX * g.c.n], m2 = u[c0 + (IVX + 1) * g.c.n],
+                 m3 = u[c0 + (IVX + 2) * g.c.n];
+    acc[0] += vol * d;
+    acc[1] += vol * m1;
+    acc[2] += vol * m2;
+    acc[3] += vol * m3;
+    acc[4] += vol * 0.5 * (m1 * m1) / d;
+    acc[5] += vol * 0.5 * (m2 * m2) / d;
+    acc[6] += vol * 0.5 * (m3 * m3) / d;
+    acc[7] += vol * u[c0 + IEN * g.c.n];
+#pragma unroll
+    for (int n = 1; n <= NM; ++n) acc[7 + n] += vol * u[c0 + n * g.c.n];
+  }
+  const long long pb = ((long long)blockIdx.y * gridDim.x + blockIdx.x) * NQ;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    sh[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[pb + q] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Minimum and maximum of each variable of w over the interior cells of every
+// block of a run (DebugOutput::WriteOutputFile, src/outputs/debug.cpp:30-84):
+// a cell is a candidate for the minimum when w < FLT_MAX and for the maximum
+// when w > FLT_MIN (the reference's initial values under strict
+// comparisons); among candidates the smallest (largest) value wins, ties to
+// the first cell in (k, j, i) order.  Per workgroup and variable: value and
+// interior index ((k - ks) nx2 + j - js) nx1 + i - is (-1: no candidate).
+__device__ inline void better_min(double v, long long x, double& bv, long long& bx) {
+  if (x >= 0 && (bx < 0 || v < bv || (v == bv && x < bx))) {
+    bv = v;
+    bx = x;
+  }
+}
+__device__ inline void better_max(double v, long long x, double& bv, long long& bx) {
+  if (x >= 0 && (bx < 0 || v > bv || (v == bv && x < bx))) {
+    bv = v;
+    bx = x;
+  }
+}
+
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_minmax(const KBlockPtr* __restrict__ bp, double* __restrict__ pv,
+             long long* __restrict__ px) {
+  constexpr int N = 5 + NM;
+  constexpr double FMAXV = 3.40282346638528859812e+38, FMINV = 1.17549435082228750797e-38;
+  __shared__ double sv[256];
+  __shared__ long long sx[256];
+  const KGrid& g = bp[blockIdx.y].g;
+  const double* w = gptr(bp[blockIdx.y].w);
+  const long long total = (long long)g.nx1 * g.nx2 * g.nx3;
+  double mn[N], mx[N];
+  long long imn[N], imx[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    mn[n] = mx[n] = 0.0;
+    imn[n] = imx[n] = -1;
+  }
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+    const long long c0 = g.at(0, k + g.ks, j + g.js, i + g.is);
+    const long long x = ((long long)k * g.nx2 + j) * g.nx1 + i;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const double v = w[c0 + n * g.c.n];
+      if (v < FMAXV) better_min(v, x, mn[n], imn[n]);
+      if (v > FMINV) better_max(v, x, mx[n], imx[n]);
+    }
+  }
+  const long long pb = ((long long)blockIdx.y * gridDim.x + blockIdx.x) * N;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    for (int side = 0; side < 2; ++side) {
+      sv[threadIdx.x] = side ? mx[n] : mn[n];
+      sx[threadIdx.x] = side ? imx[n] : imn[n];
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+          double bv = sv[threadIdx.x];
+          long long bx = sx[threadIdx.x];
+          if (side)
+            better_max(sv[threadIdx.x + o], sx[threadIdx.x + o], bv, bx);
+          else
+            better_min(sv[threadIdx.x + o], sx[threadIdx.x + o], bv, bx);
+          sv[threadIdx.x] = bv;
+          sx[threadIdx.x] = bx;
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        pv[2 * (pb + n) + side] = sv[0];
+        px[2 * (pb + n) + side] = sx[0];
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // out[0] = v (a device scalar written in stream order).
 static __global__ void k_set_scalar(double* __restrict__ out, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = v;

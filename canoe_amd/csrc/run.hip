@@ -706,6 +706,107 @@ int cg_run_block_sums(cg_run* r, int b, double* out_host) {
 
 int cg_run_sums(cg_run* r, double* out_host) { return cg_run_block_sums(r, 0, out_host); }
 
+int cg_run_history(cg_run* r, double* out_host) {
+  if (!r || !out_host) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int nb = (int)r->blocks.size(), P = 64, NQ = 8 + r->th.nmass;
+  double* part = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&part, sizeof(double) * nb * P * NQ, ctx->stream));
+  if (r->N == 5)
+    k_history<0><<<dim3(P, nb), 256, 0, ctx->stream>>>(r->d_bp, part);
+  else
+    k_history<3><<<dim3(P, nb), 256, 0, ctx->stream>>>(r->d_bp, part);
+  std::vector<double> h((size_t)nb * P * NQ);
+  hipError_t le = hipGetLastError();
+  hipError_t ce = hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(),
+                                 hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipFreeAsync(part, ctx->stream);
+  HIPCHK(ctx, le);
+  HIPCHK(ctx, ce);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < NQ; ++q) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b)
+      for (int p = 0; p < P; ++p) s += h[((size_t)b * P + p) * NQ + q];
+    out_host[q] = s;
+  }
+  return CG_OK;
+}
+
+int cg_run_debug(cg_run* r, double* vmin, double* vmax, double* xmin, double* xmax) {
+  if (!r || !vmin || !vmax || !xmin || !xmax) return CG_ERR_ARG;
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int nb = (int)r->blocks.size(), P = 64, N = r->N;
+  const size_t np = (size_t)nb * P * N * 2;
+  double* pv = nullptr;
+  long long* px = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&pv, sizeof(double) * np, ctx->stream));
+  HIPCHK(ctx, hipMallocAsync((void**)&px, sizeof(long long) * np, ctx->stream));
+  if (N == 5)
+    k_minmax<0><<<dim3(P, nb), 256, 0, ctx->stream>>>(r->d_bp, pv, px);
+  else
+    k_minmax<3><<<dim3(P, nb), 256, 0, ctx->stream>>>(r->d_bp, pv, px);
+  std::vector<double> hv(np);
+  std::vector<long long> hx(np);
+  hipError_t le = hipGetLastError();
+  hipError_t c1 = hipMemcpyAsync(hv.data(), pv, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                 ctx->stream);
+  hipError_t c2 = hipMemcpyAsync(hx.data(), px, sizeof(long long) * np,
+                                 hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipFreeAsync(pv, ctx->stream);
+  (void)hipFreeAsync(px, ctx->stream);
+  HIPCHK(ctx, le);
+  HIPCHK(ctx, c1);
+  HIPCHK(ctx, c2);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // DebugOutput's initial values and strict comparisons over the blocks in
+  // order; within a block the device candidates tie to the first cell
+  const double FMAXV = 3.40282346638528859812e+38, FMINV = 1.17549435082228750797e-38;
+  for (int n = 0; n < N; ++n) {
+    vmin[n] = FMAXV;
+    vmax[n] = FMINV;
+    for (int d = 0; d < 3; ++d) {
+      xmin[3 * n + d] = FMAXV;
+      xmax[3 * n + d] = FMINV;
+    }
+    for (int b = 0; b < nb; ++b) {
+      const cg_block& bk = r->blocks[b].blk;
+      double bv[2] = {0.0, 0.0};
+      long long bx[2] = {-1, -1};
+      for (int p = 0; p < P; ++p)
+        for (int side = 0; side < 2; ++side) {
+          const size_t e = 2 * (((size_t)b * P + p) * N + n) + side;
+          const double v = hv[e];
+          const long long x = hx[e];
+          if (x < 0) continue;
+          const bool take = bx[side] < 0 ||
+                            (side ? v > bv[side] : v < bv[side]) ||
+                            (v == bv[side] && x < bx[side]);
+          if (take) {
+            bv[side] = v;
+            bx[side] = x;
+          }
+        }
+      for (int side = 0; side < 2; ++side) {
+        if (bx[side] < 0) continue;
+        double& best = side ? vmax[n] : vmin[n];
+        if (side ? !(bv[side] > best) : !(bv[side] < best)) continue;
+        best = bv[side];
+        const long long x = bx[side];
+        const int i = (int)(x % bk.nx1), j = (int)((x / bk.nx1) % bk.nx2),
+                  k = (int)(x / ((long long)bk.nx1 * bk.nx2));
+        double* xo = side ? xmax : xmin;
+        xo[3 * n] = bk.x1min + (i + 0.5) * bk.dx1;
+        xo[3 * n + 1] = bk.x2min + (j + 0.5) * bk.dx2;
+        xo[3 * n + 2] = bk.x3min + (k + 0.5) * bk.dx3;
+      }
+    }
+  }
+  return CG_OK;
+}
+
 int cg_run_set_mlp(cg_run* r, const cg_mlp* net) {
   if (!r) return CG_ERR_ARG;
   if (net && r->cfg.satadj != CG_SATADJ_MLP)

@@ -267,6 +267,18 @@ int cg_run_device_state(cg_run* run, double** u, double** w);
 /* FP64 global sums of the conserved variables over interior cells times the
  * cell volume (the reference's `hst` history totals).  out: N doubles. */
 int cg_run_sums(cg_run* run, double* out_host);
+/* Athena++'s default history columns (upstream outputs/history.cpp), summed
+ * over every block of the run: volume-weighted u(IDN), the three momenta,
+ * the three kinetic energies 0.5 m_d^2 / u(IDN) and u(IEN), then the species
+ * densities u(1..nmass).  out: 8 + nmass doubles. */
+int cg_run_history(cg_run* run, double* out_host);
+/* Replaces DebugOutput::WriteOutputFile (src/outputs/debug.cpp:30-84): per
+ * variable n of w, the minimum and maximum over the interior cells of every
+ * block and the cell-centre coordinates (x1, x2, x3) where they occur
+ * (xmin / xmax: 3 N doubles); the reference's initial values FLT_MAX /
+ * FLT_MIN and strict comparisons in block, k, j, i order. */
+int cg_run_debug(cg_run* run, double* vmin, double* vmax, double* xmin,
+                 double* xmax);
 /* Per-kernel timing: with timing on, cg_run_step launches without the graph
  * and records HIP events around every kernel on the run's stream.
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of

@@ -1632,6 +1632,74 @@ double or_new_mesh_dt(double dt_old, double blocks_dt, double time,
   return new_mesh_dt(dt_old, blocks_dt, time, tlim);
 }
 
+// Athena++'s default history columns (upstream outputs/history.cpp, not
+// vendored): volume-weighted sums of u(IDN), the three momenta, the three
+// kinetic energies 0.5 m_d^2 / u(IDN) and u(IEN), followed here by the sums
+// of the species densities u(1..nmass).  out: 8 + nmass doubles.
+int or_history(const cg_block* blk, const double* u, double* out) {
+  Grid g(*blk);
+  const double vol = blk->dx1 * blk->dx2 * blk->dx3;
+  const int nq = 8 + g.nmass;
+  for (int q = 0; q < nq; ++q) out[q] = 0.0;
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        const double d = u[g.at(0, k, j, i)];
+        const double m1 = u[g.at(g.IVX, k, j, i)], m2 = u[g.at(g.IVY, k, j, i)],
+                     m3 = u[g.at(g.IVZ, k, j, i)];
+        out[0] += vol * d;
+        out[1] += vol * m1;
+        out[2] += vol * m2;
+        out[3] += vol * m3;
+        out[4] += vol * 0.5 * (m1 * m1) / d;
+        out[5] += vol * 0.5 * (m2 * m2) / d;
+        out[6] += vol * 0.5 * (m3 * m3) / d;
+        out[7] += vol * u[g.at(g.IPR, k, j, i)];
+        for (int n = 1; n <= g.nmass; ++n) out[7 + n] += vol * u[g.at(n, k, j, i)];
+      }
+  return CG_OK;
+}
+
+// DebugOutput::WriteOutputFile (src/outputs/debug.cpp:30-84): per variable,
+// the minimum and maximum of w over interior cells (initial values FLT_MAX
+// and FLT_MIN, strict comparisons, so the first cell in (k, j, i) order wins
+// ties) and the cell-centre coordinates (x1, x2, x3) where they occur (left
+// at FLT_MAX / FLT_MIN when no cell qualifies).  xmin, xmax: 3 N doubles.
+int or_debug(const cg_block* blk, const double* w, double* vmin, double* vmax,
+             double* xmin, double* xmax) {
+  Grid g(*blk);
+  const double fmax = 3.40282346638528859812e+38, fmin = 1.17549435082228750797e-38;
+  for (int n = 0; n < g.N; ++n) {
+    vmin[n] = fmax;
+    vmax[n] = fmin;
+    for (int d = 0; d < 3; ++d) {
+      xmin[3 * n + d] = fmax;
+      xmax[3 * n + d] = fmin;
+    }
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        for (int i = g.is; i <= g.ie; ++i) {
+          const double v = w[g.at(n, k, j, i)];
+          const double x1 = blk->x1min + (i - g.is + 0.5) * blk->dx1;
+          const double x2 = blk->x2min + (j - g.js + 0.5) * blk->dx2;
+          const double x3 = blk->x3min + (k - g.ks + 0.5) * blk->dx3;
+          if (v < vmin[n]) {
+            vmin[n] = v;
+            xmin[3 * n] = x1;
+            xmin[3 * n + 1] = x2;
+            xmin[3 * n + 2] = x3;
+          }
+          if (v > vmax[n]) {
+            vmax[n] = v;
+            xmax[3 * n] = x1;
+            xmax[3 * n + 1] = x2;
+            xmax[3 * n + 2] = x3;
+          }
+        }
+  }
+  return CG_OK;
+}
+
 int or_sums(const cg_block* blk, const double* u, double* out) {
   Grid g(*blk);
   const double vol = blk->dx1 * blk->dx2 * blk->dx3;

@@ -75,6 +75,11 @@ double or_new_block_dt(const cg_block* blk, const cg_thermo* th,
 double or_new_mesh_dt(double dt_old, double blocks_dt, double time,
                       double tlim);
 int or_sums(const cg_block* blk, const double* u, double* out);
+/* Athena++ default history sums (+ species) of one block: 8 + nmass. */
+int or_history(const cg_block* blk, const double* u, double* out);
+/* DebugOutput: per-variable min / max of w and their cell centres. */
+int or_debug(const cg_block* blk, const double* w, double* vmin, double* vmax,
+             double* xmin, double* xmax);
 int or_block_thomas(int B, int n, const double* a, const double* b,
                     const double* c, const double* r, double* x);
 int or_flux_jacobian(const cg_block* blk, double gm1, const double* w,

@@ -35,6 +35,8 @@ _SIGS = {
     "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
     "or_sums": (_I, [_P, _P, _P]),
     "or_new_block_dt": (_D, [_P, _P, _P, _P, _D]),
+    "or_history": (_I, [_P, _P, _P]),
+    "or_debug": (_I, [_P, _P, _P, _P, _P, _P]),
     "or_new_mesh_dt": (_D, [_D, _D, _D, _D]),
     "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
     "or_flux_jacobian": (_I, [_P, _D, _P, _P]),
@@ -160,6 +162,23 @@ def new_block_dt(blk, th, cfg, w, cfl):
     """Hydro::NewBlockTimeStep of the oracle (cfl times the minimum step)."""
     w = np.ascontiguousarray(w, np.float64)
     return lib().or_new_block_dt(ref(blk), ref(th), ref(cfg), ptr(w), cfl)
+
+
+def history(blk, u):
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(8 + blk.nmass)
+    lib().or_history(ref(blk), ptr(u), ptr(out))
+    return out
+
+
+def debug(blk, w):
+    """(vmin, vmax, xmin (N, 3), xmax (N, 3)) of DebugOutput."""
+    w = np.ascontiguousarray(w, np.float64)
+    N = blk.N
+    vmin, vmax = np.zeros(N), np.zeros(N)
+    xmin, xmax = np.zeros((N, 3)), np.zeros((N, 3))
+    lib().or_debug(ref(blk), ptr(w), ptr(vmin), ptr(vmax), ptr(xmin), ptr(xmax))
+    return vmin, vmax, xmin, xmax
 
 
 def new_mesh_dt(dt_old, blocks_dt, time, tlim):

@@ -40,3 +40,24 @@ def test_cpp_driver_stops_at_tlim():
     last = re.findall(r"cycle=(\d+) time=(\S+) dt=(\S+)", r.stdout)[-1]
     assert abs(float(last[1]) - 3.0) <= 1e-12
     assert int(last[0]) > 1
+
+
+def test_cpp_driver_history_and_debug_files(tmp_path):
+    """--hst / --dbg write the reference's text layouts: one history row per
+    diagnostics point (mass conserved to round-off) and N rows of extremes."""
+    base = str(tmp_path / "box")
+    r = subprocess.run([EXE, "2", "--nx", "32", "16", "16", "--nlim", "6",
+                        "--out", "2", "--cfl", "0.45", "--hst", "--dbg",
+                        "--basename", base], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    hst = open(base + ".hst").read().splitlines()
+    assert hst[0] == "# Athena++ history data" and "[10]=tot-E" in hst[1]
+    rows = [list(map(float, ln.split())) for ln in hst[2:]]
+    assert len(rows) == 4  # cycles 0, 2, 4 and the final state
+    assert [round(rw[0] > 0) for rw in rows] == [0, 1, 1, 1]
+    mass = [rw[2] for rw in rows]
+    assert max(abs(m - mass[0]) for m in mass) <= 1e-13 * mass[0]
+    dbg = open(base + ".dbg").read().splitlines()
+    assert dbg[0] == "# Athena++ debug data"
+    assert len(dbg) == 2 + 4 * 5 and " --   4  -- " in dbg[6]
