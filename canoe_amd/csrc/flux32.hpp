@@ -697,13 +697,26 @@ __device__ __forceinline__ void mix_fused_tile(
   const double wv = src.weight() / (g.dx1 * g.dx2 * g.dx3);
   const int jc = min(j, g.je), kc = min(k, g.ke);  // lanes past the block
   const long long pc = g.pcol(kc, jc) * g.ps_c;
+  // column offsets, fixed over the march: the own column and the plane cells
+  // this lane stages (the level adds i times the level stride)
+  const long long colb = (long long)kc * g.c.k + (long long)jc * g.c.j;
+  constexpr int NPL = (NP + NT - 1) / NT;
+  long long pbase[NPL];
+#pragma unroll
+  for (int m = 0; m < NPL; ++m) {
+    const int e = min(t + m * NT, NP - 1);
+    const int ej = e % FT_PJ, ek = e / FT_PJ;
+    const int gj = min(max(j0 - FT_H + ej, 0), g.nc2 - 1);
+    const int gk = min(max(k0 - FT_H + ek, 0), g.nc3 - 1);
+    pbase[m] = (long long)gk * g.c.k + (long long)gj * g.c.j;
+  }
   // x1 window (own column)
   float q[N][6];
   float rdv[6];
   auto load1 = [&](int m, int i) {
     double wc[N];
 #pragma unroll
-    for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, kc, jc, i)];
+    for (int n = 0; n < N; ++n) wc[n] = w[colb + n * g.c.n + (long long)i * g.c.i];
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
     if (decomp) {
@@ -750,20 +763,22 @@ __device__ __forceinline__ void mix_fused_tile(
     const int inew = i + 1 - 3 + HI;
     double wn[N], psvn = 0.0;
 #pragma unroll
-    for (int n = 0; n < N; ++n) wn[n] = w[g.at(n, kc, jc, inew)];
+    for (int n = 0; n < N; ++n) wn[n] = w[colb + n * g.c.n + (long long)inew * g.c.i];
     if (decomp) psvn = psv[pc + inew * g.ps_i];
     const double pfp = psf[pc + (i + 1) * g.ps_i];
     // (1) the level-i plane of the tile with its halo
-    for (int e = t; e < NP; e += NT) {
-      const int ej = e % FT_PJ, ek = e / FT_PJ;
-      const int gj = min(max(j0 - FT_H + ej, 0), g.nc2 - 1);
-      const int gk = min(max(k0 - FT_H + ek, 0), g.nc3 - 1);
-      const long long c = g.at(0, gk, gj, i);
 #pragma unroll
-      for (int n = 0; n < N; ++n) {
-        double v = w[c + n * g.c.n];
-        if (n == IPR) v -= prd;
-        plane[n][ek][ej] = (float)v;
+    for (int m = 0; m < NPL; ++m) {
+      const int e = t + m * NT;
+      if (e < NP) {
+        const long long c = pbase[m] + (long long)i * g.c.i;
+        float* pl = &plane[0][0][0] + e;
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          double v = w[c + n * g.c.n];
+          if (n == IPR) v -= prd;
+          pl[n * NP] = (float)v;
+        }
       }
     }
     __syncthreads();
@@ -842,7 +857,9 @@ __device__ __forceinline__ void mix_fused_tile(
       }
       add_sources_cell<NM>(g, th, src, w, k, j, i, d);
 #pragma unroll
-      for (int n = 0; n < N; ++n) __builtin_nontemporal_store(d[n], &du[g.at(n, k, j, i)]);
+      for (int n = 0; n < N; ++n)
+        __builtin_nontemporal_store(
+            d[n], &du[colb + n * g.c.n + (long long)i * g.c.i]);
     }
 #pragma unroll
     for (int n = 0; n < N; ++n) fm[n] = fp[n];
