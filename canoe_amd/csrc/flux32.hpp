@@ -883,6 +883,7 @@ __global__ void __launch_bounds__(FT_J * FT_K)
   __shared__ FusedLds<NM, FT_J, FT_K> L;
   const KTile tt = tiles[blockIdx.x];
   const int b = __builtin_amdgcn_readfirstlane(tt.b);
+  if (b < 0) return;  // padding of the table
   const KBlockPtr& p = bp[b];
   mix_fused_tile<NM, OH, OV, FT_J, FT_K>(
       p.g, th, src, rsolver, __builtin_amdgcn_readfirstlane(tt.j0),

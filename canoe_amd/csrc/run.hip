@@ -366,15 +366,28 @@ int ensure_tiles(cg_run* r) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return r->blocks[a].g.nx1 > r->blocks[b].g.nx1;
   });
-  std::vector<KTile> tiles;
+  // Workgroup w runs on XCD w % 8: each XCD gets a band of tile rows (k)
+  // of every block and level segment, so that a tile's halo neighbours
+  // mostly run on the same XCD and share its L2.  Queues of unequal length
+  // are padded with empty tiles (block -1).
+  std::vector<KTile> q[8];
   for (int b : order) {
     const KGrid& g = r->blocks[b].g;
     if (!(g.f2 && g.f3)) continue;
+    const int nbk = (g.nx3 + tk - 1) / tk;
     for (int c0 = g.is; c0 <= g.ie; c0 += seg)
-      for (int k0 = g.ks; k0 <= g.ke; k0 += tk)
+      for (int rk = 0; rk < nbk; ++rk) {
+        const int x = (int)((long long)rk * 8 / nbk);
         for (int j0 = g.js; j0 <= g.je; j0 += tj)
-          tiles.push_back(KTile{b, j0, k0, c0, std::min(c0 + seg, g.ie + 1)});
+          q[x].push_back(KTile{b, j0, g.ks + rk * tk, c0, std::min(c0 + seg, g.ie + 1)});
+      }
   }
+  size_t qmax = 0;
+  for (int x = 0; x < 8; ++x) qmax = std::max(qmax, q[x].size());
+  std::vector<KTile> tiles;
+  for (size_t slot = 0; slot < qmax; ++slot)
+    for (int x = 0; x < 8; ++x)
+      tiles.push_back(slot < q[x].size() ? q[x][slot] : KTile{-1, 0, 0, 0, 0});
   if (r->d_tiles) (void)hipFree(r->d_tiles);
   r->d_tiles = nullptr;
   r->ntiles = (int)tiles.size();
