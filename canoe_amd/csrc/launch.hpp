@@ -24,6 +24,11 @@ inline int check_block(cg_ctx* ctx, const cg_block* b, const cg_thermo* th) {
     return set_err(ctx, CG_ERR_ARG, "nmass must be 0 or 3 (got %d)", b->nmass);
   if (b->nx3 > 1 && b->nx2 == 1)
     return set_err(ctx, CG_ERR_ARG, "nx3 > 1 requires nx2 > 1");
+  // each side's NGHOST ghost cells are filled from NGHOST interior cells
+  if (b->nx1 < CG_NGHOST || (b->nx2 > 1 && b->nx2 < CG_NGHOST) ||
+      (b->nx3 > 1 && b->nx3 < CG_NGHOST))
+    return set_err(ctx, CG_ERR_ARG, "a direction with cells needs at least %d of them",
+                   CG_NGHOST);
   if (b->dx1 <= 0 || b->dx2 <= 0 || b->dx3 <= 0)
     return set_err(ctx, CG_ERR_ARG, "non-positive spacing");
   for (int s = 0; s < 6; ++s)

@@ -41,4 +41,5 @@ def test_python_constants_match_the_header():
     npar = int(re.search(r"#define CG_MLP_NPARAM (\d+)", src).group(1))
     assert ca.NUM_KERNEL_IDS == nk
     assert cm.NPARAM == npar
-    assert ctypes.sizeof(ca.Tune) == 5 * 4 + 4 + 8  # 5 ints, padding, double
+    assert ctypes.sizeof(ca.Tune) == 6 * 4 + 8  # 6 ints, double
+
