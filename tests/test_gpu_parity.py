@@ -179,3 +179,25 @@ def test_blocks_thinner_than_the_ghost_zone_are_rejected(ctx):
     blk, th, cfg = ca.pgen_setup(2, 16, 2, 6)
     with pytest.raises(ca.CanoeGpuError):
         ca.Run(ctx, blk, th, cfg)
+
+
+@pytest.mark.parametrize("flag", [1, 0])
+def test_step_partial_correction_and_explicit_3d(ctx, flag):
+    """The 3x3 partial correction (flag 1) and the explicit path (flag 0) on
+    a 3D block equal the oracle."""
+    blk, th, cfg, w = perturbed_state(2, (20, 12, 10), vel=1.0, dp=1e-3)
+    cfg.implicit_flag = flag
+    u = orl.prim_to_cons(blk, th, w)
+    dt = cfl_dt(blk, th, cfg, w, 0.3 if flag == 0 else 0.5)
+    if flag == 0:
+        dt = min(dt, 0.3 * blk.dx1 / 350.0)  # explicit x1: the sound speed sets it
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        run.step(dt, 2, graph=True)
+        ug = run.cons()
+    finally:
+        run.close()
+    uo, _ = orl.step(blk, th, cfg, dt, 2, u, w)
+    sk, sj, si = blk.interior()
+    assert relerr(ug[:, sk, sj, si], uo[:, sk, sj, si]) <= TOL
