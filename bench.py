@@ -566,9 +566,10 @@ def main():
                          "a few steps of the dry box, FP32 fluxes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(columns would shard by horizontal MeshBlocks; see "
-              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here "
+              "(the step would shard by horizontal MeshBlock columns, each "
+              "GPU owning whole columns, with a 3-cell halo exchange per "
+              "stage).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     global USE_GRAPH
     USE_GRAPH = not args.no_graph

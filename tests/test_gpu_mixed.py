@@ -2,8 +2,8 @@
 implicit solve) against the FP64 path on the same inputs.
 
 Bar: relative L2 error per conserved-variable group <= 1e-5 after a few
-steps (the north-star tolerance is stated after 100 steps of the full box;
-bench.py measures that one).
+steps (the tolerance of the mixed form is stated for 100 steps of the full
+box; bench.py measures that one).
 """
 import numpy as np
 import pytest
