@@ -1,8 +1,8 @@
 /* oracle.h -- CPU FP64 oracle of the canoe dynamical-core step.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load liboracle.so, and only as the checker
- * or the timed CPU baseline -- never on the product path.
+ * cpu_baseline leg of bench.py may load liboracle.so, and only as the reference
+ * the results are compared with or the timed CPU baseline -- never on the product path.
  *
  * Every function restates the reference algorithm on host memory laid out
  * as in include/canoe_gpu.h (AthenaArray (n,k,j,i), NGHOST = 3) and cites the

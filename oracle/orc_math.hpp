@@ -1,5 +1,5 @@
-// Oracle math routines -- TEST INFRASTRUCTURE ONLY (the checker, never the
-// product path).
+// Oracle math routines -- TEST INFRASTRUCTURE ONLY (results are compared
+// with them; never on the product path).
 //
 // exp and log written from their textbook definitions so that the CPU oracle
 // and the device code (which carries its own copy of the same algorithm)

@@ -1,5 +1,5 @@
 // Oracle elementwise routines of the saturation-guess network -- TEST
-// INFRASTRUCTURE ONLY (the oracle is the checker, never the product).
+// INFRASTRUCTURE ONLY (the results are compared with it; never the product).
 //
 // The device library carries its own copy of the same algorithms, so that
 // everything outside the matrix products is evaluated identically on both

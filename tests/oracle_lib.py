@@ -1,6 +1,7 @@
 """ctypes loader of the CPU oracle (oracle/lib/liboracle.so).
 
-Test infrastructure: the oracle is the checker, never the thing measured.
+Test infrastructure: results are compared with the oracle; it is never the
+thing measured.
 """
 import ctypes
 import os
