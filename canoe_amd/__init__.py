@@ -26,7 +26,7 @@ MAX_NMASS = 3
 NUM_KERNEL_IDS = 17
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
-    ERR_SIZE, ERR_NODEV = range(9)
+    ERR_SIZE, ERR_NODEV, ERR_COLD = range(10)
 BC_REFLECT, BC_PERIODIC, BC_OUTFLOW = 0, 1, 2
 RS_LMARS, RS_HLLC = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
@@ -143,6 +143,7 @@ _SIGS = {
     "cg_run_get_prim_host": (_I, [_P, _P]),
     "cg_run_step": (_I, [_P, _D, _I, _I]),
     "cg_run_new_dt": (_I, [_P, _D, ctypes.POINTER(_D)]),
+    "cg_run_check": (_I, [_P, _P]),
     "cg_run_advance": (_I, [_P, _D, _D, ctypes.c_longlong, _I,
                             ctypes.POINTER(_D), ctypes.POINTER(_D),
                             ctypes.POINTER(ctypes.c_longlong)]),
@@ -438,6 +439,16 @@ class Run:
         self.ctx._check(lib().cg_run_new_dt(self.h, cfl, ctypes.byref(out)),
                         "cg_run_new_dt")
         return out.value
+
+    def check(self):
+        """State checks of the resident w (check_hydro_variables): returns
+        (status, (block, k, j, i)) with status 0 or ERR_NAN / ERR_NEGATIVE /
+        ERR_COLD and the first failing interior cell."""
+        where = np.zeros(4, np.int32)
+        r = lib().cg_run_check(self.h, _ptr(where))
+        if r not in (0, ERR_NAN, ERR_NEGATIVE, ERR_COLD):
+            self.ctx._check(r, "cg_run_check")
+        return r, tuple(int(x) for x in where)
 
     def advance(self, cfl, tlim=float("inf"), nlim=-1, time=0.0, dt=0.0,
                 ncycle=0, graph=True):

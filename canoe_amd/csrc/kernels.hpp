@@ -1513,24 +1513,65 @@ __device__ inline double wg_min256(double m, double* sh) {
   return sh[0];
 }
 
+// State checks of one interior cell (check_hydro_variables,
+// src/snap/eos/check_hydro_variables.cpp:21-64, and check_eos_cons2prim,
+// src/checks.cpp:52-75): 0 a NaN anywhere in w, 1 a negative density,
+// mass fraction or pressure, 2 a temperature below |g| dx1 / Rd (with
+// gravity), 3 none.  (The reference's negativity loop also runs over the
+// velocities, which its message and any flow contradict; it is restated for
+// the densities and mass fractions.)
+template <int NM>
+__device__ inline int cell_check(const KGrid& g, const KThermo& th,
+                                 const double* __restrict__ w, long long c0) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  double wi[N];
+  bool nan = false, neg = false;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    wi[n] = w[c0 + n * g.c.n];
+    nan = nan || (wi[n] != wi[n]);
+    if (n <= NM || n == IPR) neg = neg || (wi[n] < 0.0);
+  }
+  if (nan) return 0;
+  if (neg) return 1;
+  if (g.grav != 0.0 && temperature_d<NM>(th, wi) < fabs(g.grav) * g.dx1 / th.rd) return 2;
+  return 3;
+}
+
 // Per-workgroup minima of the cell steps.  Blocks of a run come from the
 // table bp (blockIdx.y = block, part[blockIdx.y gridDim.x + blockIdx.x]);
-// with bp null, the single block (g0, w0) of a caller array.
+// with bp null, the single block (g0, w0) of a caller array.  With bad
+// set, the state checks of every cell: the first failing cell in (block, k,
+// j, i) order and its check, as ((block 2^40 + index) 4 + check), by an
+// atomic minimum (bad starts at all ones).
 template <int NM>
 __global__ void __launch_bounds__(256)
     k_new_dt(const KBlockPtr* __restrict__ bp, KGrid g0,
              const double* __restrict__ w0, KThermo th, double cfl, int iflag,
-             double* __restrict__ part) {
+             double* __restrict__ part,
+             unsigned long long* __restrict__ bad = nullptr) {
   const KGrid g = bp ? bp[blockIdx.y].g : g0;
   const double* w = gptr(bp ? bp[blockIdx.y].w : w0);
   const long long total = (long long)g.nx1 * g.nx2 * g.nx3;
   double m = __DBL_MAX__;
+  unsigned long long first = ~0ULL;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     int i, j, k;
     g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
-    m = fmin(m, cell_new_dt<NM>(g, th, w, g.at(0, k + g.ks, j + g.js, i + g.is), cfl, iflag));
+    const long long c0 = g.at(0, k + g.ks, j + g.js, i + g.is);
+    m = fmin(m, cell_new_dt<NM>(g, th, w, c0, cfl, iflag));
+    if (bad) {
+      const int code = cell_check<NM>(g, th, w, c0);
+      if (code < 3) {
+        const unsigned long long x = ((unsigned long long)k * g.nx2 + j) * g.nx1 + i;
+        const unsigned long long key =
+            ((((unsigned long long)blockIdx.y << 40) | x) << 2) | (unsigned long long)code;
+        first = key < first ? key : first;
+      }
+    }
   }
+  if (bad && first != ~0ULL) atomicMin(bad, first);
   __shared__ double sh[256];
   m = wg_min256(m, sh);
   if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = m;
@@ -1678,6 +1719,12 @@ __global__ void __launch_bounds__(256)
 }
 
 // out[0] = v (a device scalar written in stream order).
+static __global__ void k_set_u64(unsigned long long* __restrict__ out,
+                                 unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = v;
+}
+
+
 static __global__ void k_set_scalar(double* __restrict__ out, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = v;
 }

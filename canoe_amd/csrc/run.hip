@@ -57,7 +57,9 @@ struct cg_run {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   // d_dt[0]: the step every stage kernel reads (a graph replays with any
-  // step); d_dt[1]: the last NewBlockTimeStep minimum; h_dt: pinned copy.
+  // step); d_dt[1]: the last NewBlockTimeStep minimum; d_dt[2]: the state
+  // checks' first failing cell (64-bit key, all ones when none); h_dt:
+  // pinned copy of d_dt[1..2].
   double* d_dt = nullptr;
   double* h_dt = nullptr;
   double* dt_part = nullptr;  // per-workgroup minima, dt_parts per block
@@ -423,27 +425,53 @@ int enqueue_new_dt(cg_run* r, double cfl) {
   dim3 grid(r->dt_parts, nb);
   KGrid g0 = r->blocks[0].g;
   KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  unsigned long long* bad = reinterpret_cast<unsigned long long*>(r->d_dt + 2);
+  k_set_u64<<<1, 64, 0, s>>>(bad, ~0ULL);
   if (r->N == 5)
     TIMED(tm, KID_NEW_DT,
           (k_new_dt<0><<<grid, 256, 0, s>>>(r->d_bp, g0, nullptr, r->kt, cfl,
-                                            r->cfg.implicit_flag, r->dt_part)));
+                                            r->cfg.implicit_flag, r->dt_part, bad)));
   else
     TIMED(tm, KID_NEW_DT,
           (k_new_dt<3><<<grid, 256, 0, s>>>(r->d_bp, g0, nullptr, r->kt, cfl,
-                                            r->cfg.implicit_flag, r->dt_part)));
+                                            r->cfg.implicit_flag, r->dt_part, bad)));
   k_new_dt_finish<<<1, 256, 0, s>>>(r->dt_part, r->dt_parts * nb, cfl, r->d_dt + 1);
   HIPCHK(r->ctx, hipGetLastError());
   return CG_OK;
 }
 
-// Reads d_dt[1] (and the status word) after the queued work.
-int read_new_dt(cg_run* r, double* out) {
+// Reads d_dt[1] and the state checks' key after the queued work; with
+// report, a failing cell becomes CG_ERR_NAN / CG_ERR_NEGATIVE / CG_ERR_COLD
+// with its place (where: block, k, j, i, or -1).
+int read_new_dt(cg_run* r, double* out, bool report, int* where = nullptr) {
   cg_ctx* ctx = r->ctx;
-  HIPCHK(ctx, hipMemcpyAsync(r->h_dt, r->d_dt + 1, sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(r->h_dt, r->d_dt + 1, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *out = r->h_dt[0];
-  return CG_OK;
+  unsigned long long key;
+  std::memcpy(&key, &r->h_dt[1], sizeof(key));
+  if (where)
+    for (int q = 0; q < 4; ++q) where[q] = -1;
+  if (key == ~0ULL || !report) return CG_OK;
+  const int code = (int)(key & 3ULL);
+  const unsigned long long cell = key >> 2;
+  const int b = (int)(cell >> 40);
+  const unsigned long long x = cell & ((1ULL << 40) - 1);
+  const KGrid& g = r->blocks[b].g;
+  const int i = (int)(x % g.nx1), j = (int)((x / g.nx1) % g.nx2),
+            k = (int)(x / ((unsigned long long)g.nx1 * g.nx2));
+  if (where) {
+    where[0] = b;
+    where[1] = k;
+    where[2] = j;
+    where[3] = i;
+  }
+  static const char* what[3] = {"NaN in the state", "negative density or pressure",
+                                "temperature below |g| dx1 / Rd"};
+  const int err[3] = {CG_ERR_NAN, CG_ERR_NEGATIVE, CG_ERR_COLD};
+  return set_err(ctx, err[code], "%s at block %d, interior cell (k, j, i) = (%d, %d, %d)",
+                 what[code], b, k, j, i);
 }
 
 }  // namespace
@@ -493,8 +521,8 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
     maxcells = std::max(maxcells, (long long)rb.g.nx1 * rb.g.nx2 * rb.g.nx3);
   r->dt_parts = (int)std::min(1024LL, std::max(1LL, maxcells / (256 * 16)));
   ok = ok && hipMalloc((void**)&r->part, sizeof(double) * 1024 * 8) == hipSuccess &&
-       hipMalloc((void**)&r->d_dt, sizeof(double) * 2) == hipSuccess &&
-       hipMemset(r->d_dt, 0, sizeof(double) * 2) == hipSuccess &&
+       hipMalloc((void**)&r->d_dt, sizeof(double) * 3) == hipSuccess &&
+       hipMemset(r->d_dt, 0, sizeof(double) * 3) == hipSuccess &&
        hipHostMalloc((void**)&r->h_dt, sizeof(double) * 2, hipHostMallocDefault) ==
            hipSuccess &&
        hipMalloc((void**)&r->dt_part, sizeof(double) * r->dt_parts * nblocks) == hipSuccess;
@@ -661,7 +689,16 @@ int cg_run_new_dt(cg_run* r, double cfl, double* dt_out) {
   if (!r || !dt_out || !(cfl > 0.0)) return CG_ERR_ARG;
   HIPCHK(r->ctx, hipSetDevice(r->ctx->device));
   int e = enqueue_new_dt(r, cfl);
-  if (!e) e = read_new_dt(r, dt_out);
+  if (!e) e = read_new_dt(r, dt_out, false);
+  return e;
+}
+
+int cg_run_check(cg_run* r, int* where) {
+  if (!r) return CG_ERR_ARG;
+  HIPCHK(r->ctx, hipSetDevice(r->ctx->device));
+  double dt = 0.0;
+  int e = enqueue_new_dt(r, 1.0);
+  if (!e) e = read_new_dt(r, &dt, true, where);
   return e;
 }
 
@@ -673,7 +710,7 @@ int cg_run_advance(cg_run* r, double cfl, double tlim, long long nlim,
   int e = CG_OK;
   double bdt = 0.0;
   if (!(*dt > 0.0)) {  // first step: the blocks' minimum, cut at tlim
-    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt))) return e;
+    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt, true))) return e;
     *dt = mesh_new_dt(__DBL_MAX__, bdt, *time, tlim);
   }
   if ((e = ensure_tiles(r))) return e;
@@ -685,7 +722,7 @@ int cg_run_advance(cg_run* r, double cfl, double tlim, long long nlim,
       HIPCHK(ctx, hipGraphLaunch(r->gexec, ctx->stream));
     else if ((e = run_one_step(r)))
       return e;
-    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt)) ||
+    if ((e = enqueue_new_dt(r, cfl)) || (e = read_new_dt(r, &bdt, true)) ||
         (e = check_status(ctx)))
       return e;
     *ncycle += 1;

@@ -40,7 +40,8 @@ enum cg_status {
   CG_ERR_PIVOT = 5,     /* singular 5x5 block in the implicit solve */
   CG_ERR_CONVERGE = 6,  /* saturation-adjustment Newton did not converge */
   CG_ERR_SIZE = 7,      /* size limit exceeded */
-  CG_ERR_NODEV = 8      /* no HIP device / library built without one */
+  CG_ERR_NODEV = 8,     /* no HIP device / library built without one */
+  CG_ERR_COLD = 9       /* temperature below |g| dx1 / Rd */
 };
 
 enum cg_bc { CG_BC_REFLECT = 0, CG_BC_PERIODIC = 1, CG_BC_OUTFLOW = 2 };
@@ -253,6 +254,14 @@ int cg_run_step(cg_run* run, double dt, int nsteps, int use_graph);
 /* The minimum of cg_new_block_dt over the run's blocks, reduced on the
  * device from the resident state (Mesh::NewTimeStep's min over blocks). */
 int cg_run_new_dt(cg_run* run, double cfl, double* dt_out);
+/* State checks of the resident w (check_hydro_variables,
+ * src/snap/eos/check_hydro_variables.cpp:21-64; check_eos_cons2prim,
+ * src/checks.cpp:52-75): CG_ERR_NAN for a NaN, CG_ERR_NEGATIVE for a
+ * negative density, mass fraction or pressure, CG_ERR_COLD for a temperature
+ * below |g| dx1 / Rd, else CG_OK.  where (nullable, 4 ints): block, and the
+ * interior k, j, i of the first failing cell in (block, k, j, i) order, or
+ * -1.  cg_run_advance runs the same checks after every step. */
+int cg_run_check(cg_run* run, int* where);
 /* The main integration loop of main.cpp:114-152 on the device: while
  * *time < tlim and (nlim < 0 or *ncycle < nlim), one step of size *dt, then
  * ++*ncycle, *time += *dt, and the next step as in Mesh::NewTimeStep

@@ -1700,6 +1700,44 @@ int or_debug(const cg_block* blk, const double* w, double* vmin, double* vmax,
   return CG_OK;
 }
 
+// State checks (check_hydro_variables, src/snap/eos/check_hydro_variables.
+// cpp:21-64; check_eos_cons2prim, src/checks.cpp:52-75) over interior cells
+// in (k, j, i) order: returns 0 for the first cell holding a NaN, 1 for a
+// negative density, mass fraction or pressure, 2 for a temperature below
+// |g| dx1 / Rd (with gravity), 3 when every cell passes; where = (k, j, i)
+// interior indices of that cell.
+int or_check(const cg_block* blk, const cg_thermo* th, const double* w,
+             int* where) {
+  Grid g(*blk);
+  where[0] = where[1] = where[2] = -1;
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i) {
+        double wi[CG_MAX_NMASS + 5];
+        bool nan = false, neg = false;
+        for (int n = 0; n < g.N; ++n) {
+          wi[n] = w[g.at(n, k, j, i)];
+          if (std::isnan(wi[n])) nan = true;
+          if ((n <= g.nmass || n == g.IPR) && wi[n] < 0.0) neg = true;
+        }
+        int code = 3;
+        if (nan)
+          code = 0;
+        else if (neg)
+          code = 1;
+        else if (blk->grav != 0.0 &&
+                 temperature(*th, wi, g.IPR) < std::fabs(blk->grav) * blk->dx1 / th->rd)
+          code = 2;
+        if (code < 3) {
+          where[0] = k - g.ks;
+          where[1] = j - g.js;
+          where[2] = i - g.is;
+          return code;
+        }
+      }
+  return 3;
+}
+
 int or_sums(const cg_block* blk, const double* u, double* out) {
   Grid g(*blk);
   const double vol = blk->dx1 * blk->dx2 * blk->dx3;

@@ -75,6 +75,9 @@ double or_new_block_dt(const cg_block* blk, const cg_thermo* th,
 double or_new_mesh_dt(double dt_old, double blocks_dt, double time,
                       double tlim);
 int or_sums(const cg_block* blk, const double* u, double* out);
+/* State checks: 0 NaN, 1 negative, 2 too cold, 3 pass; where = (k, j, i). */
+int or_check(const cg_block* blk, const cg_thermo* th, const double* w,
+             int* where);
 /* Athena++ default history sums (+ species) of one block: 8 + nmass. */
 int or_history(const cg_block* blk, const double* u, double* out);
 /* DebugOutput: per-variable min / max of w and their cell centres. */

@@ -37,6 +37,7 @@ _SIGS = {
     "or_sums": (_I, [_P, _P, _P]),
     "or_new_block_dt": (_D, [_P, _P, _P, _P, _D]),
     "or_history": (_I, [_P, _P, _P]),
+    "or_check": (_I, [_P, _P, _P, _P]),
     "or_debug": (_I, [_P, _P, _P, _P, _P, _P]),
     "or_new_mesh_dt": (_D, [_D, _D, _D, _D]),
     "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
@@ -163,6 +164,14 @@ def new_block_dt(blk, th, cfg, w, cfl):
     """Hydro::NewBlockTimeStep of the oracle (cfl times the minimum step)."""
     w = np.ascontiguousarray(w, np.float64)
     return lib().or_new_block_dt(ref(blk), ref(th), ref(cfg), ptr(w), cfl)
+
+
+def check(blk, th, w):
+    """(code, (k, j, i)): 0 NaN, 1 negative, 2 too cold, 3 pass."""
+    w = np.ascontiguousarray(w, np.float64)
+    where = np.zeros(3, np.int32)
+    code = lib().or_check(ref(blk), ref(th), ptr(w), ptr(where))
+    return code, tuple(int(x) for x in where)
 
 
 def history(blk, u):

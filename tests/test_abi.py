@@ -42,4 +42,8 @@ def test_python_constants_match_the_header():
     assert ca.NUM_KERNEL_IDS == nk
     assert cm.NPARAM == npar
     assert ctypes.sizeof(ca.Tune) == 6 * 4 + 8  # 6 ints, double
+    codes = dict((m.group(1), int(m.group(2)))
+                 for m in re.finditer(r"CG_(OK|ERR_[A-Z]+) = (\d+)", src))
+    for name, v in codes.items():
+        assert getattr(ca, name) == v, name
 

@@ -89,3 +89,53 @@ def test_debug_ties_and_batch_order(ctx):
                           b1.x3min + 0.5 * b1.dx3])
     hs = sum(orl.history(b[0], u) for b, u in zip(bl, us))
     np.testing.assert_allclose(h, hs, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("n,value,code", [(0, -1e-3, ca.ERR_NEGATIVE),
+                                          (7, -5.0, ca.ERR_NEGATIVE),
+                                          (4, np.nan, ca.ERR_NAN),
+                                          (7, 1e-3, ca.ERR_COLD)])
+def test_state_checks_match_oracle(ctx, n, value, code):
+    """The run's state checks report the first failing cell of the first
+    failing block, as the oracle's serial scan; the device main loop stops
+    with the same status before stepping."""
+    specs = [(24, 8, 8), (20, 10, 6), (24, 6, 12)]
+    bl = []
+    for b, dims in enumerate(specs):
+        blk, th, cfg = ca.pgen_setup(4, *dims)
+        w = ca.pgen_init(4, 90 + b, blk, th, cfg)
+        bl.append([blk, th, cfg, w])
+    for b in (1, 2):  # bad cells in blocks 1 and 2, two in block 1
+        for (k, j, i) in ((3, 1, 2), (1, 4, 7)):
+            if b == 2 and k == 3:
+                continue
+            bl[b][3][n, 3 + k, 3 + j, 3 + i] = value
+            if code == ca.ERR_COLD:
+                bl[b][3][1:4, 3 + k, 3 + j, 3 + i] = 0.0
+    th, cfg = bl[0][1], bl[0][2]
+    run = ca.Run(ctx, None, th, cfg, blocks=[b[0] for b in bl])
+    try:
+        for i, b in enumerate(bl):
+            run.set_prim(b[3], i)
+        st, where = run.check()
+        with pytest.raises(ca.CanoeGpuError) as ex:
+            run.advance(0.45, nlim=2)
+    finally:
+        run.close()
+    want = [orl.check(b[0], b[1], b[3]) for b in bl]
+    first = next(i for i, c in enumerate(want) if c[0] < 3)
+    assert first == 1
+    assert st == code and where == (first,) + want[first][1], (st, where, want)
+    assert ex.value.code == code
+
+
+def test_state_checks_pass_on_a_physical_run(ctx):
+    blk, th, cfg, w = perturbed_state(4, (20, 12, 10), vel=3.0)
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        assert run.check() == (0, (-1, -1, -1, -1))
+        t, dt, n = run.advance(0.45, nlim=2)
+        assert n == 2 and run.check()[0] == 0
+    finally:
+        run.close()

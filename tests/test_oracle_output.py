@@ -87,3 +87,39 @@ def test_debug_reference_initial_values_and_nan():
     assert vmax[0] == FLT_MIN and np.all(xmax[0] == FLT_MIN)
     assert vmin[2] == -1.0 and not np.isnan(vmin[2])
     assert xmin[2][0] == 100.0 + 1.5 * blk.dx1  # the NaN cell (first) skipped
+
+
+def _rest_state():
+    blk, th, cfg = ca.pgen_setup(4, 8, 6, 5)
+    w = ca.pgen_init(4, 3, blk, th, cfg)
+    return blk, th, cfg, w
+
+
+def test_check_passes_a_physical_state():
+    blk, th, cfg, w = _rest_state()
+    assert orl.check(blk, th, w) == (3, (-1, -1, -1))
+
+
+@pytest.mark.parametrize("n,value,code", [(0, -1e-3, 1), (2, -1e-9, 1),
+                                          (7, -5.0, 1), (4, np.nan, 0),
+                                          (1, np.nan, 0)])
+def test_check_finds_the_first_bad_cell(n, value, code):
+    # two bad cells: the one first in (k, j, i) order is reported; a
+    # negative velocity is not an error
+    blk, th, cfg, w = _rest_state()
+    w[4, 3 + 2, 3 + 1, 3 + 4] = -30.0  # v2 < 0: fine
+    w[n, 3 + 3, 3 + 0, 3 + 1] = value
+    w[n, 3 + 1, 3 + 5, 3 + 6] = value
+    assert orl.check(blk, th, w) == (code, (1, 5, 6))
+
+
+def test_check_temperature_floor():
+    # T below |g| dx1 / Rd (the reference's minimum temperature)
+    blk, th, cfg, w = _rest_state()
+    tmin = abs(blk.grav) * blk.dx1 / th.rd
+    rho = w[0, 3 + 2, 3 + 2, 3 + 2]
+    w[7, 3 + 2, 3 + 2, 3 + 2] = 0.5 * tmin * rho * th.rd  # dry-ish cell, T < tmin
+    w[1:4, 3 + 2, 3 + 2, 3 + 2] = 0.0
+    assert orl.check(blk, th, w) == (2, (2, 2, 2))
+    w[7, 3 + 2, 3 + 2, 3 + 2] = 2.0 * tmin * rho * th.rd
+    assert orl.check(blk, th, w)[0] == 3
