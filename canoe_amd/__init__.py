@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 
 NGHOST = 3
 MAX_NMASS = 3
-NUM_KERNEL_IDS = 17
+NUM_KERNEL_IDS = 18
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV, ERR_COLD = range(10)
@@ -374,13 +374,14 @@ class ImplicitSolver:
     def solve_implicit_3d(self, du, w, dt):
         """ImplicitSolver::SolveImplicit3D(du, w, dt): returns the new du."""
         ddu, dw = self.ctx.array(du), self.ctx.array(w)
-        self.ctx._check(lib().cg_implicit_solve(
-            self.ctx.h, _ref(self.blk), _ref(self.th), ddu.ptr, dw.ptr, dt,
-            self.flag), "cg_implicit_solve")
-        out = ddu.download()
-        ddu.free()
-        dw.free()
-        return out
+        try:
+            self.ctx._check(lib().cg_implicit_solve(
+                self.ctx.h, _ref(self.blk), _ref(self.th), ddu.ptr, dw.ptr, dt,
+                self.flag), "cg_implicit_solve")
+            return ddu.download()
+        finally:
+            ddu.free()
+            dw.free()
 
 
 class Tune(ctypes.Structure):

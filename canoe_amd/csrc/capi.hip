@@ -183,21 +183,39 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if (!ctx) return CG_ERR_ARG;
   int e = check_block(ctx, blk, th);
   if (e) return e;
+  e = check_implicit_flag(ctx, *blk, flag);
+  if (e) return e;
   if (flag == 0) return CG_OK;
-  if (flag & 6) return set_err(ctx, CG_ERR_ARG, "only x1 implicit (flag 1 or 9) is supported");
-  if (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC)
-    return set_err(ctx, CG_ERR_ARG, "periodic x1 implicit solve not supported");
   if (!(dt > 0.0)) return set_err(ctx, CG_ERR_ARG, "dt must be positive");
   KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
-  double* arch = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&arch, sizeof(double) * arch_doubles(g), ctx->stream));
-  if (blk->nmass == 0)
-    launch_implicit<0>(g, kt, dt, flag, du, w, arch, ctx->d_status, ctx->stream);
-  else
-    launch_implicit<3>(g, kt, dt, flag, du, w, arch, ctx->d_status, ctx->stream);
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipFreeAsync(arch, ctx->stream));
+  bool per[3];
+  periodic_dirs(*blk, per);
+  const bool act[3] = {true, blk->nx2 > 1, blk->nx3 > 1};
+  const int B = (flag & 8) ? 5 : 3;
+  // x3 lines, then x2 lines, then x1 columns (solve_implicit_3d.cpp:25-142)
+  size_t need = 0;
+  for (int d = 0; d < 3; ++d)
+    if ((flag & (1 << d)) && act[d])
+      need = std::max(need, (d == 0 && !per[0]) ? arch_doubles(g) : line_doubles(g, d, B));
+  if (need == 0) return CG_OK;  // only directions without cells were asked
+  double* scratch = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&scratch, sizeof(double) * need, ctx->stream));
+  for (int d = 2; d >= 0; --d) {
+    if (!(flag & (1 << d)) || !act[d]) continue;
+    if (d == 0 && !per[0]) {
+      if (blk->nmass == 0)
+        launch_implicit<0>(g, kt, dt, flag, du, w, scratch, ctx->d_status, ctx->stream);
+      else
+        launch_implicit<3>(g, kt, dt, flag, du, w, scratch, ctx->d_status, ctx->stream);
+    } else if (blk->nmass == 0) {
+      launch_lines<0>(g, kt, dt, flag, d, per[d], du, w, scratch, ctx->d_status, ctx->stream);
+    } else {
+      launch_lines<3>(g, kt, dt, flag, d, per[d], du, w, scratch, ctx->d_status, ctx->stream);
+    }
+    HIPCHK(ctx, hipGetLastError());
+  }
+  HIPCHK(ctx, hipFreeAsync(scratch, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_status(ctx);
 }

@@ -9,6 +9,7 @@
 #include "cg_timer.hpp"
 #include "flux32.hpp"
 #include "kernels.hpp"
+#include "lines.hpp"
 
 namespace cg_host {
 using namespace cg;
@@ -184,6 +185,94 @@ void launch_bc(const KGrid& g, double* w, hipStream_t s, KTimer* tm = nullptr) {
 
 inline size_t arch_doubles(const KGrid& g) {
   return (size_t)g.nx1 * 30 * (size_t)g.nx2 * g.nx3;
+}
+
+// The lines of direction dir (0 = x1, 1 = x2, 2 = x3) of a block: lanes
+// along the transverse index of smaller stride; Cartesian face areas and
+// volume (x1 faces dx2 dx3, x2 faces dx1 dx3, x3 faces dx1 dx2).
+inline KLines make_lines(const KGrid& g, int dir) {
+  const int st[3] = {g.is, g.js, g.ks};
+  const int ex[3] = {g.nx1, g.nx2, g.nx3};
+  const long long sd[3] = {g.c.i, g.c.j, g.c.k};
+  int t0 = (dir + 1) % 3, t1 = (dir + 2) % 3;
+  if (sd[t1] < sd[t0]) std::swap(t0, t1);
+  KLines L;
+  L.nf = ex[t0];
+  L.nline = (long long)ex[t0] * ex[t1];
+  L.n = ex[dir];
+  L.lo = st[dir];
+  L.org0 = (long long)st[t0] * sd[t0] + (long long)st[t1] * sd[t1];
+  L.sf = sd[t0];
+  L.ss = sd[t1];
+  L.sl = sd[dir];
+  L.sn = g.c.n;
+  const double area = (dir == 0) ? g.dx2 * g.dx3 : (dir == 1) ? g.dx1 * g.dx3 : g.dx1 * g.dx2;
+  const double vol = g.dx1 * g.dx2 * g.dx3;
+  const double s2v = 1. / (2. * vol);
+  L.sla = area * s2v;
+  L.sra = area * s2v;
+  L.sda = (area - area) * s2v;
+  L.grav = (dir == 0) ? g.grav : 0.0;
+  return L;
+}
+
+// Scratch doubles of the line solve of direction dir (B x B blocks).
+inline size_t line_doubles(const KGrid& g, int dir, int B) {
+  KLines L = make_lines(g, dir);
+  return (size_t)(L.n + 1) * (3 * B * B + B) * (size_t)L.nline;
+}
+
+template <int NM, int B, int DIR>
+void launch_lines_d(const KLines& L, const KThermo& th, double dt, bool per,
+                    double* du, const double* w, double* lv, int* status,
+                    hipStream_t s, KTimer* tm) {
+  const unsigned nb = nblk(L.nline, 64);
+  TIMED(tm, KID_LINES,
+        (k_line_assemble<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, per ? 1 : 0, w, du, lv)));
+  if (per)
+    TIMED(tm, KID_LINES,
+          (k_line_sweep_periodic<NM, B, DIR><<<nb, 64, 0, s>>>(L, w, du, lv, status)));
+  else
+    TIMED(tm, KID_LINES, (k_line_sweep<NM, B, DIR><<<nb, 64, 0, s>>>(L, w, du, lv, status)));
+}
+
+// SolveImplicit3D's lines of one direction in place on du (x2 / x3 lines,
+// or periodic lines of any direction); lv holds line_doubles(g, dir, B).
+template <int NM>
+void launch_lines(const KGrid& g, const KThermo& th, double dt, int flag, int dir,
+                  bool per, double* du, const double* w, double* lv, int* status,
+                  hipStream_t s, KTimer* tm = nullptr) {
+  const KLines L = make_lines(g, dir);
+  if (flag & 8) {
+    if (dir == 0) launch_lines_d<NM, 5, 0>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 1) launch_lines_d<NM, 5, 1>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 2) launch_lines_d<NM, 5, 2>(L, th, dt, per, du, w, lv, status, s, tm);
+  } else {
+    if (dir == 0) launch_lines_d<NM, 3, 0>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 1) launch_lines_d<NM, 3, 1>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 2) launch_lines_d<NM, 3, 2>(L, th, dt, per, du, w, lv, status, s, tm);
+  }
+}
+
+// Periodic directions of a block (both boundaries periodic,
+// implicit_solver.cpp:127-134) and the checks of SolveImplicit3D's flags.
+inline void periodic_dirs(const cg_block& b, bool per[3]) {
+  for (int d = 0; d < 3; ++d)
+    per[d] = b.bc[2 * d] == CG_BC_PERIODIC && b.bc[2 * d + 1] == CG_BC_PERIODIC;
+}
+
+inline int check_implicit_flag(cg_ctx* ctx, const cg_block& b, int flag) {
+  if (flag < 0 || flag > 15)
+    return set_err(ctx, CG_ERR_ARG, "implicit_flag must be in 0..15 (got %d)", flag);
+  bool per[3];
+  periodic_dirs(b, per);
+  // x2 / x3 lines between walls with species: the reference weights the
+  // species by w at the transposed index (forward_backward.hpp:109-115)
+  if (b.nmass > 0 && (((flag & 2) && b.nx2 > 1 && !per[1]) ||
+                      ((flag & 4) && b.nx3 > 1 && !per[2])))
+    return set_err(ctx, CG_ERR_ARG,
+                   "x2/x3 implicit lines with species need periodic ends");
+  return CG_OK;
 }
 
 }  // namespace cg_host

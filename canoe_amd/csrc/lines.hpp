@@ -1,0 +1,534 @@
+// lines.hpp -- the lines of ImplicitSolver::SolveImplicit3D other than the
+// batched non-periodic x1 columns: x2 and x3 lines (implicit_flag bits 1
+// and 2) and lines whose two ends are periodic, in any direction
+// (src/snap/implicit/solve_implicit_3d.cpp:19-143,
+// periodic_forward_backward.hpp:14-232).
+//
+// One line per lane in two passes.  k_line_assemble builds the blocks
+// a_i, b_i, c_i and the right-hand side of every interior level (plus, for
+// a periodic line, the lower corner c_{is-1}) into a lane-contiguous
+// scratch; k_line_sweep (walls) or k_line_sweep_periodic (El-Mikkawy's
+// cyclic recurrences) solves and writes du back.  Splitting the assembly
+// from the recurrence keeps each kernel's live state small: the cyclic
+// recurrence alone carries four B x B matrices between levels.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace cg {
+
+// A family of lines of one direction.  Line t has transverse interior
+// indices (f, s) = (t % nf, t / nf) along the transverse index of smaller
+// stride (f, on consecutive lanes) and the other one (s); level p of the
+// line (p = 0 .. n-1) is the cell at org(t) + (lo + p) * sl.
+struct KLines {
+  long long nline;
+  int n, lo, nf;
+  long long org0, sf, ss, sl, sn;
+  double sla, sra, sda;  // aleft / (2 vol), aright / (2 vol), their difference
+  double grav;           // the x1 lines' gravity, 0 on x2 / x3 lines
+  CG_HD long long org(long long t) const {
+    return org0 + (t % nf) * sf + (t / nf) * ss;
+  }
+};
+
+// Scratch doubles of one level: a, b, c (B x B each) and the rhs (B).
+template <int B>
+CG_HD constexpr int line_level_size() {
+  return 3 * B * B + B;
+}
+
+// The velocities of a state in the frame of direction DIR: slot IVX holds
+// the normal component, IVX + 1 and IVX + 2 the next ones in cyclic order
+// (flux_decomposition.hpp:80-83, 105-107).
+template <int NM, int DIR>
+__device__ inline void to_frame(const double* src, double* dst) {
+  constexpr int N = 5 + NM, IVX = 1 + NM;
+#pragma unroll
+  for (int n = 0; n < N; ++n) dst[n] = src[n];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) dst[IVX + s] = src[IVX + (s + DIR) % 3];
+}
+
+template <int B>
+__device__ inline void mstore(const Mt<B>& m, double* p, long long st) {
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int q = 0; q < B; ++q) p[(r * B + q) * st] = m.a[r][q];
+}
+template <int B>
+__device__ inline void mload(Mt<B>& m, const double* p, long long st) {
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int q = 0; q < B; ++q) m.a[r][q] = p[(r * B + q) * st];
+}
+
+// Assembly of FullCorrection / PartialCorrection (full_correction.cpp:
+// 77-146, partial_correction.cpp:69-155) for one line: level p holds
+// a (folded at walls), b, c and rhs of cell lo + p; slot n holds the lower
+// corner c_{lo-1} of a periodic line.  Velocities enter in the frame of
+// DIR; the Roe average is formed from the unrotated states
+// (RoeAverage is called before the frame is chosen, :98-104).
+template <int NM, int B, int DIR>
+__global__ void __launch_bounds__(64)
+    k_line_assemble(KLines L, KThermo th, double dt, int per,
+                    const double* __restrict__ w, const double* __restrict__ du,
+                    double* __restrict__ lv) {
+  constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  constexpr int LS = line_level_size<B>();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nline) return;
+  const long long o = L.org(t);
+  const long long st = L.nline;
+  const double gm1 = th.gammad - 1.;
+  const double gm = 0.5 * (gm1 + gm1);
+  const double idt = 1. / dt;
+  auto load = [&](double* dst, int i) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) dst[n] = w[o + n * L.sn + (long long)i * L.sl];
+  };
+  auto jac = [&](const double* wc, Mt<B>& out) {
+    double f[N];
+    to_frame<NM, DIR>(wc, f);
+    flux_jac<NM, B>(gm1, f, out);
+  };
+  auto absA = [&](const double* wl, const double* wr, Mt<B>& out) {
+    double prim[N], pf[N];
+    roe_average<NM>(prim, gm, wl, wr);
+    to_frame<NM, DIR>(prim, pf);
+    abs_roe_fma<NM, B>(th, pf, gm, out);
+  };
+  const int lo = L.lo, hi = L.lo + L.n - 1;
+  double wm[N], wc[N], wp[N];
+  load(wm, lo - 2);
+  load(wc, lo - 1);
+  Mt<B> Am, Ap, dm, dc, dp;
+  absA(wm, wc, Am);
+  jac(wm, dm);
+  jac(wc, dc);
+  for (int i = lo - 1; i <= hi; ++i) {
+    load(wp, i + 1);
+    absA(wc, wp, Ap);
+    jac(wp, dp);
+    Mt<B> a, b, c;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        double phi = 0.0;
+        if (B == 5) {
+          if ((r == 1 && q == 0) || (r == 4 && q == 1)) phi = L.grav;
+        } else {
+          if ((r == 1 && q == 0) || (r == 2 && q == 1)) phi = L.grav;
+        }
+        double av = fma(Am.a[r][q], L.sla, fma(Ap.a[r][q], L.sra, dc.a[r][q] * L.sda));
+        if (r == q) av += idt;
+        a.a[r][q] = av - phi;
+        b.a[r][q] = -(Am.a[r][q] + dm.a[r][q]) * L.sla;
+        c.a[r][q] = -(Ap.a[r][q] - dp.a[r][q]) * L.sra;
+      }
+    if (i == lo - 1) {
+      if (per) mstore<B>(c, lv + (long long)L.n * LS * st + t, st);
+    } else {
+      if (!per) {
+        // walls: a_lo += b_lo diag(1,-1,1,..), a_hi += c_hi diag(1,-1,1,..)
+        if (i == lo) {
+#pragma unroll
+          for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int q = 0; q < B; ++q) a.a[r][q] += (q == 1) ? -b.a[r][q] : b.a[r][q];
+        }
+        if (i == hi) {
+#pragma unroll
+          for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int q = 0; q < B; ++q) a.a[r][q] += (q == 1) ? -c.a[r][q] : c.a[r][q];
+        }
+      }
+      double* base = lv + (long long)(i - lo) * LS * st + t;
+      mstore<B>(a, base, st);
+      mstore<B>(b, base + B * B * st, st);
+      mstore<B>(c, base + 2 * B * B * st, st);
+      // rhs in the frame of DIR (forward_backward.hpp:24-36)
+      const long long ci = o + (long long)i * L.sl;
+      double d[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) d[n] = du[ci + n * L.sn];
+      double r0 = d[0];
+#pragma unroll
+      for (int n = 1; n < IVX; ++n) r0 += d[n];
+      double rhs[B];
+      rhs[0] = r0 * idt;
+      if (B == 5) {
+        rhs[1] = d[IVX + DIR] * idt;
+        rhs[2] = d[IVX + (1 + DIR) % 3] * idt;
+        rhs[3] = d[IVX + (2 + DIR) % 3] * idt;
+        rhs[4] = d[IEN] * idt;
+      } else {
+        rhs[1] = d[IVX + DIR] * idt;
+        rhs[2] = d[IEN] * idt;
+      }
+#pragma unroll
+      for (int r = 0; r < B; ++r) base[(3 * B * B + r) * st] = rhs[r];
+    }
+    Am = Ap;
+    dm = dc;
+    dc = dp;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      wm[n] = wc[n];
+      wc[n] = wp[n];
+    }
+  }
+}
+
+// du of a cell from the line's solution d (frame of DIR).  Walls: the
+// write-back of BackwardSubstitution (forward_backward.hpp:100-124), the
+// total-density increment split over the species by w_n (x1 lines only:
+// x2 / x3 lines with species are rejected on the host).  Periodic: the
+// write-back of PeriodicBackwardSubstitution (periodic_forward_backward.hpp:
+// 216-229), species unchanged.  x2 / x3 lines store IDN, the normal momentum
+// and IEN only, the variables the reference copies out of its line buffer
+// (solve_implicit_3d.cpp:51-56, 82-87).
+template <int NM, int B, int DIR, bool PER>
+__device__ inline void line_write_back(const Vt<B>& d, const double* __restrict__ w,
+                                       double* __restrict__ du, long long ci,
+                                       long long sn) {
+  constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  constexpr int vn = IVX + DIR, vt1 = IVX + (1 + DIR) % 3, vt2 = IVX + (2 + DIR) % 3;
+  double o[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) o[n] = du[ci + n * sn];
+  if (!PER) {
+    double dens = o[0];
+#pragma unroll
+    for (int n = 1; n < IVX; ++n) dens += o[n];
+    dens = d.v[0] - dens;
+    if (DIR == 0) {
+#pragma unroll
+      for (int n = 1; n < IVX; ++n) o[n] += dens * w[ci + n * sn];
+    }
+  }
+  o[0] = d.v[0];
+  if (B == 5) {
+    o[vn] = d.v[1];
+    o[vt1] = d.v[2];
+    o[vt2] = d.v[3];
+    o[IEN] = d.v[4];
+  } else {
+    o[vn] = d.v[1];
+    o[IEN] = d.v[2];
+  }
+#pragma unroll
+  for (int n = 1; n < IVX; ++n) o[0] -= o[n];
+  if (DIR == 0) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) du[ci + n * sn] = o[n];
+  } else {
+    du[ci] = o[0];
+    du[ci + vn * sn] = o[vn];
+    du[ci + IEN * sn] = o[IEN];
+  }
+}
+
+// Gauss-Jordan elimination of [M | C | t] with partial pivoting (first row
+// of largest |pivot|): C = M^-1 C, t = M^-1 t.  Returns false on a zero
+// pivot.
+template <int B>
+__device__ inline bool gj_solve_d(Mt<B>& M, Mt<B>& C, double* t) {
+  bool ok = true;
+#pragma unroll
+  for (int cc = 0; cc < B; ++cc) {
+    int p = cc;
+    double best = fabs(M.a[cc][cc]);
+#pragma unroll
+    for (int r = cc + 1; r < B; ++r) {
+      double v = fabs(M.a[r][cc]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (best == 0.0) ok = false;
+#pragma unroll
+    for (int r = cc + 1; r < B; ++r) {
+      bool sw = (p == r);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        double x0 = M.a[cc][q], x1 = M.a[r][q];
+        M.a[cc][q] = sw ? x1 : x0;
+        M.a[r][q] = sw ? x0 : x1;
+        double y0 = C.a[cc][q], y1 = C.a[r][q];
+        C.a[cc][q] = sw ? y1 : y0;
+        C.a[r][q] = sw ? y0 : y1;
+      }
+      double z0 = t[cc], z1 = t[r];
+      t[cc] = sw ? z1 : z0;
+      t[r] = sw ? z0 : z1;
+    }
+    double inv = 1.0 / M.a[cc][cc];
+#pragma unroll
+    for (int q = cc + 1; q < B; ++q) M.a[cc][q] *= inv;
+#pragma unroll
+    for (int q = 0; q < B; ++q) C.a[cc][q] *= inv;
+    t[cc] *= inv;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      if (r == cc) continue;
+      double f = M.a[r][cc];
+#pragma unroll
+      for (int q = cc + 1; q < B; ++q) M.a[r][q] = fma(-f, M.a[cc][q], M.a[r][q]);
+#pragma unroll
+      for (int q = 0; q < B; ++q) C.a[r][q] = fma(-f, C.a[cc][q], C.a[r][q]);
+      t[r] = fma(-f, t[cc], t[r]);
+    }
+  }
+  return ok;
+}
+
+// ForwardSweep / BackwardSubstitution (forward_backward.hpp:14-137) of a
+// line with walls: M = a_p - b_p a'_{p-1}, a'_p = M^-1 c_p,
+// delta_p = M^-1 (rhs_p - b_p delta_{p-1}) by one Gauss-Jordan elimination;
+// a'_p and delta_p overwrite c_p and rhs_p; then delta_p -= a'_p delta_{p+1}
+// and the write-back.
+template <int NM, int B, int DIR>
+__global__ void __launch_bounds__(64)
+    k_line_sweep(KLines L, const double* __restrict__ w, double* __restrict__ du,
+                 double* __restrict__ lv, int* __restrict__ status) {
+  constexpr int LS = line_level_size<B>();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nline) return;
+  const long long st = L.nline;
+  bool ok = true;
+  Mt<B> ap;
+  Vt<B> dl;
+  for (int p = 0; p < L.n; ++p) {
+    double* base = lv + (long long)p * LS * st + t;
+    Mt<B> M, b, c;
+    double tv[B];
+    mload<B>(M, base, st);
+    mload<B>(b, base + B * B * st, st);
+    mload<B>(c, base + 2 * B * B * st, st);
+#pragma unroll
+    for (int r = 0; r < B; ++r) tv[r] = base[(3 * B * B + r) * st];
+    if (p > 0) {
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+          double s = M.a[r][q];
+#pragma unroll
+          for (int kk = 0; kk < B; ++kk) s = fma(-b.a[r][kk], ap.a[kk][q], s);
+          M.a[r][q] = s;
+        }
+        double s = tv[r];
+#pragma unroll
+        for (int kk = 0; kk < B; ++kk) s = fma(-b.a[r][kk], dl.v[kk], s);
+        tv[r] = s;
+      }
+    }
+    if (!gj_solve_d<B>(M, c, tv)) ok = false;
+    ap = c;
+#pragma unroll
+    for (int r = 0; r < B; ++r) dl.v[r] = tv[r];
+    mstore<B>(ap, base + 2 * B * B * st, st);
+#pragma unroll
+    for (int r = 0; r < B; ++r) base[(3 * B * B + r) * st] = dl.v[r];
+  }
+  if (!ok) atomicOr(status, 1);
+  const long long o = L.org(t);
+  Vt<B> dn;
+  for (int p = L.n - 1; p >= 0; --p) {
+    const double* base = lv + (long long)p * LS * st + t;
+    Vt<B> d;
+#pragma unroll
+    for (int r = 0; r < B; ++r) d.v[r] = base[(3 * B * B + r) * st];
+    if (p < L.n - 1) {
+      Mt<B> a;
+      mload<B>(a, base + 2 * B * B * st, st);
+      Vt<B> ad;
+      mvec<B>(a, dn, ad);
+#pragma unroll
+      for (int r = 0; r < B; ++r) d.v[r] = d.v[r] - ad.v[r];
+    }
+    dn = d;
+    line_write_back<NM, B, DIR, false>(d, w, du, o + (long long)(L.lo + p) * L.sl, L.sn);
+  }
+}
+
+template <int B>
+__device__ inline void msub(const Mt<B>& x, const Mt<B>& y, Mt<B>& z) {
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int q = 0; q < B; ++q) z.a[r][q] = x.a[r][q] - y.a[r][q];
+}
+
+// PeriodicForwardSweep / PeriodicBackwardSubstitution (periodic_forward_
+// backward.hpp:14-232) for a line held by one block (first_block =
+// last_block = true), with L_p = b_p, d_p = a_p, U_p = c_p and the corner
+// U_{-1} = c_{lo-1}:
+//   ainv_0 = d_0^-1, gamma_0 = L_0, beta_0 = U_{-1} ainv_0, zeta_0 = r_0;
+//   p = 1 .. n-2, T = L_p ainv_{p-1}: ainv_p = (d_p - T U_{p-1})^-1,
+//     zeta_p = r_p - T zeta_{p-1}, gamma_p = -T gamma_{p-1},
+//     beta_p = -(beta_{p-1} U_{p-1}) ainv_p, except at p = n-2:
+//     gamma_{n-2} = U_{n-2} - T gamma_{n-3},
+//     beta_{n-2} = (L_{n-1} - beta_{n-3} U_{n-3}) ainv_{n-2};
+//   ainv_{n-1} = (d_{n-1} - sum beta_p gamma_p)^-1,
+//   zeta_{n-1} = r_{n-1} - sum beta_p zeta_p (p < n-1);
+//   x_{n-1} = ainv_{n-1} zeta_{n-1},
+//   x_p = ainv_p (zeta_p - U_p x_{p+1} - gamma_p x_{n-1}), U_{n-2} = 0.
+// ainv_p, gamma_p and zeta_p overwrite a_p, b_p and rhs_p of the scratch
+// (b_{n-1} is read at p = n-2 before its level is reached).  Products left
+// to right as written, inverses by Gauss-Jordan with partial pivoting.
+template <int NM, int B, int DIR>
+__global__ void __launch_bounds__(64)
+    k_line_sweep_periodic(KLines L, const double* __restrict__ w,
+                          double* __restrict__ du, double* __restrict__ lv,
+                          int* __restrict__ status) {
+  constexpr int LS = line_level_size<B>();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nline) return;
+  const long long st = L.nline;
+  const int n = L.n;
+  auto lvl = [&](int p) { return lv + (long long)p * LS * st + t; };
+  bool ok = true;
+  Mt<B> bet, sbg;
+  Vt<B> sbz;
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    sbz.v[r] = 0.0;
+#pragma unroll
+    for (int q = 0; q < B; ++q) sbg.a[r][q] = 0.0;
+  }
+  // p = 0
+  {
+    Mt<B> d, ai, um1;
+    mload<B>(d, lvl(0), st);
+    mload<B>(um1, lvl(n), st);
+    if (!minv<B>(d, ai)) ok = false;
+    mmul<B>(um1, ai, bet);
+    mstore<B>(ai, lvl(0), st);  // gamma_0 = b_0 and zeta_0 = rhs_0 stay
+    Mt<B> g0, bg;
+    mload<B>(g0, lvl(0) + B * B * st, st);
+    Vt<B> z0, bz;
+#pragma unroll
+    for (int r = 0; r < B; ++r) z0.v[r] = lvl(0)[(3 * B * B + r) * st];
+    mmul<B>(bet, g0, bg);
+    mvec<B>(bet, z0, bz);
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      sbz.v[r] = sbz.v[r] + bz.v[r];
+#pragma unroll
+      for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
+    }
+  }
+  for (int p = 1; p <= n - 2; ++p) {
+    double* cur = lvl(p);
+    const double* prv = lvl(p - 1);
+    Mt<B> T, Lp, ap;
+    mload<B>(Lp, cur + B * B * st, st);
+    mload<B>(ap, prv, st);  // ainv_{p-1}
+    mmul<B>(Lp, ap, T);
+    Mt<B> Up, TU, d, ai;
+    mload<B>(Up, prv + 2 * B * B * st, st);  // U_{p-1}
+    mmul<B>(T, Up, TU);
+    mload<B>(d, cur, st);
+    msub<B>(d, TU, d);
+    if (!minv<B>(d, ai)) ok = false;
+    // zeta_p
+    Vt<B> zp, Tz, z;
+#pragma unroll
+    for (int r = 0; r < B; ++r) zp.v[r] = prv[(3 * B * B + r) * st];
+    mvec<B>(T, zp, Tz);
+#pragma unroll
+    for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st] - Tz.v[r];
+    // gamma_p
+    Mt<B> gp, Tg, g;
+    mload<B>(gp, prv + B * B * st, st);
+    mmul<B>(T, gp, Tg);
+    Mt<B> bnew;
+    if (p < n - 2) {
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) g.a[r][q] = -Tg.a[r][q];
+      Mt<B> bu;
+      mmul<B>(bet, Up, bu);
+      mmul<B>(bu, ai, bnew);
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) bnew.a[r][q] = -bnew.a[r][q];
+    } else {
+      Mt<B> Uc;
+      mload<B>(Uc, cur + 2 * B * B * st, st);
+      msub<B>(Uc, Tg, g);
+      Mt<B> Ln, bu, x;
+      mload<B>(Ln, lvl(n - 1) + B * B * st, st);
+      mmul<B>(bet, Up, bu);
+      msub<B>(Ln, bu, x);
+      mmul<B>(x, ai, bnew);
+    }
+    bet = bnew;
+    mstore<B>(ai, cur, st);
+    mstore<B>(g, cur + B * B * st, st);
+#pragma unroll
+    for (int r = 0; r < B; ++r) cur[(3 * B * B + r) * st] = z.v[r];
+    Mt<B> bg;
+    Vt<B> bz;
+    mmul<B>(bet, g, bg);
+    mvec<B>(bet, z, bz);
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      sbz.v[r] = sbz.v[r] + bz.v[r];
+#pragma unroll
+      for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
+    }
+  }
+  // last level
+  Vt<B> xl;
+  {
+    double* cur = lvl(n - 1);
+    Mt<B> d, ai;
+    mload<B>(d, cur, st);
+    msub<B>(d, sbg, d);
+    if (!minv<B>(d, ai)) ok = false;
+    Vt<B> z;
+#pragma unroll
+    for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st] - sbz.v[r];
+    mvec<B>(ai, z, xl);
+  }
+  if (!ok) atomicOr(status, 1);
+  const long long o = L.org(t);
+  line_write_back<NM, B, DIR, true>(xl, w, du, o + (long long)(L.lo + n - 1) * L.sl, L.sn);
+  Vt<B> xn = xl;
+  Mt<B> zero;
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int q = 0; q < B; ++q) zero.a[r][q] = 0.0;
+  for (int p = n - 2; p >= 0; --p) {
+    const double* cur = lvl(p);
+    Mt<B> U, ai, g;
+    if (p == n - 2)
+      U = zero;
+    else
+      mload<B>(U, cur + 2 * B * B * st, st);
+    mload<B>(ai, cur, st);
+    mload<B>(g, cur + B * B * st, st);
+    Vt<B> ux, gx, tv, x;
+    mvec<B>(U, xn, ux);
+    mvec<B>(g, xl, gx);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+      tv.v[r] = (cur[(3 * B * B + r) * st] - ux.v[r]) - gx.v[r];
+    mvec<B>(ai, tv, x);
+    xn = x;
+    line_write_back<NM, B, DIR, true>(x, w, du, o + (long long)(L.lo + p) * L.sl, L.sn);
+  }
+}
+
+}  // namespace cg

@@ -140,12 +140,21 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
                            const double* flux1, const double* flux2,
                            const double* flux3, double* du);
 
-/* ---- implicit vertical solve ------------------------------------------- *
+/* ---- implicit solve ----------------------------------------------------- *
  * Replaces void ImplicitSolver::SolveImplicit3D(AthenaArray<Real>& du,
  *   AthenaArray<Real>& w, Real dt)     (src/snap/implicit/solve_implicit_3d.cpp:19-20)
- * for the x1 direction: flag 9 = FullCorrection 5x5 (full_correction.cpp),
- * flag 1 = PartialCorrection 3x3 (partial_correction.cpp).  All columns of
- * the block in one launch; du is updated in place. */
+ * with implicit_flag semantics (implicit_solver.cpp:47): bit 0 = x1 columns,
+ * bit 1 = x2 lines, bit 2 = x3 lines (solved in the order x3, x2, x1, each
+ * on the du left by the previous one), bit 3 = FullCorrection 5x5
+ * (full_correction.cpp) instead of PartialCorrection 3x3
+ * (partial_correction.cpp).  A direction whose two boundaries are periodic
+ * is solved by the cyclic recurrences of PeriodicForwardSweep /
+ * PeriodicBackwardSubstitution (periodic_forward_backward.hpp:14-232), the
+ * others with reflecting walls folded in.  x2 / x3 lines write back du of
+ * IDN, the normal momentum and IEN only, as the reference does; with
+ * species they need periodic ends (CG_ERR_ARG otherwise).  `w` must have
+ * the ghost cells of every solved direction filled.  du is updated in
+ * place; all lines of a direction in one launch. */
 int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                       double* du, const double* w, double dt, int flag);
 
@@ -293,7 +302,7 @@ int cg_run_debug(cg_run* run, double* vmin, double* vmax, double* xmin,
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of
  * launches and the kernel name of kernel id 0..CG_NUM_KERNEL_IDS-1 (ids
  * with no launches report 0).  cg_run_set_timing resets the counters. */
-#define CG_NUM_KERNEL_IDS 17
+#define CG_NUM_KERNEL_IDS 18
 int cg_run_set_timing(cg_run* run, int on);
 int cg_run_kernel_stats(cg_run* run, int kernel_id, double* total_ms,
                         long long* launches, const char** name);

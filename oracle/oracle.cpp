@@ -735,12 +735,14 @@ void roe_average(const Grid& g, double* prim, double gm1, const double* wl,
 // evaluated with the zero/one pattern of R and R^-1 spelled out (terms with
 // an exact zero factor dropped, unit factors taken exactly), reciprocals of
 // 2 hp, hp and 2 cs hp, and fused multiply-adds; equal to the reference's
-// dense Eigen product up to round-off.
+// dense Eigen product up to round-off.  dir selects the normal velocity
+// u = prim[IVX + dir] and the tangential v, w in cyclic order (:80-83).
 Mat<5> abs_roe_matrix(const Grid& g, const cg_thermo& th, const double* prim,
-                      double gm1) {
+                      double gm1, int dir = 0) {
   const int IPR = g.IPR;
   double cs = sound_speed(th, prim, IPR);
-  double u = prim[g.IVX], v = prim[g.IVY], w = prim[g.IVZ];
+  double u = prim[g.IVX + dir], v = prim[g.IVX + (1 + dir) % 3],
+         w = prim[g.IVX + (2 + dir) % 3];
   double hp = (gm1 + 1.) / gm1 * prim[IPR] / prim[0];
   double ke = 0.5 * (u * u + v * v + w * w);
   double h = hp + ke;
@@ -773,11 +775,12 @@ Mat<5> abs_roe_matrix(const Grid& g, const cg_thermo& th, const double* prim,
   return out;
 }
 
-// FluxJacobian: flux_decomposition.hpp:101-120 (dir = X1)
-Mat<5> flux_jacobian(const Grid& g, double gm1, const double* w) {
-  double v1 = w[g.IVX];
-  double v2 = w[g.IVY];
-  double v3 = w[g.IVZ];
+// FluxJacobian: flux_decomposition.hpp:101-120, in the frame of direction
+// dir (v1 normal, v2 and v3 the next components in cyclic order)
+Mat<5> flux_jacobian(const Grid& g, double gm1, const double* w, int dir = 0) {
+  double v1 = w[g.IVX + dir];
+  double v2 = w[g.IVX + (1 + dir) % 3];
+  double v3 = w[g.IVX + (2 + dir) % 3];
   double rho = w[0];
   double pres = w[g.IPR];
   double s2 = v1 * v1 + v2 * v2 + v3 * v3;
@@ -829,48 +832,154 @@ Mat<B> sub5(const Mat<5>& m) {
   return s;
 }
 
-// FullCorrection (src/snap/implicit/full_correction.cpp:32-161) and
-// PartialCorrection (partial_correction.cpp:26-170) for one x1 column, with
-// ForwardSweep / BackwardSubstitution (forward_backward.hpp:14-137).
-// Single block (first_block = last_block = true), no pole.
+// Cyclic block-tridiagonal solve of PeriodicForwardSweep /
+// PeriodicBackwardSubstitution (periodic_forward_backward.hpp:14-232,
+// El-Mikkawy 2005) for one line of n >= 3 levels held by a single block
+// (first_block = last_block = true):
+//   L_p x_{p-1} + d_p x_p + U_p x_{p+1} = r_p,  x_{-1} = x_{n-1}, x_n = x_0,
+// with the lower corner U_{-1} (coupling row n-1 to x_0) passed as um1.
+// Recurrences (:75-148): ainv_0 = d_0^-1, gamma_0 = L_0, beta_0 =
+// um1 ainv_0, zeta_0 = r_0; for 1 <= p <= n-2 with T = L_p ainv_{p-1}
+// (the values the reference's loop also forms at p = n-1 are overwritten
+// before use):
+// ainv_p = (d_p - T U_{p-1})^-1, gamma_p = -T gamma_{p-1},
+// beta_p = -(beta_{p-1} U_{p-1}) ainv_p, zeta_p = r_p - T zeta_{p-1};
+// then gamma_{n-2} = U_{n-2} - L_{n-2} ainv_{n-3} gamma_{n-3},
+// beta_{n-2} = (L_{n-1} - beta_{n-3} U_{n-3}) ainv_{n-2},
+// ainv_{n-1} = (d_{n-1} - sum_{p<n-1} beta_p gamma_p)^-1,
+// zeta_{n-1} = r_{n-1} - sum_{p<n-1} beta_p zeta_p.  Back substitution
+// (:196-213): U_{n-2} = 0, x_{n-1} = ainv_{n-1} zeta_{n-1},
+// x_p = ainv_p (zeta_p - U_p x_{p+1} - gamma_p x_{n-1}).
+// Products are evaluated left to right as written; inverses by the
+// partial-pivot Gauss-Jordan above.  Returns false on a zero pivot.
 template <int B>
-bool correction_column(const Grid& g, const cg_block& blk,
-                       const cg_thermo& th, double* du, const double* w,
-                       double dt, int k, int j) {
-  const int is = g.is, ie = g.ie, nc = g.nc1, N = g.N;
-  const int IVX = g.IVX, IEN = g.IPR;
-  const double grav = blk.grav;
+bool periodic_sweep(int n, const Mat<B>* d, const Mat<B>* L, const Mat<B>* U,
+                    const Mat<B>& um1, const Vec<B>* r, Vec<B>* x) {
+  std::vector<Mat<B>> ainv(n), gam(n), bet(n);
+  std::vector<Vec<B>> zeta(n);
+  auto sub = [](const Mat<B>& p, const Mat<B>& q) {
+    Mat<B> z;
+    for (int a = 0; a < B; ++a)
+      for (int c = 0; c < B; ++c) z.a[a][c] = p.a[a][c] - q.a[a][c];
+    return z;
+  };
+  auto neg = [](Mat<B> p) {
+    for (int a = 0; a < B; ++a)
+      for (int c = 0; c < B; ++c) p.a[a][c] = -p.a[a][c];
+    return p;
+  };
+  auto vsub = [](const Vec<B>& p, const Vec<B>& q) {
+    Vec<B> z;
+    for (int a = 0; a < B; ++a) z.v[a] = p.v[a] - q.v[a];
+    return z;
+  };
+  if (!inverse(d[0], &ainv[0])) return false;
+  gam[0] = L[0];
+  bet[0] = mul(um1, ainv[0]);
+  zeta[0] = r[0];
+  for (int p = 1; p < n - 1; ++p) {
+    Mat<B> T = mul(L[p], ainv[p - 1]);
+    if (!inverse(sub(d[p], mul(T, U[p - 1])), &ainv[p])) return false;
+    gam[p] = neg(mul(T, gam[p - 1]));
+    bet[p] = neg(mul(mul(bet[p - 1], U[p - 1]), ainv[p]));
+    zeta[p] = vsub(r[p], mul(T, zeta[p - 1]));
+  }
+  gam[n - 2] = sub(U[n - 2], mul(mul(L[n - 2], ainv[n - 3]), gam[n - 3]));
+  bet[n - 2] = mul(sub(L[n - 1], mul(bet[n - 3], U[n - 3])), ainv[n - 2]);
+  Mat<B> sbg;
+  sbg.zero();
+  Vec<B> sbz;
+  for (int a = 0; a < B; ++a) sbz.v[a] = 0.0;
+  for (int p = 0; p < n - 1; ++p) {
+    Mat<B> bg = mul(bet[p], gam[p]);
+    Vec<B> bz = mul(bet[p], zeta[p]);
+    for (int a = 0; a < B; ++a) {
+      for (int c = 0; c < B; ++c) sbg.a[a][c] = sbg.a[a][c] + bg.a[a][c];
+      sbz.v[a] = sbz.v[a] + bz.v[a];
+    }
+  }
+  if (!inverse(sub(d[n - 1], sbg), &ainv[n - 1])) return false;
+  zeta[n - 1] = vsub(r[n - 1], sbz);
+  x[n - 1] = mul(ainv[n - 1], zeta[n - 1]);
+  Mat<B> zero;
+  zero.zero();
+  for (int p = n - 2; p >= 0; --p) {
+    const Mat<B>& up = (p == n - 2) ? zero : U[p];
+    Vec<B> t = vsub(vsub(zeta[p], mul(up, x[p + 1])), mul(gam[p], x[n - 1]));
+    x[p] = mul(ainv[p], t);
+  }
+  return true;
+}
+
+// FullCorrection (src/snap/implicit/full_correction.cpp:32-161) and
+// PartialCorrection (partial_correction.cpp:26-170) for one line of
+// direction dir through the transverse pair (k, j) in the reference's
+// shuffled order (solve_implicit_3d.cpp:28-31, 60-63): x1 lines k = x3,
+// j = x2; x2 lines k = x1, j = x3; x3 lines k = x2, j = x1.  The velocities
+// enter in the frame of dir (CopyPrimitives / FluxJacobian / Eigenvector,
+// flux_decomposition.hpp:14-32, 76-83, 101-110); gravity (Phi) only on x1.
+// Non-periodic lines fold the walls and run ForwardSweep /
+// BackwardSubstitution (forward_backward.hpp:14-137); lines with both ends
+// periodic run periodic_sweep.  Single block, no pole.  On x2 / x3 lines
+// only du of IDN, the normal momentum and IEN is written back: the
+// reference shuffles only those three out of its line buffer
+// (solve_implicit_3d.cpp:51-56, 82-87).
+template <int B>
+bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                     double* du, const double* w, double dt, int dir, int k,
+                     int j, bool periodic) {
+  const int N = g.N, IVX = g.IVX, IEN = g.IPR;
+  int is, ie, nc;
+  if (dir == 0) {
+    is = g.is; ie = g.ie; nc = g.nc1;
+  } else if (dir == 1) {
+    is = g.js; ie = g.je; nc = g.nc2;
+  } else {
+    is = g.ks; ie = g.ke; nc = g.nc3;
+  }
+  auto at = [&](int n, int i) -> size_t {
+    if (dir == 0) return g.at(n, k, j, i);
+    if (dir == 1) return g.at(n, j, i, k);
+    return g.at(n, i, k, j);
+  };
+  const int vn = IVX + dir, vt1 = IVX + (1 + dir) % 3, vt2 = IVX + (2 + dir) % 3;
+  const double grav = (dir == 0) ? blk.grav : 0.0;
   Mat<B> Phi;
   Phi.zero();
-  if (B == 5) {
-    Phi.a[1][0] = grav;
-    Phi.a[4][1] = grav;
-  } else {
-    Phi.a[1][0] = grav;
-    Phi.a[2][1] = grav;
+  if (dir == 0) {
+    if (B == 5) {
+      Phi.a[1][0] = grav;
+      Phi.a[4][1] = grav;
+    } else {
+      Phi.a[1][0] = grav;
+      Phi.a[2][1] = grav;
+    }
   }
-  const bool periodic = (blk.bc[0] == CG_BC_PERIODIC);
   std::vector<Mat<B>> a(nc), b(nc), c(nc), dfdq(nc);
   std::vector<Vec<B>> delta(nc);
   const double gm1 = th.gammad - 1.;
   double wl[5 + CG_MAX_NMASS], wr[5 + CG_MAX_NMASS], prim[5 + CG_MAX_NMASS];
   auto load = [&](double* dst, int i) {
-    for (int n = 0; n < N; ++n) dst[n] = w[g.at(n, k, j, i)];
+    for (int n = 0; n < N; ++n) dst[n] = w[at(n, i)];
   };
   for (int i = is - 2; i <= ie + 1; ++i) {
     load(wr, i);
-    dfdq[i] = sub5<B>(flux_jacobian(g, gm1, wr));
+    dfdq[i] = sub5<B>(flux_jacobian(g, gm1, wr, dir));
   }
   load(wl, is - 2);
   load(wr, is - 1);
   double gm = 0.5 * (gm1 + gm1);
   roe_average(g, prim, gm, wl, wr);
-  Mat<B> Am = sub5<B>(abs_roe_matrix(g, th, prim, gm));
-  const double aleft = blk.dx2 * blk.dx3, aright = blk.dx2 * blk.dx3;
+  Mat<B> Am = sub5<B>(abs_roe_matrix(g, th, prim, gm, dir));
+  // Cartesian face areas and volume (Athena++ cartesian.cpp Face{1,2,3}Area,
+  // CellVolume): x1 faces dx2 dx3, x2 faces dx1 dx3, x3 faces dx1 dx2
+  const double area = (dir == 0) ? blk.dx2 * blk.dx3
+                                 : (dir == 1) ? blk.dx1 * blk.dx3 : blk.dx1 * blk.dx2;
+  const double aleft = area, aright = area;
   const double vol = blk.dx1 * blk.dx2 * blk.dx3;
   // a_i = (A- aleft + A+ aright + (aright - aleft) dfdq_i) / (2 vol)
   //       + I/dt - Phi,  b_i = -(A- + dfdq_{i-1}) aleft / (2 vol),
-  //       c_i = -(A+ - dfdq_{i+1}) aright / (2 vol)   (:276-280),
+  //       c_i = -(A+ - dfdq_{i+1}) aright / (2 vol)   (:133-137),
   // with the division by 2 vol taken as a product with its reciprocal.
   const double s2v = 1. / (2. * vol);
   const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
@@ -880,7 +989,7 @@ bool correction_column(const Grid& g, const cg_block& blk,
     load(wr, i + 1);
     gm = 0.5 * (gm1 + gm1);
     roe_average(g, prim, gm, wl, wr);
-    Mat<B> Ap = sub5<B>(abs_roe_matrix(g, th, prim, gm));
+    Mat<B> Ap = sub5<B>(abs_roe_matrix(g, th, prim, gm, dir));
     for (int r = 0; r < B; ++r)
       for (int q = 0; q < B; ++q) {
         double av = std::fma(Am.a[r][q], sl,
@@ -892,6 +1001,24 @@ bool correction_column(const Grid& g, const cg_block& blk,
       }
     Am = Ap;
   }
+  auto rhs_at = [&](int i) {
+    Vec<B> rhs;
+    double r0 = du[at(0, i)];
+    for (int n = 1; n < IVX; ++n) r0 += du[at(n, i)];
+    rhs.v[0] = r0 * idt;
+    if (B == 5) {
+      rhs.v[1] = du[at(vn, i)] * idt;
+      rhs.v[2] = du[at(vt1, i)] * idt;
+      rhs.v[3] = du[at(vt2, i)] * idt;
+      rhs.v[4] = du[at(IEN, i)] * idt;
+    } else {
+      rhs.v[1] = du[at(vn, i)] * idt;
+      rhs.v[2] = du[at(IEN, i)] * idt;
+    }
+    return rhs;
+  };
+  // the line buffer's values after the write-back; stored into du below
+  std::vector<double> out((size_t)(ie - is + 1) * N);
   if (!periodic) {
     // a_is += b_is Bnd, a_ie += c_ie Bnd with Bnd = diag(1, -1, 1, ...)
     for (int r = 0; r < B; ++r)
@@ -900,93 +1027,139 @@ bool correction_column(const Grid& g, const cg_block& blk,
     for (int r = 0; r < B; ++r)
       for (int q = 0; q < B; ++q)
         a[ie].a[r][q] += (q == 1) ? -c[ie].a[r][q] : c[ie].a[r][q];
-  }
-  // ForwardSweep (forward_backward.hpp:14-84): with M = a_i - b_i a'_{i-1}
-  // and t = rhs_i - b_i delta_{i-1}, a'_i = M^-1 c_i and delta_i = M^-1 t.
-  // The reference forms M^-1 (Eigen .inverse()) and multiplies; here one
-  // Gauss-Jordan elimination of [M | c_i | t] with partial pivoting gives
-  // both (equal in exact arithmetic).
-  auto rhs_at = [&](int i) {
-    Vec<B> rhs;
-    double r0 = du[g.at(0, k, j, i)];
-    for (int n = 1; n < IVX; ++n) r0 += du[g.at(n, k, j, i)];
-    rhs.v[0] = r0 * idt;
-    if (B == 5) {
-      rhs.v[1] = du[g.at(IVX, k, j, i)] * idt;
-      rhs.v[2] = du[g.at(IVX + 1, k, j, i)] * idt;
-      rhs.v[3] = du[g.at(IVX + 2, k, j, i)] * idt;
-      rhs.v[4] = du[g.at(IEN, k, j, i)] * idt;
-    } else {
-      rhs.v[1] = du[g.at(IVX, k, j, i)] * idt;
-      rhs.v[2] = du[g.at(IEN, k, j, i)] * idt;
-    }
-    return rhs;
-  };
-  for (int i = is; i <= ie; ++i) {
-    Vec<B> t = rhs_at(i);
-    Mat<B> M = a[i], C = c[i];
-    if (i > is) {
-      for (int r = 0; r < B; ++r) {
-        for (int q = 0; q < B; ++q) {
-          double s = M.a[r][q];
+    // ForwardSweep (forward_backward.hpp:14-84): with M = a_i - b_i a'_{i-1}
+    // and t = rhs_i - b_i delta_{i-1}, a'_i = M^-1 c_i and delta_i = M^-1 t.
+    // The reference forms M^-1 (Eigen .inverse()) and multiplies; here one
+    // Gauss-Jordan elimination of [M | c_i | t] with partial pivoting gives
+    // both (equal in exact arithmetic).
+    for (int i = is; i <= ie; ++i) {
+      Vec<B> t = rhs_at(i);
+      Mat<B> M = a[i], C = c[i];
+      if (i > is) {
+        for (int r = 0; r < B; ++r) {
+          for (int q = 0; q < B; ++q) {
+            double s = M.a[r][q];
+            for (int kk = 0; kk < B; ++kk)
+              s = std::fma(-b[i].a[r][kk], a[i - 1].a[kk][q], s);
+            M.a[r][q] = s;
+          }
+          double s = t.v[r];
           for (int kk = 0; kk < B; ++kk)
-            s = std::fma(-b[i].a[r][kk], a[i - 1].a[kk][q], s);
-          M.a[r][q] = s;
+            s = std::fma(-b[i].a[r][kk], delta[i - 1].v[kk], s);
+          t.v[r] = s;
         }
-        double s = t.v[r];
-        for (int kk = 0; kk < B; ++kk)
-          s = std::fma(-b[i].a[r][kk], delta[i - 1].v[kk], s);
-        t.v[r] = s;
       }
+      if (!gj_solve(M, C, t)) return false;
+      a[i] = C;
+      delta[i] = t;
     }
-    if (!gj_solve(M, C, t)) return false;
-    a[i] = C;
-    delta[i] = t;
-  }
-  // BackwardSubstitution (forward_backward.hpp:86-137)
-  for (int i = ie - 1; i >= is; --i) {
-    Vec<B> ad = mul(a[i], delta[i + 1]);
-    for (int r = 0; r < B; ++r) delta[i].v[r] = delta[i].v[r] - ad.v[r];
+    // BackwardSubstitution (forward_backward.hpp:86-137)
+    for (int i = ie - 1; i >= is; --i) {
+      Vec<B> ad = mul(a[i], delta[i + 1]);
+      for (int r = 0; r < B; ++r) delta[i].v[r] = delta[i].v[r] - ad.v[r];
+    }
+    for (int i = is; i <= ie; ++i) {
+      double* o = &out[(size_t)(i - is) * N];
+      for (int n = 0; n < N; ++n) o[n] = du[at(n, i)];
+      double dens = o[0];
+      for (int n = 1; n < IVX; ++n) dens += o[n];
+      dens = delta[i].v[0] - dens;
+      o[0] = delta[i].v[0];
+      // the species weights are w at the line's own cell (x1 lines; x2 / x3
+      // lines with species are rejected by implicit_solve)
+      for (int n = 1; n < IVX; ++n) o[n] += dens * w[at(n, i)];
+      if (B == 5) {
+        o[vn] = delta[i].v[1];
+        o[vt1] = delta[i].v[2];
+        o[vt2] = delta[i].v[3];
+        o[IEN] = delta[i].v[4];
+      } else {
+        o[vn] = delta[i].v[1];
+        o[IEN] = delta[i].v[2];
+      }
+      for (int n = 1; n < IVX; ++n) o[0] -= o[n];
+    }
+  } else {
+    const int n = ie - is + 1;
+    std::vector<Vec<B>> rhs(n), x(n);
+    for (int i = is; i <= ie; ++i) rhs[i - is] = rhs_at(i);
+    if (!periodic_sweep<B>(n, &a[is], &b[is], &c[is], c[is - 1], rhs.data(),
+                           x.data()))
+      return false;
+    // PeriodicBackwardSubstitution's write-back (:216-229): no species split
+    for (int i = is; i <= ie; ++i) {
+      double* o = &out[(size_t)(i - is) * N];
+      const Vec<B>& d = x[i - is];
+      for (int q = 0; q < N; ++q) o[q] = du[at(q, i)];
+      o[0] = d.v[0];
+      if (B == 5) {
+        o[vn] = d.v[1];
+        o[vt1] = d.v[2];
+        o[vt2] = d.v[3];
+        o[IEN] = d.v[4];
+      } else {
+        o[vn] = d.v[1];
+        o[IEN] = d.v[2];
+      }
+      for (int q = 1; q < IVX; ++q) o[0] -= o[q];
+    }
   }
   for (int i = is; i <= ie; ++i) {
-    double dens = du[g.at(0, k, j, i)];
-    for (int n = 1; n < IVX; ++n) dens += du[g.at(n, k, j, i)];
-    dens = delta[i].v[0] - dens;
-    du[g.at(0, k, j, i)] = delta[i].v[0];
-    for (int n = 1; n < IVX; ++n)
-      du[g.at(n, k, j, i)] += dens * w[g.at(n, k, j, i)];
-    if (B == 5) {
-      du[g.at(IVX, k, j, i)] = delta[i].v[1];
-      du[g.at(IVX + 1, k, j, i)] = delta[i].v[2];
-      du[g.at(IVX + 2, k, j, i)] = delta[i].v[3];
-      du[g.at(IEN, k, j, i)] = delta[i].v[4];
+    const double* o = &out[(size_t)(i - is) * N];
+    if (dir == 0) {
+      for (int q = 0; q < N; ++q) du[at(q, i)] = o[q];
     } else {
-      du[g.at(IVX, k, j, i)] = delta[i].v[1];
-      du[g.at(IEN, k, j, i)] = delta[i].v[2];
+      du[at(0, i)] = o[0];
+      du[at(vn, i)] = o[vn];
+      du[at(IEN, i)] = o[IEN];
     }
-    for (int n = 1; n < IVX; ++n)
-      du[g.at(0, k, j, i)] -= du[g.at(n, k, j, i)];
   }
   return true;
 }
 
-// ImplicitSolver::SolveImplicit3D, x1 branch (solve_implicit_3d.cpp:91-142)
+// ImplicitSolver::SolveImplicit3D (solve_implicit_3d.cpp:19-143): x3 lines
+// (flag bit 2, nx3 > 1), then x2 lines (bit 1, nx2 > 1), then x1 columns
+// (bit 0); bit 3 selects the 5 x 5 FullCorrection, else the 3 x 3
+// PartialCorrection.  A direction is periodic when both of its boundaries
+// are (implicit_solver.cpp:127-134).
 int implicit_solve(const Grid& g, const cg_block& blk, const cg_thermo& th,
                    double* du, const double* w, double dt, int flag) {
   if (flag == 0) return CG_OK;
-  if ((flag & 6) != 0) return CG_ERR_ARG;  // x2/x3 implicit not supported
-  if (blk.bc[0] == CG_BC_PERIODIC) return CG_ERR_ARG;
+  if (flag & ~15) return CG_ERR_ARG;
+  const bool per[3] = {blk.bc[0] == CG_BC_PERIODIC && blk.bc[1] == CG_BC_PERIODIC,
+                       blk.bc[2] == CG_BC_PERIODIC && blk.bc[3] == CG_BC_PERIODIC,
+                       blk.bc[4] == CG_BC_PERIODIC && blk.bc[5] == CG_BC_PERIODIC};
+  // x2 / x3 walls with species: the reference weights the species by w at
+  // the shuffled (transposed) index (forward_backward.hpp:109-115)
+  if (g.nmass > 0 && (((flag & 2) && g.f2 && !per[1]) || ((flag & 4) && g.f3 && !per[2])))
+    return CG_ERR_ARG;
   int err = CG_OK;
-#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
-  for (int k = g.ks; k <= g.ke; ++k)
-    for (int j = g.js; j <= g.je; ++j) {
-      bool ok = (flag & 8) ? correction_column<5>(g, blk, th, du, w, dt, k, j)
-                           : correction_column<3>(g, blk, th, du, w, dt, k, j);
-      if (!ok) {
-#pragma omp atomic write
-        err = CG_ERR_PIVOT;
-      }
+  const int dirs[3] = {2, 1, 0};
+  for (int dd = 0; dd < 3; ++dd) {
+    const int dir = dirs[dd];
+    if (!(flag & (1 << dir))) continue;
+    if ((dir == 1 && !g.f2) || (dir == 2 && !g.f3)) continue;
+    int k0, k1, j0, j1;
+    if (dir == 0) {
+      k0 = g.ks; k1 = g.ke; j0 = g.js; j1 = g.je;
+    } else if (dir == 1) {
+      k0 = g.is; k1 = g.ie; j0 = g.ks; j1 = g.ke;
+    } else {
+      k0 = g.js; k1 = g.je; j0 = g.is; j1 = g.ie;
     }
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+    for (int k = k0; k <= k1; ++k)
+      for (int j = j0; j <= j1; ++j) {
+        bool ok = (flag & 8)
+                      ? correction_line<5>(g, blk, th, du, w, dt, dir, k, j, per[dir])
+                      : correction_line<3>(g, blk, th, du, w, dt, dir, k, j, per[dir]);
+        if (!ok) {
+#pragma omp atomic write
+          err = CG_ERR_PIVOT;
+        }
+      }
+    if (err) return err;
+  }
   return err;
 }
 
@@ -1592,6 +1765,38 @@ int or_block_thomas(int B, int n, const double* a, const double* b,
     }
     for (int i = 0; i < n; ++i)
       for (int q = 0; q < BB; ++q) x[(size_t)i * BB + q] = d[i].v[q];
+    return CG_OK;
+  };
+  if (B == 5) return run(std::integral_constant<int, 5>());
+  if (B == 3) return run(std::integral_constant<int, 3>());
+  return CG_ERR_ARG;
+}
+
+// Cyclic block-tridiagonal solve of n >= 3 levels (periodic_sweep):
+// b_i x_{i-1} + a_i x_i + c_i x_{i+1} = r_i with x_{-1} = x_{n-1} and
+// x_n = x_0; corner couples row n-1 to x_0.  Row-major blocks.
+int or_block_thomas_periodic(int B, int n, const double* a, const double* b,
+                             const double* c, const double* corner,
+                             const double* r, double* x) {
+  if (n < 3) return CG_ERR_ARG;
+  auto run = [&](auto tag) {
+    constexpr int BB = decltype(tag)::value;
+    std::vector<Mat<BB>> A(n), L(n), U(n);
+    std::vector<Vec<BB>> R(n), X(n);
+    Mat<BB> um1;
+    for (int i = 0; i < n; ++i) {
+      for (int q = 0; q < BB * BB; ++q) {
+        A[i].a[q / BB][q % BB] = a[(size_t)i * BB * BB + q];
+        L[i].a[q / BB][q % BB] = b[(size_t)i * BB * BB + q];
+        U[i].a[q / BB][q % BB] = c[(size_t)i * BB * BB + q];
+      }
+      for (int q = 0; q < BB; ++q) R[i].v[q] = r[(size_t)i * BB + q];
+    }
+    for (int q = 0; q < BB * BB; ++q) um1.a[q / BB][q % BB] = corner[q];
+    if (!periodic_sweep<BB>(n, A.data(), L.data(), U.data(), um1, R.data(), X.data()))
+      return CG_ERR_PIVOT;
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < BB; ++q) x[(size_t)i * BB + q] = X[i].v[q];
     return CG_OK;
   };
   if (B == 5) return run(std::integral_constant<int, 5>());

@@ -85,6 +85,9 @@ int or_debug(const cg_block* blk, const double* w, double* vmin, double* vmax,
              double* xmin, double* xmax);
 int or_block_thomas(int B, int n, const double* a, const double* b,
                     const double* c, const double* r, double* x);
+int or_block_thomas_periodic(int B, int n, const double* a, const double* b,
+                             const double* c, const double* corner,
+                             const double* r, double* x);
 int or_flux_jacobian(const cg_block* blk, double gm1, const double* w,
                      double* out);
 int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,

@@ -41,6 +41,7 @@ _SIGS = {
     "or_debug": (_I, [_P, _P, _P, _P, _P, _P]),
     "or_new_mesh_dt": (_D, [_D, _D, _D, _D]),
     "or_block_thomas": (_I, [_I, _I, _P, _P, _P, _P, _P]),
+    "or_block_thomas_periodic": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "or_flux_jacobian": (_I, [_P, _D, _P, _P]),
     "or_roe_matrices": (_I, [_P, _P, _D, _P, _P, _P, _P]),
     "or_mlp_nparam": (_I, []),

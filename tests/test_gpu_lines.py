@@ -1,0 +1,93 @@
+"""GPU parity of SolveImplicit3D's x2 / x3 lines and periodic lines
+(cg_implicit_solve with implicit_flag bits 1 and 2, and directions whose two
+boundaries are periodic) against the oracle on the same seeded inputs.
+
+Both evaluate the same expressions in FP64 without contraction, so the bar
+is the north-star 1e-12 relative per cell (in practice equality).
+"""
+import numpy as np
+import pytest
+
+import canoe_amd as ca
+import oracle_lib as orl
+from helpers import perturbed_state, relerr
+from test_oracle_lines import _case, _set_bc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _check(ctx, blk, th, w, du, dt, flag):
+    got = ca.ImplicitSolver(ctx, blk, th, flag).solve_implicit_3d(du, w, dt)
+    ref, r = orl.implicit_solve(blk, th, du, w, dt, flag)
+    assert r == 0
+    sk, sj, si = blk.interior()
+    e = relerr(got[:, sk, sj, si], ref[:, sk, sj, si])
+    _say("  relerr %.3e" % e)
+    assert e <= TOL
+    # cells outside the interior are left alone
+    mask = np.ones(blk.shape(), bool)
+    mask[:, sk, sj, si] = False
+    np.testing.assert_array_equal(got[mask], du[mask])
+    return got
+
+
+@pytest.mark.parametrize("flag", [10, 12, 14, 15, 2, 4, 7])
+@pytest.mark.parametrize("bc", [ca.BC_PERIODIC, ca.BC_REFLECT])
+def test_x2_x3_lines(ctx, flag, bc):
+    blk, th, w, du = _case(bc, bc)
+    _say("lines flag=%d bc=%d" % (flag, bc))
+    _check(ctx, blk, th, w, du, 0.7, flag)
+
+
+@pytest.mark.parametrize("flag", [9, 1, 15])
+def test_periodic_x1_columns(ctx, flag):
+    blk, th, w, du = _case(ca.BC_PERIODIC, ca.BC_PERIODIC)
+    bc = list(blk.bc)
+    bc[0] = bc[1] = ca.BC_PERIODIC
+    _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
+    _say("periodic x1 flag=%d" % flag)
+    _check(ctx, blk, th, w, du, 0.7, flag)
+
+
+def test_uneven_extents_and_2d_block(ctx):
+    # lanes run along the transverse index of smaller stride; extents that
+    # are not multiples of the wavefront and a block with nx3 = 1
+    blk, th, w, du = _case(ca.BC_PERIODIC, ca.BC_REFLECT, dims=(13, 7, 5), seed=3)
+    _say("lines uneven 13x7x5")
+    _check(ctx, blk, th, w, du, 0.4, 15)
+    blk, th, cfg, w = perturbed_state(1, (12, 20, 1), vel=20.0)
+    du = np.zeros(blk.shape())
+    sk, sj, si = blk.interior()
+    du[:, sk, sj, si] = np.random.default_rng(5).normal(
+        size=du[:, sk, sj, si].shape) * np.array(
+            [1e-3, 1e-2, 1e-2, 1e-2, 3e2])[:, None, None, None]
+    _say("lines 2D 12x20x1")
+    for flag in (10, 14, 11):  # x3 is skipped: nx3 = 1
+        _check(ctx, blk, th, w, du, 0.4, flag)
+
+
+def test_moist_periodic_lines(ctx):
+    blk, th, cfg, w = perturbed_state(4, (8, 10, 6), vel=10.0)
+    du = np.zeros(blk.shape())
+    sk, sj, si = blk.interior()
+    du[:, sk, sj, si] = np.random.default_rng(8).normal(
+        size=du[:, sk, sj, si].shape) * 1e-3
+    _say("moist periodic lines")
+    _check(ctx, blk, th, w, du, 0.5, 15)
+
+
+def test_species_with_walled_lines_rejected(ctx):
+    blk, th, cfg, w = perturbed_state(4, (8, 10, 6), vel=10.0)
+    bc = list(blk.bc)
+    bc[2] = bc[3] = ca.BC_REFLECT
+    _set_bc(blk, bc)
+    with pytest.raises(ca.CanoeGpuError):
+        ca.ImplicitSolver(ctx, blk, th, 10).solve_implicit_3d(
+            np.zeros(blk.shape()), w, 0.5)
