@@ -209,9 +209,11 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
       else
         launch_implicit<3>(g, kt, dt, flag, du, w, scratch, ctx->d_status, ctx->stream);
     } else if (blk->nmass == 0) {
-      launch_lines<0>(g, kt, dt, flag, d, per[d], du, w, scratch, ctx->d_status, ctx->stream);
+      launch_lines<0>(g, kt, dt, nullptr, flag, d, per[d], du, w, scratch, ctx->d_status,
+                      ctx->stream);
     } else {
-      launch_lines<3>(g, kt, dt, flag, d, per[d], du, w, scratch, ctx->d_status, ctx->stream);
+      launch_lines<3>(g, kt, dt, nullptr, flag, d, per[d], du, w, scratch, ctx->d_status,
+                      ctx->stream);
     }
     HIPCHK(ctx, hipGetLastError());
   }

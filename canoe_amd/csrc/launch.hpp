@@ -223,12 +223,13 @@ inline size_t line_doubles(const KGrid& g, int dir, int B) {
 }
 
 template <int NM, int B, int DIR>
-void launch_lines_d(const KLines& L, const KThermo& th, double dt, bool per,
-                    double* du, const double* w, double* lv, int* status,
+void launch_lines_d(const KLines& L, const KThermo& th, double dt, const double* dtp,
+                    bool per, double* du, const double* w, double* lv, int* status,
                     hipStream_t s, KTimer* tm) {
   const unsigned nb = nblk(L.nline, 64);
   TIMED(tm, KID_LINES,
-        (k_line_assemble<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, per ? 1 : 0, w, du, lv)));
+        (k_line_assemble<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, dtp, per ? 1 : 0, w, du,
+                                                       lv)));
   if (per)
     TIMED(tm, KID_LINES,
           (k_line_sweep_periodic<NM, B, DIR><<<nb, 64, 0, s>>>(L, w, du, lv, status)));
@@ -238,20 +239,24 @@ void launch_lines_d(const KLines& L, const KThermo& th, double dt, bool per,
 
 // SolveImplicit3D's lines of one direction in place on du (x2 / x3 lines,
 // or periodic lines of any direction); lv holds line_doubles(g, dir, B).
+// The step is dt, or dt * dtp[0] when dtp is given.
 template <int NM>
-void launch_lines(const KGrid& g, const KThermo& th, double dt, int flag, int dir,
-                  bool per, double* du, const double* w, double* lv, int* status,
-                  hipStream_t s, KTimer* tm = nullptr) {
+void launch_lines(const KGrid& g, const KThermo& th, double dt, const double* dtp,
+                  int flag, int dir, bool per, double* du, const double* w, double* lv,
+                  int* status, hipStream_t s, KTimer* tm = nullptr) {
   const KLines L = make_lines(g, dir);
+#define CG_LINES(B, D) \
+  launch_lines_d<NM, B, D>(L, th, dt, dtp, per, du, w, lv, status, s, tm)
   if (flag & 8) {
-    if (dir == 0) launch_lines_d<NM, 5, 0>(L, th, dt, per, du, w, lv, status, s, tm);
-    if (dir == 1) launch_lines_d<NM, 5, 1>(L, th, dt, per, du, w, lv, status, s, tm);
-    if (dir == 2) launch_lines_d<NM, 5, 2>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 0) CG_LINES(5, 0);
+    if (dir == 1) CG_LINES(5, 1);
+    if (dir == 2) CG_LINES(5, 2);
   } else {
-    if (dir == 0) launch_lines_d<NM, 3, 0>(L, th, dt, per, du, w, lv, status, s, tm);
-    if (dir == 1) launch_lines_d<NM, 3, 1>(L, th, dt, per, du, w, lv, status, s, tm);
-    if (dir == 2) launch_lines_d<NM, 3, 2>(L, th, dt, per, du, w, lv, status, s, tm);
+    if (dir == 0) CG_LINES(3, 0);
+    if (dir == 1) CG_LINES(3, 1);
+    if (dir == 2) CG_LINES(3, 2);
   }
+#undef CG_LINES
 }
 
 // Periodic directions of a block (both boundaries periodic,

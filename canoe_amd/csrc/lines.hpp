@@ -70,12 +70,13 @@ __device__ inline void mload(Mt<B>& m, const double* p, long long st) {
 // a (folded at walls), b, c and rhs of cell lo + p; slot n holds the lower
 // corner c_{lo-1} of a periodic line.  Velocities enter in the frame of
 // DIR; the Roe average is formed from the unrotated states
-// (RoeAverage is called before the frame is chosen, :98-104).
+// (RoeAverage is called before the frame is chosen, :98-104).  The step
+// is dt, or dt * dtp[0] with a device step (runs: dt is the stage weight).
 template <int NM, int B, int DIR>
 __global__ void __launch_bounds__(64)
-    k_line_assemble(KLines L, KThermo th, double dt, int per,
-                    const double* __restrict__ w, const double* __restrict__ du,
-                    double* __restrict__ lv) {
+    k_line_assemble(KLines L, KThermo th, double dt, const double* __restrict__ dtp,
+                    int per, const double* __restrict__ w,
+                    const double* __restrict__ du, double* __restrict__ lv) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   constexpr int LS = line_level_size<B>();
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(64)
   const long long st = L.nline;
   const double gm1 = th.gammad - 1.;
   const double gm = 0.5 * (gm1 + gm1);
-  const double idt = 1. / dt;
+  const double idt = 1. / (dtp ? dt * dtp[0] : dt);
   auto load = [&](double* dst, int i) {
 #pragma unroll
     for (int n = 0; n < N; ++n) dst[n] = w[o + n * L.sn + (long long)i * L.sl];

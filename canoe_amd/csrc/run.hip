@@ -47,6 +47,7 @@ struct cg_run {
   int AS = 30;  // archive doubles per level (B*B + B)
   std::vector<RunBlock> blocks;
   double* arch = nullptr;
+  double* lines = nullptr;  // scratch of the x2 / x3 line solves
   KBlockPtr* d_bp = nullptr;
   KGroup* d_grp = nullptr;
   int ngroups = 0;
@@ -153,7 +154,9 @@ int run_stage(cg_run* r, int s) {
   src.chi = (r->th.gammad - 1.) / r->th.gammad;
   src.heat_flux = r->cfg.heat_flux;
   KTimer* tm = r->tm.on ? &r->tm : nullptr;
-  const bool imp = r->cfg.implicit_flag != 0;
+  // x1 columns in the batched solve fused with the update; x2 / x3 lines
+  // (implicit_flag bits 1, 2) on du before it
+  const bool imp = (r->cfg.implicit_flag & 1) != 0;
   const int bs = r->tune.flux_block;
   KUpdate up;
   up.stage = s;
@@ -190,6 +193,17 @@ int run_stage(cg_run* r, int s) {
   if (tiled)
     launch_fused_batch<NM>(r->d_bp, r->d_tiles, r->ntiles, r->kt, r->cfg, src,
                            r->tune.face_variant, st, tm);
+  // SolveImplicit3D's x3 then x2 lines (solve_implicit_3d.cpp:25-88), with
+  // the stage's step beta * dt read from the device
+  for (int d = 2; d >= 1; --d) {
+    if (!(r->cfg.implicit_flag & (1 << d))) continue;
+    for (RunBlock& b : r->blocks) {
+      if ((d == 1 && !b.g.f2) || (d == 2 && !b.g.f3)) continue;
+      const bool per = b.blk.bc[2 * d] == CG_BC_PERIODIC && b.blk.bc[2 * d + 1] == CG_BC_PERIODIC;
+      launch_lines<NM>(b.g, r->kt, sbeta[s], r->d_dt, r->cfg.implicit_flag, d, per, b.du,
+                       b.w, r->lines, r->ctx->d_status, st, tm);
+    }
+  }
   if (imp) {
     launch_columns<NM>(r, sbeta[s], up, tm);
     for (RunBlock& b : r->blocks) {
@@ -243,10 +257,12 @@ int check_config(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   if ((cfg->order_h != 2 && cfg->order_h != 5) ||
       (cfg->order_v != 2 && cfg->order_v != 5))
     return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
-  if (cfg->implicit_flag != 0 && cfg->implicit_flag != 1 && cfg->implicit_flag != 9)
-    return set_err(ctx, CG_ERR_ARG, "implicit_flag must be 0, 1 or 9");
-  if (cfg->implicit_flag && (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC))
-    return set_err(ctx, CG_ERR_ARG, "periodic x1 implicit solve not supported");
+  e = check_implicit_flag(ctx, *blk, cfg->implicit_flag);
+  if (e) return e;
+  // a run's x1 columns are solved between walls, fused with the update
+  // (periodic x1 columns: cg_implicit_solve)
+  if ((cfg->implicit_flag & 1) && (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC))
+    return set_err(ctx, CG_ERR_ARG, "a run solves x1 columns between walls, not periodic ones");
   if (cfg->satadj != CG_SATADJ_OFF && (th->nmass != 3 || th->ivap <= 0 || th->icld <= 0))
     return set_err(ctx, CG_ERR_ARG, "saturation adjustment needs a vapour/cloud pair");
   if (cfg->precision != CG_PREC_FP64 && cfg->precision != CG_PREC_FP32)
@@ -546,7 +562,16 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
          hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
   }
-  if (ok && cfg->implicit_flag) {
+  if (ok && (cfg->implicit_flag & 6)) {
+    size_t n = 0;
+    const int B = (cfg->implicit_flag & 8) ? 5 : 3;
+    for (const RunBlock& rb : r->blocks)
+      for (int d = 1; d <= 2; ++d)
+        if ((cfg->implicit_flag & (1 << d)) && (d == 1 ? rb.g.f2 : rb.g.f3))
+          n = std::max(n, line_doubles(rb.g, d, B));
+    ok = hipMalloc((void**)&r->lines, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess;
+  }
+  if (ok && (cfg->implicit_flag & 1)) {
     std::vector<KGroup> groups;
     size_t arch_n = 0;
     build_groups(r->blocks, r->AS, groups, arch_n);
@@ -580,6 +605,7 @@ int cg_run_destroy(cg_run* r) {
   drop_graph(r);
   for (RunBlock& b : r->blocks) free_block(b);
   if (r->arch) (void)hipFree(r->arch);
+  if (r->lines) (void)hipFree(r->lines);
   if (r->d_bp) (void)hipFree(r->d_bp);
   if (r->d_grp) (void)hipFree(r->d_grp);
   if (r->part) (void)hipFree(r->part);
@@ -979,8 +1005,9 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     int wpb = wpbs[wi];
     if (!(r->cfg.implicit_flag & 8) && wpb != 1) continue;
     const int grids[4] = {0, r->ncu * 4 / wpb, r->ncu * 2 / wpb, r->ncu / wpb};
-    for (int gi = 0; gi < (r->cfg.implicit_flag ? 4 : 1); ++gi)
-      for (int fu = 1; fu >= (r->cfg.implicit_flag ? 0 : 1); --fu) {
+    const bool cols = (r->cfg.implicit_flag & 1) != 0;
+    for (int gi = 0; gi < (cols ? 4 : 1); ++gi)
+      for (int fu = 1; fu >= (cols ? 0 : 1); --fu) {
         cg_tune t = best;
         t.implicit_wpb = wpb;
         t.implicit_grid = grids[gi];

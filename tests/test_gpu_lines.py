@@ -91,3 +91,61 @@ def test_species_with_walled_lines_rejected(ctx):
     with pytest.raises(ca.CanoeGpuError):
         ca.ImplicitSolver(ctx, blk, th, 10).solve_implicit_3d(
             np.zeros(blk.shape()), w, 0.5)
+
+
+@pytest.mark.parametrize("flag", [15, 14, 11, 7])
+@pytest.mark.parametrize("graph", [False, True])
+def test_run_with_lines_matches_oracle(ctx, flag, graph):
+    """A run with x2 / x3 lines (SolveImplicit3D's x3, x2, then x1 solves in
+    every stage, the step read from the device) against the oracle's step."""
+    blk, th, cfg, w = perturbed_state(2, (20, 12, 10), vel=1.0, dp=1e-3)
+    cfg.implicit_flag = flag
+    dt = orl.new_block_dt(blk, th, cfg, w, 0.5)
+    _say("run lines flag=%d graph=%s" % (flag, graph))
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        run.step(dt, 3, graph=graph)
+        ug = run.cons()
+        dtg = run.new_dt(0.5)
+        wg = run.prim()
+    finally:
+        run.close()
+    u0 = orl.prim_to_cons(blk, th, w)
+    uo, _ = orl.step(blk, th, cfg, dt, 3, u0, w)
+    sk, sj, si = blk.interior()
+    e = relerr(ug[:, sk, sj, si], uo[:, sk, sj, si])
+    _say("  relerr %.3e" % e)
+    assert e <= TOL
+    assert dtg == orl.new_block_dt(blk, th, cfg, wg, 0.5)
+
+
+def test_mixed_run_with_lines(ctx):
+    blk, th, cfg, w = perturbed_state(2, (24, 16, 12), vel=1.0, dp=1e-3)
+    cfg.implicit_flag = 15
+    dt = orl.new_block_dt(blk, th, cfg, w, 0.5)
+    us = {}
+    for prec in (ca.PREC_FP64, ca.PREC_FP32):
+        cfg.precision = prec
+        _say("mixed run lines prec=%d" % prec)
+        run = ca.Run(ctx, blk, th, cfg)
+        try:
+            run.set_prim(w)
+            run.step(dt, 10, graph=True)
+            us[prec] = run.cons()
+        finally:
+            run.close()
+    sk, sj, si = blk.interior()
+    a, b = us[ca.PREC_FP32][:, sk, sj, si], us[ca.PREC_FP64][:, sk, sj, si]
+    for idx in ([0], [1, 2, 3], [4]):
+        e = np.sqrt(np.sum((a[idx] - b[idx]) ** 2) / np.sum(b[idx] ** 2))
+        assert e <= 1e-5, (idx, e)
+
+
+def test_run_rejects_periodic_x1_columns(ctx):
+    blk, th, cfg, w = perturbed_state(2, (20, 12, 10))
+    bc = list(blk.bc)
+    bc[0] = bc[1] = ca.BC_PERIODIC
+    _set_bc(blk, bc)
+    with pytest.raises(ca.CanoeGpuError):
+        ca.Run(ctx, blk, th, cfg)
