@@ -426,67 +426,87 @@ __global__ void __launch_bounds__(64)
       for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
     }
   }
+  // Each level forms zeta_p and gamma_p first and parks them in the
+  // scratch, so that only the pivoted inverse and the running sums are live
+  // across it.
   for (int p = 1; p <= n - 2; ++p) {
     double* cur = lvl(p);
     const double* prv = lvl(p - 1);
-    Mt<B> T, Lp, ap;
-    mload<B>(Lp, cur + B * B * st, st);
-    mload<B>(ap, prv, st);  // ainv_{p-1}
-    mmul<B>(Lp, ap, T);
-    Mt<B> Up, TU, d, ai;
-    mload<B>(Up, prv + 2 * B * B * st, st);  // U_{p-1}
-    mmul<B>(T, Up, TU);
-    mload<B>(d, cur, st);
-    msub<B>(d, TU, d);
-    if (!minv<B>(d, ai)) ok = false;
-    // zeta_p
-    Vt<B> zp, Tz, z;
-#pragma unroll
-    for (int r = 0; r < B; ++r) zp.v[r] = prv[(3 * B * B + r) * st];
-    mvec<B>(T, zp, Tz);
-#pragma unroll
-    for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st] - Tz.v[r];
-    // gamma_p
-    Mt<B> gp, Tg, g;
-    mload<B>(gp, prv + B * B * st, st);
-    mmul<B>(T, gp, Tg);
-    Mt<B> bnew;
-    if (p < n - 2) {
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int q = 0; q < B; ++q) g.a[r][q] = -Tg.a[r][q];
-      Mt<B> bu;
-      mmul<B>(bet, Up, bu);
-      mmul<B>(bu, ai, bnew);
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int q = 0; q < B; ++q) bnew.a[r][q] = -bnew.a[r][q];
-    } else {
-      Mt<B> Uc;
-      mload<B>(Uc, cur + 2 * B * B * st, st);
-      msub<B>(Uc, Tg, g);
-      Mt<B> Ln, bu, x;
-      mload<B>(Ln, lvl(n - 1) + B * B * st, st);
-      mmul<B>(bet, Up, bu);
-      msub<B>(Ln, bu, x);
-      mmul<B>(x, ai, bnew);
+    Mt<B> T;
+    {
+      Mt<B> Lp, ap;
+      mload<B>(Lp, cur + B * B * st, st);
+      mload<B>(ap, prv, st);  // ainv_{p-1}
+      mmul<B>(Lp, ap, T);
     }
-    bet = bnew;
-    mstore<B>(ai, cur, st);
-    mstore<B>(g, cur + B * B * st, st);
+    {  // zeta_p = r_p - T zeta_{p-1}
+      Vt<B> zp, Tz;
 #pragma unroll
-    for (int r = 0; r < B; ++r) cur[(3 * B * B + r) * st] = z.v[r];
-    Mt<B> bg;
-    Vt<B> bz;
-    mmul<B>(bet, g, bg);
-    mvec<B>(bet, z, bz);
+      for (int r = 0; r < B; ++r) zp.v[r] = prv[(3 * B * B + r) * st];
+      mvec<B>(T, zp, Tz);
 #pragma unroll
-    for (int r = 0; r < B; ++r) {
-      sbz.v[r] = sbz.v[r] + bz.v[r];
+      for (int r = 0; r < B; ++r)
+        cur[(3 * B * B + r) * st] = cur[(3 * B * B + r) * st] - Tz.v[r];
+    }
+    {  // gamma_p = -T gamma_{p-1}, or U_{n-2} - T gamma_{n-3}
+      Mt<B> gp, Tg, g;
+      mload<B>(gp, prv + B * B * st, st);
+      mmul<B>(T, gp, Tg);
+      if (p < n - 2) {
 #pragma unroll
-      for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+          for (int q = 0; q < B; ++q) g.a[r][q] = -Tg.a[r][q];
+      } else {
+        Mt<B> Uc;
+        mload<B>(Uc, cur + 2 * B * B * st, st);
+        msub<B>(Uc, Tg, g);
+      }
+      mstore<B>(g, cur + B * B * st, st);  // b_p is no longer needed
+    }
+    Mt<B> ai;
+    {  // ainv_p = (d_p - T U_{p-1})^-1
+      Mt<B> Up, TU, d;
+      mload<B>(Up, prv + 2 * B * B * st, st);
+      mmul<B>(T, Up, TU);
+      mload<B>(d, cur, st);
+      msub<B>(d, TU, d);
+      if (!minv<B>(d, ai)) ok = false;
+      mstore<B>(ai, cur, st);
+    }
+    {  // beta_p = -(beta_{p-1} U_{p-1}) ainv_p, or
+       // (L_{n-1} - beta_{n-3} U_{n-3}) ainv_{n-2}
+      Mt<B> Up, bu, bnew;
+      mload<B>(Up, prv + 2 * B * B * st, st);
+      mmul<B>(bet, Up, bu);
+      if (p < n - 2) {
+        mmul<B>(bu, ai, bnew);
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+          for (int q = 0; q < B; ++q) bnew.a[r][q] = -bnew.a[r][q];
+      } else {
+        Mt<B> Ln, x;
+        mload<B>(Ln, lvl(n - 1) + B * B * st, st);
+        msub<B>(Ln, bu, x);
+        mmul<B>(x, ai, bnew);
+      }
+      bet = bnew;
+    }
+    {  // sums of beta_p gamma_p and beta_p zeta_p
+      Mt<B> g, bg;
+      Vt<B> z, bz;
+      mload<B>(g, cur + B * B * st, st);
+#pragma unroll
+      for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st];
+      mmul<B>(bet, g, bg);
+      mvec<B>(bet, z, bz);
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        sbz.v[r] = sbz.v[r] + bz.v[r];
+#pragma unroll
+        for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
+      }
     }
   }
   // last level
