@@ -466,6 +466,39 @@ def bench_moist(ctx, nx, steps, warmup, acc_steps, npairs, train_steps):
     return out
 
 
+def bench_lines(ctx, nx, steps, warmup):
+    """SolveImplicit3D in every direction on the dry box, plain FP64:
+    implicit_flag 15 = x3 lines, then x2 lines (both with periodic ends,
+    PeriodicForwardSweep's cyclic recurrences), then the x1 columns, in each
+    of the three stages.  Algorithmic bytes of one direction's solve: read w,
+    read du, write du, 24 N bytes per cell; the line kernels' time per
+    direction and stage is their time per step over 6."""
+    blk, th, cfg = ca.pgen_setup(2, *nx)
+    w0 = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0)
+    cfg.implicit_flag = 15
+    cfg.precision = ca.PREC_FP64
+    cells = blk.ncells()
+    ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps, warmup)
+    lk = stats.get("k_line_*", (0.0, 0))
+    line_ms = lk[0] * lk[1]
+    per_dir = line_ms / 6.0
+    alg = 24 * blk.N * cells
+    ach = alg / (per_dir * 1e-3) / 1e9 if per_dir > 0 else None
+    log("lines (flag 15): %.3f ms/step, line kernels %.3f ms/step"
+        % (ms, line_ms))
+    return {"workload": "dry box %dx%dx%d, implicit_flag 15 (x3, x2 periodic "
+                        "lines + x1 columns), plain FP64"
+                        % (blk.nx2, blk.nx3, blk.nx1),
+            "ms_per_step": round(ms, 4),
+            "line_kernels_ms_per_step": round(line_ms, 4),
+            "line_ms_per_direction_stage": round(per_dir, 4),
+            "algorithmic_bytes_per_direction_stage": alg,
+            "achieved_GBs": round(ach, 1) if ach else None,
+            "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
+            "kernels_avg_ms": {k: round(v[0], 4) for k, v in stats.items()}}
+
+
 def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     bl = ca.batch_blocks(seed, nblocks)
     blocks = [b[0] for b in bl]
@@ -564,6 +597,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for rocprofv3 (copy calibration + "
                          "a few steps of the dry box, FP32 fluxes)")
+    ap.add_argument("--lines", action="store_true",
+                    help="only the dry box with implicit x2/x3 lines "
+                         "(implicit_flag 15)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path here "
@@ -576,12 +612,19 @@ def main():
     if args.profile:
         profile_run(args.nx, max(1, min(args.steps, 5)))
         return
+    if args.lines:
+        ctx = ca.Context(0)
+        print(json.dumps(bench_lines(ctx, args.nx, max(1, args.steps),
+                                     args.warmup)))
+        ctx.close()
+        return
 
     ctx = ca.Context(0)
     blk, cfg, dt, cells, forms, acc = bench_box(ctx, args.nx, args.steps,
                                                 args.warmup, args.acc_steps)
-    moist = batch = None
+    moist = batch = lines = None
     if not args.quick:
+        lines = bench_lines(ctx, args.nx, max(2, args.steps // 5), 1)
         moist = bench_moist(ctx, args.nx, args.steps, args.warmup,
                             args.acc_steps, args.pairs, args.train_steps)
         batch = bench_batch(ctx, 2024, 32, max(3, args.steps // 4),
@@ -630,6 +673,7 @@ def main():
         "forms": forms,
         "moist_box": moist,
         "batch": batch,
+        "implicit_lines": lines,
     }
     print(json.dumps(out))
 
