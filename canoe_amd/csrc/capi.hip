@@ -197,7 +197,7 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   size_t need = 0;
   for (int d = 0; d < 3; ++d)
     if ((flag & (1 << d)) && act[d])
-      need = std::max(need, (d == 0 && !per[0]) ? arch_doubles(g) : line_doubles(g, d, B));
+      need = std::max(need, (d == 0 && !per[0]) ? arch_doubles(g) : line_doubles(g, d, B, per[d]));
   if (need == 0) return CG_OK;  // only directions without cells were asked
   double* scratch = nullptr;
   HIPCHK(ctx, hipMallocAsync((void**)&scratch, sizeof(double) * need, ctx->stream));

@@ -217,9 +217,10 @@ inline KLines make_lines(const KGrid& g, int dir) {
 }
 
 // Scratch doubles of the line solve of direction dir (B x B blocks).
-inline size_t line_doubles(const KGrid& g, int dir, int B) {
+inline size_t line_doubles(const KGrid& g, int dir, int B, bool per) {
   KLines L = make_lines(g, dir);
-  return (size_t)(L.n + 1) * (3 * B * B + B) * (size_t)L.nline;
+  const size_t ls = per ? 3 * B * B + B : B * B + B;
+  return (size_t)L.n * ls * (size_t)L.nline;
 }
 
 template <int NM, int B, int DIR>
@@ -227,18 +228,16 @@ void launch_lines_d(const KLines& L, const KThermo& th, double dt, const double*
                     bool per, double* du, const double* w, double* lv, int* status,
                     hipStream_t s, KTimer* tm) {
   const unsigned nb = nblk(L.nline, 64);
-  TIMED(tm, KID_LINES,
-        (k_line_assemble<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, dtp, per ? 1 : 0, w, du,
-                                                       lv)));
   if (per)
     TIMED(tm, KID_LINES,
-          (k_line_sweep_periodic<NM, B, DIR><<<nb, 64, 0, s>>>(L, w, du, lv, status)));
+          (k_line_periodic<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, dtp, w, du, lv, status)));
   else
-    TIMED(tm, KID_LINES, (k_line_sweep<NM, B, DIR><<<nb, 64, 0, s>>>(L, w, du, lv, status)));
+    TIMED(tm, KID_LINES,
+          (k_line_walls<NM, B, DIR><<<nb, 64, 0, s>>>(L, th, dt, dtp, w, du, lv, status)));
 }
 
 // SolveImplicit3D's lines of one direction in place on du (x2 / x3 lines,
-// or periodic lines of any direction); lv holds line_doubles(g, dir, B).
+// or periodic lines of any direction); lv holds line_doubles(g, dir, B, per).
 // The step is dt, or dt * dtp[0] when dtp is given.
 template <int NM>
 void launch_lines(const KGrid& g, const KThermo& th, double dt, const double* dtp,

@@ -4,13 +4,11 @@
 // (src/snap/implicit/solve_implicit_3d.cpp:19-143,
 // periodic_forward_backward.hpp:14-232).
 //
-// One line per lane in two passes.  k_line_assemble builds the blocks
-// a_i, b_i, c_i and the right-hand side of every interior level (plus, for
-// a periodic line, the lower corner c_{is-1}) into a lane-contiguous
-// scratch; k_line_sweep (walls) or k_line_sweep_periodic (El-Mikkawy's
-// cyclic recurrences) solves and writes du back.  Splitting the assembly
-// from the recurrence keeps each kernel's live state small: the cyclic
-// recurrence alone carries four B x B matrices between levels.
+// One line per lane, one launch per direction: the blocks a_i, b_i, c_i
+// and the right-hand side of each level are assembled in registers as the
+// lane climbs the line and fed straight into the recurrence; only what the
+// way back needs goes to a lane-contiguous scratch (walls: a'_i and
+// delta_i; periodic ends: ainv_i, gamma_i, zeta_i and c_i).
 #pragma once
 
 #include "kernels.hpp"
@@ -32,10 +30,11 @@ struct KLines {
   }
 };
 
-// Scratch doubles of one level: a, b, c (B x B each) and the rhs (B).
+// Scratch doubles of one level: walls a'_i, delta_i (B x B + B); periodic
+// ends ainv_i, gamma_i, c_i (B x B each) and zeta_i (B).
 template <int B>
-CG_HD constexpr int line_level_size() {
-  return 3 * B * B + B;
+CG_HD constexpr int line_level_size(bool per) {
+  return per ? 3 * B * B + B : B * B + B;
 }
 
 // The velocities of a state in the frame of direction DIR: slot IVX holds
@@ -66,54 +65,60 @@ __device__ inline void mload(Mt<B>& m, const double* p, long long st) {
 }
 
 // Assembly of FullCorrection / PartialCorrection (full_correction.cpp:
-// 77-146, partial_correction.cpp:69-155) for one line: level p holds
-// a (folded at walls), b, c and rhs of cell lo + p; slot n holds the lower
-// corner c_{lo-1} of a periodic line.  Velocities enter in the frame of
-// DIR; the Roe average is formed from the unrotated states
-// (RoeAverage is called before the frame is chosen, :98-104).  The step
-// is dt, or dt * dtp[0] with a device step (runs: dt is the stage weight).
+// 77-146, partial_correction.cpp:69-155) along one line, one level per
+// call: level(i) returns a_i (unfolded), b_i, c_i and rhs_i of cell i and
+// moves the window up.  Velocities enter in the frame of DIR; the Roe
+// average is formed from the unrotated states (RoeAverage precedes the
+// choice of frame, :98-104).  start(i0) primes the window so that the
+// first level() is cell i0.
 template <int NM, int B, int DIR>
-__global__ void __launch_bounds__(64)
-    k_line_assemble(KLines L, KThermo th, double dt, const double* __restrict__ dtp,
-                    int per, const double* __restrict__ w,
-                    const double* __restrict__ du, double* __restrict__ lv) {
-  constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
-  constexpr int LS = line_level_size<B>();
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.nline) return;
-  const long long o = L.org(t);
-  const long long st = L.nline;
-  const double gm1 = th.gammad - 1.;
-  const double gm = 0.5 * (gm1 + gm1);
-  const double idt = 1. / (dtp ? dt * dtp[0] : dt);
-  auto load = [&](double* dst, int i) {
+struct LineAsm {
+  static constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  const KLines& L;
+  const KThermo& th;
+  const double* __restrict__ w;
+  const double* __restrict__ du;
+  long long o;
+  double gm1, gm, idt;
+  double wc[N];
+  Mt<B> Am, dm, dc;
+  __device__ LineAsm(const KLines& L_, const KThermo& th_, const double* w_,
+                     const double* du_, long long o_, double dt)
+      : L(L_), th(th_), w(w_), du(du_), o(o_) {
+    gm1 = th.gammad - 1.;
+    gm = 0.5 * (gm1 + gm1);
+    idt = 1. / dt;
+  }
+  __device__ void load(double* dst, int i) const {
 #pragma unroll
     for (int n = 0; n < N; ++n) dst[n] = w[o + n * L.sn + (long long)i * L.sl];
-  };
-  auto jac = [&](const double* wc, Mt<B>& out) {
+  }
+  __device__ void jac(const double* wv, Mt<B>& out) const {
     double f[N];
-    to_frame<NM, DIR>(wc, f);
+    to_frame<NM, DIR>(wv, f);
     flux_jac<NM, B>(gm1, f, out);
-  };
-  auto absA = [&](const double* wl, const double* wr, Mt<B>& out) {
+  }
+  __device__ void absA(const double* wl, const double* wr, Mt<B>& out) const {
     double prim[N], pf[N];
     roe_average<NM>(prim, gm, wl, wr);
     to_frame<NM, DIR>(prim, pf);
     abs_roe_fma<NM, B>(th, pf, gm, out);
-  };
-  const int lo = L.lo, hi = L.lo + L.n - 1;
-  double wm[N], wc[N], wp[N];
-  load(wm, lo - 2);
-  load(wc, lo - 1);
-  Mt<B> Am, Ap, dm, dc, dp;
-  absA(wm, wc, Am);
-  jac(wm, dm);
-  jac(wc, dc);
-  for (int i = lo - 1; i <= hi; ++i) {
+  }
+  __device__ void start(int i0) {
+    double wm[N];
+    load(wm, i0 - 1);
+    load(wc, i0);
+    absA(wm, wc, Am);
+    jac(wm, dm);
+    jac(wc, dc);
+  }
+  __device__ void level(int i, Mt<B>& a, Mt<B>& b, Mt<B>& c, double* rhs,
+                        bool want_rhs) {
+    double wp[N];
     load(wp, i + 1);
+    Mt<B> Ap, dp;
     absA(wc, wp, Ap);
     jac(wp, dp);
-    Mt<B> a, b, c;
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
@@ -130,29 +135,7 @@ __global__ void __launch_bounds__(64)
         b.a[r][q] = -(Am.a[r][q] + dm.a[r][q]) * L.sla;
         c.a[r][q] = -(Ap.a[r][q] - dp.a[r][q]) * L.sra;
       }
-    if (i == lo - 1) {
-      if (per) mstore<B>(c, lv + (long long)L.n * LS * st + t, st);
-    } else {
-      if (!per) {
-        // walls: a_lo += b_lo diag(1,-1,1,..), a_hi += c_hi diag(1,-1,1,..)
-        if (i == lo) {
-#pragma unroll
-          for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int q = 0; q < B; ++q) a.a[r][q] += (q == 1) ? -b.a[r][q] : b.a[r][q];
-        }
-        if (i == hi) {
-#pragma unroll
-          for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int q = 0; q < B; ++q) a.a[r][q] += (q == 1) ? -c.a[r][q] : c.a[r][q];
-        }
-      }
-      double* base = lv + (long long)(i - lo) * LS * st + t;
-      mstore<B>(a, base, st);
-      mstore<B>(b, base + B * B * st, st);
-      mstore<B>(c, base + 2 * B * B * st, st);
-      // rhs in the frame of DIR (forward_backward.hpp:24-36)
+    if (want_rhs) {  // rhs in the frame of DIR (forward_backward.hpp:24-36)
       const long long ci = o + (long long)i * L.sl;
       double d[N];
 #pragma unroll
@@ -160,7 +143,6 @@ __global__ void __launch_bounds__(64)
       double r0 = d[0];
 #pragma unroll
       for (int n = 1; n < IVX; ++n) r0 += d[n];
-      double rhs[B];
       rhs[0] = r0 * idt;
       if (B == 5) {
         rhs[1] = d[IVX + DIR] * idt;
@@ -171,19 +153,23 @@ __global__ void __launch_bounds__(64)
         rhs[1] = d[IVX + DIR] * idt;
         rhs[2] = d[IEN] * idt;
       }
-#pragma unroll
-      for (int r = 0; r < B; ++r) base[(3 * B * B + r) * st] = rhs[r];
     }
+    // the next level's b: -(A_{i+1/2} + dfdq_i) aleft / (2 vol)
     Am = Ap;
     dm = dc;
     dc = dp;
 #pragma unroll
-    for (int n = 0; n < N; ++n) {
-      wm[n] = wc[n];
-      wc[n] = wp[n];
-    }
+    for (int n = 0; n < N; ++n) wc[n] = wp[n];
   }
-}
+  // b of the next level without moving the window: -(Am + dm) sla after
+  // the last level() call
+  __device__ void next_b(Mt<B>& b) const {
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int q = 0; q < B; ++q) b.a[r][q] = -(Am.a[r][q] + dm.a[r][q]) * L.sla;
+  }
+};
 
 // du of a cell from the line's solution d (frame of DIR).  Walls: the
 // write-back of BackwardSubstitution (forward_backward.hpp:100-124), the
@@ -289,32 +275,54 @@ __device__ inline bool gj_solve_d(Mt<B>& M, Mt<B>& C, double* t) {
   return ok;
 }
 
+template <int B>
+__device__ inline void msub(const Mt<B>& x, const Mt<B>& y, Mt<B>& z) {
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int q = 0; q < B; ++q) z.a[r][q] = x.a[r][q] - y.a[r][q];
+}
+
 // ForwardSweep / BackwardSubstitution (forward_backward.hpp:14-137) of a
-// line with walls: M = a_p - b_p a'_{p-1}, a'_p = M^-1 c_p,
-// delta_p = M^-1 (rhs_p - b_p delta_{p-1}) by one Gauss-Jordan elimination;
-// a'_p and delta_p overwrite c_p and rhs_p; then delta_p -= a'_p delta_{p+1}
-// and the write-back.
+// line with walls (full_correction.cpp:144-145 folds them into a_lo and
+// a_hi): M = a_p - b_p a'_{p-1}, a'_p = M^-1 c_p and
+// delta_p = M^-1 (rhs_p - b_p delta_{p-1}) from one Gauss-Jordan elimination
+// of [M | c_p | rhs_p], archived; then delta_p -= a'_p delta_{p+1} down the
+// line and the write-back.  The step is dt, or dt * dtp[0] with a device
+// step (runs: dt is the stage weight).
 template <int NM, int B, int DIR>
 __global__ void __launch_bounds__(64)
-    k_line_sweep(KLines L, const double* __restrict__ w, double* __restrict__ du,
+    k_line_walls(KLines L, KThermo th, double dt, const double* __restrict__ dtp,
+                 const double* __restrict__ w, double* __restrict__ du,
                  double* __restrict__ lv, int* __restrict__ status) {
-  constexpr int LS = line_level_size<B>();
+  constexpr int AS = line_level_size<B>(false);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L.nline) return;
+  const long long o = L.org(t);
   const long long st = L.nline;
+  const int lo = L.lo, hi = L.lo + L.n - 1;
+  LineAsm<NM, B, DIR> as(L, th, w, du, o, dtp ? dt * dtp[0] : dt);
+  as.start(lo);
   bool ok = true;
   Mt<B> ap;
   Vt<B> dl;
-  for (int p = 0; p < L.n; ++p) {
-    double* base = lv + (long long)p * LS * st + t;
+  for (int i = lo; i <= hi; ++i) {
     Mt<B> M, b, c;
     double tv[B];
-    mload<B>(M, base, st);
-    mload<B>(b, base + B * B * st, st);
-    mload<B>(c, base + 2 * B * B * st, st);
+    as.level(i, M, b, c, tv, true);
+    if (i == lo) {
 #pragma unroll
-    for (int r = 0; r < B; ++r) tv[r] = base[(3 * B * B + r) * st];
-    if (p > 0) {
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) M.a[r][q] += (q == 1) ? -b.a[r][q] : b.a[r][q];
+    }
+    if (i == hi) {
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) M.a[r][q] += (q == 1) ? -c.a[r][q] : c.a[r][q];
+    }
+    if (i > lo) {
 #pragma unroll
       for (int r = 0; r < B; ++r) {
 #pragma unroll
@@ -334,37 +342,29 @@ __global__ void __launch_bounds__(64)
     ap = c;
 #pragma unroll
     for (int r = 0; r < B; ++r) dl.v[r] = tv[r];
-    mstore<B>(ap, base + 2 * B * B * st, st);
+    double* base = lv + (long long)(i - lo) * AS * st + t;
+    mstore<B>(ap, base, st);
 #pragma unroll
-    for (int r = 0; r < B; ++r) base[(3 * B * B + r) * st] = dl.v[r];
+    for (int r = 0; r < B; ++r) base[(B * B + r) * st] = dl.v[r];
   }
   if (!ok) atomicOr(status, 1);
-  const long long o = L.org(t);
   Vt<B> dn;
   for (int p = L.n - 1; p >= 0; --p) {
-    const double* base = lv + (long long)p * LS * st + t;
+    const double* base = lv + (long long)p * AS * st + t;
     Vt<B> d;
 #pragma unroll
-    for (int r = 0; r < B; ++r) d.v[r] = base[(3 * B * B + r) * st];
+    for (int r = 0; r < B; ++r) d.v[r] = base[(B * B + r) * st];
     if (p < L.n - 1) {
       Mt<B> a;
-      mload<B>(a, base + 2 * B * B * st, st);
+      mload<B>(a, base, st);
       Vt<B> ad;
       mvec<B>(a, dn, ad);
 #pragma unroll
       for (int r = 0; r < B; ++r) d.v[r] = d.v[r] - ad.v[r];
     }
     dn = d;
-    line_write_back<NM, B, DIR, false>(d, w, du, o + (long long)(L.lo + p) * L.sl, L.sn);
+    line_write_back<NM, B, DIR, false>(d, w, du, o + (long long)(lo + p) * L.sl, L.sn);
   }
-}
-
-template <int B>
-__device__ inline void msub(const Mt<B>& x, const Mt<B>& y, Mt<B>& z) {
-#pragma unroll
-  for (int r = 0; r < B; ++r)
-#pragma unroll
-    for (int q = 0; q < B; ++q) z.a[r][q] = x.a[r][q] - y.a[r][q];
 }
 
 // PeriodicForwardSweep / PeriodicBackwardSubstitution (periodic_forward_
@@ -381,20 +381,26 @@ __device__ inline void msub(const Mt<B>& x, const Mt<B>& y, Mt<B>& z) {
 //   zeta_{n-1} = r_{n-1} - sum beta_p zeta_p (p < n-1);
 //   x_{n-1} = ainv_{n-1} zeta_{n-1},
 //   x_p = ainv_p (zeta_p - U_p x_{p+1} - gamma_p x_{n-1}), U_{n-2} = 0.
-// ainv_p, gamma_p and zeta_p overwrite a_p, b_p and rhs_p of the scratch
-// (b_{n-1} is read at p = n-2 before its level is reached).  Products left
-// to right as written, inverses by Gauss-Jordan with partial pivoting.
+// Each level forms zeta_p and gamma_p first and parks them in the scratch,
+// so that only the pivoted inverse, beta and the running sums are live
+// across it; L_{n-1} comes from the assembly window at level n-2.
+// Products left to right as written, inverses by Gauss-Jordan with partial
+// pivoting.
 template <int NM, int B, int DIR>
 __global__ void __launch_bounds__(64)
-    k_line_sweep_periodic(KLines L, const double* __restrict__ w,
-                          double* __restrict__ du, double* __restrict__ lv,
-                          int* __restrict__ status) {
-  constexpr int LS = line_level_size<B>();
+    k_line_periodic(KLines L, KThermo th, double dt, const double* __restrict__ dtp,
+                    const double* __restrict__ w, double* __restrict__ du,
+                    double* __restrict__ lv, int* __restrict__ status) {
+  constexpr int AS = line_level_size<B>(true);
+  constexpr int OG = B * B, OU = 2 * B * B, OZ = 3 * B * B;  // gamma, c, zeta
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L.nline) return;
+  const long long o = L.org(t);
   const long long st = L.nline;
-  const int n = L.n;
-  auto lvl = [&](int p) { return lv + (long long)p * LS * st + t; };
+  const int n = L.n, lo = L.lo;
+  auto lvl = [&](int p) { return lv + (long long)p * AS * st + t; };
+  LineAsm<NM, B, DIR> as(L, th, w, du, o, dtp ? dt * dtp[0] : dt);
+  as.start(lo - 1);
   bool ok = true;
   Mt<B> bet, sbg;
   Vt<B> sbz;
@@ -404,21 +410,26 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int q = 0; q < B; ++q) sbg.a[r][q] = 0.0;
   }
-  // p = 0
-  {
-    Mt<B> d, ai, um1;
-    mload<B>(d, lvl(0), st);
-    mload<B>(um1, lvl(n), st);
-    if (!minv<B>(d, ai)) ok = false;
-    mmul<B>(um1, ai, bet);
-    mstore<B>(ai, lvl(0), st);  // gamma_0 = b_0 and zeta_0 = rhs_0 stay
-    Mt<B> g0, bg;
-    mload<B>(g0, lvl(0) + B * B * st, st);
-    Vt<B> z0, bz;
+  {  // the corner c_{lo-1}, then level 0
+    Mt<B> a, b, um1;
+    double rhs[B];
+    as.level(lo - 1, a, b, um1, rhs, false);
+    Mt<B> c, ai;
+    as.level(lo, a, b, c, rhs, true);
+    double* cur = lvl(0);
+    mstore<B>(c, cur + OU * st, st);
+    mstore<B>(b, cur + OG * st, st);  // gamma_0 = L_0
 #pragma unroll
-    for (int r = 0; r < B; ++r) z0.v[r] = lvl(0)[(3 * B * B + r) * st];
-    mmul<B>(bet, g0, bg);
-    mvec<B>(bet, z0, bz);
+    for (int r = 0; r < B; ++r) cur[(OZ + r) * st] = rhs[r];
+    if (!minv<B>(a, ai)) ok = false;
+    mstore<B>(ai, cur, st);
+    mmul<B>(um1, ai, bet);
+    Mt<B> bg;
+    Vt<B> z, bz;
+    mmul<B>(bet, b, bg);
+#pragma unroll
+    for (int r = 0; r < B; ++r) z.v[r] = rhs[r];
+    mvec<B>(bet, z, bz);
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       sbz.v[r] = sbz.v[r] + bz.v[r];
@@ -426,58 +437,57 @@ __global__ void __launch_bounds__(64)
       for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
     }
   }
-  // Each level forms zeta_p and gamma_p first and parks them in the
-  // scratch, so that only the pivoted inverse and the running sums are live
-  // across it.
   for (int p = 1; p <= n - 2; ++p) {
     double* cur = lvl(p);
     const double* prv = lvl(p - 1);
-    Mt<B> T;
+    Mt<B> d;
     {
-      Mt<B> Lp, ap;
-      mload<B>(Lp, cur + B * B * st, st);
-      mload<B>(ap, prv, st);  // ainv_{p-1}
-      mmul<B>(Lp, ap, T);
-    }
-    {  // zeta_p = r_p - T zeta_{p-1}
-      Vt<B> zp, Tz;
-#pragma unroll
-      for (int r = 0; r < B; ++r) zp.v[r] = prv[(3 * B * B + r) * st];
-      mvec<B>(T, zp, Tz);
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-        cur[(3 * B * B + r) * st] = cur[(3 * B * B + r) * st] - Tz.v[r];
-    }
-    {  // gamma_p = -T gamma_{p-1}, or U_{n-2} - T gamma_{n-3}
-      Mt<B> gp, Tg, g;
-      mload<B>(gp, prv + B * B * st, st);
-      mmul<B>(T, gp, Tg);
-      if (p < n - 2) {
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-          for (int q = 0; q < B; ++q) g.a[r][q] = -Tg.a[r][q];
-      } else {
-        Mt<B> Uc;
-        mload<B>(Uc, cur + 2 * B * B * st, st);
-        msub<B>(Uc, Tg, g);
+      Mt<B> b, c;
+      double rhs[B];
+      as.level(lo + p, d, b, c, rhs, true);
+      mstore<B>(c, cur + OU * st, st);
+      Mt<B> T;
+      {
+        Mt<B> ap;
+        mload<B>(ap, prv, st);  // ainv_{p-1}
+        mmul<B>(b, ap, T);
       }
-      mstore<B>(g, cur + B * B * st, st);  // b_p is no longer needed
+      {  // zeta_p = r_p - T zeta_{p-1}
+        Vt<B> zp, Tz;
+#pragma unroll
+        for (int r = 0; r < B; ++r) zp.v[r] = prv[(OZ + r) * st];
+        mvec<B>(T, zp, Tz);
+#pragma unroll
+        for (int r = 0; r < B; ++r) cur[(OZ + r) * st] = rhs[r] - Tz.v[r];
+      }
+      {  // gamma_p = -T gamma_{p-1}, or U_{n-2} - T gamma_{n-3}
+        Mt<B> gp, Tg, g;
+        mload<B>(gp, prv + OG * st, st);
+        mmul<B>(T, gp, Tg);
+        if (p < n - 2) {
+#pragma unroll
+          for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int q = 0; q < B; ++q) g.a[r][q] = -Tg.a[r][q];
+        } else {
+          msub<B>(c, Tg, g);
+        }
+        mstore<B>(g, cur + OG * st, st);
+      }
+      {  // d_p - T U_{p-1}
+        Mt<B> Up, TU;
+        mload<B>(Up, prv + OU * st, st);
+        mmul<B>(T, Up, TU);
+        msub<B>(d, TU, d);
+      }
     }
     Mt<B> ai;
-    {  // ainv_p = (d_p - T U_{p-1})^-1
-      Mt<B> Up, TU, d;
-      mload<B>(Up, prv + 2 * B * B * st, st);
-      mmul<B>(T, Up, TU);
-      mload<B>(d, cur, st);
-      msub<B>(d, TU, d);
-      if (!minv<B>(d, ai)) ok = false;
-      mstore<B>(ai, cur, st);
-    }
+    if (!minv<B>(d, ai)) ok = false;
+    mstore<B>(ai, cur, st);
     {  // beta_p = -(beta_{p-1} U_{p-1}) ainv_p, or
        // (L_{n-1} - beta_{n-3} U_{n-3}) ainv_{n-2}
       Mt<B> Up, bu, bnew;
-      mload<B>(Up, prv + 2 * B * B * st, st);
+      mload<B>(Up, prv + OU * st, st);
       mmul<B>(bet, Up, bu);
       if (p < n - 2) {
         mmul<B>(bu, ai, bnew);
@@ -487,7 +497,7 @@ __global__ void __launch_bounds__(64)
           for (int q = 0; q < B; ++q) bnew.a[r][q] = -bnew.a[r][q];
       } else {
         Mt<B> Ln, x;
-        mload<B>(Ln, lvl(n - 1) + B * B * st, st);
+        as.next_b(Ln);
         msub<B>(Ln, bu, x);
         mmul<B>(x, ai, bnew);
       }
@@ -496,9 +506,9 @@ __global__ void __launch_bounds__(64)
     {  // sums of beta_p gamma_p and beta_p zeta_p
       Mt<B> g, bg;
       Vt<B> z, bz;
-      mload<B>(g, cur + B * B * st, st);
+      mload<B>(g, cur + OG * st, st);
 #pragma unroll
-      for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st];
+      for (int r = 0; r < B; ++r) z.v[r] = cur[(OZ + r) * st];
       mmul<B>(bet, g, bg);
       mvec<B>(bet, z, bz);
 #pragma unroll
@@ -512,19 +522,18 @@ __global__ void __launch_bounds__(64)
   // last level
   Vt<B> xl;
   {
-    double* cur = lvl(n - 1);
-    Mt<B> d, ai;
-    mload<B>(d, cur, st);
+    Mt<B> d, b, c, ai;
+    double rhs[B];
+    as.level(lo + n - 1, d, b, c, rhs, true);
     msub<B>(d, sbg, d);
     if (!minv<B>(d, ai)) ok = false;
     Vt<B> z;
 #pragma unroll
-    for (int r = 0; r < B; ++r) z.v[r] = cur[(3 * B * B + r) * st] - sbz.v[r];
+    for (int r = 0; r < B; ++r) z.v[r] = rhs[r] - sbz.v[r];
     mvec<B>(ai, z, xl);
   }
   if (!ok) atomicOr(status, 1);
-  const long long o = L.org(t);
-  line_write_back<NM, B, DIR, true>(xl, w, du, o + (long long)(L.lo + n - 1) * L.sl, L.sn);
+  line_write_back<NM, B, DIR, true>(xl, w, du, o + (long long)(lo + n - 1) * L.sl, L.sn);
   Vt<B> xn = xl;
   Mt<B> zero;
 #pragma unroll
@@ -537,18 +546,17 @@ __global__ void __launch_bounds__(64)
     if (p == n - 2)
       U = zero;
     else
-      mload<B>(U, cur + 2 * B * B * st, st);
+      mload<B>(U, cur + OU * st, st);
     mload<B>(ai, cur, st);
-    mload<B>(g, cur + B * B * st, st);
+    mload<B>(g, cur + OG * st, st);
     Vt<B> ux, gx, tv, x;
     mvec<B>(U, xn, ux);
     mvec<B>(g, xl, gx);
 #pragma unroll
-    for (int r = 0; r < B; ++r)
-      tv.v[r] = (cur[(3 * B * B + r) * st] - ux.v[r]) - gx.v[r];
+    for (int r = 0; r < B; ++r) tv.v[r] = (cur[(OZ + r) * st] - ux.v[r]) - gx.v[r];
     mvec<B>(ai, tv, x);
     xn = x;
-    line_write_back<NM, B, DIR, true>(x, w, du, o + (long long)(L.lo + p) * L.sl, L.sn);
+    line_write_back<NM, B, DIR, true>(x, w, du, o + (long long)(lo + p) * L.sl, L.sn);
   }
 }
 

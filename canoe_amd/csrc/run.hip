@@ -568,7 +568,9 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
     for (const RunBlock& rb : r->blocks)
       for (int d = 1; d <= 2; ++d)
         if ((cfg->implicit_flag & (1 << d)) && (d == 1 ? rb.g.f2 : rb.g.f3))
-          n = std::max(n, line_doubles(rb.g, d, B));
+          n = std::max(n, line_doubles(rb.g, d, B,
+                                       rb.blk.bc[2 * d] == CG_BC_PERIODIC &&
+                                           rb.blk.bc[2 * d + 1] == CG_BC_PERIODIC));
     ok = hipMalloc((void**)&r->lines, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess;
   }
   if (ok && (cfg->implicit_flag & 1)) {
