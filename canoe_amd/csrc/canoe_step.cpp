@@ -63,7 +63,7 @@ void usage() {
                "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
                "[--nlim N] [--tlim T] [--cfl C] [--fp32] "
                "[--no-graph] [--seed S] [--out K] [--hst] [--dbg] "
-               "[--basename NAME]\n");
+               "[--basename NAME] [--implicit-flag F]\n");
 }
 
 int check(cg_ctx* ctx, int rc, const char* what) {
@@ -85,6 +85,7 @@ int main(int argc, char** argv) {
   long long nlim = 10, out_every = 1;
   double tlim = 1e300, cfl = 0.9;
   bool fp32 = false, graph = true, hst = false, dbg = false;
+  int iflag = -1;  // <hydro> implicit_flag (implicit_solver.cpp:47)
   std::string base = "canoe";
   uint64_t seed = 1;
   for (int a = 2; a < argc; ++a) {
@@ -124,6 +125,9 @@ int main(int argc, char** argv) {
       fp32 = true;
     } else if (s == "--no-graph") {
       graph = false;
+    } else if (s == "--implicit-flag") {
+      need(1);
+      iflag = std::atoi(argv[++a]);
     } else {
       usage();
       return 2;
@@ -137,6 +141,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   if (fp32) cfg.precision = CG_PREC_FP32;
+  if (iflag >= 0) cfg.implicit_flag = iflag;
   Dims d(blk);
   std::vector<double> w((size_t)d.N * d.sn);
   if (cg_pgen_init(config, seed, &blk, &th, &cfg, w.data()) != CG_OK) return 1;

@@ -15,7 +15,9 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 
 @pytest.mark.parametrize("args", [["2", "--nx", "32", "16", "16"],
                                   ["1", "--nx", "16", "64", "1"],
-                                  ["2", "--nx", "32", "16", "16", "--fp32"]])
+                                  ["2", "--nx", "32", "16", "16", "--fp32"],
+                                  ["2", "--nx", "32", "16", "16",
+                                   "--implicit-flag", "11"]])
 def test_cpp_driver(args):
     print("canoe_step", " ".join(args), flush=True)
     r = subprocess.run([EXE] + args + ["--nlim", "6", "--out", "2"],
