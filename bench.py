@@ -253,6 +253,34 @@ def traffic_from_profiles(op, tune):
         return None
 
 
+def fp64_from_profiles(avg_ms, cells):
+    """FP64 rate of the column solve: the FP64 lane operations per cell of
+    each column kernel, counted with rocprofv3 (SQ_INSTS_VALU_*_F64, a
+    committed summary under profiles/), over the kernels' live event-timed
+    durations."""
+    p = os.path.join(ROOT, "profiles", "fp64_ops_box256.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        per = json.load(open(p))["flop_per_cell"]
+    except Exception:
+        return None
+    names = {"k_implicit_fused": "k_implicit_fused_batch",
+             "k_implicit_fwd": "k_implicit_fwd_batch",
+             "k_implicit_bwd": "k_implicit_bwd_update_batch"}
+    flop = ms = 0.0
+    for k, key in names.items():
+        if k in avg_ms and key in per:
+            flop += per[key] * cells
+            ms += avg_ms[k]
+    if ms <= 0:
+        return None
+    tf = flop / (ms * 1e-3) / 1e12
+    return {"flop_per_cell": round(flop / cells, 1), "achieved": round(tf, 2),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
+
+
 def cpu_baseline(seconds, threads):
     """Oracle step on a bounded sample of the workload: 128 levels x 32 x 32
     columns (same physics, same column height)."""
@@ -639,6 +667,15 @@ def main():
           "achieved": op["achieved_GBs"], "peak": HBM_PEAK_GBS,
           "unit": "GB/s", "frac": op["frac"],
           "traffic": traffic_from_profiles(dom, chosen["tune"])}
+    # the FP64 side of the column solve, and the whole step against the
+    # step's algorithmic bytes B_step = 88 N per cell-update (3 stages x
+    # 24 N for U_s, U_n, U_{s+1} + 16 N for the adjustment pass)
+    if dom == "implicit_solve":
+        rf["fp64"] = fp64_from_profiles(chosen["kernels_avg_ms"], cells)
+    bstep = 88 * blk.N
+    sgb = bstep * chosen["cell_updates_per_s"] / 1e9
+    rf["step"] = {"bytes_per_cell_update": bstep, "achieved": round(sgb, 1),
+                  "unit": "GB/s", "frac": round(sgb / HBM_PEAK_GBS, 4)}
 
     cpu = None
     if not args.no_cpu:
