@@ -188,8 +188,11 @@ void hllc(const cg_thermo& th, int ivx, int il, int iu, const double* wl,
     double rhoa = .5 * (wli[IDN] + wri[IDN]);
     double ca = .5 * (cl + cr);
     double pmid = .5 * (wli[IPR] + wri[IPR] + (wli[IVX] - wri[IVX]) * rhoa * ca);
+    // the PVRS velocity (:121-124) feeds only the GENERAL_EOS branch of the
+    // shock factors (:128-136); the ideal-gas branch below does not use it
     double umid =
         .5 * (wli[IVX] + wri[IVX] + (wli[IPR] - wri[IPR]) / (rhoa * ca));
+    (void)umid;
 
     double ql = (pmid <= wli[IPR])
                     ? 1.0
