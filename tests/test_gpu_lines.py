@@ -149,3 +149,19 @@ def test_run_rejects_periodic_x1_columns(ctx):
     _set_bc(blk, bc)
     with pytest.raises(ca.CanoeGpuError):
         ca.Run(ctx, blk, th, cfg)
+
+
+def test_batch_with_lines_matches_oracle(ctx):
+    """Every block of an uneven batch (partial 64-column groups, one shared
+    line scratch) with x3, x2 and x1 solves equals its own oracle run."""
+    from test_gpu_batch import _blocks, _oracle, _run_batch
+    bl = _blocks(seed=21)
+    for b in bl:
+        b[2].implicit_flag = 15
+    dt = min(orl.new_block_dt(b[0], b[1], b[2], b[3], 0.5) for b in bl)
+    _say("batch lines dt=%.4f" % dt)
+    ug = _run_batch(ctx, bl, dt, 2)
+    uo = _oracle(bl, dt, 2)
+    for i, (blk, th, cfg, w) in enumerate(bl):
+        sk, sj, si = blk.interior()
+        np.testing.assert_array_equal(ug[i][:, sk, sj, si], uo[i][:, sk, sj, si])
