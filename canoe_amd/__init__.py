@@ -142,6 +142,7 @@ _SIGS = {
     "cg_run_get_cons_host": (_I, [_P, _P]),
     "cg_run_get_prim_host": (_I, [_P, _P]),
     "cg_run_step": (_I, [_P, _D, _I, _I]),
+    "cg_run_stage": (_I, [_P, _I, _D, _P]),
     "cg_run_new_dt": (_I, [_P, _D, ctypes.POINTER(_D)]),
     "cg_run_check": (_I, [_P, _P]),
     "cg_run_advance": (_I, [_P, _D, _D, ctypes.c_longlong, _I,
@@ -291,6 +292,11 @@ class Context:
             raise CanoeGpuError(r, "%s: %s" % (what, msg.decode()
                                                if msg else ""))
 
+    def synchronize(self):
+        """Wait for the context's stream; reports kernel status (zero
+        pivots, launch errors) of asynchronous calls."""
+        self._check(lib().cg_synchronize(self.h), "cg_synchronize")
+
     def array(self, a=None, shape=None, dtype=np.float64):
         if a is not None:
             a = np.ascontiguousarray(a, dtype=dtype)
@@ -432,6 +438,12 @@ class Run:
     def step(self, dt, nsteps=1, graph=True):
         self.ctx._check(lib().cg_run_step(self.h, dt, nsteps, int(graph)),
                         "cg_run_step")
+
+    def stage(self, s, dt=0.0, dt_dev=None):
+        """One SSPRK(3,3) stage s (0, 1, 2), asynchronous; the step is dt or
+        the double at device pointer dt_dev."""
+        self.ctx._check(lib().cg_run_stage(self.h, s, dt, dt_dev),
+                        "cg_run_stage")
 
     def new_dt(self, cfl):
         """Minimum over the run's blocks of NewBlockTimeStep, reduced on the

@@ -713,6 +713,28 @@ int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
   return check_status(ctx);
 }
 
+static __global__ void k_copy_scalar(double* __restrict__ out,
+                                     const double* __restrict__ in) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = in[0];
+}
+
+int cg_run_stage(cg_run* r, int stage, double dt, const double* dt_dev) {
+  if (!r || stage < 0 || stage > 2) return CG_ERR_ARG;
+  if (!dt_dev && !(dt > 0.0)) return set_err(r->ctx, CG_ERR_ARG, "dt must be positive");
+  cg_ctx* ctx = r->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int e = ensure_tiles(r);
+  if (e) return e;
+  if (dt_dev)
+    k_copy_scalar<<<1, 64, 0, ctx->stream>>>(r->d_dt, dt_dev);
+  else if ((e = set_step(r, dt)))
+    return e;
+  e = (r->N == 5) ? run_stage<0>(r, stage) : run_stage<3>(r, stage);
+  if (e) return e;
+  HIPCHK(ctx, hipGetLastError());
+  return CG_OK;
+}
+
 int cg_run_new_dt(cg_run* r, double cfl, double* dt_out) {
   if (!r || !dt_out || !(cfl > 0.0)) return CG_ERR_ARG;
   HIPCHK(r->ctx, hipSetDevice(r->ctx->device));

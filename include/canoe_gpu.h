@@ -260,6 +260,18 @@ int cg_run_get_prim_host(cg_run* run, double* w_host);
  * stage kernels read the step from device memory, so one captured graph
  * serves every dt. */
 int cg_run_step(cg_run* run, double dt, int nsteps, int use_graph);
+/* One SSPRK(3,3) stage (0, 1 or 2) of the run's blocks: the per-stage tasks
+ * of ImplicitHydroTasks (src/tasklist/implicit_hydro_tasks.cpp:134-313) --
+ * fluxes, AddFluxDivergence with weight beta_s dt and the sources, the
+ * implicit x3 / x2 / x1 solves, WeightedAve and the update, the saturation
+ * adjustment on stage 2, ghost cells.  The step is read from device memory
+ * at dt_dev (one double) when dt_dev is not null, else it is dt.  Stages go
+ * in order 0, 1, 2 per step.  Asynchronous on the run's stream: kernel
+ * status (zero pivots) is reported by the next cg_synchronize or
+ * cg_run_step.  After the first call on a run (which builds its launch
+ * tables) the call launches kernels only and may be captured into a
+ * caller's hipGraph. */
+int cg_run_stage(cg_run* run, int stage, double dt, const double* dt_dev);
 /* The minimum of cg_new_block_dt over the run's blocks, reduced on the
  * device from the resident state (Mesh::NewTimeStep's min over blocks). */
 int cg_run_new_dt(cg_run* run, double cfl, double* dt_out);

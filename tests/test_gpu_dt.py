@@ -160,3 +160,37 @@ def test_one_graph_serves_every_dt(ctx):
         finally:
             run.close()
     np.testing.assert_array_equal(out[1], out[0])
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2]])
+def test_run_stage_by_stage_equals_steps(ctx, case):
+    """cg_run_stage driven stage by stage, with the step given by value or
+    read from device memory, equals cg_run_step bit for bit."""
+    config, dims = case
+    blk, th, cfg, w = perturbed_state(config, dims, vel=1.0, dp=1e-3)
+    cfg.implicit_flag = 15 if config == 2 else 9
+    dt = orl.new_block_dt(blk, th, cfg, w, 0.5)
+    out = []
+    ddt = ctx.array(np.array([dt]))
+    try:
+        for mode in ("step", "value", "device"):
+            run = ca.Run(ctx, blk, th, cfg)
+            try:
+                run.set_prim(w)
+                if mode == "step":
+                    run.step(dt, 2, graph=False)
+                else:
+                    for _ in range(2):
+                        for s in range(3):
+                            if mode == "value":
+                                run.stage(s, dt)
+                            else:
+                                run.stage(s, dt_dev=ddt.ptr)
+                    ctx.synchronize()
+                out.append(run.cons())
+            finally:
+                run.close()
+    finally:
+        ddt.free()
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
