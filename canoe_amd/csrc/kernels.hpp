@@ -726,15 +726,22 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
         }
       }
       if (best == 0.0) ok = false;
-      if (p != cc) {
+      // the row exchange runs only when a lane of the wavefront pivots (a
+      // uniform branch: columns 2..4, and every column at the small step
+      // of the second stage, keep their diagonal pivot); columns left of cc
+      // are not read again
+      if (__any(p != cc)) {
 #pragma unroll
         for (int r = cc + 1; r < B; ++r) {
           bool sw = (p == r);
 #pragma unroll
-          for (int q = 0; q < B; ++q) {
+          for (int q = cc; q < B; ++q) {
             double x0 = M.a[cc][q], x1 = M.a[r][q];
             M.a[cc][q] = sw ? x1 : x0;
             M.a[r][q] = sw ? x0 : x1;
+          }
+#pragma unroll
+          for (int q = 0; q < B; ++q) {
             double y0 = c.a[cc][q], y1 = c.a[r][q];
             c.a[cc][q] = sw ? y1 : y0;
             c.a[r][q] = sw ? y0 : y1;
