@@ -3,7 +3,7 @@
 boundaries are periodic) against the oracle on the same seeded inputs.
 
 Both evaluate the same expressions in FP64 without contraction, so the bar
-is the north-star 1e-12 relative per cell (in practice equality).
+is 1e-12 relative per cell (in practice equality).
 """
 import numpy as np
 import pytest
