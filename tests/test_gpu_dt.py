@@ -194,3 +194,15 @@ def test_run_stage_by_stage_equals_steps(ctx, case):
         ddt.free()
     np.testing.assert_array_equal(out[1], out[0])
     np.testing.assert_array_equal(out[2], out[0])
+
+
+def test_run_stage_rejects_bad_arguments(ctx):
+    blk, th, cfg, w = perturbed_state(2, (20, 12, 10), vel=1.0, dp=1e-3)
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        for s, dt in ((3, 0.1), (-1, 0.1), (0, 0.0), (0, -1.0)):
+            with pytest.raises(ca.CanoeGpuError):
+                run.stage(s, dt)
+    finally:
+        run.close()

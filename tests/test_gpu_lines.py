@@ -165,3 +165,15 @@ def test_batch_with_lines_matches_oracle(ctx):
     for i, (blk, th, cfg, w) in enumerate(bl):
         sk, sj, si = blk.interior()
         np.testing.assert_array_equal(ug[i][:, sk, sj, si], uo[i][:, sk, sj, si])
+
+
+def test_implicit_flag_out_of_range_rejected(ctx):
+    blk, th, w, du = _case(ca.BC_PERIODIC, ca.BC_PERIODIC)
+    for flag in (16, -1):
+        with pytest.raises(ca.CanoeGpuError):
+            ca.ImplicitSolver(ctx, blk, th, flag).solve_implicit_3d(du, w, 0.5)
+    # only directions without cells: nothing to do, du unchanged
+    blk2, th2, cfg2, w2 = perturbed_state(1, (12, 20, 1), vel=20.0)
+    du2 = np.random.default_rng(1).normal(size=blk2.shape())
+    got = ca.ImplicitSolver(ctx, blk2, th2, 12).solve_implicit_3d(du2, w2, 0.5)
+    np.testing.assert_array_equal(got, du2)
