@@ -26,7 +26,7 @@ MAX_NMASS = 3
 NUM_KERNEL_IDS = 18
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
-    ERR_SIZE, ERR_NODEV, ERR_COLD = range(10)
+    ERR_SIZE, ERR_NODEV, ERR_COLD, ERR_IO = range(11)
 BC_REFLECT, BC_PERIODIC, BC_OUTFLOW = 0, 1, 2
 RS_LMARS, RS_HLLC = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
@@ -170,6 +170,7 @@ _SIGS = {
     "cg_apply_bc_host": (_I, [_P, _P]),
     "cg_pgen_setup": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "cg_pgen_init": (_I, [_I, ctypes.c_uint64, _P, _P, _P, _P]),
+    "cg_netcdf_write": (_I, [ctypes.c_char_p, _P, _D, _I, _P, _P, _P, _P, _I]),
 }
 
 
@@ -233,6 +234,26 @@ def batch_blocks(seed, nblocks=32, heights=(40, 64, 96, 128, 200, 256),
         w = pgen_init(5, int(seed) * 1000 + b, blk, th, cfg)
         out.append((blk, th, cfg, w))
     return out
+
+
+def netcdf_write(path, blk, time, fields, append=False):
+    """NetCDF dump of a block in the layout of the reference's netcdf output
+    (cg_netcdf_write).  fields: list of (name, units, long_name, array) with
+    arrays (k, j, i) of the block including ghost cells."""
+    n = len(fields)
+    arrs = [np.ascontiguousarray(f[3], dtype=np.float64) for f in fields]
+    for a in arrs:
+        assert a.shape == blk.shape()[1:], (a.shape, blk.shape())
+    enc = [[f[i].encode() for f in fields] for i in range(3)]
+    cs = [(ctypes.c_char_p * max(n, 1))(*e) for e in enc]
+    ps = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in arrs])
+    r = lib().cg_netcdf_write(path.encode(), _ref(blk), float(time), n,
+                              ctypes.cast(cs[0], ctypes.c_void_p),
+                              ctypes.cast(cs[1], ctypes.c_void_p),
+                              ctypes.cast(cs[2], ctypes.c_void_p),
+                              ctypes.cast(ps, ctypes.c_void_p), int(append))
+    if r:
+        raise CanoeGpuError(r, "cg_netcdf_write(%s)" % path)
 
 
 def apply_bc_host(blk, w):

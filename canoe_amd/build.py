@@ -32,7 +32,7 @@ DEVICE = [
 ]
 
 SOURCES_HIP = ["capi.hip", "run.hip", "mlp.hip"]
-SOURCES_CPP = ["pgen.cpp"]
+SOURCES_CPP = ["pgen.cpp", "netcdf.cpp"]
 
 
 def run(cmd):

@@ -10,7 +10,7 @@
 //
 //   canoe_step <config> [--nx nx1 nx2 nx3] [--nlim N] [--tlim T] [--cfl C]
 //              [--fp32] [--no-graph] [--seed S] [--out K] [--hst] [--dbg]
-//              [--basename NAME]
+//              [--nc] [--basename NAME] [--implicit-flag F]
 //
 // The cycles run on the device through cg_run_advance: each step is
 // followed by the patched Hydro::NewBlockTimeStep reduced over the resident
@@ -19,7 +19,10 @@
 // K cycles (--out).  At the same points --hst appends Athena++'s history
 // columns to NAME.hst and --dbg the per-variable extremes of DebugOutput
 // (src/outputs/debug.cpp:137-182) to NAME.dbg, in the reference's text
-// layouts with its %15.12e data format.
+// layouts with its %15.12e data format, and --nc one record of the
+// primitive variables to NAME.prim.nc and of temperature and potential
+// temperature (the user output variables of straka.cpp:88-115) to
+// NAME.uov.nc, in the layout of the reference's netcdf output.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -63,7 +66,7 @@ void usage() {
                "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
                "[--nlim N] [--tlim T] [--cfl C] [--fp32] "
                "[--no-graph] [--seed S] [--out K] [--hst] [--dbg] "
-               "[--basename NAME] [--implicit-flag F]\n");
+               "[--nc] [--basename NAME] [--implicit-flag F]\n");
 }
 
 int check(cg_ctx* ctx, int rc, const char* what) {
@@ -84,7 +87,7 @@ int main(int argc, char** argv) {
   int nx[3] = {0, 0, 0};
   long long nlim = 10, out_every = 1;
   double tlim = 1e300, cfl = 0.9;
-  bool fp32 = false, graph = true, hst = false, dbg = false;
+  bool fp32 = false, graph = true, hst = false, dbg = false, nc = false;
   int iflag = -1;  // <hydro> implicit_flag (implicit_solver.cpp:47)
   std::string base = "canoe";
   uint64_t seed = 1;
@@ -118,6 +121,8 @@ int main(int argc, char** argv) {
       hst = true;
     } else if (s == "--dbg") {
       dbg = true;
+    } else if (s == "--nc") {
+      nc = true;
     } else if (s == "--basename") {
       need(1);
       base = argv[++a];
@@ -195,7 +200,69 @@ int main(int argc, char** argv) {
   }
   std::vector<double> hv(8 + blk.nmass), vmin(d.N), vmax(d.N), xmin(3 * d.N),
       xmax(3 * d.N);
+  // NetCDF dumps: prim (rho, press, vel1..3, species) and uov (temp, theta)
+  std::vector<double> wp, tt, th_;
+  int nc_records = 0;
+  auto nc_dump = [&]() -> int {
+    wp.resize((size_t)d.N * d.sn);
+    tt.assign(d.sn, 0.0);
+    th_.assign(d.sn, 0.0);
+    if (check(ctx, cg_run_get_prim_host(run, wp.data()), "get_prim")) return 1;
+    const int IVX = 1 + blk.nmass, IPR = IVX + 3;
+    const double chi = (th.gammad - 1.) / th.gammad;
+    for (int k = d.ks; k < d.ks + blk.nx3; ++k)
+      for (int j = d.js; j < d.js + blk.nx2; ++j)
+        for (int i = d.is; i < d.is + blk.nx1; ++i) {
+          // T = p / (rho Rd feps), feps = 1 + sum_vap q inv_mu_ratio_m1
+          // - sum_cld q; theta = T (p0 / p)^((gamma_d - 1) / gamma_d)
+          double feps = 1.;
+          for (int n = 1; n <= blk.nmass; ++n) {
+            double q = wp[d.at(n, k, j, i)];
+            feps += (n <= th.nvapor) ? q * th.inv_mu_ratio_m1[n - 1] : -q;
+          }
+          const double p = wp[d.at(IPR, k, j, i)], rho = wp[d.at(0, k, j, i)];
+          const double T = p / (rho * th.rd * feps);
+          const size_t c = d.at(0, k, j, i);
+          tt[c] = T;
+          th_[c] = T * std::pow(cfg.p0 / p, chi);
+        }
+    std::vector<std::string> nm = {"rho", "press", "vel1", "vel2", "vel3"};
+    std::vector<std::string> un = {"kg/m^3", "pa", "m/s", "m/s", "m/s"};
+    std::vector<std::string> ln = {"density", "pressure", "vertical velocity",
+                                   "horizontal velocity", "horizontal velocity"};
+    std::vector<const double*> fp = {&wp[d.at(0, 0, 0, 0)], &wp[d.at(IPR, 0, 0, 0)],
+                                     &wp[d.at(IVX, 0, 0, 0)], &wp[d.at(IVX + 1, 0, 0, 0)],
+                                     &wp[d.at(IVX + 2, 0, 0, 0)]};
+    for (int n = 1; n <= blk.nmass; ++n) {
+      nm.push_back("q" + std::to_string(n));
+      un.push_back("kg/kg");
+      ln.push_back("mass mixing ratio of species " + std::to_string(n));
+      fp.push_back(&wp[d.at(n, 0, 0, 0)]);
+    }
+    std::vector<const char*> a, b2, c2;
+    for (size_t q = 0; q < nm.size(); ++q) {
+      a.push_back(nm[q].c_str());
+      b2.push_back(un[q].c_str());
+      c2.push_back(ln[q].c_str());
+    }
+    const char* un_names[2] = {"temp", "theta"};
+    const char* un_units[2] = {"K", "K"};
+    const char* un_long[2] = {"temperature", "potential temperature"};
+    const double* un_f[2] = {tt.data(), th_.data()};
+    int e = cg_netcdf_write((base + ".prim.nc").c_str(), &blk, time, (int)nm.size(),
+                            a.data(), b2.data(), c2.data(), fp.data(), nc_records > 0);
+    if (!e)
+      e = cg_netcdf_write((base + ".uov.nc").c_str(), &blk, time, 2, un_names, un_units,
+                          un_long, un_f, nc_records > 0);
+    if (e) {
+      std::fprintf(stderr, "canoe_step: netcdf output failed (%d)\n", e);
+      return e;
+    }
+    ++nc_records;
+    return 0;
+  };
   auto outputs = [&]() {
+    if (nc && nc_dump()) return;
     if (fh && !check(ctx, cg_run_history(run, hv.data()), "cg_run_history")) {
       std::fprintf(fh, fmt, time);
       std::fprintf(fh, fmt, dt);

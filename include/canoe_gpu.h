@@ -41,7 +41,8 @@ enum cg_status {
   CG_ERR_CONVERGE = 6,  /* saturation-adjustment Newton did not converge */
   CG_ERR_SIZE = 7,      /* size limit exceeded */
   CG_ERR_NODEV = 8,     /* no HIP device / library built without one */
-  CG_ERR_COLD = 9       /* temperature below |g| dx1 / Rd */
+  CG_ERR_COLD = 9,      /* temperature below |g| dx1 / Rd */
+  CG_ERR_IO = 10        /* a file could not be opened, read or written */
 };
 
 enum cg_bc { CG_BC_REFLECT = 0, CG_BC_PERIODIC = 1, CG_BC_OUTFLOW = 2 };
@@ -309,6 +310,20 @@ int cg_run_history(cg_run* run, double* out_host);
  * FLT_MIN and strict comparisons in block, k, j, i order. */
 int cg_run_debug(cg_run* run, double* vmin, double* vmax, double* xmin,
                  double* xmax);
+/* Replaces the netcdf output (NetCDFOutput::WriteOutputFile,
+ * src/outputs/netcdf.cpp:100-420) for one block, written in NetCDF's classic
+ * 64-bit-offset format: dimensions time (record), x1, x1f, x2, x2f, x3, x3f
+ * (a face dimension only for a direction with more than one cell),
+ * coordinate variables with axis / units / long_name / domain_decomposition,
+ * and one float variable (time, x1, x2, x3) per field with its units and
+ * long_name.  fields[f] is a host array (k, j, i) of the block with ghost
+ * cells; its interior is written.  append = 0 creates the file with one
+ * record; append != 0 adds a record to a file this function created with
+ * the same block and fields.  Host code only: no device is used. */
+int cg_netcdf_write(const char* path, const cg_block* blk, double time, int nfield,
+                    const char* const* names, const char* const* units,
+                    const char* const* long_names, const double* const* fields,
+                    int append);
 /* Per-kernel timing: with timing on, cg_run_step launches without the graph
  * and records HIP events around every kernel on the run's stream.
  * cg_run_kernel_stats returns the accumulated milliseconds, the number of

@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -63,3 +64,33 @@ def test_cpp_driver_history_and_debug_files(tmp_path):
     dbg = open(base + ".dbg").read().splitlines()
     assert dbg[0] == "# Athena++ debug data"
     assert len(dbg) == 2 + 4 * 5 and " --   4  -- " in dbg[6]
+
+
+def test_cpp_driver_netcdf_dumps(tmp_path):
+    """--nc writes the primitive variables and (temp, theta) in the layout of
+    the reference's netcdf output; the Straka run to 900 s read back the way
+    examples/2019-Li-snap/test_straka.py reads it (theta[-1, :, :, 0])."""
+    from scipy.io import netcdf_file
+    base = str(tmp_path / "straka")
+    r = subprocess.run([EXE, "1", "--tlim", "900", "--cfl", "0.45", "--nlim",
+                        "100000", "--out", "2500", "--nc", "--basename", base],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with netcdf_file(base + ".uov.nc", "r", mmap=False) as nc:
+        t = nc.variables["time"][:]
+        # cycles 0, 2500, 5000 and the last (t = 900 s)
+        assert t[0] == 0.0 and t[-1] == 900.0 and len(t) == 4
+        theta = nc.variables["theta"][-1, :, :, 0].astype(np.float64)
+        assert theta.shape == (64, 512)
+        tp = theta - 300.0
+        assert -10.0 <= tp.min() <= -8.3 and tp.max() < 0.5
+        # the initial record: the bubble is -15 K of temperature at 3 km,
+        # i.e. theta' = -15 (p0 / p)^(R/cp) = -15 x 300 / 270.7 = -16.6 K
+        th0 = nc.variables["theta"][0, :, :, 0].astype(np.float64)
+        assert -16.8 <= th0.min() - 300.0 <= -16.4
+    with netcdf_file(base + ".prim.nc", "r", mmap=False) as nc:
+        assert [v for v in ("rho", "press", "vel1", "vel2", "vel3")
+                if v not in nc.variables] == []
+        rho = nc.variables["rho"][-1, :, :, 0]
+        assert rho.shape == (64, 512) and np.all(rho > 0)
+        assert nc.variables["x2"][:].shape == (512,)
