@@ -1,0 +1,110 @@
+"""Properties of the full-size workloads (the 256 x 256 x 128 boxes), where
+the oracle is too slow to compare cell by cell.
+
+* Translation invariance: the box is periodic in x2 and x3 and its forcing
+  (bottom heating, gravity) is horizontally uniform, so stepping a state
+  shifted by whole cells gives the shifted result of stepping the state.
+  Every cell runs the same arithmetic wherever it sits in a face tile, a
+  64-column group or an XCD's share, so the equality is bit for bit; a
+  wrong tile, group or halo index breaks it.
+* Conservation: mass to round-off with reflecting walls and periodic sides;
+  the moist box keeps its total water through transport and the saturation
+  adjustment (the implicit correction's species split does not: by the
+  reference's design).
+"""
+import numpy as np
+import pytest
+
+import canoe_amd as ca
+from helpers import cfl_dt
+
+pytestmark = pytest.mark.gpu
+
+SHIFT = (5, 17)  # cells along x3, x2
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _roll(blk, a):
+    sk, sj, si = blk.interior()
+    out = a.copy()
+    inner = a[:, sk, sj, si]
+    out[:, sk, sj, si] = np.roll(inner, SHIFT, axis=(1, 2))
+    return out
+
+
+@pytest.mark.parametrize("prec", [ca.PREC_FP64, ca.PREC_FP32])
+def test_full_box_translation_invariance(ctx, prec):
+    blk, th, cfg = ca.pgen_setup(2)
+    cfg.precision = prec
+    w0 = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0, 0.9)
+    w1 = ca.apply_bc_host(blk, _roll(blk, w0))
+    out = []
+    for w in (w0, w1):
+        _say("full box prec=%d" % prec)
+        run = ca.Run(ctx, blk, th, cfg)
+        try:
+            run.set_prim(w)
+            run.step(dt, 3, graph=True)
+            out.append(run.cons())
+        finally:
+            run.close()
+    sk, sj, si = blk.interior()
+    a = _roll(blk, out[0])[:, sk, sj, si]
+    b = out[1][:, sk, sj, si]
+    assert np.all(np.isfinite(b))
+    np.testing.assert_array_equal(a, b)
+
+
+def test_full_box_mass_conservation(ctx):
+    blk, th, cfg = ca.pgen_setup(2)
+    cfg.precision = ca.PREC_FP32
+    w0 = ca.pgen_init(2, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0, 0.9)
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w0)
+        s0 = run.sums()
+        run.step(dt, 10, graph=True)
+        s1 = run.sums()
+    finally:
+        run.close()
+    _say("full box mass %.17e -> %.17e" % (s0[0], s1[0]))
+    assert abs(s1[0] / s0[0] - 1) < 1e-13
+
+
+@pytest.mark.parametrize("flag", [9, 0])
+def test_full_moist_box_mass_and_water(ctx, flag):
+    """Total mass (dry air + species) is conserved with or without the
+    implicit solve.  Total water is conserved by transport and the
+    saturation adjustment, but not by the implicit correction: its
+    write-back splits each cell's total-density increment over the species
+    by their mass fractions (forward_backward.hpp:101-124), which the
+    oracle reproduces (a 128 x 16 x 16 moist box loses 3.4e-6 of its water
+    in 5 steps with flag 9, 3e-16 without)."""
+    blk, th, cfg = ca.pgen_setup(4)
+    cfg.precision = ca.PREC_FP32
+    cfg.implicit_flag = flag
+    w0 = ca.pgen_init(4, 1, blk, th, cfg)
+    dt = cfl_dt(blk, th, cfg, w0, 0.9 if flag else 0.5)
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w0)
+        s0 = run.sums()
+        run.step(dt, 5, graph=True)
+        s1 = run.sums()
+        w = run.prim()
+    finally:
+        run.close()
+    m0, m1 = s0[:4].sum(), s1[:4].sum()
+    water0, water1 = s0[1:4].sum(), s1[1:4].sum()
+    _say("moist box flag %d: mass %.3e, water %.3e"
+         % (flag, m1 / m0 - 1, water1 / water0 - 1))
+    assert abs(m1 / m0 - 1) < 1e-13
+    if flag == 0:
+        assert abs(water1 / water0 - 1) < 1e-12
+    sk, sj, si = blk.interior()
+    assert np.all(np.isfinite(w[:, sk, sj, si]))
