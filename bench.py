@@ -527,6 +527,45 @@ def bench_lines(ctx, nx, steps, warmup):
             "kernels_avg_ms": {k: round(v[0], 4) for k, v in stats.items()}}
 
 
+def mlp_run(nx, train_steps):
+    """A short, fixed run of the saturation-guess network for rocprofv3:
+    pairs from the moist box's initial state (2^20), train_steps Adam steps
+    of batch 8192, then 3 forwards of 2^23 samples; prints the measured
+    forward rate against the bf16 MFMA peak."""
+    ctx = ca.Context(0)
+    blk, th, cfg = ca.pgen_setup(4, *nx)
+    w0 = ca.pgen_init(4, 1, blk, th, cfg)
+    rho, ie, q = moist_states(blk, th, w0)
+    dx, dtt, _ = cm.make_pairs(ctx, th, rho, ie, q, 1 << 20, seed=5,
+                               device=True)
+    net = cm.Mlp(ctx, seed=7)
+    t0 = time.perf_counter()
+    net.train(dx, dtt, steps=train_steps, batch=8192, lr=3e-3, lr_final=1e-5,
+              seed=9)
+    t1 = time.perf_counter()
+    nbig = 1 << 23
+    xb = ctx.array(shape=(nbig, 4), dtype=np.float32)
+    xb.upload(np.tile(dx.download()[:1 << 20], (nbig >> 20, 1)))
+    yb = ctx.array(shape=(nbig, 3), dtype=np.float32)
+    L = cm._lib()
+    reps = 3
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        ctx._check(L.cg_mlp_forward(net.h, nbig, xb.ptr, yb.ptr), "forward")
+    t3 = time.perf_counter()
+    tf = MLP_FLOP * nbig * reps / (t3 - t2) / 1e12
+    for a in (xb, yb, dx, dtt):
+        a.free()
+    net.close()
+    ctx.close()
+    print(json.dumps({"profile": "saturation-guess network 4-64-64-64-3",
+                      "train_steps": train_steps, "batch": 8192,
+                      "train_ms_per_step": round(1e3 * (t1 - t0) /
+                                                 max(train_steps, 1), 4),
+                      "forward_samples": nbig, "forward_tflops": round(tf, 2),
+                      "forward_mfma_frac": round(tf / BF16_PEAK_TFLOPS, 5)}))
+
+
 def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     bl = ca.batch_blocks(seed, nblocks)
     blocks = [b[0] for b in bl]
@@ -625,6 +664,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for rocprofv3 (copy calibration + "
                          "a few steps of the dry box, FP32 fluxes)")
+    ap.add_argument("--mlp", action="store_true",
+                    help="short fixed run of the network for rocprofv3 "
+                         "(pairs, a few training steps, 3 forwards of 8M)")
     ap.add_argument("--lines", action="store_true",
                     help="only the dry box with implicit x2/x3 lines "
                          "(implicit_flag 15)")
@@ -639,6 +681,9 @@ def main():
     USE_GRAPH = not args.no_graph
     if args.profile:
         profile_run(args.nx, max(1, min(args.steps, 5)))
+        return
+    if args.mlp:
+        mlp_run(args.nx, 200)
         return
     if args.lines:
         ctx = ca.Context(0)
