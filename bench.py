@@ -86,7 +86,8 @@ def make_run(ctx, blocks, th, cfg, w0s, net=None, tune=None):
 def time_form(ctx, blocks, th, cfg, w0s, dt, steps, warmup, net=None,
               tune=None, autotune=False):
     """ms per step (graph replay, or direct launches when graphs are off),
-    per-kernel event-timed averages, and the launch shape used."""
+    per-kernel event-timed averages, the launch shape used, and the median
+    of 20 single synchronised steps."""
     run = make_run(ctx, blocks, th, cfg, w0s, net, tune)
     try:
         if autotune:
@@ -99,6 +100,13 @@ def time_form(ctx, blocks, th, cfg, w0s, dt, steps, warmup, net=None,
         run.step(dt, steps, graph=USE_GRAPH)       # returns after stream sync
         t1 = time.perf_counter()
         ms = 1e3 * (t1 - t0) / steps
+        # the median of single synchronised steps (each a graph replay)
+        single = []
+        for _ in range(20):
+            s0 = time.perf_counter()
+            run.step(dt, 1, graph=USE_GRAPH)
+            single.append(1e3 * (time.perf_counter() - s0))
+        median = float(np.median(single))
         for b, w in enumerate(w0s):
             run.set_prim(w, b)
         run.set_timing(True)
@@ -107,7 +115,7 @@ def time_form(ctx, blocks, th, cfg, w0s, dt, steps, warmup, net=None,
         stats = {k: (v[0] / v[1], v[1] / nt)
                  for k, v in run.kernel_stats().items()}
         run.set_timing(False)
-        return ms, stats, t.as_dict()
+        return ms, stats, t.as_dict(), median
     finally:
         run.close()
 
@@ -316,10 +324,11 @@ def bench_box(ctx, nx, steps, warmup, acc_steps):
     tunes = {}
     for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
         cfg.precision = prec
-        ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
+        ms, stats, tune, med = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
                                     warmup, autotune=True)
         dom, ops = roofline(blk, cfg, stats)
         forms[name] = {"ms_per_step": round(ms, 4),
+                       "ms_per_step_median_of_20": round(med, 4),
                        "cell_updates_per_s": cells / (ms * 1e-3),
                        "dominant": dom, "ops": ops, "tune": tune,
                        "kernels_avg_ms": {k: round(v[0], 4)
@@ -464,10 +473,11 @@ def bench_moist(ctx, nx, steps, warmup, acc_steps, npairs, train_steps):
                                  net)):
         cfg.precision = prec
         cfg.satadj = mode
-        ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
+        ms, stats, tune, med = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
                                     warmup, net=n, autotune=True)
         dom, ops = roofline(blk, cfg, stats)
         forms[name] = {"ms_per_step": round(ms, 4),
+                       "ms_per_step_median_of_20": round(med, 4),
                        "cell_updates_per_s": cells / (ms * 1e-3),
                        "dominant": dom, "ops": ops, "tune": tune,
                        "kernels_avg_ms": {k: round(v[0], 4)
@@ -507,7 +517,7 @@ def bench_lines(ctx, nx, steps, warmup):
     cfg.implicit_flag = 15
     cfg.precision = ca.PREC_FP64
     cells = blk.ncells()
-    ms, stats, tune = time_form(ctx, [blk], th, cfg, [w0], dt, steps, warmup)
+    ms, stats, tune, med = time_form(ctx, [blk], th, cfg, [w0], dt, steps, warmup)
     lk = stats.get("k_line_*", (0.0, 0))
     line_ms = lk[0] * lk[1]
     per_dir = line_ms / 6.0
@@ -584,9 +594,10 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     forms = {}
     for name, prec in (("plain_fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
         cfg.precision = prec
-        ms, stats, tune = time_form(ctx, blocks, th, cfg, w0s, dt, steps,
+        ms, stats, tune, med = time_form(ctx, blocks, th, cfg, w0s, dt, steps,
                                     warmup, autotune=True)
         forms[name] = {"ms_per_step": round(ms, 4),
+                       "ms_per_step_median_of_20": round(med, 4),
                        "cell_updates_per_s": cells / (ms * 1e-3),
                        "tune": tune,
                        "kernels_avg_ms": {k: round(v[0], 4)
