@@ -47,7 +47,8 @@ struct cg_run {
   int AS = 30;  // archive doubles per level (B*B + B)
   std::vector<RunBlock> blocks;
   double* arch = nullptr;
-  double* lines = nullptr;  // scratch of the x2 / x3 line solves
+  double* lines = nullptr;  // scratch of the line solves (x2 / x3, periodic x1)
+  bool x1_periodic = false; // x1 columns solved as periodic lines, not fused
   KBlockPtr* d_bp = nullptr;
   KGroup* d_grp = nullptr;
   int ngroups = 0;
@@ -155,8 +156,8 @@ int run_stage(cg_run* r, int s) {
   src.heat_flux = r->cfg.heat_flux;
   KTimer* tm = r->tm.on ? &r->tm : nullptr;
   // x1 columns in the batched solve fused with the update; x2 / x3 lines
-  // (implicit_flag bits 1, 2) on du before it
-  const bool imp = (r->cfg.implicit_flag & 1) != 0;
+  // (implicit_flag bits 1, 2), and periodic x1 columns, on du before it
+  const bool imp = (r->cfg.implicit_flag & 1) != 0 && !r->x1_periodic;
   const int bs = r->tune.flux_block;
   KUpdate up;
   up.stage = s;
@@ -204,6 +205,11 @@ int run_stage(cg_run* r, int s) {
                        b.w, r->lines, r->ctx->d_status, st, tm);
     }
   }
+  // periodic x1 columns by the cyclic recurrences, then the plain update
+  if ((r->cfg.implicit_flag & 1) && r->x1_periodic)
+    for (RunBlock& b : r->blocks)
+      launch_lines<NM>(b.g, r->kt, sbeta[s], r->d_dt, r->cfg.implicit_flag, 0, true, b.du,
+                       b.w, r->lines, r->ctx->d_status, st, tm);
   if (imp) {
     launch_columns<NM>(r, sbeta[s], up, tm);
     for (RunBlock& b : r->blocks) {
@@ -259,10 +265,9 @@ int check_config(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
   e = check_implicit_flag(ctx, *blk, cfg->implicit_flag);
   if (e) return e;
-  // a run's x1 columns are solved between walls, fused with the update
-  // (periodic x1 columns: cg_implicit_solve)
-  if ((cfg->implicit_flag & 1) && (blk->bc[0] == CG_BC_PERIODIC || blk->bc[1] == CG_BC_PERIODIC))
-    return set_err(ctx, CG_ERR_ARG, "a run solves x1 columns between walls, not periodic ones");
+  // x1 periodic at one end only: no such mesh in the reference
+  if ((blk->bc[0] == CG_BC_PERIODIC) != (blk->bc[1] == CG_BC_PERIODIC))
+    return set_err(ctx, CG_ERR_ARG, "x1 must be periodic at both ends or at neither");
   if (cfg->satadj != CG_SATADJ_OFF && (th->nmass != 3 || th->ivap <= 0 || th->icld <= 0))
     return set_err(ctx, CG_ERR_ARG, "saturation adjustment needs a vapour/cloud pair");
   if (cfg->precision != CG_PREC_FP64 && cfg->precision != CG_PREC_FP32)
@@ -502,9 +507,13 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   for (int b = 0; b < nblocks; ++b) {
     int e = check_config(ctx, &blks[b], th, cfg);
     if (e) return e;
+    // one x1 treatment for every block of a run
+    if ((blks[b].bc[0] == CG_BC_PERIODIC) != (blks[0].bc[0] == CG_BC_PERIODIC))
+      return set_err(ctx, CG_ERR_ARG, "the blocks of a run must agree on periodic x1");
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   cg_run* r = new cg_run();
+  r->x1_periodic = blks[0].bc[0] == CG_BC_PERIODIC;
   r->ctx = ctx;
   r->th = *th;
   r->cfg = *cfg;
@@ -562,18 +571,21 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
          hipMemcpy(r->d_bp, bp.data(), sizeof(KBlockPtr) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
   }
-  if (ok && (cfg->implicit_flag & 6)) {
+  const bool x1_lines = (cfg->implicit_flag & 1) && r->x1_periodic;
+  if (ok && ((cfg->implicit_flag & 6) || x1_lines)) {
     size_t n = 0;
     const int B = (cfg->implicit_flag & 8) ? 5 : 3;
-    for (const RunBlock& rb : r->blocks)
+    for (const RunBlock& rb : r->blocks) {
       for (int d = 1; d <= 2; ++d)
         if ((cfg->implicit_flag & (1 << d)) && (d == 1 ? rb.g.f2 : rb.g.f3))
           n = std::max(n, line_doubles(rb.g, d, B,
                                        rb.blk.bc[2 * d] == CG_BC_PERIODIC &&
                                            rb.blk.bc[2 * d + 1] == CG_BC_PERIODIC));
+      if (x1_lines) n = std::max(n, line_doubles(rb.g, 0, B, true));
+    }
     ok = hipMalloc((void**)&r->lines, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess;
   }
-  if (ok && (cfg->implicit_flag & 1)) {
+  if (ok && (cfg->implicit_flag & 1) && !r->x1_periodic) {
     std::vector<KGroup> groups;
     size_t arch_n = 0;
     build_groups(r->blocks, r->AS, groups, arch_n);
@@ -1029,7 +1041,7 @@ int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
     int wpb = wpbs[wi];
     if (!(r->cfg.implicit_flag & 8) && wpb != 1) continue;
     const int grids[4] = {0, r->ncu * 4 / wpb, r->ncu * 2 / wpb, r->ncu / wpb};
-    const bool cols = (r->cfg.implicit_flag & 1) != 0;
+    const bool cols = (r->cfg.implicit_flag & 1) != 0 && !r->x1_periodic;
     for (int gi = 0; gi < (cols ? 4 : 1); ++gi)
       for (int fu = 1; fu >= (cols ? 0 : 1); --fu) {
         cg_tune t = best;
