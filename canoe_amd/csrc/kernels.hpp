@@ -370,7 +370,10 @@ __device__ inline void mvec(const Mt<B>& x, const Vt<B>& y, Vt<B>& z) {
 }
 
 // Gauss-Jordan with partial pivoting (first row of largest |pivot|), pivot
-// row scaled by the reciprocal; row exchanges by register selects.
+// row scaled by the reciprocal; row exchanges by register selects, run only
+// when a lane of the wavefront pivots (a uniform branch).  Columns of m left
+// of the pivot column are not updated: neither x nor any later pivot or
+// factor reads them, so x equals the full-row elimination bit for bit.
 template <int B>
 __device__ inline bool minv(Mt<B> m, Mt<B>& x) {
 #pragma unroll
@@ -391,23 +394,27 @@ __device__ inline bool minv(Mt<B> m, Mt<B>& x) {
       }
     }
     if (best == 0.0) ok = false;
+    if (__any(p != c)) {
 #pragma unroll
-    for (int r = c + 1; r < B; ++r) {
-      bool sw = (p == r);
+      for (int r = c + 1; r < B; ++r) {
+        bool sw = (p == r);
 #pragma unroll
-      for (int q = 0; q < B; ++q) {
-        double mc = m.a[c][q], mr = m.a[r][q];
-        m.a[c][q] = sw ? mr : mc;
-        m.a[r][q] = sw ? mc : mr;
-        double xc = x.a[c][q], xr = x.a[r][q];
-        x.a[c][q] = sw ? xr : xc;
-        x.a[r][q] = sw ? xc : xr;
+        for (int q = 0; q < B; ++q) {
+          if (q >= c) {
+            double mc = m.a[c][q], mr = m.a[r][q];
+            m.a[c][q] = sw ? mr : mc;
+            m.a[r][q] = sw ? mc : mr;
+          }
+          double xc = x.a[c][q], xr = x.a[r][q];
+          x.a[c][q] = sw ? xr : xc;
+          x.a[r][q] = sw ? xc : xr;
+        }
       }
     }
     double inv = 1.0 / m.a[c][c];
 #pragma unroll
     for (int q = 0; q < B; ++q) {
-      m.a[c][q] = m.a[c][q] * inv;
+      if (q >= c) m.a[c][q] = m.a[c][q] * inv;
       x.a[c][q] = x.a[c][q] * inv;
     }
 #pragma unroll
@@ -416,7 +423,7 @@ __device__ inline bool minv(Mt<B> m, Mt<B>& x) {
       double f = m.a[r][c];
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        m.a[r][q] = m.a[r][q] - f * m.a[c][q];
+        if (q >= c) m.a[r][q] = m.a[r][q] - f * m.a[c][q];
         x.a[r][q] = x.a[r][q] - f * x.a[c][q];
       }
     }

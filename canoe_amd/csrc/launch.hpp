@@ -219,7 +219,7 @@ inline KLines make_lines(const KGrid& g, int dir) {
 // Scratch doubles of the line solve of direction dir (B x B blocks).
 inline size_t line_doubles(const KGrid& g, int dir, int B, bool per) {
   KLines L = make_lines(g, dir);
-  const size_t ls = per ? 3 * B * B + B : B * B + B;
+  const size_t ls = B == 5 ? line_level_size<5>(per) : line_level_size<3>(per);
   return (size_t)L.n * ls * (size_t)L.nline;
 }
 

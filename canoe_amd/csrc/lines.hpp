@@ -34,7 +34,7 @@ struct KLines {
 // ends ainv_i, gamma_i, c_i (B x B each) and zeta_i (B).
 template <int B>
 CG_HD constexpr int line_level_size(bool per) {
-  return per ? 3 * B * B + B : B * B + B;
+  return per ? 4 * B * B + B : B * B + B;
 }
 
 // The velocities of a state in the frame of direction DIR: slot IVX holds
@@ -70,7 +70,9 @@ __device__ inline void mload(Mt<B>& m, const double* p, long long st) {
 // moves the window up.  Velocities enter in the frame of DIR; the Roe
 // average is formed from the unrotated states (RoeAverage precedes the
 // choice of frame, :98-104).  start(i0) primes the window so that the
-// first level() is cell i0.
+// first level() is cell i0.  The w of cell i + 2 and the du of cell i + 1
+// are requested at the top of level(i), one level ahead of their use, up to
+// the line's last cell.
 template <int NM, int B, int DIR>
 struct LineAsm {
   static constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
@@ -80,7 +82,8 @@ struct LineAsm {
   const double* __restrict__ du;
   long long o;
   double gm1, gm, idt;
-  double wc[N];
+  int hi;  // last cell of the line
+  double wc[N], wn[N], dn[N];
   Mt<B> Am, dm, dc;
   __device__ LineAsm(const KLines& L_, const KThermo& th_, const double* w_,
                      const double* du_, long long o_, double dt)
@@ -88,10 +91,15 @@ struct LineAsm {
     gm1 = th.gammad - 1.;
     gm = 0.5 * (gm1 + gm1);
     idt = 1. / dt;
+    hi = L.lo + L.n - 1;
   }
   __device__ void load(double* dst, int i) const {
 #pragma unroll
     for (int n = 0; n < N; ++n) dst[n] = w[o + n * L.sn + (long long)i * L.sl];
+  }
+  __device__ void load_du(double* dst, int i) const {
+#pragma unroll
+    for (int n = 0; n < N; ++n) dst[n] = du[o + n * L.sn + (long long)i * L.sl];
   }
   __device__ void jac(const double* wv, Mt<B>& out) const {
     double f[N];
@@ -108,14 +116,29 @@ struct LineAsm {
     double wm[N];
     load(wm, i0 - 1);
     load(wc, i0);
+    load(wn, i0 + 1);
+    if (i0 >= L.lo) {
+      load_du(dn, i0);
+    } else {
+#pragma unroll
+      for (int n = 0; n < N; ++n) dn[n] = 0.0;
+    }
     absA(wm, wc, Am);
     jac(wm, dm);
     jac(wc, dc);
   }
   __device__ void level(int i, Mt<B>& a, Mt<B>& b, Mt<B>& c, double* rhs,
                         bool want_rhs) {
-    double wp[N];
-    load(wp, i + 1);
+    double wp[N], d[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      wp[n] = wn[n];
+      d[n] = dn[n];
+    }
+    if (i + 1 <= hi) {
+      load(wn, i + 2);
+      load_du(dn, i + 1);
+    }
     Mt<B> Ap, dp;
     absA(wc, wp, Ap);
     jac(wp, dp);
@@ -136,10 +159,6 @@ struct LineAsm {
         c.a[r][q] = -(Ap.a[r][q] - dp.a[r][q]) * L.sra;
       }
     if (want_rhs) {  // rhs in the frame of DIR (forward_backward.hpp:24-36)
-      const long long ci = o + (long long)i * L.sl;
-      double d[N];
-#pragma unroll
-      for (int n = 0; n < N; ++n) d[n] = du[ci + n * L.sn];
       double r0 = d[0];
 #pragma unroll
       for (int n = 1; n < IVX; ++n) r0 += d[n];
@@ -221,8 +240,9 @@ __device__ inline void line_write_back(const Vt<B>& d, const double* __restrict_
 }
 
 // Gauss-Jordan elimination of [M | C | t] with partial pivoting (first row
-// of largest |pivot|): C = M^-1 C, t = M^-1 t.  Returns false on a zero
-// pivot.
+// of largest |pivot|): C = M^-1 C, t = M^-1 t.  Row exchanges run only when
+// a lane of the wavefront pivots, over the columns of M still read.
+// Returns false on a zero pivot.
 template <int B>
 __device__ inline bool gj_solve_d(Mt<B>& M, Mt<B>& C, double* t) {
   bool ok = true;
@@ -239,21 +259,25 @@ __device__ inline bool gj_solve_d(Mt<B>& M, Mt<B>& C, double* t) {
       }
     }
     if (best == 0.0) ok = false;
+    if (__any(p != cc)) {
 #pragma unroll
-    for (int r = cc + 1; r < B; ++r) {
-      bool sw = (p == r);
+      for (int r = cc + 1; r < B; ++r) {
+        bool sw = (p == r);
 #pragma unroll
-      for (int q = 0; q < B; ++q) {
-        double x0 = M.a[cc][q], x1 = M.a[r][q];
-        M.a[cc][q] = sw ? x1 : x0;
-        M.a[r][q] = sw ? x0 : x1;
-        double y0 = C.a[cc][q], y1 = C.a[r][q];
-        C.a[cc][q] = sw ? y1 : y0;
-        C.a[r][q] = sw ? y0 : y1;
+        for (int q = 0; q < B; ++q) {
+          if (q >= cc) {
+            double x0 = M.a[cc][q], x1 = M.a[r][q];
+            M.a[cc][q] = sw ? x1 : x0;
+            M.a[r][q] = sw ? x0 : x1;
+          }
+          double y0 = C.a[cc][q], y1 = C.a[r][q];
+          C.a[cc][q] = sw ? y1 : y0;
+          C.a[r][q] = sw ? y0 : y1;
+        }
+        double z0 = t[cc], z1 = t[r];
+        t[cc] = sw ? z1 : z0;
+        t[r] = sw ? z0 : z1;
       }
-      double z0 = t[cc], z1 = t[r];
-      t[cc] = sw ? z1 : z0;
-      t[r] = sw ? z0 : z1;
     }
     double inv = 1.0 / M.a[cc][cc];
 #pragma unroll
@@ -382,8 +406,10 @@ __global__ void __launch_bounds__(64)
 //   x_{n-1} = ainv_{n-1} zeta_{n-1},
 //   x_p = ainv_p (zeta_p - U_p x_{p+1} - gamma_p x_{n-1}), U_{n-2} = 0.
 // Each level forms zeta_p and gamma_p first and parks them in the scratch,
-// so that only the pivoted inverse, beta and the running sums are live
-// across it; L_{n-1} comes from the assembly window at level n-2.
+// so that only the pivoted inverse and beta are live across it; beta_p is
+// parked too, and the two sums run as their own pass over the scratch
+// (p = 0 .. n-2, the order of the recurrence); L_{n-1} comes from the
+// assembly window at level n-2.
 // Products left to right as written, inverses by Gauss-Jordan with partial
 // pivoting.
 template <int NM, int B, int DIR>
@@ -392,7 +418,8 @@ __global__ void __launch_bounds__(64)
                     const double* __restrict__ w, double* __restrict__ du,
                     double* __restrict__ lv, int* __restrict__ status) {
   constexpr int AS = line_level_size<B>(true);
-  constexpr int OG = B * B, OU = 2 * B * B, OZ = 3 * B * B;  // gamma, c, zeta
+  // gamma, c, zeta, beta after ainv
+  constexpr int OG = B * B, OU = 2 * B * B, OZ = 3 * B * B, OB = 3 * B * B + B;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L.nline) return;
   const long long o = L.org(t);
@@ -402,14 +429,7 @@ __global__ void __launch_bounds__(64)
   LineAsm<NM, B, DIR> as(L, th, w, du, o, dtp ? dt * dtp[0] : dt);
   as.start(lo - 1);
   bool ok = true;
-  Mt<B> bet, sbg;
-  Vt<B> sbz;
-#pragma unroll
-  for (int r = 0; r < B; ++r) {
-    sbz.v[r] = 0.0;
-#pragma unroll
-    for (int q = 0; q < B; ++q) sbg.a[r][q] = 0.0;
-  }
+  Mt<B> bet;
   {  // the corner c_{lo-1}, then level 0
     Mt<B> a, b, um1;
     double rhs[B];
@@ -424,22 +444,11 @@ __global__ void __launch_bounds__(64)
     if (!minv<B>(a, ai)) ok = false;
     mstore<B>(ai, cur, st);
     mmul<B>(um1, ai, bet);
-    Mt<B> bg;
-    Vt<B> z, bz;
-    mmul<B>(bet, b, bg);
-#pragma unroll
-    for (int r = 0; r < B; ++r) z.v[r] = rhs[r];
-    mvec<B>(bet, z, bz);
-#pragma unroll
-    for (int r = 0; r < B; ++r) {
-      sbz.v[r] = sbz.v[r] + bz.v[r];
-#pragma unroll
-      for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
-    }
+    mstore<B>(bet, cur + OB * st, st);
   }
-  for (int p = 1; p <= n - 2; ++p) {
-    double* cur = lvl(p);
-    const double* prv = lvl(p - 1);
+  // level p reads level p - 1's scratch and writes level p's: disjoint
+  // entries, declared so that the reads may be issued before the writes
+  auto step = [&](int p, double* __restrict__ cur, const double* __restrict__ prv) {
     Mt<B> d;
     {
       Mt<B> b, c;
@@ -502,21 +511,34 @@ __global__ void __launch_bounds__(64)
         mmul<B>(x, ai, bnew);
       }
       bet = bnew;
+      mstore<B>(bet, cur + OB * st, st);
     }
-    {  // sums of beta_p gamma_p and beta_p zeta_p
-      Mt<B> g, bg;
-      Vt<B> z, bz;
-      mload<B>(g, cur + OG * st, st);
+  };
+  for (int p = 1; p <= n - 2; ++p) step(p, lvl(p), lvl(p - 1));
+  // sums of beta_p gamma_p and beta_p zeta_p
+  Mt<B> sbg;
+  Vt<B> sbz;
 #pragma unroll
-      for (int r = 0; r < B; ++r) z.v[r] = cur[(OZ + r) * st];
-      mmul<B>(bet, g, bg);
-      mvec<B>(bet, z, bz);
+  for (int r = 0; r < B; ++r) {
+    sbz.v[r] = 0.0;
 #pragma unroll
-      for (int r = 0; r < B; ++r) {
-        sbz.v[r] = sbz.v[r] + bz.v[r];
+    for (int q = 0; q < B; ++q) sbg.a[r][q] = 0.0;
+  }
+  for (int p = 0; p <= n - 2; ++p) {
+    const double* cur = lvl(p);
+    Mt<B> bp, g, bg;
+    Vt<B> z, bz;
+    mload<B>(bp, cur + OB * st, st);
+    mload<B>(g, cur + OG * st, st);
 #pragma unroll
-        for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
-      }
+    for (int r = 0; r < B; ++r) z.v[r] = cur[(OZ + r) * st];
+    mmul<B>(bp, g, bg);
+    mvec<B>(bp, z, bz);
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      sbz.v[r] = sbz.v[r] + bz.v[r];
+#pragma unroll
+      for (int q = 0; q < B; ++q) sbg.a[r][q] = sbg.a[r][q] + bg.a[r][q];
     }
   }
   // last level
