@@ -73,6 +73,22 @@ def test_uneven_extents_and_2d_block(ctx):
         _check(ctx, blk, th, w, du, 0.4, flag)
 
 
+@pytest.mark.parametrize("bc", [ca.BC_PERIODIC, ca.BC_REFLECT])
+def test_shortest_lines(ctx, bc):
+    # three cells, the smallest extent a block takes: the first level, the
+    # cyclic corner terms and the last level of every line meet
+    blk, th, w, du = _case(bc, bc, dims=(3, 3, 3), seed=21)
+    for flag in (15, 7):
+        _say("lines 3x3x3 flag=%d bc=%d" % (flag, bc))
+        _check(ctx, blk, th, w, du, 0.5, flag)
+    bcs = list(blk.bc)
+    bcs[0] = bcs[1] = ca.BC_PERIODIC
+    _set_bc(blk, bcs)
+    w = ca.apply_bc_host(blk, w)
+    _say("periodic x1 3x3x3 bc=%d" % bc)
+    _check(ctx, blk, th, w, du, 0.5, 15)
+
+
 def test_moist_periodic_lines(ctx):
     blk, th, cfg, w = perturbed_state(4, (8, 10, 6), vel=10.0)
     du = np.zeros(blk.shape())
