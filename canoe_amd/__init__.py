@@ -131,6 +131,10 @@ _SIGS = {
     "cg_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cg_add_flux_divergence": (_I, [_P, _P, _D, _P, _P, _P, _P]),
     "cg_implicit_solve": (_I, [_P, _P, _P, _P, _P, _D, _I]),
+    "cg_riemann_host": (_I, [_P, _P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
+                             ctypes.c_ssize_t]),
+    "cg_calculate_fluxes_host": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "cg_implicit_solve_host": (_I, [_P, _P, _P, _P, _P, _D, _I]),
     "cg_cons_to_prim": (_I, [_P, _P, _P, _P, _P]),
     "cg_prim_to_cons": (_I, [_P, _P, _P, _P, _P]),
     "cg_apply_bc": (_I, [_P, _P, _P]),

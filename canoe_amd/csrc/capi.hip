@@ -222,6 +222,91 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   return check_status(ctx);
 }
 
+// ---- host-array variants ---------------------------------------------------
+// Device copies of host arrays, freed on every return path.
+namespace {
+struct Staged {
+  double* p = nullptr;
+  size_t n = 0;
+  ~Staged() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int stage_in(cg_ctx* ctx, Staged& b, const double* h, size_t n) {
+  b.n = n;
+  if (n == 0) return CG_OK;
+  if (!h) return set_err(ctx, CG_ERR_ARG, "null host array");
+  HIPCHK(ctx, hipMalloc((void**)&b.p, n * sizeof(double)));
+  HIPCHK(ctx, hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+  return CG_OK;
+}
+
+int stage_out(cg_ctx* ctx, double* h, const Staged& b) {
+  if (b.n == 0) return CG_OK;
+  HIPCHK(ctx, hipMemcpyAsync(h, b.p, b.n * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+}  // namespace
+
+int cg_riemann_host(cg_ctx* ctx, const cg_thermo* th, int solver, int ivx, int n_faces,
+                    const double* wl, const double* wr, ptrdiff_t ld, double* flx,
+                    ptrdiff_t flx_ld) {
+  if (!ctx || !th) return CG_ERR_ARG;
+  if (th->nmass != 0 && th->nmass != 3)
+    return set_err(ctx, CG_ERR_ARG, "nmass must be 0 or 3");
+  if (n_faces <= 0) return cg_riemann(ctx, th, solver, ivx, n_faces, nullptr, nullptr, ld,
+                                      nullptr, flx_ld);
+  if (ld < n_faces || flx_ld < n_faces) return set_err(ctx, CG_ERR_ARG, "ld < n_faces");
+  const size_t N = 5 + (size_t)th->nmass;
+  Staged l, r, f;
+  int e = stage_in(ctx, l, wl, N * ld);
+  if (!e) e = stage_in(ctx, r, wr, N * ld);
+  if (!e) e = stage_in(ctx, f, flx, N * flx_ld);  // faces past n_faces kept
+  if (!e) e = cg_riemann(ctx, th, solver, ivx, n_faces, l.p, r.p, ld, f.p, flx_ld);
+  if (!e) e = stage_out(ctx, flx, f);
+  return e;
+}
+
+int cg_calculate_fluxes_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                             const cg_config* cfg, const double* w, double* flux1,
+                             double* flux2, double* flux3) {
+  if (!ctx) return CG_ERR_ARG;
+  int e = check_block(ctx, blk, th);
+  if (e) return e;
+  KGrid g = make_grid(*blk, 0);
+  const int N = 5 + blk->nmass;
+  Staged dw, f1, f2, f3;
+  e = stage_in(ctx, dw, w, cell_doubles(g, N));
+  // faces the sweep does not write keep the caller's values
+  if (!e) e = stage_in(ctx, f1, flux1, f1_doubles(g, N));
+  if (!e) e = stage_in(ctx, f2, flux2, f2_doubles(g, N));
+  if (!e) e = stage_in(ctx, f3, flux3, f3_doubles(g, N));
+  if (!e) e = cg_calculate_fluxes(ctx, blk, th, cfg, dw.p, f1.p, f2.p, f3.p);
+  if (!e) e = stage_out(ctx, flux1, f1);
+  if (!e) e = stage_out(ctx, flux2, f2);
+  if (!e) e = stage_out(ctx, flux3, f3);
+  return e;
+}
+
+int cg_implicit_solve_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                           double* du, const double* w, double dt, int flag) {
+  if (!ctx) return CG_ERR_ARG;
+  int e = check_block(ctx, blk, th);
+  if (e) return e;
+  KGrid g = make_grid(*blk, 0);
+  const size_t n = cell_doubles(g, 5 + blk->nmass);
+  Staged dd, dw;
+  e = stage_in(ctx, dd, du, n);
+  if (!e) e = stage_in(ctx, dw, w, n);
+  if (!e) e = cg_implicit_solve(ctx, blk, th, dd.p, dw.p, dt, flag);
+  if (!e) e = stage_out(ctx, du, dd);
+  return e;
+}
+
 int cg_cons_to_prim(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                     const double* u, double* w) {
   if (!ctx) return CG_ERR_ARG;

@@ -159,6 +159,23 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
 int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                       double* du, const double* w, double dt, int flag);
 
+/* ---- host-array variants ------------------------------------------------ *
+ * The same calls on host arrays of the same layouts, for a caller that keeps
+ * its AthenaArrays in host memory (the reference's own buffers: wl_ / wr_
+ * rows and flx of lmars.cpp:19-22, w and the flux arrays of
+ * Hydro::CalculateFluxes, du and w of solve_implicit_3d.cpp:19-20).  Each
+ * copies its inputs to the device, runs the device call and copies the
+ * outputs back before returning; entries the device call leaves alone
+ * (faces past n_faces, faces of ghost cells) keep the caller's values. */
+int cg_riemann_host(cg_ctx* ctx, const cg_thermo* th, int solver, int ivx,
+                    int n_faces, const double* wl, const double* wr, ptrdiff_t ld,
+                    double* flx, ptrdiff_t flx_ld);
+int cg_calculate_fluxes_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                             const cg_config* cfg, const double* w, double* flux1,
+                             double* flux2, double* flux3);
+int cg_implicit_solve_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                           double* du, const double* w, double dt, int flag);
+
 /* ---- EOS and boundaries ------------------------------------------------- *
  * Replace EquationOfState::ConservedToPrimitive / PrimitiveToConserved
  * (src/snap/eos/ideal_moist_hydro.cpp:47-81) on interior cells, in the
@@ -384,6 +401,22 @@ int cg_run_set_tune(cg_run* run, const cg_tune* tune);
 /* Times one graph-replayed step of size dt per candidate shape (the state is
  * restored after every trial) and keeps the fastest. */
 int cg_run_autotune(cg_run* run, double dt, cg_tune* out);
+
+/* ---- problem generators and host boundaries (host code) ----------------- *
+ * The configurations' MeshBlock::ProblemGenerator and input-file settings
+ * (examples/2019-Li-snap/straka.cpp:220-268 and straka.inp for config 1;
+ * the dry and moist boxes and one block of the uneven-column batch for
+ * configs 2, 4 and 5).  cg_pgen_setup fills the block, species constants and
+ * run settings (nx* = 0: the configuration's own size; other sizes keep the
+ * grid spacing); cg_pgen_init writes the seeded primitive state, interior and
+ * ghosts, into a host array.  cg_apply_bc_host fills the ghost cells of a
+ * host primitive array in the order of the device boundary kernels (x1, x2,
+ * x3). */
+int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
+                  cg_thermo* th, cg_config* cfg);
+int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
+                 const cg_thermo* th, const cg_config* cfg, double* w);
+int cg_apply_bc_host(const cg_block* blk, double* w);
 
 /* Library build string (arch, flags). */
 const char* cg_version(void);
