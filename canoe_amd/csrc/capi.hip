@@ -146,17 +146,17 @@ int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
   KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
-  double *psf = nullptr, *psv = nullptr;
   long long ncol = (long long)g.nx2 * g.nx3;
-  HIPCHK(ctx, hipMallocAsync((void**)&psf, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
-  HIPCHK(ctx, hipMallocAsync((void**)&psv, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
+  AsyncScratch psf_s(ctx->stream), psv_s(ctx->stream);
+  HIPCHK(ctx, hipMallocAsync(&psf_s.p, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
+  HIPCHK(ctx, hipMallocAsync(&psv_s.p, sizeof(double) * ncol * (g.nc1 + 1), ctx->stream));
+  double* psf = (double*)psf_s.p;
+  double* psv = (double*)psv_s.p;
   if (blk->nmass == 0)
     launch_fluxes<0>(g, kt, *cfg, w, psf, psv, flux1, flux2, flux3, ctx->stream);
   else
     launch_fluxes<3>(g, kt, *cfg, w, psf, psv, flux1, flux2, flux3, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipFreeAsync(psf, ctx->stream));
-  HIPCHK(ctx, hipFreeAsync(psv, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
 }
@@ -199,8 +199,9 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     if ((flag & (1 << d)) && act[d])
       need = std::max(need, (d == 0 && !per[0]) ? arch_doubles(g) : line_doubles(g, d, B, per[d]));
   if (need == 0) return CG_OK;  // only directions without cells were asked
-  double* scratch = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&scratch, sizeof(double) * need, ctx->stream));
+  AsyncScratch scratch_s(ctx->stream);
+  HIPCHK(ctx, hipMallocAsync(&scratch_s.p, sizeof(double) * need, ctx->stream));
+  double* scratch = (double*)scratch_s.p;
   for (int d = 2; d >= 0; --d) {
     if (!(flag & (1 << d)) || !act[d]) continue;
     if (d == 0 && !per[0]) {
@@ -217,7 +218,6 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     }
     HIPCHK(ctx, hipGetLastError());
   }
-  HIPCHK(ctx, hipFreeAsync(scratch, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_status(ctx);
 }
@@ -350,8 +350,9 @@ int cg_new_block_dt(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   KGrid g = make_grid(*blk, 0);
   KThermo kt = make_thermo(*th);
   constexpr int parts = 256;
-  double* part = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&part, sizeof(double) * (parts + 1), ctx->stream));
+  AsyncScratch part_s(ctx->stream);
+  HIPCHK(ctx, hipMallocAsync(&part_s.p, sizeof(double) * (parts + 1), ctx->stream));
+  double* part = (double*)part_s.p;
   if (blk->nmass == 0)
     k_new_dt<0><<<parts, 256, 0, ctx->stream>>>(nullptr, g, w, kt, cfl,
                                                  cfg->implicit_flag, part);
@@ -362,7 +363,6 @@ int cg_new_block_dt(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(dt_out, part + parts, sizeof(double), hipMemcpyDeviceToHost,
                              ctx->stream));
-  HIPCHK(ctx, hipFreeAsync(part, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
 }

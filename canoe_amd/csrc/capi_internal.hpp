@@ -38,6 +38,19 @@ inline int set_err(cg_ctx* ctx, int code, const char* fmt, ...) {
       return set_err(ctx, CG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
+// Stream-ordered scratch of one entry point, released on every return path
+// (a failed HIP call returns early through HIPCHK).
+struct AsyncScratch {
+  hipStream_t s;
+  void* p = nullptr;
+  explicit AsyncScratch(hipStream_t st) : s(st) {}
+  ~AsyncScratch() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  AsyncScratch(const AsyncScratch&) = delete;
+  AsyncScratch& operator=(const AsyncScratch&) = delete;
+};
+
 inline cg::KThermo make_thermo(const cg_thermo& t) {
   cg::KThermo k;
   memset(&k, 0, sizeof(k));
