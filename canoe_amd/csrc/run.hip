@@ -329,14 +329,14 @@ void build_groups(const std::vector<RunBlock>& blocks, int AS,
 int run_download(cg_run* r, const RunBlock& b, const double* src, double* host) {
   cg_ctx* ctx = r->ctx;
   size_t nc = cell_doubles(b.g, r->N);
-  double* tmp = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&tmp, sizeof(double) * nc, ctx->stream));
+  AsyncScratch tmp_s(ctx->stream);
+  HIPCHK(ctx, hipMallocAsync(&tmp_s.p, sizeof(double) * nc, ctx->stream));
+  double* tmp = (double*)tmp_s.p;
   k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(b.g, b.ga, r->N, src,
                                                                tmp);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(host, tmp, sizeof(double) * nc, hipMemcpyDeviceToHost,
                              ctx->stream));
-  HIPCHK(ctx, hipFreeAsync(tmp, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
 }
@@ -852,10 +852,11 @@ int cg_run_debug(cg_run* r, double* vmin, double* vmax, double* xmin, double* xm
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int nb = (int)r->blocks.size(), P = 64, N = r->N;
   const size_t np = (size_t)nb * P * N * 2;
-  double* pv = nullptr;
-  long long* px = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&pv, sizeof(double) * np, ctx->stream));
-  HIPCHK(ctx, hipMallocAsync((void**)&px, sizeof(long long) * np, ctx->stream));
+  AsyncScratch pv_s(ctx->stream), px_s(ctx->stream);
+  HIPCHK(ctx, hipMallocAsync(&pv_s.p, sizeof(double) * np, ctx->stream));
+  HIPCHK(ctx, hipMallocAsync(&px_s.p, sizeof(long long) * np, ctx->stream));
+  double* pv = (double*)pv_s.p;
+  long long* px = (long long*)px_s.p;
   if (N == 5)
     k_minmax<0><<<dim3(P, nb), 256, 0, ctx->stream>>>(r->d_bp, pv, px);
   else
@@ -867,8 +868,6 @@ int cg_run_debug(cg_run* r, double* vmin, double* vmax, double* xmin, double* xm
                                  ctx->stream);
   hipError_t c2 = hipMemcpyAsync(hx.data(), px, sizeof(long long) * np,
                                  hipMemcpyDeviceToHost, ctx->stream);
-  (void)hipFreeAsync(pv, ctx->stream);
-  (void)hipFreeAsync(px, ctx->stream);
   HIPCHK(ctx, le);
   HIPCHK(ctx, c1);
   HIPCHK(ctx, c2);
