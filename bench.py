@@ -537,6 +537,44 @@ def bench_lines(ctx, nx, steps, warmup):
             "kernels_avg_ms": {k: round(v[0], 4) for k, v in stats.items()}}
 
 
+def bench_host_solve(ctx, nx, reps=3):
+    """cg_implicit_solve_host on the dry box (implicit_flag 9, plain FP64):
+    du and w stay in pageable host memory, as the reference's AthenaArrays
+    do; the call copies both to the device, solves the x1 columns and copies
+    du back, so its time includes PCIe transfers of 3 N cells_with_ghosts x 8
+    bytes.  The rate of the host-buffer boundary, beside (not instead of) the
+    resident step."""
+    blk, th, cfg = ca.pgen_setup(2, *nx)
+    w = np.ascontiguousarray(ca.pgen_init(2, 1, blk, th, cfg), np.float64)
+    dt = cfl_dt(blk, th, cfg, w)
+    du = np.zeros(blk.shape())
+    sk, sj, si = blk.interior()
+    du[:, sk, sj, si] = np.random.default_rng(3).normal(
+        size=du[:, sk, sj, si].shape) * np.array(
+            [1e-3, 1e-2, 1e-2, 1e-2, 3e2])[:, None, None, None]
+    times = []
+    for r in range(reps + 1):
+        d = du.copy()
+        t0 = time.perf_counter()
+        rc = ca.lib().cg_implicit_solve_host(ctx.h, ca._ref(blk), ca._ref(th),
+                                             ca._ptr(d), ca._ptr(w), dt, 9)
+        t1 = time.perf_counter()
+        if rc != ca.OK:
+            raise ca.CanoeGpuError(rc, "cg_implicit_solve_host")
+        if r:
+            times.append(t1 - t0)
+    ms = float(np.median(times)) * 1e3
+    cells = blk.ncells()
+    pcie = 3 * du.nbytes
+    log("implicit solve on host arrays: %.2f ms per call" % ms)
+    return {"workload": "cg_implicit_solve_host, dry box %dx%dx%d, flag 9, "
+                        "pageable host du and w" % (blk.nx2, blk.nx3, blk.nx1),
+            "ms_per_call": round(ms, 3),
+            "pcie_bytes": pcie,
+            "pcie_inclusive_GBs": round(pcie / (ms * 1e-3) / 1e9, 2),
+            "cell_solves_per_s": cells / (ms * 1e-3)}
+
+
 def mlp_run(nx, train_steps):
     """A short, fixed run of the saturation-guess network for rocprofv3:
     pairs from the moist box's initial state (2^20), train_steps Adam steps
@@ -706,9 +744,10 @@ def main():
     ctx = ca.Context(0)
     blk, cfg, dt, cells, forms, acc = bench_box(ctx, args.nx, args.steps,
                                                 args.warmup, args.acc_steps)
-    moist = batch = lines = None
+    moist = batch = lines = host = None
     if not args.quick:
         lines = bench_lines(ctx, args.nx, max(2, args.steps // 5), 1)
+        host = bench_host_solve(ctx, args.nx)
         moist = bench_moist(ctx, args.nx, args.steps, args.warmup,
                             args.acc_steps, args.pairs, args.train_steps)
         batch = bench_batch(ctx, 2024, 32, max(3, args.steps // 4),
@@ -767,6 +806,7 @@ def main():
         "moist_box": moist,
         "batch": batch,
         "implicit_lines": lines,
+        "implicit_solve_host": host,
     }
     print(json.dumps(out))
 
