@@ -126,6 +126,8 @@ _SIGS = {
     "cg_memcpy_h2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cg_memcpy_d2h": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cg_memcpy_d2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "cg_timer_start": (_I, [_P]),
+    "cg_timer_stop": (_I, [_P, ctypes.POINTER(_D)]),
     "cg_riemann": (_I, [_P, _P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
                         ctypes.c_ssize_t]),
     "cg_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -321,6 +323,17 @@ class Context:
         """Wait for the context's stream; reports kernel status (zero
         pivots, launch errors) of asynchronous calls."""
         self._check(lib().cg_synchronize(self.h), "cg_synchronize")
+
+    def timer_start(self):
+        """Record a HIP event on the context's stream (cg_timer_start)."""
+        self._check(lib().cg_timer_start(self.h), "cg_timer_start")
+
+    def timer_stop(self):
+        """Device milliseconds since timer_start (cg_timer_stop)."""
+        ms = ctypes.c_double()
+        self._check(lib().cg_timer_stop(self.h, ctypes.byref(ms)),
+                    "cg_timer_stop")
+        return ms.value
 
     def array(self, a=None, shape=None, dtype=np.float64):
         if a is not None:

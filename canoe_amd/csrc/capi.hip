@@ -50,8 +50,29 @@ int cg_destroy(cg_ctx* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->d_status) hipFree(ctx->d_status);
+  if (ctx->t0) hipEventDestroy(ctx->t0);
+  if (ctx->t1) hipEventDestroy(ctx->t1);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
+  return CG_OK;
+}
+
+int cg_timer_start(cg_ctx* ctx) {
+  if (!ctx) return CG_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!ctx->t0) HIPCHK(ctx, hipEventCreate(&ctx->t0));
+  if (!ctx->t1) HIPCHK(ctx, hipEventCreate(&ctx->t1));
+  HIPCHK(ctx, hipEventRecord(ctx->t0, ctx->stream));
+  return CG_OK;
+}
+
+int cg_timer_stop(cg_ctx* ctx, double* ms) {
+  if (!ctx || !ms || !ctx->t0 || !ctx->t1) return CG_ERR_ARG;
+  HIPCHK(ctx, hipEventRecord(ctx->t1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(ctx->t1));
+  float f = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+  *ms = f;
   return CG_OK;
 }
 
@@ -282,13 +303,16 @@ int cg_calculate_fluxes_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* 
   Staged dw, f1, f2, f3;
   e = stage_in(ctx, dw, w, cell_doubles(g, N));
   // faces the sweep does not write keep the caller's values
+  // a direction without cells has no flux array (flux2 / flux3 may be null,
+  // as the reference allocates x2flux / x3flux only where the mesh has f2 /
+  // f3): nothing is staged for it
   if (!e) e = stage_in(ctx, f1, flux1, f1_doubles(g, N));
-  if (!e) e = stage_in(ctx, f2, flux2, f2_doubles(g, N));
-  if (!e) e = stage_in(ctx, f3, flux3, f3_doubles(g, N));
+  if (!e && g.f2) e = stage_in(ctx, f2, flux2, f2_doubles(g, N));
+  if (!e && g.f3) e = stage_in(ctx, f3, flux3, f3_doubles(g, N));
   if (!e) e = cg_calculate_fluxes(ctx, blk, th, cfg, dw.p, f1.p, f2.p, f3.p);
   if (!e) e = stage_out(ctx, flux1, f1);
-  if (!e) e = stage_out(ctx, flux2, f2);
-  if (!e) e = stage_out(ctx, flux3, f3);
+  if (!e && g.f2) e = stage_out(ctx, flux2, f2);
+  if (!e && g.f3) e = stage_out(ctx, flux3, f3);
   return e;
 }
 

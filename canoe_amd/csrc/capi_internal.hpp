@@ -16,6 +16,7 @@ struct cg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int* d_status = nullptr;  // device error flags (bit 0 pivot, bit 1 Newton)
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // cg_timer_start / cg_timer_stop
   char err[512] = {0};
 };
 

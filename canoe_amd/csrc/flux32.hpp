@@ -292,6 +292,36 @@ __device__ __forceinline__ void recon_pair_f(const float* q, float* l, float* r)
   }
 }
 
+// Decomposition of x1 stencil cell i of the column at colb (offset of
+// n = 0, i = 0): T in float and the perturbation p - psv as an FP64
+// difference rounded to float; a cell beyond an outflow x1 side takes the
+// boundary cell's (decomp_cell, change_to_temperature.cpp:113-127).  wc
+// holds cell i.  Returns R of cell i itself (its gamma_ in the face
+// restore, :158).
+template <int NM>
+__device__ __forceinline__ double decomp32(const KGrid& g, const KThermo& th,
+                                           const double* __restrict__ w,
+                                           long long colb,
+                                           const double* __restrict__ psv,
+                                           long long pc, int i, const double* wc,
+                                           float& T, float& pp) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
+  const double r = decomp_r<NM>(th, wc);
+  const int ib = decomp_cell(g, i);
+  double pb = wc[IPR], db = wc[0], rb = r;
+  if (ib != i) {
+    double wb[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) wb[n] = w[colb + n * g.c.n + (long long)ib * g.c.i];
+    pb = wb[IPR];
+    db = wb[0];
+    rb = decomp_r<NM>(th, wb);
+  }
+  T = fdiv((float)pb, (float)db * (float)rb);
+  pp = (float)(pb - psv[pc + ib * g.ps_i]);
+  return r;
+}
+
 // x1 faces in float: decomposition in FP64 (T, p - psv), reconstruction and
 // LMARS/HLLC in float about pb = psf(i).
 template <int NM, int ORDER>
@@ -318,12 +348,9 @@ __global__ void __launch_bounds__(256)
     for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i - 3 + m)];
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
-    if (decomp) {
-      // T in float; the perturbation p - psv in FP64 before rounding
-      rdv[m] = decomp_r<NM>(th, wc);
-      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)rdv[m]);
-      q[IPR][m] = (float)(wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i]);
-    }
+    if (decomp)  // T in float; the perturbation p - psv in FP64 before rounding
+      rdv[m] = decomp32<NM>(g, th, w, g.at(0, k, j, 0), psv, pc, i - 3 + m, wc,
+                            q[0][m], q[IPR][m]);
   }
   float wl[N], wr[N];
 #pragma unroll
@@ -388,8 +415,8 @@ __global__ void __launch_bounds__(256)
     k_divergence32(KGrid g, KThermo th, KSource src,
                    const float* __restrict__ f1, const float* __restrict__ f2,
                    const float* __restrict__ f3, const double* __restrict__ psf,
-                   int decomp, const double* __restrict__ w,
-                   double* __restrict__ du) {
+                   const double* __restrict__ psv, int decomp,
+                   const double* __restrict__ w, double* __restrict__ du) {
   constexpr int N = 5 + NM, IVX = 1 + NM;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
@@ -418,7 +445,7 @@ __global__ void __launch_bounds__(256)
               A3 * (double)f3[g.f[2].at(n, k, j, i)];
     d[n] = 0.0 - wv * dflx;
   }
-  add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+  add_sources_cell<NM>(g, th, src, w, decomp ? psv : nullptr, k, j, i, d);
 #pragma unroll
   for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 }
@@ -582,12 +609,9 @@ __global__ void __launch_bounds__(256)
     for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i)];
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
-    if (decomp) {
-      double r = decomp_r<NM>(th, wc);
-      rdv[m] = (float)r;
-      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)r);
-      q[IPR][m] = (float)(wc[IPR] - psv[pc + i * g.ps_i]);
-    }
+    if (decomp)
+      rdv[m] = (float)decomp32<NM>(g, th, w, g.at(0, k, j, 0), psv, pc, i, wc,
+                                   q[0][m], q[IPR][m]);
   };
   // face f (between cells f-1 and f) uses window cells f-3 .. f+2
   auto face = [&](int f, float* fx) {
@@ -635,7 +659,7 @@ __global__ void __launch_bounds__(256)
                       (double)f3[g.f[2].at(n, k, j, i)]);
       d[n] = 0.0 - wv * dflx;
     }
-    add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+    add_sources_cell<NM>(g, th, src, w, decomp ? psv : nullptr, k, j, i, d);
 #pragma unroll
     for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 #pragma unroll
@@ -719,12 +743,9 @@ __device__ __forceinline__ void mix_fused_tile(
     for (int n = 0; n < N; ++n) wc[n] = w[colb + n * g.c.n + (long long)i * g.c.i];
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
-    if (decomp) {
-      double r = decomp_r<NM>(th, wc);
-      rdv[m] = (float)r;
-      q[0][m] = fdiv((float)wc[IPR], (float)wc[0] * (float)r);
-      q[IPR][m] = (float)(wc[IPR] - psv[pc + i * g.ps_i]);
-    }
+    if (decomp)
+      rdv[m] = (float)decomp32<NM>(g, th, w, colb, psv, pc, i, wc, q[0][m],
+                                   q[IPR][m]);
   };
   auto face1 = [&](int f, float* fx) {
     float wl[N], wr[N];
@@ -834,10 +855,15 @@ __device__ __forceinline__ void mix_fused_tile(
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][HI] = (float)wn[n];
     if (decomp) {
-      double r = decomp_r<NM>(th, wn);
-      rdv[HI] = (float)r;
-      q[0][HI] = fdiv((float)wn[IPR], (float)wn[0] * (float)r);
-      q[IPR][HI] = (float)(wn[IPR] - psvn);
+      if (decomp_cell(g, inew) != inew) {  // outflow ghost
+        rdv[HI] = (float)decomp32<NM>(g, th, w, colb, psv, pc, inew, wn,
+                                      q[0][HI], q[IPR][HI]);
+      } else {
+        double r = decomp_r<NM>(th, wn);
+        rdv[HI] = (float)r;
+        q[0][HI] = fdiv((float)wn[IPR], (float)wn[0] * (float)r);
+        q[IPR][HI] = (float)(wn[IPR] - psvn);
+      }
     }
     float fp[N];
     face1(i + 1, fp);
@@ -855,7 +881,7 @@ __device__ __forceinline__ void mix_fused_tile(
         dflx += A3 * ((double)L.fx[s3][f3 + FT_J] - (double)L.fx[s3][f3]);
         d[n] = 0.0 - wv * dflx;
       }
-      add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+      add_sources_cell<NM>(g, th, src, w, decomp ? psv : nullptr, k, j, i, d);
 #pragma unroll
       for (int n = 0; n < N; ++n)
         __builtin_nontemporal_store(
@@ -949,7 +975,7 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
     }
     TIMED(tm, KID_DIVERGENCE,
           (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
-              g, th, src, f1, f2, f3, psf, decomp, w, du)));
+              g, th, src, f1, f2, f3, psf, psv, decomp, w, du)));
     return;
   }
   if (g.f2) {
@@ -983,7 +1009,7 @@ void launch_fluxes32(const KGrid& g, const KThermo& th, const cg_config& cfg,
                                  g, th, cfg.rsolver, decomp, w, psf, psv, f1)));
     TIMED(tm, KID_DIVERGENCE,
           (k_divergence32<NM><<<(unsigned)((ncell + bs - 1) / bs), bs, 0, s>>>(
-              g, th, src, f1, f2, f3, psf, decomp, w, du)));
+              g, th, src, f1, f2, f3, psf, psv, decomp, w, du)));
     return;
   }
   const long long nx1t = ncol * ((g.nx1 + seg - 1) / seg);

@@ -29,6 +29,34 @@ CG_HD double log_mean(double lo, double hi) {
   return (lo - hi) / log_d(lo / hi);
 }
 
+// Cell whose temperature and pressure perturbation the decomposition gives
+// cell i of a column: a ghost beyond an outflow x1 side takes those of the
+// boundary cell (change_to_temperature.cpp:113-127), every other cell its
+// own.
+CG_HD int decomp_cell(const KGrid& g, int i) {
+  if (i < g.is && g.bc[0] == 2) return g.is;
+  if (i > g.ie && g.bc[1] == 2) return g.ie;
+  return i;
+}
+
+// An outflow x1 ghost as the reference's stage reads it after
+// CalculateFluxes: RestoreFromTemperature (:150-153) puts the boundary
+// cell's T and p' back over the ghost's own hydrostatic pressure,
+//   p_g = (p_b - psv_b) + psv_g,  rho_g = p_g / (T_b R_g),
+//   T_b = p_b / (rho_b R_b),
+// and those values stay in w until the next boundary fill, so the sources
+// and the implicit solve of the stage see them.  wb: boundary cell, wg: the
+// ghost (density and pressure replaced in place).
+template <int NM>
+CG_HD void outflow_ghost_w(const KThermo& th, const double* wb, double psv_b,
+                           double psv_g, double* wg) {
+  constexpr int IPR = 4 + NM;
+  const double Tb = wb[IPR] / (wb[0] * decomp_r<NM>(th, wb));
+  const double pg = (wb[IPR] - psv_b) + psv_g;
+  wg[IPR] = pg;
+  wg[0] = pg / (Tb * decomp_r<NM>(th, wg));
+}
+
 // ---------------------------------------------------------------------------
 // K2: hydrostatic face pressure psf and log-mean cell pressure psv per
 // column (Decomposition::ChangeToTemperature, :40-111).  One column per
@@ -103,9 +131,18 @@ __global__ void __launch_bounds__(256)
     for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i - 3 + m)];
     if (decomp) {
       rdv[m] = decomp_r<NM>(th, wc);
-      double T = wc[IPR] / (wc[0] * rdv[m]);
-      wc[0] = T;
-      wc[IPR] = wc[IPR] - psv[pc + (i - 3 + m) * g.ps_i];
+      const int ib = decomp_cell(g, i - 3 + m);
+      double pb = wc[IPR], db = wc[0], rb = rdv[m];
+      if (ib != i - 3 + m) {  // outflow ghost: the boundary cell's T, p'
+        double wb[N];
+#pragma unroll
+        for (int n = 0; n < N; ++n) wb[n] = w[g.at(n, k, j, ib)];
+        pb = wb[IPR];
+        db = wb[0];
+        rb = decomp_r<NM>(th, wb);
+      }
+      wc[0] = pb / (db * rb);
+      wc[IPR] = pb - psv[pc + ib * g.ps_i];
     }
 #pragma unroll
     for (int n = 0; n < N; ++n) q[n][m] = wc[n];
@@ -232,11 +269,14 @@ struct KSource {
 // Source terms of one interior cell added to d[] in HydroSourceTerms order:
 // constant acceleration (upstream form, see patches/25.constant_acc.patch
 // context lines), the straka diffusion (examples/2019-Li-snap/straka.cpp:
-// 126-190), the bottom heating.
+// 126-190), the bottom heating.  psv: the stage's cell pressures (null
+// without gravity); the diffusion then reads ghosts beyond an outflow x1
+// side as CalculateFluxes left them (outflow_ghost_w).
 template <int NM>
 __device__ inline void add_sources_cell(const KGrid& g, const KThermo& th,
                                         const KSource& src,
-                                        const double* __restrict__ w, int k,
+                                        const double* __restrict__ w,
+                                        const double* __restrict__ psv, int k,
                                         int j, int i, double* d) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IVY = IVX + 1, IEN = 4 + NM;
   const long long c0 = g.at(0, k, j, i);
@@ -252,6 +292,15 @@ __device__ inline void add_sources_cell(const KGrid& g, const KThermo& th,
       double wc[N];
 #pragma unroll
       for (int n = 0; n < N; ++n) wc[n] = W(n, jj, ii);
+      const int ib = decomp_cell(g, ii);
+      if (psv && g.grav != 0.0 && ib != ii) {
+        double wb[N];
+#pragma unroll
+        for (int n = 0; n < N; ++n) wb[n] = W(n, jj, ib);
+        const long long pc = g.pcol(k, jj) * g.ps_c;
+        outflow_ghost_w<NM>(th, wb, psv[pc + ib * g.ps_i],
+                            psv[pc + ii * g.ps_i], wc);
+      }
       double T = temperature_d<NM>(th, wc);
       if (tmp) *tmp = T;
       return T * pow_d(src.p0 / wc[4 + NM], src.chi);
@@ -280,7 +329,7 @@ __global__ void __launch_bounds__(256)
     k_divergence(KGrid g, KThermo th, KSource src,
                  const double* __restrict__ f1, const double* __restrict__ f2,
                  const double* __restrict__ f3, const double* __restrict__ w,
-                 double* __restrict__ du) {
+                 const double* __restrict__ psv, double* __restrict__ du) {
   constexpr int N = 5 + NM;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return;
@@ -302,7 +351,7 @@ __global__ void __launch_bounds__(256)
       dflx += A3 * f3[g.f[2].at(n, k + 1, j, i)] - A3 * f3[g.f[2].at(n, k, j, i)];
     d[n] = 0.0 - wght * dflx / vol;
   }
-  add_sources_cell<NM>(g, th, src, w, k, j, i, d);
+  add_sources_cell<NM>(g, th, src, w, psv, k, j, i, d);
 #pragma unroll
   for (int n = 0; n < N; ++n) du[g.at(n, k, j, i)] = d[n];
 }
@@ -595,14 +644,19 @@ __device__ inline void abs_roe_fma(const KThermo& th, const double* prim,
 // [M | c_i | rhs] with partial pivoting (the reference forms M^-1 and then
 // multiplies; the two agree to round-off).
 // One column: c0 = offset of (n = 0, k, j, i = 0) in w / du; the archive of
-// level lvl, entry e goes to arch[(lvl (B*B+B) + e) as + ao].  Returns false
-// on a zero pivot.
+// level lvl, entry e goes to arch[(lvl (B*B+B) + e) as + ao].  psvr (runs
+// with gravity, else null) and pc: the stage's cell pressures psv of the
+// column, with which ghosts beyond an outflow x1 side are read as the
+// reference's CalculateFluxes left them (outflow_ghost_w).  Returns false on
+// a zero pivot.
 template <int NM, int B>
 __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
                                            double dt, const double* __restrict__ w,
                                            const double* __restrict__ du,
                                            long long c0, double* __restrict__ arch,
-                                           long long as, long long ao) {
+                                           long long as, long long ao,
+                                           const double* __restrict__ psvr = nullptr,
+                                           long long pc = 0) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   constexpr int AS = B * B + B;
   const int is = g.is, ie = g.ie;
@@ -622,6 +676,10 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
   double wm[N], wc[N], wp[N], prim[N];
   load(wm, is - 1);
   load(wc, is);
+  const bool rw = psvr != nullptr && grav != 0.0;
+  if (rw && g.bc[0] == 2)
+    outflow_ghost_w<NM>(th, wc, psvr[pc + is * g.ps_i],
+                        psvr[pc + (is - 1) * g.ps_i], wm);
   Mt<B> Am, Ap, dm, dc, dp;
   roe_average<NM>(prim, gm, wm, wc);
   abs_roe_fma<NM, B>(th, prim, gm, Am);
@@ -650,6 +708,9 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
       load(wn, i + 2);
       load_du(dun, i + 1);
     }
+    if (i == ie && rw && g.bc[1] == 2)
+      outflow_ghost_w<NM>(th, wc, psvr[pc + ie * g.ps_i],
+                          psvr[pc + (ie + 1) * g.ps_i], wp);
     roe_average<NM>(prim, gm, wc, wp);
     abs_roe_fma<NM, B>(th, prim, gm, Ap);
     flux_jac<NM, B>(gm1, wp, dp);
@@ -1219,7 +1280,8 @@ __global__ void __launch_bounds__(64 * WPB)
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       if (!fwd_column<NM, B>(g, th, dt, gptr(bp[b].w), gptr(bp[b].du),
                              g.at(0, k, j, 0), arch + aoff,
-                             (long long)g.nx2 * g.nx3, col))
+                             (long long)g.nx2 * g.nx3, col, gptr(bp[b].psv),
+                             col * g.ps_c))
         atomicOr(status, 1);
     }
   }
@@ -1277,7 +1339,7 @@ __global__ void __launch_bounds__(64 * WPB)
       const KBlockPtr& p = bp[b];
       const long long ncol = (long long)g.nx2 * g.nx3;
       if (!fwd_column<NM, B>(g, th, dt, gptr(p.w), gptr(p.du), g.at(0, k, j, 0),
-                             arch + aoff, ncol, col))
+                             arch + aoff, ncol, col, gptr(p.psv), col * g.ps_c))
         atomicOr(status, 1);
       bwd_update_column<NM, B>(g, th, up, arch + aoff, ncol, col, gptr(p.du),
                                gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),

@@ -188,7 +188,8 @@ int run_stage(cg_run* r, int s) {
                         tm, !imp, bs);
       TIMED(tm, KID_DIVERGENCE,
             (k_divergence<NM><<<nblk(ncell, bs), bs, 0, st>>>(
-                g, r->kt, src, b.f1, b.f2, b.f3, b.w, b.du)));
+                g, r->kt, src, b.f1, b.f2, b.f3, b.w,
+                g.grav != 0.0 ? b.psv : nullptr, b.du)));
     }
   }
   if (tiled)
@@ -291,13 +292,23 @@ bool alloc_block(RunBlock& b, int N, const cg_config& cfg) {
     ok = hipMalloc((void**)&b.ff, sizeof(float) * (f1_doubles(g, N) + f2_doubles(g, N) +
                                                    f3_doubles(g, N))) == hipSuccess &&
          alloc(&b.pref, g.nc1);
+  // Every buffer starts as NaN (all bytes 0xFF), so that a kernel reading
+  // an entry no earlier kernel wrote spreads NaN into the state, where the
+  // step's checks and the tests see it, instead of a plausible zero.
+  auto poison = [&](void* p, size_t bytes) {
+    return !p || hipMemset(p, 0xFF, bytes) == hipSuccess;
+  };
   if (ok)
-    ok = hipMemset(b.u, 0, sizeof(double) * nc) == hipSuccess &&
-         hipMemset(b.w, 0, sizeof(double) * nc) == hipSuccess &&
-         hipMemset(b.u1, 0, sizeof(double) * nc) == hipSuccess &&
-         hipMemset(b.du, 0, sizeof(double) * nc) == hipSuccess &&
-         hipMemset(b.psf, 0, sizeof(double) * psf_doubles(g)) == hipSuccess &&
-         hipMemset(b.psv, 0, sizeof(double) * psf_doubles(g)) == hipSuccess;
+    ok = poison(b.u, sizeof(double) * nc) && poison(b.w, sizeof(double) * nc) &&
+         poison(b.u1, sizeof(double) * nc) && poison(b.du, sizeof(double) * nc) &&
+         poison(b.psf, sizeof(double) * psf_doubles(g)) &&
+         poison(b.psv, sizeof(double) * psf_doubles(g)) &&
+         poison(b.f1, sizeof(double) * f1_doubles(g, N)) &&
+         poison(b.f2, sizeof(double) * f2_doubles(g, N)) &&
+         poison(b.f3, sizeof(double) * f3_doubles(g, N)) &&
+         poison(b.ff, sizeof(float) * (f1_doubles(g, N) + f2_doubles(g, N) +
+                                       f3_doubles(g, N))) &&
+         poison(b.pref, sizeof(double) * g.nc1);
   return ok;
 }
 
@@ -583,7 +594,8 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
                                            rb.blk.bc[2 * d + 1] == CG_BC_PERIODIC));
       if (x1_lines) n = std::max(n, line_doubles(rb.g, 0, B, true));
     }
-    ok = hipMalloc((void**)&r->lines, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess;
+    ok = hipMalloc((void**)&r->lines, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess &&
+         hipMemset(r->lines, 0xFF, sizeof(double) * std::max<size_t>(n, 1)) == hipSuccess;
   }
   if (ok && (cfg->implicit_flag & 1) && !r->x1_periodic) {
     std::vector<KGroup> groups;
@@ -591,6 +603,8 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
     build_groups(r->blocks, r->AS, groups, arch_n);
     r->ngroups = (int)groups.size();
     ok = hipMalloc((void**)&r->arch, sizeof(double) * std::max<size_t>(arch_n, 1)) ==
+             hipSuccess &&
+         hipMemset(r->arch, 0xFF, sizeof(double) * std::max<size_t>(arch_n, 1)) ==
              hipSuccess &&
          hipMalloc((void**)&r->d_grp, sizeof(KGroup) * std::max(r->ngroups, 1)) ==
              hipSuccess &&
@@ -646,6 +660,9 @@ int cg_run_block_set_prim_host(cg_run* r, int b, const double* w_host) {
   size_t nc = cell_doubles(rb.g, r->N);
   HIPCHK(ctx, hipMemcpyAsync(rb.du, w_host, sizeof(double) * nc, hipMemcpyHostToDevice,
                              ctx->stream));
+  // the step never reads the conserved ghosts (only interior u is set from
+  // w below); they are handed back as zeros
+  HIPCHK(ctx, hipMemsetAsync(rb.u, 0, sizeof(double) * nc, ctx->stream));
   k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(rb.ga, rb.g, r->N, rb.du,
                                                                rb.w);
   if (r->N == 5)

@@ -110,6 +110,12 @@ int cg_memcpy_h2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Device-to-device copy through a streaming kernel (the copy used to
  * calibrate HBM traffic counters). */
 int cg_memcpy_d2d(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Event timing on the context's stream (no reference counterpart): start
+ * records an event, stop records a second one, waits for it and returns the
+ * device time between the two in milliseconds; the work enqueued between the
+ * calls is what is timed, without host launch or copy overheads. */
+int cg_timer_start(cg_ctx* ctx);
+int cg_timer_stop(cg_ctx* ctx, double* ms);
 int cg_memcpy_d2h(cg_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- Riemann plugin ----------------------------------------------------- *
@@ -130,7 +136,12 @@ int cg_riemann(cg_ctx* ctx, const cg_thermo* th, int solver, int ivx,
  * src/snap/decomposition/change_to_temperature.cpp:40-167) around the x1
  * reconstruction, then the Riemann solver along x1, x2, x3.  `w` must have
  * its ghost cells filled.  Pure: `w` is not modified (the reference mutates
- * and restores it).  Fluxes are computed on every face of interior cells. */
+ * and restores it, which leaves the ghosts of an outflow x1 side rewritten
+ * with the boundary cell's temperature and pressure perturbation,
+ * change_to_temperature.cpp:113-127 and :150-153; runs apply that rewrite
+ * where the stage reads those ghosts).  Fluxes are computed on every face
+ * of interior cells.  flux2 / flux3 are not read or written for a direction
+ * without cells (nx2 = 1 / nx3 = 1) and may then be null. */
 int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                         const cg_config* cfg, const double* w, double* flux1,
                         double* flux2, double* flux3);

@@ -386,6 +386,41 @@ void decompose_column(const Grid& g, const cg_block& blk, const cg_thermo& th,
   }
 }
 
+// The x1 ghosts of w that the reference's CalculateFluxes leaves behind on
+// an outflow side.  ChangeToTemperature gives those ghosts T and p' of the
+// boundary cell b (change_to_temperature.cpp:113-127) and
+// RestoreFromTemperature (:150-153) puts them back over the ghost's own
+// hydrostatic pressure, so until the next boundary fill the stage's sources
+// and implicit solve read
+//   p_g = (p_b - psv_b) + psv_g,  rho_g = p_g / (T_b R_g),
+//   T_b = p_b / (rho_b R_b).
+// (On every other cell the restore undoes the change up to rounding; the
+// oracle's pure flux sweep leaves those cells alone.)  Interior (k, j)
+// rows, the ones later readers use.
+void restore_outflow_ghosts(const Grid& g, const cg_block& blk,
+                            const cg_thermo& th, double* w) {
+  if (blk.grav == 0. ||
+      (blk.bc[0] != CG_BC_OUTFLOW && blk.bc[1] != CG_BC_OUTFLOW))
+    return;
+  const int IPR = g.IPR;
+  std::vector<double> psf(g.nc1 + 1), psv(g.nc1), rdv(g.nc1);
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j) {
+      decompose_column(g, blk, th, w, k, j, psf.data(), psv.data(), rdv.data());
+      auto side = [&](int b, int lo, int hi) {
+        double Tb = w[g.at(IPR, k, j, b)] / (w[g.at(0, k, j, b)] * rdv[b]);
+        double pb = w[g.at(IPR, k, j, b)] - psv[b];
+        for (int i = lo; i <= hi; ++i) {
+          double pg = pb + psv[i];
+          w[g.at(IPR, k, j, i)] = pg;
+          w[g.at(0, k, j, i)] = pg / (Tb * rdv[i]);
+        }
+      };
+      if (blk.bc[0] == CG_BC_OUTFLOW) side(g.is, g.is - NG, g.is - 1);
+      if (blk.bc[1] == CG_BC_OUTFLOW) side(g.ie, g.ie + 1, g.ie + NG);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch:19-185), pure.
 void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
@@ -409,6 +444,18 @@ void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
           col[0 * nc1 + i] = col[IPR * nc1 + i] / (col[0 * nc1 + i] * rdv[i]);
           col[IPR * nc1 + i] -= psv[i];
         }
+        // outflow sides: the ghosts take T and p' of the boundary cell
+        // (:113-127)
+        if (blk.bc[0] == CG_BC_OUTFLOW)
+          for (int i = g.is - NG; i < g.is; ++i) {
+            col[0 * nc1 + i] = col[0 * nc1 + g.is];
+            col[IPR * nc1 + i] = col[IPR * nc1 + g.is];
+          }
+        if (blk.bc[1] == CG_BC_OUTFLOW)
+          for (int i = g.ie + 1; i <= g.ie + NG; ++i) {
+            col[0 * nc1 + i] = col[0 * nc1 + g.ie];
+            col[IPR * nc1 + i] = col[IPR * nc1 + g.ie];
+          }
       }
       for (int n = 0; n < N; ++n)
         recon_line(cfg.order_v, &col[n * nc1], 1, g.is - 1, g.ie + 1,
@@ -1518,6 +1565,7 @@ int step(const cg_block& blk, const cg_thermo& th, const cg_config& cfg,
     std::fill(u1.begin(), u1.end(), 0.0);
     for (int s = 0; s < 3; ++s) {
       calculate_fluxes(g, blk, th, cfg, w, f1.data(), f2.data(), f3.data());
+      restore_outflow_ghosts(g, blk, th, w);
       for (int n = 0; n < g.N; ++n)
         for (int k = g.ks; k <= g.ke; ++k)
           for (int j = g.js; j <= g.je; ++j)
@@ -1637,6 +1685,13 @@ int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
                         double* flux2, double* flux3) {
   Grid g(*blk);
   calculate_fluxes(g, *blk, *th, *cfg, w, flux1, flux2, flux3);
+  return CG_OK;
+}
+
+int or_restore_outflow_ghosts(const cg_block* blk, const cg_thermo* th,
+                              double* w) {
+  Grid g(*blk);
+  restore_outflow_ghosts(g, *blk, *th, w);
   return CG_OK;
 }
 

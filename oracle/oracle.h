@@ -44,6 +44,8 @@ int or_decompose_column(const cg_block* blk, const cg_thermo* th,
 int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
                         const cg_config* cfg, const double* w, double* flux1,
                         double* flux2, double* flux3);
+int or_restore_outflow_ghosts(const cg_block* blk, const cg_thermo* th,
+                              double* w);
 int or_add_flux_divergence(const cg_block* blk, double wght,
                            const double* flux1, const double* flux2,
                            const double* flux3, double* du);

@@ -25,6 +25,7 @@ _SIGS = {
                         ctypes.c_ssize_t]),
     "or_decompose_column": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "or_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "or_restore_outflow_ghosts": (_I, [_P, _P, _P]),
     "or_add_flux_divergence": (_I, [_P, _D, _P, _P, _P, _P]),
     "or_add_sources": (_I, [_P, _P, _P, _D, _P, _P]),
     "or_implicit_solve": (_I, [_P, _P, _P, _P, _D, _I]),
@@ -104,6 +105,14 @@ def calculate_fluxes(blk, th, cfg, w):
     lib().or_calculate_fluxes(ref(blk), ref(th), ref(cfg), ptr(w), ptr(fs[0]),
                               ptr(fs[1]), ptr(fs[2]))
     return fs
+
+
+def restore_outflow_ghosts(blk, th, w):
+    """The x1 ghosts of w that the reference's CalculateFluxes leaves on an
+    outflow side (returns a copy of w)."""
+    w = np.array(w, dtype=np.float64, copy=True)
+    lib().or_restore_outflow_ghosts(ref(blk), ref(th), ptr(w))
+    return w
 
 
 def add_flux_divergence(blk, wght, fs, du):

@@ -132,3 +132,19 @@ def test_host_variants(ctx):
     mask = np.ones(blk.shape(), bool)
     mask[:, sk, sj, si] = False
     np.testing.assert_array_equal(got[mask], du[mask])
+
+
+def test_host_fluxes_2d_without_x3_array(ctx):
+    """A 2D block (config 1, nx3 = 1) has no x3 flux array: the host flux
+    sweep takes a null flux3 and equals the oracle on flux1 / flux2."""
+    L = ca.lib()
+    blk, th, cfg, w = perturbed_state(1, (24, 40, 1))
+    w = np.ascontiguousarray(w, np.float64)
+    f1 = np.zeros(blk.flux_shapes()[0])
+    f2 = np.zeros(blk.flux_shapes()[1])
+    _say("host calculate_fluxes 2D, flux3 = NULL")
+    assert L.cg_calculate_fluxes_host(ctx.h, ca._ref(blk), ca._ref(th), ca._ref(cfg),
+                                      ca._ptr(w), ca._ptr(f1), ca._ptr(f2),
+                                      None) == ca.OK
+    ref = orl.calculate_fluxes(blk, th, cfg, w)
+    assert relerr(f1, ref[0]) <= 1e-12 and relerr(f2, ref[1]) <= 1e-12
