@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libcanoe_gpu.so")
 NGHOST = 3
 MAX_NMASS = 3
 NUM_KERNEL_IDS = 18
+# bytes of the fixed device scratch of cg_implicit_solve_noalloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV, ERR_COLD, ERR_IO = range(11)
@@ -133,6 +135,7 @@ _SIGS = {
     "cg_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cg_add_flux_divergence": (_I, [_P, _P, _D, _P, _P, _P, _P]),
     "cg_implicit_solve": (_I, [_P, _P, _P, _P, _P, _D, _I]),
+    "cg_implicit_solve_noalloc": (_I, [_P, _P, _P, _P, _P, _D, _I]),
     "cg_riemann_host": (_I, [_P, _P, _I, _I, _I, _P, _P, ctypes.c_ssize_t, _P,
                              ctypes.c_ssize_t]),
     "cg_calculate_fluxes_host": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),

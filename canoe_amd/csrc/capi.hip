@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,37 @@
 
 using namespace cg;
 using namespace cg_host;
+
+// Fixed device scratch of cg_implicit_solve_noalloc: CG_NOALLOC_SCRATCH_BYTES
+// (64 MiB) per device, held from the library's first use on that device
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it only stands in for the symbol): cg_create looks up
+// the device address once per device and every launch takes that address.
+static __device__ __align__(256) double scratch[CG_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// on the first call for that device.
+hipError_t fixed_scratch_address(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (double*)ptr;
+    g_scratch_done[device] = true;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
 
 extern "C" {
 
@@ -37,7 +69,10 @@ int cg_create(int device, cg_ctx** out) {
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(int)) != hipSuccess ||
-      hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess) {
+      hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess ||
+      fixed_scratch_address(device, &c->fixed_scratch) != hipSuccess) {
+    if (c->d_status) (void)hipFree(c->d_status);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CG_ERR_HIP;
   }
@@ -199,8 +234,15 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
   return CG_OK;
 }
 
-int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
-                      double* du, const double* w, double dt, int flag) {
+}  // extern "C"
+
+namespace {
+// SolveImplicit3D on caller arrays.  The archive / line scratch is
+// `fixed` (fixed_doubles long) when given, else stream-ordered memory of
+// exactly the size needed, released on every return path.
+int implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th, double* du,
+                   const double* w, double dt, int flag, double* fixed,
+                   size_t fixed_doubles) {
   if (!ctx) return CG_ERR_ARG;
   int e = check_block(ctx, blk, th);
   if (e) return e;
@@ -221,26 +263,52 @@ int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
       need = std::max(need, (d == 0 && !per[0]) ? arch_doubles(g) : line_doubles(g, d, B, per[d]));
   if (need == 0) return CG_OK;  // only directions without cells were asked
   AsyncScratch scratch_s(ctx->stream);
-  HIPCHK(ctx, hipMallocAsync(&scratch_s.p, sizeof(double) * need, ctx->stream));
-  double* scratch = (double*)scratch_s.p;
+  double* lv = fixed;
+  if (fixed) {
+    if (need > fixed_doubles)
+      return set_err(ctx, CG_ERR_ARG,
+                     "implicit solve needs %zu bytes of scratch, the fixed scratch "
+                     "holds %zu",
+                     need * sizeof(double), fixed_doubles * sizeof(double));
+  } else {
+    HIPCHK(ctx, hipMallocAsync(&scratch_s.p, sizeof(double) * need, ctx->stream));
+    lv = (double*)scratch_s.p;
+  }
   for (int d = 2; d >= 0; --d) {
     if (!(flag & (1 << d)) || !act[d]) continue;
     if (d == 0 && !per[0]) {
       if (blk->nmass == 0)
-        launch_implicit<0>(g, kt, dt, flag, du, w, scratch, ctx->d_status, ctx->stream);
+        launch_implicit<0>(g, kt, dt, flag, du, w, lv, ctx->d_status, ctx->stream);
       else
-        launch_implicit<3>(g, kt, dt, flag, du, w, scratch, ctx->d_status, ctx->stream);
+        launch_implicit<3>(g, kt, dt, flag, du, w, lv, ctx->d_status, ctx->stream);
     } else if (blk->nmass == 0) {
-      launch_lines<0>(g, kt, dt, nullptr, flag, d, per[d], du, w, scratch, ctx->d_status,
+      launch_lines<0>(g, kt, dt, nullptr, flag, d, per[d], du, w, lv, ctx->d_status,
                       ctx->stream);
     } else {
-      launch_lines<3>(g, kt, dt, nullptr, flag, d, per[d], du, w, scratch, ctx->d_status,
+      launch_lines<3>(g, kt, dt, nullptr, flag, d, per[d], du, w, lv, ctx->d_status,
                       ctx->stream);
     }
     HIPCHK(ctx, hipGetLastError());
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_status(ctx);
+}
+}  // namespace
+
+extern "C" {
+
+int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                      double* du, const double* w, double dt, int flag) {
+  return implicit_solve(ctx, blk, th, du, w, dt, flag, nullptr, 0);
+}
+
+int cg_implicit_solve_noalloc(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                              double* du, const double* w, double dt, int flag) {
+  if (!ctx) return CG_ERR_ARG;
+  if (!ctx->fixed_scratch)
+    return set_err(ctx, CG_ERR_HIP, "fixed scratch of this device not available");
+  return implicit_solve(ctx, blk, th, du, w, dt, flag, ctx->fixed_scratch,
+                        CG_NOALLOC_SCRATCH_BYTES / sizeof(double));
 }
 
 // ---- host-array variants ---------------------------------------------------

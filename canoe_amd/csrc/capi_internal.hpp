@@ -17,6 +17,9 @@ struct cg_ctx {
   hipStream_t stream = nullptr;
   int* d_status = nullptr;  // device error flags (bit 0 pivot, bit 1 Newton)
   hipEvent_t t0 = nullptr, t1 = nullptr;  // cg_timer_start / cg_timer_stop
+  // Device address of this device's fixed scratch array (the implicit
+  // solve's entry point that allocates nothing); null if the lookup failed.
+  double* fixed_scratch = nullptr;
   char err[512] = {0};
 };
 

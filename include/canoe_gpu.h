@@ -170,6 +170,23 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
 int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                       double* du, const double* w, double dt, int flag);
 
+/* No counterpart in the reference.  cg_implicit_solve without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * archive / line scratch is a fixed array of CG_NOALLOC_SCRATCH_BYTES per
+ * device, held from the first cg_create on that device until the process
+ * ends.  A solve that needs more scratch (30 doubles per cell for x1
+ * columns between walls; 30 or 105 doubles per cell of the largest line
+ * direction otherwise, e.g. 64 MiB = 279,620 cells of 5x5 x1 columns) is
+ * rejected with CG_ERR_ARG before any launch.  Every call on one device
+ * shares that array, which keeps the last call's contents (nothing in it
+ * needs to start at zero: each level is written before it is read): calls
+ * that overlap on one device, from any contexts or threads, race.  Results
+ * equal cg_implicit_solve's bit for bit. */
+#define CG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int cg_implicit_solve_noalloc(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                              double* du, const double* w, double dt, int flag);
+
 /* ---- host-array variants ------------------------------------------------ *
  * The same calls on host arrays of the same layouts, for a caller that keeps
  * its AthenaArrays in host memory (the reference's own buffers: wl_ / wr_

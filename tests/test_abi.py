@@ -41,6 +41,8 @@ def test_python_constants_match_the_header():
     npar = int(re.search(r"#define CG_MLP_NPARAM (\d+)", src).group(1))
     assert ca.NUM_KERNEL_IDS == nk
     assert cm.NPARAM == npar
+    m = re.search(r"#define CG_NOALLOC_SCRATCH_BYTES \(\(size_t\)(\d+) << (\d+)\)", src)
+    assert ca.NOALLOC_SCRATCH_BYTES == int(m.group(1)) << int(m.group(2))
     assert ctypes.sizeof(ca.Tune) == 6 * 4 + 8  # 6 ints, double
     codes = dict((m.group(1), int(m.group(2)))
                  for m in re.finditer(r"CG_(OK|ERR_[A-Z]+) = (\d+)", src))
