@@ -104,6 +104,8 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int), ("satadj", ctypes.c_int),
         ("visc", ctypes.c_double), ("p0", ctypes.c_double),
         ("heat_flux", ctypes.c_double),
+        ("k_auto", ctypes.c_double), ("k_accr", ctypes.c_double),
+        ("k_evap", ctypes.c_double), ("v_fall", ctypes.c_double),
     ]
 
 
@@ -141,6 +143,7 @@ _SIGS = {
     "cg_calculate_fluxes_host": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cg_implicit_solve_host": (_I, [_P, _P, _P, _P, _P, _D, _I]),
     "cg_cons_to_prim": (_I, [_P, _P, _P, _P, _P]),
+    "cg_microphysics": (_I, [_P, _P, _P, _P, _D, _P, _P]),
     "cg_prim_to_cons": (_I, [_P, _P, _P, _P, _P]),
     "cg_apply_bc": (_I, [_P, _P, _P]),
     "cg_new_block_dt": (_I, [_P, _P, _P, _P, _P, _D, ctypes.POINTER(_D)]),

@@ -399,6 +399,22 @@ int cg_implicit_solve_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th
   return e;
 }
 
+int cg_microphysics(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const cg_config* cfg, double dt, double* u, double* w) {
+  if (!ctx || !cfg) return CG_ERR_ARG;
+  int e = check_block(ctx, blk, th);
+  if (e) return e;
+  if (!(dt >= 0.0)) return set_err(ctx, CG_ERR_ARG, "dt must be >= 0");
+  KKessler kz;
+  e = make_kessler(ctx, *cfg, *th, &kz);
+  if (e) return e;
+  KGrid g = make_grid(*blk, 0);
+  launch_kessler<3>(g, make_thermo(*th), kz, dt, nullptr, u, w, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CG_OK;
+}
+
 int cg_cons_to_prim(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                     const double* u, double* w) {
   if (!ctx) return CG_ERR_ARG;

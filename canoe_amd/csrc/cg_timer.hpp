@@ -9,14 +9,14 @@
 enum KernelId {
   KID_SCAN = 0, KID_FLUX_X1, KID_FLUX_X2, KID_FLUX_X3, KID_DIVERGENCE,
   KID_IMPLICIT_FWD, KID_IMPLICIT_BWD, KID_UPDATE, KID_BC_X1, KID_BC_X2,
-  KID_BC_X3, KID_FLUX32, KID_MIRROR32, KID_SATADJ, KID_NEW_DT, KID_IMPLICIT_FUSED, KID_MIX_FUSED, KID_LINES, KID_COUNT
+  KID_BC_X3, KID_FLUX32, KID_KESSLER, KID_SATADJ, KID_NEW_DT, KID_IMPLICIT_FUSED, KID_MIX_FUSED, KID_LINES, KID_COUNT
 };
 static const char* kKernelNames[KID_COUNT] = {
     "k_hydrostatic_scan", "k_flux_x1",        "k_flux_h<x2>",
     "k_flux_h<x3>",       "k_divergence",     "k_implicit_fwd",
     "k_implicit_bwd",     "k_update",         "k_bc_x1",
     "k_bc_x2",            "k_bc_x3",          "k_flux32",
-    "k_mirror32",         "k_satadj_run_mlp", "k_new_dt",
+    "k_kessler",          "k_satadj_run_mlp", "k_new_dt",
     "k_implicit_fused",   "k_mix_fused",      "k_line_*"};
 
 struct KTimer {

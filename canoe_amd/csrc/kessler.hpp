@@ -1,0 +1,125 @@
+// kessler.hpp -- Kessler warm-rain microphysics and sedimentation of the
+// precipitation tracer, one column per lane, swept top down.
+//
+// No counterpart in the reference's live stage (its sedimentation hook is
+// commented out at src/tasklist/implicit_hydro_tasks.cpp:292; the rates live
+// in kintera, which is not vendored).  Reactions and rate constants:
+// examples/2019-Li-snap/earth-precip.yaml:44-53, water-kessler.yaml:4-18;
+// the sedimentation mass flux (velocity times upwind density) of
+// src/snap/riemann/noflux.cpp:61-71.  Per cell, with the full step dt:
+//   a = min(rho_c, dt (k_auto + k_accr q_p) rho_c)        cloud -> precip
+//   b = min(rho_p, rho_s - rho_v, dt k_evap rho_p (1 - rho_v / rho_s))
+//                                  precip -> vapour while sub-saturated,
+// E -= L b (the moist internal energy ie - rho L (q_c + q_p) is kept), then
+// a fraction c = min(1, dt v_fall / dx1) of the cell's precipitation falls
+// into the cell below with the cell's velocity and its own sensible energy
+// cv_p T + |v|^2 / 2; nothing crosses the walls.
+#pragma once
+#include "cg_physics.hpp"
+#include "kernels.hpp"
+
+namespace cg {
+
+struct KKessler {
+  double k_auto, k_accr, k_evap, v_fall;
+  int ip;  // 0-based species of the precipitation
+};
+
+// One column (k, j): u (conserved) in / out, w (primitive) out, interior
+// cells only.  dt is the full step.
+template <int NM>
+__device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th,
+                                               const KKessler& kz, double dt,
+                                               double* __restrict__ u,
+                                               double* __restrict__ w, int k, int j) {
+  constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
+  const int rv_ = 1 + th.iv, rc_ = 1 + th.ic, rp_ = 1 + kz.ip;
+  const double cvp = th.cvd * (1. + th.cv_ratio_m1[kz.ip]);
+  double cf = dt * kz.v_fall / g.dx1;
+  if (cf > 1.) cf = 1.;
+  const long long sn = g.c.n;
+  double in_p = 0., in_m0 = 0., in_m1 = 0., in_m2 = 0., in_e = 0.;
+  for (int i = g.ie; i >= g.is; --i) {
+    const long long c0 = g.at(0, k, j, i);
+    double uc[N], wc[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) uc[n] = u[c0 + n * sn];
+    double rho = uc[0];
+#pragma unroll
+    for (int n = 1; n <= NM; ++n) rho += uc[n];
+    cons_to_prim_cell<NM>(th, uc, wc);
+    const double T = temperature_d<NM>(th, wc);
+    // species picked by runtime index from registers: select chains
+    double rc = 0., rp = 0., rv = 0.;
+#pragma unroll
+    for (int n = 1; n <= NM; ++n) {
+      if (n == rc_) rc = uc[n];
+      if (n == rp_) rp = uc[n];
+      if (n == rv_) rv = uc[n];
+    }
+    double a = dt * (kz.k_auto + kz.k_accr * (rp / rho)) * rc;
+    if (a > rc) a = rc;
+    rc = rc - a;
+    rp = rp + a;
+    const double rs = qsat_d(th, T, rho) * rho;
+    const double def = rs - rv;
+    double b = 0.;
+    if (def > 0. && rp > 0.) {
+      b = dt * kz.k_evap * rp * (def / rs);
+      if (b > rp) b = rp;
+      if (b > def) b = def;
+    }
+    rp = rp - b;
+    rv = rv + b;
+#pragma unroll
+    for (int n = 1; n <= NM; ++n) {
+      if (n == rc_) uc[n] = rc;
+      if (n == rp_) uc[n] = rp;
+      if (n == rv_) uc[n] = rv;
+    }
+    uc[IEN] = uc[IEN] - th.L * b;
+    double out_p = 0., out_m0 = 0., out_m1 = 0., out_m2 = 0., out_e = 0.;
+    if (i > g.is && cf > 0.) {
+      cons_to_prim_cell<NM>(th, uc, wc);
+      const double T2 = temperature_d<NM>(th, wc);
+      const double v1 = wc[IVX], v2 = wc[IVX + 1], v3 = wc[IVX + 2];
+      out_p = cf * rp;
+      out_m0 = out_p * v1;
+      out_m1 = out_p * v2;
+      out_m2 = out_p * v3;
+      out_e = out_p * (cvp * T2 + 0.5 * (v1 * v1 + v2 * v2 + v3 * v3));
+    }
+#pragma unroll
+    for (int n = 1; n <= NM; ++n)
+      if (n == rp_) uc[n] = (uc[n] - out_p) + in_p;
+    uc[IVX] = (uc[IVX] - out_m0) + in_m0;
+    uc[IVX + 1] = (uc[IVX + 1] - out_m1) + in_m1;
+    uc[IVX + 2] = (uc[IVX + 2] - out_m2) + in_m2;
+    uc[IEN] = (uc[IEN] - out_e) + in_e;
+    cons_to_prim_cell<NM>(th, uc, wc);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      u[c0 + n * sn] = uc[n];
+      w[c0 + n * sn] = wc[n];
+    }
+    in_p = out_p;
+    in_m0 = out_m0;
+    in_m1 = out_m1;
+    in_m2 = out_m2;
+    in_e = out_e;
+  }
+}
+
+// Every column of a block; the step is dt, or dt * dtp[0] when dtp is given.
+template <int NM>
+__global__ void __launch_bounds__(64)
+    k_kessler(KGrid g, KThermo th, KKessler kz, double dt, const double* __restrict__ dtp,
+              double* __restrict__ u, double* __restrict__ w) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx2 * g.nx3) return;
+  const int j = g.js + (int)(t % g.nx2);
+  const int k = g.ks + (int)(t / g.nx2);
+  kessler_column<NM>(g, th, kz, dtp ? dt * dtp[0] : dt, u, w, k, j);
+}
+
+}  // namespace cg

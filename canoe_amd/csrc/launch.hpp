@@ -9,6 +9,7 @@
 #include "cg_timer.hpp"
 #include "flux32.hpp"
 #include "kernels.hpp"
+#include "kessler.hpp"
 #include "lines.hpp"
 
 namespace cg_host {
@@ -277,6 +278,42 @@ inline int check_implicit_flag(cg_ctx* ctx, const cg_block& b, int flag) {
     return set_err(ctx, CG_ERR_ARG,
                    "x2/x3 implicit lines with species need periodic ends");
   return CG_OK;
+}
+
+// Kessler microphysics of a configuration: on when any rate is non-zero;
+// the precipitation is the condensate that is not the adjusted cloud.
+inline bool kessler_on(const cg_config& c) {
+  return c.k_auto != 0. || c.k_accr != 0. || c.k_evap != 0. || c.v_fall != 0.;
+}
+
+inline int make_kessler(cg_ctx* ctx, const cg_config& c, const cg_thermo& th,
+                        KKessler* out) {
+  out->k_auto = c.k_auto;
+  out->k_accr = c.k_accr;
+  out->k_evap = c.k_evap;
+  out->v_fall = c.v_fall;
+  out->ip = -1;
+  if (!(c.k_auto >= 0. && c.k_accr >= 0. && c.k_evap >= 0. && c.v_fall >= 0.))
+    return set_err(ctx, CG_ERR_ARG, "Kessler rates and fall speed must be >= 0");
+  if (th.ivap > 0 && th.icld > 0)
+    for (int n = th.nvapor; n < th.nmass; ++n)
+      if (n != th.ivap - 1 && n != th.icld - 1) {
+        out->ip = n;
+        break;
+      }
+  if (out->ip < 0)
+    return set_err(ctx, CG_ERR_ARG,
+                   "Kessler microphysics needs vapour, cloud and a precipitation species");
+  return CG_OK;
+}
+
+template <int NM>
+void launch_kessler(const KGrid& g, const KThermo& th, const KKessler& kz, double dt,
+                    const double* dtp, double* u, double* w, hipStream_t s,
+                    KTimer* tm = nullptr) {
+  TIMED(tm, KID_KESSLER,
+        (k_kessler<NM><<<nblk((long long)g.nx2 * g.nx3, 64), 64, 0, s>>>(g, th, kz, dt, dtp,
+                                                                          u, w)));
 }
 
 }  // namespace cg_host

@@ -56,6 +56,8 @@ struct cg_run {
   int ncu = 256;
   double* part = nullptr;
   const cg_mlp* mlp = nullptr;  // network of the guided adjustment
+  bool kes = false;             // Kessler microphysics after the last stage
+  KKessler kz;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   // d_dt[0]: the step every stage kernel reads (a graph replays with any
@@ -211,14 +213,20 @@ int run_stage(cg_run* r, int s) {
     for (RunBlock& b : r->blocks)
       launch_lines<NM>(b.g, r->kt, sbeta[s], r->d_dt, r->cfg.implicit_flag, 0, true, b.du,
                        b.w, r->lines, r->ctx->d_status, st, tm);
+  // Kessler microphysics and sedimentation after the last stage's update
+  // and adjustment, with the full step (NM = 3 only: it needs the
+  // precipitation species)
+  const bool kes = (s == 2) && r->kes && NM == 3;
   if (imp) {
     launch_columns<NM>(r, sbeta[s], up, tm);
     for (RunBlock& b : r->blocks) {
       const KGrid& g = b.g;
-      if (guided) {
-        // adjusted cells change w: x1 ghosts and hydrostatic pressures again
+      if (guided)
         TIMED(tm, KID_SATADJ,
               (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
+      if (kes) launch_kessler<NM>(g, r->kt, r->kz, 1.0, r->d_dt, b.u, b.w, st, tm);
+      if (guided || kes) {
+        // changed cells change w: x1 ghosts and hydrostatic pressures again
         TIMED(tm, KID_BC_X1,
               (k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, st>>>(g, b.w)));
         if (g.grav != 0.0)
@@ -243,6 +251,7 @@ int run_stage(cg_run* r, int s) {
       if (guided)
         TIMED(tm, KID_SATADJ,
               (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
+      if (kes) launch_kessler<NM>(g, r->kt, r->kz, 1.0, r->d_dt, b.u, b.w, st, tm);
       launch_bc<NM>(g, b.w, st, tm);
     }
   }
@@ -273,6 +282,11 @@ int check_config(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
     return set_err(ctx, CG_ERR_ARG, "saturation adjustment needs a vapour/cloud pair");
   if (cfg->precision != CG_PREC_FP64 && cfg->precision != CG_PREC_FP32)
     return set_err(ctx, CG_ERR_ARG, "bad precision");
+  if (kessler_on(*cfg)) {
+    KKessler kz;
+    e = make_kessler(ctx, *cfg, *th, &kz);
+    if (e) return e;
+  }
   return CG_OK;
 }
 
@@ -529,6 +543,7 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   r->th = *th;
   r->cfg = *cfg;
   r->kt = make_thermo(*th);
+  r->kes = kessler_on(*cfg) && make_kessler(ctx, *cfg, *th, &r->kz) == CG_OK;
   r->N = 5 + th->nmass;
   r->AS = (cfg->implicit_flag & 8) ? 30 : 12;
   r->tune.implicit_wpb = 1;

@@ -91,6 +91,13 @@ typedef struct cg_config {
   double visc;        /* constant viscosity/diffusivity K, m^2/s (0 = off) */
   double p0;          /* reference pressure for potential temperature */
   double heat_flux;   /* W/m^2 deposited in the lowest interior layer */
+  /* Kessler warm-rain microphysics and precipitation sedimentation, applied
+   * after the last stage (all zero = off; needs vapour, cloud and a third
+   * condensate, the precipitation): rates of autoconversion cloud -> precip,
+   * accretion, evaporation of precip (1/s; examples/2019-Li-snap/
+   * earth-precip.yaml:44-53, water-kessler.yaml:4-18) and the fall speed of
+   * precipitation along -x1 (m/s). */
+  double k_auto, k_accr, k_evap, v_fall;
 } cg_config;
 
 typedef struct cg_ctx cg_ctx;
@@ -212,6 +219,16 @@ int cg_cons_to_prim(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                     const double* u, double* w);
 int cg_prim_to_cons(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                     const double* w, double* u);
+/* Kessler microphysics + precipitation sedimentation over the step dt on
+ * caller arrays: u (conserved, interior cells) updated in place, w
+ * (primitive) rewritten from it on interior cells.  No counterpart in the
+ * reference's live stage (the sedimentation hook of UpdateAllConserved is
+ * commented out, src/tasklist/implicit_hydro_tasks.cpp:292, and the
+ * reaction rates are kintera's, src/snap/riemann/noflux.cpp:61-71 gives the
+ * sedimentation mass flux); runs apply it after the last stage when a rate
+ * of cfg is non-zero.  CG_ERR_ARG without a precipitation species. */
+int cg_microphysics(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
+                    const cg_config* cfg, double dt, double* u, double* w);
 /* Fills the ghost cells of a primitive array (reflecting / periodic /
  * outflow), as the upstream BoundaryValues do after CONS2PRIM. */
 int cg_apply_bc(cg_ctx* ctx, const cg_block* blk, double* w);

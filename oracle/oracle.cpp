@@ -1543,6 +1543,112 @@ int update_all_conserved(const Grid& g, const cg_thermo& th, double* u) {
 }
 
 // ---------------------------------------------------------------------------
+// Kessler warm-rain microphysics and sedimentation of the precipitation
+// tracer (no counterpart in the reference's live stage: its sedimentation
+// hook is commented out at src/tasklist/implicit_hydro_tasks.cpp:292 and the
+// rates live in kintera, which is not vendored).  The reactions and rate
+// constants are those of examples/2019-Li-snap/earth-precip.yaml:44-53 and
+// water-kessler.yaml:4-18 (species dry, H2O, H2O(l), H2O(p)); the rate laws
+// and the integration over one step are this project's definition:
+//   autoconversion  H2O(l) -> H2O(p):        rate k_auto rho_c
+//   accretion       H2O(l) + H2O(p) -> 2 H2O(p): rate k_accr rho_c q_p
+//   evaporation     H2O(p) -> H2O:           rate k_evap rho_p (1 - rho_v / rho_s(T))
+//                                            while sub-saturated
+// applied once per step with the full dt after the saturation adjustment,
+// each conversion capped by what is there (and evaporation by the deficit);
+// evaporation takes the latent heat L per kg from the total energy (the
+// moist internal energy ie - rho L (q_c + q_p) is conserved).  Precipitation
+// then falls with speed v_fall (toward -x1) by first-order upwinding
+// (the mass flux of src/snap/riemann/noflux.cpp:61-71: velocity times the
+// upwind density): a fraction c = min(1, dt v_fall / dx1) of each cell's
+// precipitation moves to the cell below, carrying the cell's velocity and
+// its own sensible energy cv_p T + |v|^2 / 2, so that the donor keeps its
+// temperature and velocity; nothing crosses the walls (the column's water,
+// mass, momentum and energy are conserved).  Columns are swept top down.
+void kessler_column(const Grid& g, const cg_block& blk, const cg_thermo& th,
+                    const cg_config& cfg, const SatParams& s, int ip,
+                    double dt, double* u, int k, int j) {
+  const int N = g.N, IVX = g.IVX, IEN = g.IPR;
+  const int rv_ = 1 + s.iv, rc_ = 1 + s.ic, rp_ = 1 + ip;
+  const double cvp = s.cvd * (1. + th.cv_ratio_m1[ip]);
+  double cf = dt * cfg.v_fall / blk.dx1;
+  if (cf > 1.) cf = 1.;
+  double in_p = 0., in_m[3] = {0., 0., 0.}, in_e = 0.;
+  for (int i = g.ie; i >= g.is; --i) {
+    double uc[5 + CG_MAX_NMASS], wc[5 + CG_MAX_NMASS];
+    for (int n = 0; n < N; ++n) uc[n] = u[g.at(n, k, j, i)];
+    double rho = uc[0];
+    for (int n = 1; n <= g.nmass; ++n) rho += uc[n];
+    cons_to_prim_cell(th, g.nmass, uc, wc);
+    double T = temperature(th, wc, IEN);
+    // conversions (densities of the species; rho is unchanged)
+    double rc = uc[rc_], rp = uc[rp_], rv = uc[rv_];
+    double a = dt * (cfg.k_auto + cfg.k_accr * (rp / rho)) * rc;
+    if (a > rc) a = rc;
+    rc = rc - a;
+    rp = rp + a;
+    double rs = qsat(s, T, rho) * rho;
+    double def = rs - rv;
+    double b = 0.;
+    if (def > 0. && rp > 0.) {
+      b = dt * cfg.k_evap * rp * (def / rs);
+      if (b > rp) b = rp;
+      if (b > def) b = def;
+    }
+    rp = rp - b;
+    rv = rv + b;
+    uc[rc_] = rc;
+    uc[rp_] = rp;
+    uc[rv_] = rv;
+    uc[IEN] = uc[IEN] - s.L * b;
+    // sedimentation: what leaves this cell downward, then what arrives
+    double out_p = 0., out_m[3] = {0., 0., 0.}, out_e = 0.;
+    if (i > g.is && cf > 0.) {
+      cons_to_prim_cell(th, g.nmass, uc, wc);
+      double T2 = temperature(th, wc, IEN);
+      double v1 = wc[IVX], v2 = wc[IVX + 1], v3 = wc[IVX + 2];
+      out_p = cf * rp;
+      out_m[0] = out_p * v1;
+      out_m[1] = out_p * v2;
+      out_m[2] = out_p * v3;
+      out_e = out_p * (cvp * T2 + 0.5 * (v1 * v1 + v2 * v2 + v3 * v3));
+    }
+    uc[rp_] = (uc[rp_] - out_p) + in_p;
+    for (int d = 0; d < 3; ++d) uc[IVX + d] = (uc[IVX + d] - out_m[d]) + in_m[d];
+    uc[IEN] = (uc[IEN] - out_e) + in_e;
+    for (int n = 0; n < N; ++n) u[g.at(n, k, j, i)] = uc[n];
+    in_p = out_p;
+    for (int d = 0; d < 3; ++d) in_m[d] = out_m[d];
+    in_e = out_e;
+  }
+}
+
+bool kessler_on(const cg_config& cfg) {
+  return cfg.k_auto != 0. || cfg.k_accr != 0. || cfg.k_evap != 0. || cfg.v_fall != 0.;
+}
+
+// Species index (0-based) of the precipitation: the species that is
+// neither the adjusted vapour nor the adjusted cloud; -1 if none.
+int precip_species(const cg_thermo& th) {
+  if (th.ivap <= 0 || th.icld <= 0) return -1;
+  for (int n = 0; n < th.nmass; ++n)
+    if (n != th.ivap - 1 && n != th.icld - 1 && n >= th.nvapor) return n;
+  return -1;
+}
+
+int kessler(const Grid& g, const cg_block& blk, const cg_thermo& th,
+            const cg_config& cfg, double dt, double* u) {
+  if (!kessler_on(cfg)) return CG_OK;
+  const int ip = precip_species(th);
+  if (ip < 0) return CG_ERR_ARG;
+  SatParams s = sat_params(th);
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads)
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j) kessler_column(g, blk, th, cfg, s, ip, dt, u, k, j);
+  return CG_OK;
+}
+
+// ---------------------------------------------------------------------------
 // One SSPRK(3,3) step of ImplicitHydroTasks
 // (src/tasklist/implicit_hydro_tasks.cpp:134-313; weights
 //  src/integrator/integrators.cpp:14-44):
@@ -1594,6 +1700,10 @@ int step(const cg_block& blk, const cg_thermo& th, const cg_config& cfg,
             }
       if (s == 2 && cfg.satadj != CG_SATADJ_OFF) {
         err = update_all_conserved(g, th, u);
+        if (err) return err;
+      }
+      if (s == 2) {
+        err = kessler(g, blk, th, cfg, dt, u);
         if (err) return err;
       }
       cons_to_prim(g, th, u, w);
@@ -1760,6 +1870,12 @@ int or_mlp_features(const cg_thermo* th, int ncell, const double* rho,
     if (tu) tu[c] = f.tu;
   }
   return CG_OK;
+}
+
+int or_kessler(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+               double dt, double* u) {
+  orc::Grid g(*blk);
+  return orc::kessler(g, *blk, *th, *cfg, dt, u);
 }
 
 int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,

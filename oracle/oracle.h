@@ -62,6 +62,10 @@ int or_apply_bc(const cg_block* blk, double* w);
 int or_saturation_adjust(const cg_thermo* th, int ncell, const double* rho,
                          double* ie, double* y, const double* t_guess,
                          int* iters);
+/* Kessler microphysics + precipitation sedimentation over dt on the
+ * conserved state u (interior cells), as applied after the last stage. */
+int or_kessler(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+               double dt, double* u);
 /* nsteps full SSPRK(3,3) steps of ImplicitHydroTasks on (u, w): u interior is
  * the state, w (with ghosts) is recomputed from u at the end of each stage. */
 int or_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,

@@ -34,6 +34,7 @@ _SIGS = {
     "or_apply_bc": (_I, [_P, _P]),
     "or_saturation_adjust": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "or_step": (_I, [_P, _P, _P, _D, _I, _P, _P]),
+    "or_kessler": (_I, [_P, _P, _P, _D, _P]),
     "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
     "or_sums": (_I, [_P, _P, _P]),
     "or_new_block_dt": (_D, [_P, _P, _P, _P, _D]),
@@ -168,6 +169,14 @@ def step(blk, th, cfg, dt, nsteps, u, w, threads=1):
                               ptr(w), threads)
     assert r == 0, r
     return u, w
+
+
+def kessler(blk, th, cfg, dt, u):
+    """Kessler microphysics + sedimentation of the oracle on conserved u."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    r = lib().or_kessler(ref(blk), ref(th), ref(cfg), dt, ptr(u))
+    assert r == 0, r
+    return u
 
 
 def new_block_dt(blk, th, cfg, w, cfl):
