@@ -29,7 +29,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_NEGATIVE, ERR_PIVOT, ERR_CONVERGE, \
     ERR_SIZE, ERR_NODEV, ERR_COLD, ERR_IO = range(11)
-BC_REFLECT, BC_PERIODIC, BC_OUTFLOW = 0, 1, 2
+BC_REFLECT, BC_PERIODIC, BC_OUTFLOW, BC_PANEL = 0, 1, 2, 3
 RS_LMARS, RS_HLLC = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 SATADJ_OFF, SATADJ_NEWTON, SATADJ_MLP = 0, 1, 2
@@ -109,6 +109,17 @@ class Config(ctypes.Structure):
     ]
 
 
+class Sphere(ctypes.Structure):
+    """cg_sphere: rotation and Held-Suarez forcing of a cubed-sphere run."""
+    _fields_ = [
+        ("omega", ctypes.c_double), ("hs94", ctypes.c_int),
+        ("ts", ctypes.c_double), ("dt_eq", ctypes.c_double),
+        ("dtheta", ctypes.c_double), ("p0", ctypes.c_double),
+        ("kf", ctypes.c_double), ("ka", ctypes.c_double),
+        ("ks", ctypes.c_double), ("sigmab", ctypes.c_double),
+    ]
+
+
 class CanoeGpuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("canoe_gpu error %d: %s" % (code, msg))
@@ -144,6 +155,9 @@ _SIGS = {
     "cg_implicit_solve_host": (_I, [_P, _P, _P, _P, _P, _D, _I]),
     "cg_cons_to_prim": (_I, [_P, _P, _P, _P, _P]),
     "cg_microphysics": (_I, [_P, _P, _P, _P, _D, _P, _P]),
+    "cg_sphere_create": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "cg_pgen_sphere": (_I, [_P]),
+    "cg_pgen_sphere_init": (_I, [ctypes.c_uint64, _P, _P, _P]),
     "cg_prim_to_cons": (_I, [_P, _P, _P, _P, _P]),
     "cg_apply_bc": (_I, [_P, _P, _P]),
     "cg_new_block_dt": (_I, [_P, _P, _P, _P, _P, _D, ctypes.POINTER(_D)]),
@@ -228,6 +242,25 @@ def pgen_init(config, seed, blk, th, cfg):
                            _ptr(w))
     if r:
         raise CanoeGpuError(r, "cg_pgen_init(%d)" % config)
+    return w
+
+
+def pgen_sphere():
+    """Rotation and Held-Suarez parameters of hs94.inp (cg_pgen_sphere)."""
+    sp = Sphere()
+    r = lib().cg_pgen_sphere(_ref(sp))
+    if r:
+        raise CanoeGpuError(r, "cg_pgen_sphere")
+    return sp
+
+
+def pgen_sphere_init(seed, blk, th):
+    """Six panels' primitive states (6, N, nc3, nc2, nc1): isothermal 300 K
+    at rest plus 0.1 K noise on the interior, ghosts zero."""
+    w = np.zeros((6,) + blk.shape(), dtype=np.float64)
+    r = lib().cg_pgen_sphere_init(seed, _ref(blk), _ref(th), _ptr(w))
+    if r:
+        raise CanoeGpuError(r, "cg_pgen_sphere_init")
     return w
 
 
@@ -583,3 +616,30 @@ class Run:
         if self.h is not None:
             lib().cg_run_destroy(self.h)
             self.h = None
+
+
+class SphereRun(Run):
+    """The six panels of a cubed-sphere run (cg_sphere_create); panel b
+    (0-5) is block b.  set_prim / prim take contravariant velocities, cons
+    holds covariant momenta."""
+
+    def __init__(self, ctx, panel, th, cfg, sp):
+        self.ctx, self.th, self.cfg, self.sp = ctx, th, cfg, sp
+        self.blocks = [panel] * 6
+        self.blk = panel
+        h = ctypes.c_void_p()
+        ctx._check(lib().cg_sphere_create(ctx.h, _ref(panel), _ref(th),
+                                          _ref(cfg), _ref(sp),
+                                          ctypes.byref(h)), "cg_sphere_create")
+        self.h = h
+
+    def set_all(self, w6):
+        for b in range(6):
+            self.set_prim(w6[b], b)
+
+    def cons_all(self):
+        return np.stack([self.cons(b) for b in range(6)])
+
+    def prim_all(self):
+        return np.stack([self.prim(b) for b in range(6)])
+

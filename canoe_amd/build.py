@@ -32,7 +32,7 @@ DEVICE = [
 ]
 
 SOURCES_HIP = ["capi.hip", "run.hip", "mlp.hip"]
-SOURCES_CPP = ["pgen.cpp", "netcdf.cpp"]
+SOURCES_CPP = ["pgen.cpp", "netcdf.cpp", "sphere_geom.cpp"]
 
 
 def run(cmd):

@@ -649,14 +649,25 @@ __device__ inline void abs_roe_fma(const KThermo& th, const double* prim,
 // column, with which ghosts beyond an outflow x1 side are read as the
 // reference's CalculateFluxes left them (outflow_ghost_w).  Returns false on
 // a zero pivot.
-template <int NM, int B>
+// Column of a cubed-sphere panel (GEO): face areas dx2f_ang dx3f_ang x1f^2
+// sin(theta) and volumes dx1 dx2f_ang dx3f_ang x1v^2 sin(theta) per level,
+// tangential velocities projected onto the orthonormal frame
+// (partial_correction.cpp:113-137, solve_implicit_3d.cpp:103-136).
+struct KColGeo {
+  const double* x1f;
+  const double* x1v;
+  double dx1, d2a, d3a, cosc, sinc;
+};
+
+template <int NM, int B, bool GEO = false>
 __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
                                            double dt, const double* __restrict__ w,
                                            const double* __restrict__ du,
                                            long long c0, double* __restrict__ arch,
                                            long long as, long long ao,
                                            const double* __restrict__ psvr = nullptr,
-                                           long long pc = 0) {
+                                           long long pc = 0,
+                                           const KColGeo* geo = nullptr) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   constexpr int AS = B * B + B;
   const int is = g.is, ie = g.ie;
@@ -666,12 +677,16 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
   const double aleft = g.dx2 * g.dx3, aright = g.dx2 * g.dx3;
   const double vol = g.dx1 * g.dx2 * g.dx3;
   const double s2v = 1. / (2. * vol);
-  const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
+  const double sl0 = aleft * s2v, sr0 = aright * s2v, sd0 = (aright - aleft) * s2v;
   const double idt = 1. / dt;
   const long long sn = g.c.n, si = g.c.i;
   auto load = [&](double* dst, int i) {
 #pragma unroll
     for (int n = 0; n < N; ++n) dst[n] = w[c0 + n * sn + i * si];
+    if (GEO) {
+      dst[IVX + 1] = dst[IVX + 1] + dst[IVX + 2] * geo->cosc;
+      dst[IVX + 2] = dst[IVX + 2] * geo->sinc;
+    }
   };
   double wm[N], wc[N], wp[N], prim[N];
   load(wm, is - 1);
@@ -714,6 +729,16 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
     roe_average<NM>(prim, gm, wc, wp);
     abs_roe_fma<NM, B>(th, prim, gm, Ap);
     flux_jac<NM, B>(gm1, wp, dp);
+    double sl = sl0, sr = sr0, sd = sd0;
+    if (GEO) {
+      const double al = geo->d2a * geo->d3a * geo->x1f[i] * geo->x1f[i] * geo->sinc;
+      const double ar = geo->d2a * geo->d3a * geo->x1f[i + 1] * geo->x1f[i + 1] * geo->sinc;
+      const double v = geo->dx1 * geo->d2a * geo->d3a * geo->x1v[i] * geo->x1v[i] * geo->sinc;
+      const double s2 = 1. / (2. * v);
+      sl = al * s2;
+      sr = ar * s2;
+      sd = (ar - al) * s2;
+    }
     Mt<B> M, b, c;
     double t[B];
     // rhs (forward_backward.hpp:30-36)

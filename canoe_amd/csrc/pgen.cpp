@@ -215,11 +215,80 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
     cfg->heat_flux = 50.0;
     cfg->p0 = 1e5;
     cfg->satadj = (config == 4) ? CG_SATADJ_NEWTON : CG_SATADJ_OFF;
+  } else if (config == 3) {
+    // one panel of the cubed sphere (hs94.inp: R = 6.371e6 m, 25 km deep,
+    // g = 9.81; BASELINE's 6 x 96 x 96 x 64)
+    blk->nx1 = nx1 > 0 ? nx1 : 64;
+    blk->nx2 = nx2 > 0 ? nx2 : 96;
+    blk->nx3 = nx3 > 0 ? nx3 : blk->nx2;
+    blk->x1min = 6.371e6;
+    blk->dx1 = 25000.0 / blk->nx1;
+    blk->dx2 = M_PI / (2.0 * blk->nx2);
+    blk->dx3 = M_PI / (2.0 * blk->nx3);
+    blk->x2min = -0.25 * M_PI;
+    blk->x3min = -0.25 * M_PI;
+    blk->grav = -9.81;
+    blk->bc[0] = blk->bc[1] = CG_BC_REFLECT;
+    blk->bc[2] = blk->bc[3] = blk->bc[4] = blk->bc[5] = CG_BC_PANEL;
+    dry_thermo(th, 1.4, 287.0);
+    cfg->order_h = 5;
+    cfg->order_v = 5;
+    cfg->rsolver = CG_RS_HLLC;
+    cfg->implicit_flag = 1;  // hs94.inp
+    cfg->p0 = 1e5;
   } else {
     return CG_ERR_ARG;
   }
   blk->nmass = th->nmass;
   cfg->precision = CG_PREC_FP64;
+  return CG_OK;
+}
+
+// Rotation and Held-Suarez parameters of examples/2023-Chen-exo3/hs94.inp
+// (Omega, Ts, dT, dtheta, p0, Kf, Ka, Ks in 1/day, sigmab).
+int cg_pgen_sphere(cg_sphere* sp) {
+  if (!sp) return CG_ERR_ARG;
+  const double day = 8.64e4;
+  sp->omega = 7.292e-5;
+  sp->hs94 = 1;
+  sp->ts = 315.0;
+  sp->dt_eq = 60.0;
+  sp->dtheta = 10.0;
+  sp->p0 = 1e5;
+  sp->kf = 1.0 / day;
+  sp->ka = 0.025 / day;
+  sp->ks = 0.25 / day;
+  sp->sigmab = 0.7;
+  return CG_OK;
+}
+
+// Isothermal 300 K atmosphere at rest, p = p0 exp(-g z / (Rd T0)) at the
+// cell centres, plus seeded Gaussian 0.1 K temperature noise (Box-Muller of
+// two uniforms per cell and panel).
+int cg_pgen_sphere_init(uint64_t seed, const cg_block* blk, const cg_thermo* th,
+                        double* w6) {
+  if (!blk || !th || !w6) return CG_ERR_ARG;
+  Dims d(*blk);
+  const size_t ps = (size_t)d.N * d.sn;
+  memset(w6, 0, sizeof(double) * ps * 6);
+  const double T0 = 300.0, p0 = 1e5, g = -blk->grav, Rd = th->rd;
+  for (int p = 0; p < 6; ++p) {
+    double* w = w6 + p * ps;
+    for (int k = d.ks; k < d.ks + blk->nx3; ++k)
+      for (int j = d.js; j < d.js + blk->nx2; ++j)
+        for (int i = d.is; i < d.is + blk->nx1; ++i) {
+          const double z = (i - d.is + 0.5) * blk->dx1;
+          const double pr = p0 * std::exp(-g * z / (Rd * T0));
+          const uint64_t idx =
+              ((((uint64_t)p * blk->nx3 + (k - d.ks)) * blk->nx2 + (j - d.js)) * blk->nx1 +
+               (i - d.is)) * 2;
+          const double u1 = urand(seed, idx), u2 = urand(seed, idx + 1);
+          const double gz = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(2.0 * M_PI * u2);
+          const double T = T0 + 0.1 * gz;
+          w[d.at(0, k, j, i)] = pr / (Rd * T);
+          w[d.at(d.IPR, k, j, i)] = pr;
+        }
+  }
   return CG_OK;
 }
 

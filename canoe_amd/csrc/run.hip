@@ -19,6 +19,8 @@
 
 #include "../../include/canoe_gpu.h"
 #include "launch.hpp"
+#include "sphere.hpp"
+#include "sphere_geom.hpp"
 
 using namespace cg;
 using namespace cg_host;
@@ -37,6 +39,23 @@ struct RunBlock {
 };
 
 }  // namespace
+
+// Device state of a cubed-sphere run (six panel blocks; sphere.hpp).
+struct SphRun {
+  cg_sphere sp;
+  KSph k;                        // kernel arguments (panel pointers, tables)
+  KSphPtrs* d_ptr = nullptr;     // panel pointers for the table kernels
+  double* kj = nullptr;          // six panels' (k, j) tables
+  double* x1 = nullptr;          // x1f (nc1 + 1) then x1v (nc1)
+  double* lr = nullptr;          // face states
+  double* arch = nullptr;        // column archive, arch_panel doubles per panel
+  long long arch_panel = 0;
+  SphGhost* d_ghosts = nullptr;
+  int nghost = 0;
+  SphEdge* d_edges = nullptr;
+  int nedge = 0;
+  bool dirty = true;             // ghosts and hydrostatic pressures to refresh
+};
 
 struct cg_run {
   cg_ctx* ctx = nullptr;
@@ -74,6 +93,7 @@ struct cg_run {
   int ntiles = 0, tile_variant = -1, tile_seg = -1;
   unsigned bc2_grid = 1, bc3_grid = 1;  // workgroups per block of the ghost kernels
   KTimer tm;
+  SphRun* sph = nullptr;  // cubed-sphere run (cg_sphere_create)
 };
 
 namespace {
@@ -258,7 +278,98 @@ int run_stage(cg_run* r, int s) {
   return CG_OK;
 }
 
+// Ghost zones of the six panels from the neighbours, x1 walls, and the
+// hydrostatic pressures the next x1 fluxes read.
+void sphere_fill(cg_run* r) {
+  SphRun& S = *r->sph;
+  hipStream_t st = r->ctx->stream;
+  KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  TIMED(tm, KID_BC_X2,
+        (k_sph_ghosts<<<nblk((long long)S.nghost * S.k.nr, 256), 256, 0, st>>>(
+            S.k, S.d_ptr, S.d_ghosts, S.nghost)));
+  for (RunBlock& b : r->blocks) {
+    const KGrid& g = b.g;
+    TIMED(tm, KID_BC_X1,
+          (k_bc_x1<0><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, st>>>(g, b.w)));
+    if (g.grav != 0.0)
+      TIMED(tm, KID_SCAN,
+            (k_hydrostatic_scan<0><<<nblk((long long)g.nx2 * g.nx3, 256), 256, 0, st>>>(
+                g, r->kt, b.w, b.psf, b.psv)));
+  }
+  S.dirty = false;
+}
+
+// One SSPRK(3,3) stage of the cubed sphere (FP64, dry)
+int sphere_stage(cg_run* r, int s) {
+  const double sbeta[3] = {1.0, 0.25, 2. / 3.};
+  const double sg1[3] = {0.0, 0.25, 2. / 3.};
+  const double sg2[3] = {1.0, 0.75, 1. / 3.};
+  SphRun& S = *r->sph;
+  const KSph& ks = S.k;
+  const KGrid& g = ks.g;
+  hipStream_t st = r->ctx->stream;
+  KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
+  const long long nf1 = (long long)(g.nx1 + 1) * g.nx2 * g.nx3;
+  const long long nface = (long long)ks.nr * ks.n * (ks.n + 1);
+  for (RunBlock& b : r->blocks)
+    TIMED(tm, KID_FLUX_X1,
+          (k_flux_x1<0, 5><<<nblk(nf1, 256), 256, 0, st>>>(g, r->kt, CG_RS_HLLC,
+                                                           g.grav != 0.0 ? 1 : 0, b.w, b.psf,
+                                                           b.psv, b.f1)));
+  TIMED(tm, KID_FLUX_X1, (k_sph_x1_project<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks)));
+  if (r->cfg.order_h == 5) {
+    TIMED(tm, KID_FLUX_X2, (k_sph_faces<5, 1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
+    TIMED(tm, KID_FLUX_X3, (k_sph_faces<5, 2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
+  } else {
+    TIMED(tm, KID_FLUX_X2, (k_sph_faces<2, 1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
+    TIMED(tm, KID_FLUX_X3, (k_sph_faces<2, 2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
+  }
+  TIMED(tm, KID_FLUX_X2,
+        (k_sph_exchange<<<nblk((long long)S.nedge * ks.nr, 256), 256, 0, st>>>(ks, S.d_edges,
+                                                                             S.nedge)));
+  TIMED(tm, KID_FLUX_X2,
+        (k_sph_face_flux<1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  TIMED(tm, KID_FLUX_X3,
+        (k_sph_face_flux<2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  TIMED(tm, KID_DIVERGENCE,
+        (k_sph_divergence<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, sbeta[s], r->d_dt)));
+  if (r->cfg.implicit_flag & 1) {
+    const long long ncol = (long long)g.nx2 * g.nx3;
+    if (r->cfg.implicit_flag & 8) {
+      TIMED(tm, KID_IMPLICIT_FWD,
+            (k_sph_implicit_fwd<5><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(
+                ks, r->kt, sbeta[s], r->d_dt, S.arch, S.arch_panel, r->ctx->d_status)));
+      TIMED(tm, KID_IMPLICIT_BWD,
+            (k_sph_implicit_bwd<5><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(ks, S.arch,
+                                                                         S.arch_panel)));
+    } else {
+      TIMED(tm, KID_IMPLICIT_FWD,
+            (k_sph_implicit_fwd<3><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(
+                ks, r->kt, sbeta[s], r->d_dt, S.arch, S.arch_panel, r->ctx->d_status)));
+      TIMED(tm, KID_IMPLICIT_BWD,
+            (k_sph_implicit_bwd<3><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(ks, S.arch,
+                                                                         S.arch_panel)));
+    }
+  }
+  KUpdate up;
+  up.stage = s;
+  up.g1 = sg1[s];
+  up.g2 = sg2[s];
+  up.satadj = 0;
+  TIMED(tm, KID_UPDATE, (k_sph_update<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, up)));
+  sphere_fill(r);
+  return CG_OK;
+}
+
 int run_one_step(cg_run* r) {
+  if (r->sph) {
+    for (int s = 0; s < 3; ++s) {
+      int e = sphere_stage(r, s);
+      if (e) return e;
+    }
+    return CG_OK;
+  }
   for (int s = 0; s < 3; ++s) {
     int e = (r->N == 5) ? run_stage<0>(r, s) : run_stage<3>(r, s);
     if (e) return e;
@@ -642,6 +753,123 @@ int cg_run_create(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   return cg_batch_create(ctx, 1, blk, th, cfg, out);
 }
 
+int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
+                     const cg_config* cfg, const cg_sphere* sp, cg_run** out) {
+  if (!ctx || !panel || !th || !cfg || !sp || !out) return CG_ERR_ARG;
+  *out = nullptr;
+  if (th->nmass != 0 || panel->nmass != 0)
+    return set_err(ctx, CG_ERR_ARG, "the cubed sphere runs dry gas (nmass 0)");
+  if (panel->nx2 != panel->nx3 || panel->nx2 < 2 * NG || panel->nx2 % 2)
+    return set_err(ctx, CG_ERR_ARG, "panels must be square with an even side of >= 6 cells");
+  if (panel->bc[0] != CG_BC_REFLECT || panel->bc[1] != CG_BC_REFLECT)
+    return set_err(ctx, CG_ERR_ARG, "the cubed sphere needs reflecting x1 walls");
+  for (int s = 2; s < 6; ++s)
+    if (panel->bc[s] != CG_BC_PANEL)
+      return set_err(ctx, CG_ERR_ARG, "x2 / x3 sides of a panel must be CG_BC_PANEL");
+  if (cfg->precision != CG_PREC_FP64 || cfg->rsolver != CG_RS_HLLC ||
+      (cfg->implicit_flag != 0 && cfg->implicit_flag != 1 && cfg->implicit_flag != 9) ||
+      cfg->satadj != CG_SATADJ_OFF || cfg->visc != 0.0 || cfg->heat_flux != 0.0 ||
+      kessler_on(*cfg))
+    return set_err(ctx, CG_ERR_ARG,
+                   "cubed sphere: FP64, HLLC, implicit_flag 0 / 1 / 9, no adjustment, "
+                   "diffusion, heating or microphysics");
+  if (!(panel->x1min > 0.0)) return set_err(ctx, CG_ERR_ARG, "x1min is the inner radius");
+  // six blocks; their x2 / x3 flags are never read by the sphere's kernels
+  cg_block b = *panel;
+  for (int s = 2; s < 6; ++s) b.bc[s] = CG_BC_PERIODIC;
+  std::vector<cg_block> blks(6, b);
+  cg_config c = *cfg;
+  c.order_v = 5;  // Weno5X1 under CUBED_SPHERE (patches/19.calculate_fluxes.patch:29-31)
+  c.implicit_flag = 0;  // the batch's own column solve is not used
+  cg_run* r = nullptr;
+  int e = cg_batch_create(ctx, 6, blks.data(), th, &c, &r);
+  if (e) return e;
+  r->cfg.implicit_flag = cfg->implicit_flag;
+  r->sph = new SphRun();
+  SphRun& S = *r->sph;
+  S.sp = *sp;
+  const int n = panel->nx2, nr = panel->nx1;
+  SphGeom G = sphere_geometry(n, nr, panel->x1min, panel->dx1);
+  const size_t nkj = (size_t)SKJ_COUNT * G.nc * G.nc;
+  const long long ncol = (long long)n * n;
+  const int B = (cfg->implicit_flag & 8) ? 5 : 3;
+  S.arch_panel = (long long)(B * B + B) * nr * ncol;
+  S.nghost = (int)G.ghosts.size();
+  S.nedge = (int)G.edges.size();
+  KSph& k = S.k;
+  k.g = r->blocks[0].g;
+  k.lrn = (long long)5 * nr * n * (n + 1);
+  bool ok = hipMalloc((void**)&S.kj, sizeof(double) * nkj * 6) == hipSuccess &&
+            hipMalloc((void**)&S.x1, sizeof(double) * (2 * G.nc1 + 1)) == hipSuccess &&
+            hipMalloc((void**)&S.lr, sizeof(double) * 24 * k.lrn) == hipSuccess &&
+            hipMemset(S.lr, 0xFF, sizeof(double) * 24 * k.lrn) == hipSuccess &&
+            hipMalloc((void**)&S.d_ghosts, sizeof(SphGhost) * S.nghost) == hipSuccess &&
+            hipMalloc((void**)&S.d_edges, sizeof(SphEdge) * S.nedge) == hipSuccess &&
+            hipMalloc((void**)&S.d_ptr, sizeof(KSphPtrs)) == hipSuccess;
+  if (ok && (cfg->implicit_flag & 1))
+    ok = hipMalloc((void**)&S.arch, sizeof(double) * 6 * S.arch_panel) == hipSuccess &&
+         hipMemset(S.arch, 0xFF, sizeof(double) * 6 * S.arch_panel) == hipSuccess;
+  if (ok) {
+    for (int p = 0; p < 6; ++p)
+      ok = ok && hipMemcpy(S.kj + p * nkj, G.kj[p].data(), sizeof(double) * nkj,
+                           hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok &&
+         hipMemcpy(S.x1, G.x1f.data(), sizeof(double) * (G.nc1 + 1), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(S.x1 + G.nc1 + 1, G.x1v.data(), sizeof(double) * G.nc1,
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(S.d_ghosts, G.ghosts.data(), sizeof(SphGhost) * S.nghost,
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(S.d_edges, G.edges.data(), sizeof(SphEdge) * S.nedge,
+                   hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok) {
+    KSphPtrs hp;
+    for (int p = 0; p < 6; ++p) {
+      const RunBlock& rb = r->blocks[p];
+      k.kj[p] = S.kj + p * nkj;
+      k.u[p] = rb.u;
+      k.w[p] = rb.w;
+      k.u1[p] = rb.u1;
+      k.du[p] = rb.du;
+      k.f1[p] = rb.f1;
+      k.f2[p] = rb.f2;
+      k.f3[p] = rb.f3;
+      hp.u[p] = rb.u;
+      hp.w[p] = rb.w;
+      hp.kj[p] = k.kj[p];
+    }
+    ok = hipMemcpy(S.d_ptr, &hp, sizeof(KSphPtrs), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    cg_run_destroy(r);
+    return set_err(ctx, CG_ERR_HIP, "allocation failed");
+  }
+  k.lr = S.lr;
+  k.x1f = S.x1;
+  k.x1v = S.x1 + G.nc1 + 1;
+  k.n = n;
+  k.nc = G.nc;
+  k.nr = nr;
+  k.dx1 = panel->dx1;
+  k.gm1 = th->gammad - 1.;
+  k.omega = sp->omega;
+  k.hs94 = sp->hs94;
+  k.ts = sp->ts;
+  k.dt_eq = sp->dt_eq;
+  k.dtheta = sp->dtheta;
+  k.p0 = sp->p0;
+  k.kf = sp->kf;
+  k.ka = sp->ka;
+  k.ks = sp->ks;
+  k.sigmab = sp->sigmab;
+  k.cp = th->gammad / (th->gammad - 1.) * th->rd;
+  k.rd = th->rd;
+  k.kappa = th->rd / k.cp;
+  *out = r;
+  return CG_OK;
+}
+
 int cg_run_destroy(cg_run* r) {
   if (!r) return CG_ERR_ARG;
   (void)hipStreamSynchronize(r->ctx->stream);
@@ -658,6 +886,13 @@ int cg_run_destroy(cg_run* r) {
   if (r->h_dt) (void)hipHostFree(r->h_dt);
   if (r->tm.e0) (void)hipEventDestroy(r->tm.e0);
   if (r->tm.e1) (void)hipEventDestroy(r->tm.e1);
+  if (r->sph) {
+    SphRun& S = *r->sph;
+    for (void* q : {(void*)S.d_ptr, (void*)S.kj, (void*)S.x1, (void*)S.lr, (void*)S.arch,
+                    (void*)S.d_ghosts, (void*)S.d_edges})
+      if (q) (void)hipFree(q);
+    delete r->sph;
+  }
   delete r;
   return CG_OK;
 }
@@ -680,10 +915,18 @@ int cg_run_block_set_prim_host(cg_run* r, int b, const double* w_host) {
   HIPCHK(ctx, hipMemsetAsync(rb.u, 0, sizeof(double) * nc, ctx->stream));
   k_relayout<<<nblk((long long)nc, 256), 256, 0, ctx->stream>>>(rb.ga, rb.g, r->N, rb.du,
                                                                rb.w);
-  if (r->N == 5)
+  if (r->sph) {
+    // interior u with covariant momenta; the panels' ghosts and hydrostatic
+    // pressures are refreshed before the next step
+    const KGrid& g = rb.g;
+    k_sph_prim_to_cons<<<nblk((long long)g.nx1 * g.nx2 * g.nx3, 256), 256, 0, ctx->stream>>>(
+        r->sph->k, b);
+    r->sph->dirty = true;
+  } else if (r->N == 5) {
     prime_block<0>(r, rb);
-  else
+  } else {
     prime_block<3>(r, rb);
+  }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CG_OK;
@@ -745,6 +988,7 @@ int cg_run_step(cg_run* r, double dt, int nsteps, int use_graph) {
   int e = set_step(r, dt);
   if (!e) e = ensure_tiles(r);
   if (e) return e;
+  if (r->sph && r->sph->dirty) sphere_fill(r);
   if (use_graph && !r->tm.on) {
     if (!r->gexec && (e = capture(r))) return e;
     for (int n = 0; n < nsteps; ++n) HIPCHK(ctx, hipGraphLaunch(r->gexec, ctx->stream));
@@ -773,7 +1017,9 @@ int cg_run_stage(cg_run* r, int stage, double dt, const double* dt_dev) {
     k_copy_scalar<<<1, 64, 0, ctx->stream>>>(r->d_dt, dt_dev);
   else if ((e = set_step(r, dt)))
     return e;
-  e = (r->N == 5) ? run_stage<0>(r, stage) : run_stage<3>(r, stage);
+  if (r->sph && r->sph->dirty) sphere_fill(r);
+  e = r->sph ? sphere_stage(r, stage)
+             : (r->N == 5) ? run_stage<0>(r, stage) : run_stage<3>(r, stage);
   if (e) return e;
   HIPCHK(ctx, hipGetLastError());
   return CG_OK;
@@ -781,6 +1027,7 @@ int cg_run_stage(cg_run* r, int stage, double dt, const double* dt_dev) {
 
 int cg_run_new_dt(cg_run* r, double cfl, double* dt_out) {
   if (!r || !dt_out || !(cfl > 0.0)) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   HIPCHK(r->ctx, hipSetDevice(r->ctx->device));
   int e = enqueue_new_dt(r, cfl);
   if (!e) e = read_new_dt(r, dt_out, false);
@@ -789,6 +1036,7 @@ int cg_run_new_dt(cg_run* r, double cfl, double* dt_out) {
 
 int cg_run_check(cg_run* r, int* where) {
   if (!r) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   HIPCHK(r->ctx, hipSetDevice(r->ctx->device));
   double dt = 0.0;
   int e = enqueue_new_dt(r, 1.0);
@@ -799,6 +1047,7 @@ int cg_run_check(cg_run* r, int* where) {
 int cg_run_advance(cg_run* r, double cfl, double tlim, long long nlim,
                    int use_graph, double* time, double* dt, long long* ncycle) {
   if (!r || !time || !dt || !ncycle || !(cfl > 0.0)) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int e = CG_OK;
@@ -830,6 +1079,7 @@ int cg_run_block_sums(cg_run* r, int b, double* out_host) {
   if (!r || !out_host || b < 0 || b >= (int)r->blocks.size()) return CG_ERR_ARG;
   cg_ctx* ctx = r->ctx;
   const RunBlock& rb = r->blocks[b];
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   const int nb = 1024;
   if (r->N == 5)
     k_sums<0><<<nb, 256, 0, ctx->stream>>>(rb.g, rb.u, r->part);
@@ -852,6 +1102,7 @@ int cg_run_sums(cg_run* r, double* out_host) { return cg_run_block_sums(r, 0, ou
 
 int cg_run_history(cg_run* r, double* out_host) {
   if (!r || !out_host) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int nb = (int)r->blocks.size(), P = 64, NQ = 8 + r->th.nmass;
@@ -880,6 +1131,7 @@ int cg_run_history(cg_run* r, double* out_host) {
 
 int cg_run_debug(cg_run* r, double* vmin, double* vmax, double* xmin, double* xmax) {
   if (!r || !vmin || !vmax || !xmin || !xmax) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   cg_ctx* ctx = r->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int nb = (int)r->blocks.size(), P = 64, N = r->N;
@@ -952,6 +1204,7 @@ int cg_run_debug(cg_run* r, double* vmin, double* vmax, double* xmin, double* xm
 
 int cg_run_set_mlp(cg_run* r, const cg_mlp* net) {
   if (!r) return CG_ERR_ARG;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   if (net && r->cfg.satadj != CG_SATADJ_MLP)
     return set_err(r->ctx, CG_ERR_ARG, "the run's config does not use the guided adjustment");
   r->mlp = net;
@@ -992,6 +1245,7 @@ int cg_run_set_tune(cg_run* r, const cg_tune* t) {
 int cg_run_autotune(cg_run* r, double dt, cg_tune* out) {
   if (!r || !(dt > 0.0)) return CG_ERR_ARG;
   cg_ctx* ctx = r->ctx;
+  if (r->sph) return set_err(r->ctx, CG_ERR_ARG, "not available for cubed-sphere runs");
   if (r->tm.on) return set_err(ctx, CG_ERR_ARG, "switch timing off before autotune");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // save the state a step reads

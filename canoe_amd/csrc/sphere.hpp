@@ -1,0 +1,526 @@
+// sphere.hpp -- device kernels of the cubed-sphere run (config 3): six
+// gnomonic-equiangle panels (src/exo3/gnomonic_equiangle.cpp), each a block
+// of the run in the level-major layout, dry gas, FP64.
+//
+// Per stage: x1 fluxes (k_flux_x1: decomposition, WENO5, HLLC) with the
+// covariant projection of the tangential momentum fluxes (k_sph_x1_project,
+// hllc_transform.cpp:235-272); x2 / x3 face states of every panel
+// (k_sph_faces); the panel-edge exchange of face states (k_sph_exchange,
+// cubed_sphere.cpp:250-565); HLLC in the face's orthonormal frame
+// (k_sph_face_flux: PrimToLocal2/3, FluxToGlobal2/3 of
+// gnomonic_equiangle.cpp:559-828, then the covariant projection); flux
+// divergence with the panel's areas and volumes, the geometric source terms
+// (gnomonic_equiangle.cpp:831-877), gravity and the Held-Suarez / Coriolis
+// forcing of examples/2023-Chen-exo3/hs94.cpp:56-151 (k_sph_divergence);
+// the x1 column solve with the panel's areas (k_sph_implicit_fwd and
+// k_implicit_bwd); the stage update with ConservedToPrimitive for covariant
+// momenta (k_sph_update, patches/26.adiabatic_hydro.patch:21-46); the ghost
+// zones of x2 / x3 interpolated from the neighbour panels (k_sph_ghosts,
+// cubed_sphere_utility.cpp:80-400); x1 walls and the hydrostatic scan.
+// Arithmetic is written in the order of the oracle's restatement so that
+// the two agree to rounding.
+#pragma once
+#include "cg_physics.hpp"
+#include "kernels.hpp"
+#include "sphere_geom.hpp"
+
+namespace cg {
+
+using cg_host::SphEdge;
+using cg_host::SphGhost;
+
+// Arguments of the sphere kernels; arrays are indexed by the panel (0-5),
+// which is wavefront uniform (blockIdx.y) except in the table kernels,
+// which read the panel pointers from ptr (device memory).
+struct KSph {
+  KGrid g;  // level-major grid of a panel (the same for all six)
+  const double* kj[6];
+  double* u[6];
+  double* w[6];
+  double* u1[6];
+  double* du[6];
+  double* f1[6];
+  double* f2[6];
+  double* f3[6];
+  double* lr;  // face states: 4 buffers (l2, r2, l3, r3) per panel, lrn each
+  long long lrn;
+  const double* x1f;
+  const double* x1v;
+  int n, nc, nr;
+  double dx1, gm1;
+  double omega, ts, dt_eq, dtheta, p0, kf, ka, ks, sigmab, cp, rd, kappa;
+  int hs94;
+};
+
+struct KSphPtrs {
+  double* u[6];
+  double* w[6];
+  const double* kj[6];
+};
+
+__device__ __forceinline__ double skj(const KSph& s, int p, int a, int k, int j) {
+  return s.kj[p][(long long)a * s.nc * s.nc + (long long)k * s.nc + j];
+}
+
+// face-state buffer b (0: l2, 1: r2, 2: l3, 3: r3) of panel p: variable v,
+// level ii (0-based), along index t, face f
+__device__ __forceinline__ long long slr(const KSph& s, int p, int b, int v, int ii, int t,
+                                         int f) {
+  const long long base = ((long long)p * 4 + b) * s.lrn;
+  if (b < 2) return base + (((long long)v * s.nr + ii) * s.n + t) * (s.n + 1) + f;
+  return base + (((long long)v * s.nr + ii) * (s.n + 1) + f) * s.n + t;
+}
+
+// ConservedToPrimitive / PrimitiveToConserved with covariant momenta
+CG_HD void sph_cons_to_prim(double gm1, double cth, const double* u, double* w) {
+  double di = 1.0 / u[0];
+  w[0] = u[0];
+  w[1] = u[1] * di;
+  double v = u[2] * di, x = u[3] * di;
+  double sth2 = 1. - cth * cth;
+  w[2] = v / sth2 - x * cth / sth2;
+  w[3] = -v * cth / sth2 + x / sth2;
+  double e_k = 0.5 * (u[1] * w[1] + u[2] * w[2] + u[3] * w[3]);
+  w[4] = gm1 * (u[4] - e_k);
+}
+
+CG_HD void sph_prim_to_cons(double gm1, double cth, const double* w, double* u) {
+  double igm1 = 1.0 / gm1;
+  u[1] = w[1] * w[0];
+  double m2 = w[2] * w[0], m3 = w[3] * w[0];
+  u[2] = m2 + m3 * cth;
+  u[3] = m3 + m2 * cth;
+  u[4] = w[4] * igm1 + 0.5 * (u[1] * w[1] + u[2] * w[2] + u[3] * w[3]);
+  u[0] = w[0];
+}
+
+// interior cell (k, j, i) of item t, panel blockIdx.y
+__device__ __forceinline__ bool sph_cell(const KGrid& g, long long t, int& i, int& j, int& k) {
+  if (t >= (long long)g.nx1 * g.nx2 * g.nx3) return false;
+  g.decode(t, g.nx1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_sph_prim_to_cons(KSph s, int p) {
+  int i, j, k;
+  if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const KGrid& g = s.g;
+  const long long c0 = g.at(0, k, j, i);
+  double w[5], u[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) w[v] = s.w[p][c0 + v * g.c.n];
+  sph_prim_to_cons(s.gm1, skj(s, p, cg_host::SKJ_COSC, k, j), w, u);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) s.u[p][c0 + v * g.c.n] = u[v];
+}
+
+// covariant projection of the x1 face fluxes' tangential momenta
+__global__ void __launch_bounds__(256) k_sph_x1_project(KSph s) {
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  int i, j, k;
+  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const double cth = skj(s, p, cg_host::SKJ_HC, k, j);
+  double* f = s.f1[p];
+  const long long a = g.f[0].at(2, k, j, i), b = g.f[0].at(3, k, j, i);
+  double ty = f[a], tz = f[b];
+  f[a] = ty + tz * cth;
+  f[b] = tz + ty * cth;
+}
+
+// x2 (DIR = 1) or x3 (DIR = 2) face states of every panel into the face
+// buffers; ORDER 5 (WENO5) or 2 (PLM)
+template <int ORDER, int DIR>
+__global__ void __launch_bounds__(256) k_sph_faces(KSph s) {
+  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const int n = s.n;
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= (long long)s.nr * n * (n + 1)) return;
+  int t, f, ii;
+  if (DIR == 1) {
+    f = (int)(item % (n + 1));
+    const long long r = item / (n + 1);
+    t = (int)(r % n);
+    ii = (int)(r / n);
+  } else {
+    t = (int)(item % n);
+    const long long r = item / n;
+    f = (int)(r % (n + 1));
+    ii = (int)(r / (n + 1));
+  }
+  const int i = g.is + ii;
+  const int j = (DIR == 1) ? g.js + f : g.js + t;
+  const int k = (DIR == 1) ? g.ks + t : g.ks + f;
+  const long long st = (DIR == 1) ? g.c.j : g.c.k;
+  const long long base = g.at(0, k, j, i);
+  const double* w = s.w[p];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double q[6];
+#pragma unroll
+    for (int m = LO; m <= HI; ++m) q[m] = w[base + v * g.c.n + (m - 3) * st];
+    double qm, qp, wl, wr;
+    if (ORDER == 5) {
+      weno5_cell<double>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
+      wl = qp;
+      weno5_cell<double>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
+      wr = qm;
+    } else {
+      plm_cell<double>(q[1], q[2], q[3], &qm, &qp);
+      wl = qp;
+      plm_cell<double>(q[2], q[3], q[4], &qm, &qp);
+      wr = qm;
+    }
+    const int bl = (DIR == 1) ? 0 : 2;
+    s.lr[slr(s, p, bl, v, ii, t, f)] = wl;
+    s.lr[slr(s, p, bl + 1, v, ii, t, f)] = wr;
+  }
+}
+
+// Panel-edge exchange: a panel's outer face state at each edge face becomes
+// the neighbour's inner state there, its contravariant velocity carried into
+// this panel's frame through (U, V).  Outer and inner slots are distinct, so the items are
+// independent.
+__global__ void __launch_bounds__(256)
+    k_sph_exchange(KSph s, const SphEdge* __restrict__ edges, int nedge) {
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= (long long)nedge * s.nr) return;
+  const SphEdge e = edges[item / s.nr];
+  const int ii = (int)(item % s.nr);
+  const int n = s.n;
+  // inner state of q at its side sq: the state from its cell next to the edge
+  const int bq = (e.sq < 2) ? ((e.sq == 0) ? 1 : 0) : ((e.sq == 2) ? 3 : 2);
+  const int fq = (e.sq % 2 == 0) ? 0 : n;
+  const int bp = (e.s < 2) ? ((e.s == 0) ? 0 : 1) : ((e.s == 2) ? 2 : 3);
+  const int fp = (e.s % 2 == 0) ? 0 : n;
+  double q[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) q[v] = s.lr[slr(s, e.q, bq, v, ii, e.tq, fq)];
+  const double U = e.r00 * q[2] + e.r01 * q[3];
+  const double V = e.r10 * q[2] + e.r11 * q[3];
+  q[2] = e.a00 * U + e.a01 * V;
+  q[3] = e.a10 * U + e.a11 * V;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) s.lr[slr(s, e.p, bp, v, ii, e.t, fp)] = q[v];
+}
+
+// HLLC at x2 (DIR = 1) / x3 (DIR = 2) faces in the face's orthonormal frame
+template <int DIR>
+__global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const int n = s.n;
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= (long long)s.nr * n * (n + 1)) return;
+  int t, f, ii;
+  if (DIR == 1) {
+    f = (int)(item % (n + 1));
+    const long long r = item / (n + 1);
+    t = (int)(r % n);
+    ii = (int)(r / n);
+  } else {
+    t = (int)(item % n);
+    const long long r = item / n;
+    f = (int)(r % (n + 1));
+    ii = (int)(r / (n + 1));
+  }
+  const int i = g.is + ii;
+  const int j = (DIR == 1) ? g.js + f : g.js + t;
+  const int k = (DIR == 1) ? g.ks + t : g.ks + f;
+  const int bl = (DIR == 1) ? 0 : 2;
+  double a[5], b[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    a[v] = s.lr[slr(s, p, bl, v, ii, t, f)];
+    b[v] = s.lr[slr(s, p, bl + 1, v, ii, t, f)];
+  }
+  const double cf = skj(s, p, DIR == 1 ? cg_host::SKJ_COSF2 : cg_host::SKJ_COSF3, k, j);
+  const double sf = skj(s, p, DIR == 1 ? cg_host::SKJ_SINF2 : cg_host::SKJ_SINF3, k, j);
+  const double hc = skj(s, p, DIR == 1 ? cg_host::SKJ_HF2 : cg_host::SKJ_HF3, k, j);
+  const double gi = 1.0 / (sf * sf);
+  if (DIR == 1) {  // PrimToLocal2
+    const double T22 = 1.0 / sqrt(gi), T32 = cf / sqrt(1.0), T33 = sqrt(1.0);
+    double uu2 = a[2], uu3 = a[3];
+    a[2] = T22 * uu2;
+    a[3] = T32 * uu2 + T33 * uu3;
+    uu2 = b[2];
+    uu3 = b[3];
+    b[2] = T22 * uu2;
+    b[3] = T32 * uu2 + T33 * uu3;
+  } else {  // PrimToLocal3
+    const double T22 = sqrt(1.0), T23 = cf / sqrt(1.0), T33 = 1.0 / sqrt(gi);
+    double uu2 = a[2], uu3 = a[3];
+    a[2] = T22 * uu2 + T23 * uu3;
+    a[3] = T33 * uu3;
+    uu2 = b[2];
+    uu3 = b[3];
+    b[2] = T22 * uu2 + T23 * uu3;
+    b[3] = T33 * uu3;
+  }
+  double fx[5];
+  riemann<0, DIR, double>(th, CG_RS_HLLC, a, b, fx);
+  const double txx = fx[1], txy = fx[2], txz = fx[3];
+  if (DIR == 1) {  // FluxToGlobal2
+    const double T22 = sqrt(gi), T32 = -sqrt(gi) * cf / 1.0, T33 = 1.0 / sqrt(1.0);
+    fx[1] = 1.0 * txx;
+    fx[2] = T22 * txy;
+    fx[3] = T32 * txy + T33 * txz;
+  } else {  // FluxToGlobal3
+    const double T22 = 1.0 / sqrt(1.0), T23 = -cf / 1.0 * sqrt(gi), T33 = sqrt(gi);
+    fx[1] = 1.0 * txx;
+    fx[2] = T22 * txy + T23 * txz;
+    fx[3] = T33 * txz;
+  }
+  const double ty = fx[2], tz = fx[3];
+  fx[2] = ty + tz * hc;
+  fx[3] = tz + ty * hc;
+  double* F = (DIR == 1) ? s.f2[p] : s.f3[p];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = fx[v];
+}
+
+// areas and volume of a panel cell (gnomonic_equiangle.cpp:221-260, 407-416)
+__device__ __forceinline__ double sph_a1(const KSph& s, int p, int k, int j, int i) {
+  return skj(s, p, cg_host::SKJ_D2A, k, j) * skj(s, p, cg_host::SKJ_D3A, k, j) * s.x1f[i] *
+         s.x1f[i] * skj(s, p, cg_host::SKJ_SINC, k, j);
+}
+__device__ __forceinline__ double sph_a2(const KSph& s, int p, int k, int j, int i) {
+  return s.x1v[i] * skj(s, p, cg_host::SKJ_D3AF2, k, j) * s.dx1;
+}
+__device__ __forceinline__ double sph_a3(const KSph& s, int p, int k, int j, int i) {
+  return s.x1v[i] * skj(s, p, cg_host::SKJ_D2AF3, k, j) * s.dx1;
+}
+__device__ __forceinline__ double sph_v(const KSph& s, int p, int k, int j, int i) {
+  return s.dx1 * skj(s, p, cg_host::SKJ_D2A, k, j) * skj(s, p, cg_host::SKJ_D3A, k, j) *
+         s.x1v[i] * s.x1v[i] * skj(s, p, cg_host::SKJ_SINC, k, j);
+}
+
+// du = -wght div F (areas, volume) + geometric terms + gravity + forcing
+__global__ void __launch_bounds__(256)
+    k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp) {
+  const int p = blockIdx.y;
+  int i, j, k;
+  if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const KGrid& g = s.g;
+  const double wght = beta * dtp[0];
+  const double* F1 = s.f1[p];
+  const double* F2 = s.f2[p];
+  const double* F3 = s.f3[p];
+  const double a1l = sph_a1(s, p, k, j, i), a1r = sph_a1(s, p, k, j, i + 1);
+  const double a2l = sph_a2(s, p, k, j, i), a2r = sph_a2(s, p, k, j + 1, i);
+  const double a3l = sph_a3(s, p, k, j, i), a3r = sph_a3(s, p, k + 1, j, i);
+  const double vol = sph_v(s, p, k, j, i);
+  double d[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double dflx = a1r * F1[g.f[0].at(v, k, j, i + 1)] - a1l * F1[g.f[0].at(v, k, j, i)];
+    dflx += a2r * F2[g.f[1].at(v, k, j + 1, i)] - a2l * F2[g.f[1].at(v, k, j, i)];
+    dflx += a3r * F3[g.f[2].at(v, k + 1, j, i)] - a3l * F3[g.f[2].at(v, k, j, i)];
+    d[v] = 0.0 - wght * dflx / vol;
+  }
+  const long long c0 = g.at(0, k, j, i);
+  const double* w = s.w[p];
+  const double rho = w[c0], v1 = w[c0 + g.c.n], v2 = w[c0 + 2 * g.c.n],
+               v3 = w[c0 + 3 * g.c.n], pr = w[c0 + 4 * g.c.n];
+  // AddCoordTermsDivergence
+  {
+    const double cosine = skj(s, p, cg_host::SKJ_COSC, k, j);
+    const double sinc = skj(s, p, cg_host::SKJ_SINC, k, j);
+    const double sine2 = sinc * sinc;
+    const double radius = s.x1v[i];
+    const double v_2 = v2 + v3 * cosine;
+    const double v_3 = v3 + v2 * cosine;
+    const double src1 = (2.0 * pr + rho * (v2 * v_2 + v3 * v_3)) / radius;
+    d[1] += wght * src1;
+    const double xr = (a2l * skj(s, p, cg_host::SKJ_SINF2, k, j) -
+                       a2r * skj(s, p, cg_host::SKJ_SINF2, k, j + 1)) /
+                      vol;
+    const double yr = (a3l * skj(s, p, cg_host::SKJ_SINF3, k, j) -
+                       a3r * skj(s, p, cg_host::SKJ_SINF3, k + 1, j)) /
+                      vol;
+    const double src2 = -xr * (pr + rho * v3 * v3 * sine2) - rho * v1 * v_2 / radius;
+    d[2] += wght * src2;
+    const double src3 = -yr * (pr + rho * v2 * v2 * sine2) - rho * v1 * v_3 / radius;
+    d[3] += wght * src3;
+  }
+  // gravity (ConstantAcceleration)
+  if (g.grav != 0.0) {
+    const double src = wght * rho * g.grav;
+    d[1] += src;
+    d[4] += src * v1;
+  }
+  const double cth = skj(s, p, cg_host::SKJ_COSC, k, j);
+  const double sl = skj(s, p, cg_host::SKJ_SINLAT, k, j);
+  const double cl = skj(s, p, cg_host::SKJ_COSLAT, k, j);
+  if (s.omega != 0.) {  // Coriolis in (U, V) (hs94.cpp:77-96)
+    const double f = 2. * s.omega * sl;
+    double U = skj(s, p, cg_host::SKJ_R00, k, j) * v2 + skj(s, p, cg_host::SKJ_R01, k, j) * v3;
+    const double V =
+        skj(s, p, cg_host::SKJ_R10, k, j) * v2 + skj(s, p, cg_host::SKJ_R11, k, j) * v3;
+    U = -U;
+    const double m2 = rho * U, m3 = rho * V;
+    const double accU = f * m3, accV = -f * m2;
+    const double a2 = skj(s, p, cg_host::SKJ_A00, k, j) * (-accU) +
+                      skj(s, p, cg_host::SKJ_A01, k, j) * accV;
+    const double a3 = skj(s, p, cg_host::SKJ_A10, k, j) * (-accU) +
+                      skj(s, p, cg_host::SKJ_A11, k, j) * accV;
+    const double b2 = a2 + a3 * cth, b3 = a3 + a2 * cth;
+    d[2] += wght * b2;
+    d[3] += wght * b3;
+  }
+  if (s.hs94) {  // Newtonian relaxation and Rayleigh drag (hs94.cpp:103-150)
+    const double scaled_z = pr / s.p0;
+    const double sigma = pr / s.p0;
+    double sigma_p = (sigma - s.sigmab) / (1. - s.sigmab);
+    const double Kv = (sigma_p < 0.0) ? 0.0 : sigma_p * s.kf;
+    double Teq_p = s.ts - s.dt_eq * (sl * sl) - s.dtheta * log_d(scaled_z) * (cl * cl);
+    Teq_p *= exp_d(s.kappa * log_d(scaled_z));
+    const double Teq = (Teq_p > 200.) ? Teq_p : 200.;
+    const double temp = pr / rho / s.rd;
+    sigma_p = (sigma_p < 0.0) ? 0.0 : sigma_p * (cl * cl * cl * cl);
+    const double Kt = s.ka + (s.ks - s.ka) * sigma_p;
+    const double m1 = rho * v1, m2 = rho * v2, m3 = rho * v3;
+    const double n2 = m2 + m3 * cth, n3 = m3 + m2 * cth;
+    d[1] += -wght * Kv * m1;
+    d[2] += -wght * Kv * n2;
+    d[3] += -wght * Kv * n3;
+    d[4] += -wght * (s.cp - s.rd) * rho * Kt * (temp - Teq);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) s.du[p][c0 + v * g.c.n] = d[v];
+}
+
+// x1 columns of panel blockIdx.y: forward sweep with the panel geometry
+template <int B>
+__global__ void __launch_bounds__(64)
+    k_sph_implicit_fwd(KSph s, KThermo th, double beta, const double* __restrict__ dtp,
+                       double* __restrict__ arch, long long arch_panel,
+                       int* __restrict__ status) {
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int j = g.js + (int)(col % g.nx2);
+  const int k = g.ks + (int)(col / g.nx2);
+  KColGeo cgeo;
+  cgeo.x1f = s.x1f;
+  cgeo.x1v = s.x1v;
+  cgeo.dx1 = s.dx1;
+  cgeo.d2a = skj(s, p, cg_host::SKJ_D2A, k, j);
+  cgeo.d3a = skj(s, p, cg_host::SKJ_D3A, k, j);
+  cgeo.cosc = skj(s, p, cg_host::SKJ_COSC, k, j);
+  cgeo.sinc = skj(s, p, cg_host::SKJ_SINC, k, j);
+  if (!fwd_column<0, B, true>(g, th, beta * dtp[0], s.w[p], s.du[p], g.at(0, k, j, 0),
+                              arch + p * arch_panel, ncol, col, nullptr, 0, &cgeo))
+    atomicOr(status, 1);
+}
+
+template <int B>
+__global__ void __launch_bounds__(64)
+    k_sph_implicit_bwd(KSph s, const double* __restrict__ arch, long long arch_panel) {
+  constexpr int AS = B * B + B;
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int j = g.js + (int)(col % g.nx2);
+  const int k = g.ks + (int)(col / g.nx2);
+  const double* ap = arch + p * arch_panel;
+  double* du = s.du[p];
+  const long long sn = g.c.n;
+  Vt<B> dn;
+  for (int i = g.ie; i >= g.is; --i) {
+    Vt<B> d;
+    back_step<B>(ap, (long long)(i - g.is) * AS, ncol, col, i == g.ie, dn, d);
+    dn = d;
+    const long long ci = g.at(0, k, j, i);
+    double duo[5], wsp[5], dun[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      duo[n] = du[ci + n * sn];
+      wsp[n] = 0.0;
+    }
+    write_back<0, B>(d, duo, wsp, dun);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) du[ci + n * sn] = dun[n];
+  }
+}
+
+// end of stage: u1 / u averaging, u += du, ConservedToPrimitive
+__global__ void __launch_bounds__(256) k_sph_update(KSph s, KUpdate up) {
+  const int p = blockIdx.y;
+  int i, j, k;
+  if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const KGrid& g = s.g;
+  const long long c0 = g.at(0, k, j, i);
+  double uc[5], wc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const long long c = c0 + v * g.c.n;
+    const double un = s.u[p][c];
+    double x;
+    if (up.stage == 0) {
+      x = 0.0 + 1.0 * un;
+      s.u1[p][c] = x;
+    } else {
+      x = up.g1 * un + up.g2 * s.u1[p][c];
+    }
+    uc[v] = x + s.du[p][c];
+  }
+  sph_cons_to_prim(s.gm1, skj(s, p, cg_host::SKJ_COSC, k, j), uc, wc);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    s.u[p][c0 + v * g.c.n] = uc[v];
+    s.w[p][c0 + v * g.c.n] = wc[v];
+  }
+}
+
+// x2 / x3 ghost zones of every panel from the neighbours' conserved state
+__global__ void __launch_bounds__(256)
+    k_sph_ghosts(KSph s, const KSphPtrs* __restrict__ ptr, const SphGhost* __restrict__ gh,
+                 int nghost) {
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= (long long)nghost * s.nr) return;
+  const SphGhost h = gh[item / s.nr];
+  const int i = s.g.is + (int)(item % s.nr);
+  const KGrid& g = s.g;
+  const double* uq = ptr->u[h.q];
+  double* wp = ptr->w[h.p];
+  const long long a1 = g.at(0, h.k1, h.j1, i), a2 = g.at(0, h.k2, h.j2, i);
+  double ug[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double x1 = uq[a1 + v * g.c.n], x2 = uq[a2 + v * g.c.n];
+    ug[v] = ((h.y2 - h.yg) * x1 + (h.yg - h.y1) * x2) / (h.y2 - h.y1);
+  }
+  const double vy = ug[2], vz = ug[3];
+  const double c2 = vy / h.sq / h.sq - vz * h.cq / h.sq / h.sq;
+  const double c3 = vz / h.sq / h.sq - vy * h.cq / h.sq / h.sq;
+  const double U = h.r00 * c2 + h.r01 * c3;
+  const double V = h.r10 * c2 + h.r11 * c3;
+  const double d2 = h.a00 * U + h.a01 * V;
+  const double d3 = h.a10 * U + h.a11 * V;
+  ug[2] = d2 + d3 * h.cp;
+  ug[3] = d3 + d2 * h.cp;
+  double wg[5];
+  sph_cons_to_prim(s.gm1, ptr->kj[h.p][(long long)cg_host::SKJ_COSC * s.nc * s.nc +
+                                       (long long)h.kg * s.nc + h.jg],
+                   ug, wg);
+  const long long c0 = g.at(0, h.kg, h.jg, i);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) wp[c0 + v * g.c.n] = wg[v];
+}
+
+}  // namespace cg

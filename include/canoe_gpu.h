@@ -45,7 +45,12 @@ enum cg_status {
   CG_ERR_IO = 10        /* a file could not be opened, read or written */
 };
 
-enum cg_bc { CG_BC_REFLECT = 0, CG_BC_PERIODIC = 1, CG_BC_OUTFLOW = 2 };
+enum cg_bc {
+  CG_BC_REFLECT = 0,
+  CG_BC_PERIODIC = 1,
+  CG_BC_OUTFLOW = 2,
+  CG_BC_PANEL = 3 /* x2 / x3 side of a cubed-sphere panel: the neighbour panel */
+};
 enum cg_rsolver { CG_RS_LMARS = 0, CG_RS_HLLC = 1 };
 enum cg_precision { CG_PREC_FP64 = 0, CG_PREC_FP32 = 1 };
 enum cg_satadj { CG_SATADJ_OFF = 0, CG_SATADJ_NEWTON = 1, CG_SATADJ_MLP = 2 };
@@ -99,6 +104,20 @@ typedef struct cg_config {
    * precipitation along -x1 (m/s). */
   double k_auto, k_accr, k_evap, v_fall;
 } cg_config;
+
+/* Cubed-sphere run (config 3; src/exo3, examples/2023-Chen-exo3/hs94.cpp):
+ * rotation and the Held-Suarez (1994) forcing of hs94.cpp:56-151 (Newtonian
+ * relaxation toward T_eq and Rayleigh drag below sigma_b). */
+typedef struct cg_sphere {
+  double omega;   /* planetary rotation, 1/s (Coriolis force; 0 = off) */
+  int hs94;       /* 1: Held-Suarez relaxation and drag, 0: off */
+  double ts;      /* T_eq at the surface and equator, K (Ts) */
+  double dt_eq;   /* equator-pole temperature difference, K (dT) */
+  double dtheta;  /* vertical potential temperature difference, K */
+  double p0;      /* reference surface pressure, Pa */
+  double kf, ka, ks; /* drag, free-atmosphere and surface relaxation rates, 1/s */
+  double sigmab;  /* top of the frictional boundary layer, p / p0 */
+} cg_sphere;
 
 typedef struct cg_ctx cg_ctx;
 
@@ -447,6 +466,26 @@ int cg_run_set_tune(cg_run* run, const cg_tune* tune);
  * restored after every trial) and keeps the fastest. */
 int cg_run_autotune(cg_run* run, double dt, cg_tune* out);
 
+/* ---- cubed sphere (config 3) ------------------------------------------- *
+ * Six gnomonic-equiangle panels (src/exo3/gnomonic_equiangle.cpp, panel
+ * numbering of src/exo3/cs_find_block_id.cpp:11-70 and positions of
+ * cubed_sphere.cpp:78-89) as the six blocks of one run: `panel` describes
+ * each (nx1 levels from the inner radius x1min with spacing dx1; nx2 = nx3
+ * cells of pi / (2 nx2) per side; reflecting x1 walls; x2 / x3 sides
+ * CG_BC_PANEL).  The step is the reference's CUBED_SPHERE path: WENO5 x1
+ * reconstruction, HLLC with the panel frames (hllc_transform.cpp), the x2 /
+ * x3 face states exchanged across panel edges (cubed_sphere.cpp:250-565),
+ * ghost zones interpolated from the neighbour panels
+ * (cubed_sphere_utility.cpp:80-400), areas, volumes and geometric source
+ * terms of the panel metric, covariant momenta in u and contravariant
+ * velocities in w (patches/26.adiabatic_hydro.patch), the x1 column solve
+ * with the panel areas (implicit_flag 0, 1 or 9), gravity and the Coriolis /
+ * Held-Suarez forcing of cg_sphere.  Dry gas, FP64.  The cg_run_* calls
+ * taking a block index address panel b (0-5: panels 1-6); the time-step,
+ * check, history, debug, sums and autotune calls return CG_ERR_ARG. */
+int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
+                     const cg_config* cfg, const cg_sphere* sp, cg_run** out);
+
 /* ---- problem generators and host boundaries (host code) ----------------- *
  * The configurations' MeshBlock::ProblemGenerator and input-file settings
  * (examples/2019-Li-snap/straka.cpp:220-268 and straka.inp for config 1;
@@ -462,6 +501,16 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
 int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
                  const cg_thermo* th, const cg_config* cfg, double* w);
 int cg_apply_bc_host(const cg_block* blk, double* w);
+/* Config 3 (BASELINE's cubed-sphere GCM step; examples/2023-Chen-exo3/
+ * hs94.cpp, hs94.inp): cg_pgen_setup(3, ...) gives the panel block (nx1 = 64
+ * levels over 25 km from R = 6.371e6 m, 96 x 96 cells per panel, g = 9.81),
+ * dry air and the run settings; cg_pgen_sphere fills the rotation and
+ * Held-Suarez parameters of hs94.inp; cg_pgen_sphere_init writes the six
+ * panels' interior primitives (isothermal 300 K hydrostatic atmosphere at
+ * rest plus seeded 0.1 K white noise), six consecutive arrays. */
+int cg_pgen_sphere(cg_sphere* sp);
+int cg_pgen_sphere_init(uint64_t seed, const cg_block* panel, const cg_thermo* th,
+                        double* w6);
 
 /* Library build string (arch, flags). */
 const char* cg_version(void);

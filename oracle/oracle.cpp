@@ -974,10 +974,23 @@ bool periodic_sweep(int n, const Mat<B>* d, const Mat<B>* L, const Mat<B>* U,
 // only du of IDN, the normal momentum and IEN is written back: the
 // reference shuffles only those three out of its line buffer
 // (solve_implicit_3d.cpp:51-56, 82-87).
+// Geometry of one x1 column of a cubed-sphere panel (GnomonicEquiangle,
+// src/exo3/gnomonic_equiangle.cpp:221-231, 407-416): face areas
+// dx2f_ang dx3f_ang x1f^2 sin(theta), volumes dx1f dx2f_ang dx3f_ang x1v^2
+// sin(theta); the column's tangential velocities are projected onto the
+// local orthonormal frame around the solve (solve_implicit_3d.cpp:103-136:
+// v2 += v3 cos(theta), v3 *= sin(theta); the projection is undone on w
+// afterwards, so a projected copy is used here).
+struct LineGeo {
+  const double* x1f;  // face radii, nc1 + 1
+  const double* x1v;  // cell radii, nc1
+  double dx1, d2a, d3a, cosc, sinc;
+};
+
 template <int B>
 bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
                      double* du, const double* w, double dt, int dir, int k,
-                     int j, bool periodic) {
+                     int j, bool periodic, const LineGeo* lg = nullptr) {
   const int N = g.N, IVX = g.IVX, IEN = g.IPR;
   int is, ie, nc;
   if (dir == 0) {
@@ -1011,6 +1024,10 @@ bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
   double wl[5 + CG_MAX_NMASS], wr[5 + CG_MAX_NMASS], prim[5 + CG_MAX_NMASS];
   auto load = [&](double* dst, int i) {
     for (int n = 0; n < N; ++n) dst[n] = w[at(n, i)];
+    if (lg) {
+      dst[g.IVY] = dst[g.IVY] + dst[g.IVZ] * lg->cosc;
+      dst[g.IVZ] = dst[g.IVZ] * lg->sinc;
+    }
   };
   for (int i = is - 2; i <= ie + 1; ++i) {
     load(wr, i);
@@ -1032,7 +1049,7 @@ bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
   //       c_i = -(A+ - dfdq_{i+1}) aright / (2 vol)   (:133-137),
   // with the division by 2 vol taken as a product with its reciprocal.
   const double s2v = 1. / (2. * vol);
-  const double sl = aleft * s2v, sr = aright * s2v, sd = (aright - aleft) * s2v;
+  const double sl0 = aleft * s2v, sr0 = aright * s2v, sd0 = (aright - aleft) * s2v;
   const double idt = 1. / dt;
   for (int i = is - 1; i <= ie; ++i) {
     load(wl, i);
@@ -1040,6 +1057,16 @@ bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
     gm = 0.5 * (gm1 + gm1);
     roe_average(g, prim, gm, wl, wr);
     Mat<B> Ap = sub5<B>(abs_roe_matrix(g, th, prim, gm, dir));
+    double sl = sl0, sr = sr0, sd = sd0;
+    if (lg) {  // GetFace1Area(k, j, i), (k, j, i + 1), GetCellVolume(k, j, i)
+      const double al = lg->d2a * lg->d3a * lg->x1f[i] * lg->x1f[i] * lg->sinc;
+      const double ar = lg->d2a * lg->d3a * lg->x1f[i + 1] * lg->x1f[i + 1] * lg->sinc;
+      const double v = lg->dx1 * lg->d2a * lg->d3a * lg->x1v[i] * lg->x1v[i] * lg->sinc;
+      const double s2 = 1. / (2. * v);
+      sl = al * s2;
+      sr = ar * s2;
+      sd = (ar - al) * s2;
+    }
     for (int r = 0; r < B; ++r)
       for (int q = 0; q < B; ++q) {
         double av = std::fma(Am.a[r][q], sl,
@@ -1757,6 +1784,8 @@ double new_mesh_dt(double dt_old, double blocks_dt, double time, double tlim) {
   return dt;
 }
 
+#include "oracle_sphere.inc"
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -2128,6 +2157,100 @@ int or_sums(const cg_block* blk, const double* u, double* out) {
     out[n] = s;
   }
   return CG_OK;
+}
+
+// ---- cubed sphere (oracle_sphere.inc) ------------------------------------
+int or_sph_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+                const cg_sphere* sp, double dt, int nsteps, double* u, double* w) {
+  g_threads = 1;
+  return sph_step(*blk, *th, *cfg, *sp, dt, nsteps, u, w);
+}
+
+int or_sph_fill_ghosts(const cg_block* blk, const cg_thermo* th, const double* u,
+                       double* w) {
+  Grid g(*blk);
+  if (g.nmass != 0 || g.nx2 != g.nx3) return CG_ERR_ARG;
+  std::vector<SphPanel> P;
+  for (int p = 1; p <= 6; ++p) P.push_back(sph_panel(*blk, p));
+  sph_fill_ghosts(g, P, th->gammad - 1., u, w);
+  sph_x1_walls(g, *blk, w);
+  return CG_OK;
+}
+
+int or_sph_prim_to_cons(const cg_block* blk, const cg_thermo* th, const double* w,
+                        double* u) {
+  Grid g(*blk);
+  if (g.nmass != 0 || g.nx2 != g.nx3) return CG_ERR_ARG;
+  const size_t ps = (size_t)g.N * g.ncell;
+  for (int p = 1; p <= 6; ++p) {
+    SphPanel P = sph_panel(*blk, p);
+    for (int k = g.ks; k <= g.ke; ++k)
+      for (int j = g.js; j <= g.je; ++j)
+        for (int i = g.is; i <= g.ie; ++i) {
+          double wc[5], uc[5];
+          for (int v = 0; v < 5; ++v) wc[v] = w[(p - 1) * ps + g.at(v, k, j, i)];
+          sph_prim_to_cons_cell(th->gammad - 1., P.cosc[P.kj(k, j)], wc, uc);
+          for (int v = 0; v < 5; ++v) u[(p - 1) * ps + g.at(v, k, j, i)] = uc[v];
+        }
+  }
+  return CG_OK;
+}
+
+/* metric array `what` of panel p over the (k, j) range of the block with
+ * ghosts (nc x nc): 0 cos, 1 sin at cells, 2 cos, 3 sin at x2 faces, 4 cos,
+ * 5 sin at x3 faces, 6 dx2f_ang, 7 dx3f_ang, 8 dx2f_ang at x3 faces, 9
+ * dx3f_ang at x2 faces, 10 latitude, 11 longitude; 12 cell volumes and 13
+ * x1-face areas (lower faces) of the interior, (nx3, nx2, nx1). */
+int or_sph_metric(const cg_block* blk, int p, int what, double* out) {
+  Grid g(*blk);
+  SphPanel P = sph_panel(*blk, p);
+  const std::vector<double>* a[12] = {&P.cosc, &P.sinc, &P.cosf2, &P.sinf2, &P.cosf3,
+                                      &P.sinf3, &P.d2a, &P.d3a, &P.d2af3, &P.d3af2,
+                                      &P.lat, &P.lon};
+  if (what >= 0 && what < 12) {
+    std::copy(a[what]->begin(), a[what]->end(), out);
+    return CG_OK;
+  }
+  size_t o = 0;
+  for (int k = g.ks; k <= g.ke; ++k)
+    for (int j = g.js; j <= g.je; ++j)
+      for (int i = g.is; i <= g.ie; ++i)
+        out[o++] = (what == 12) ? sph_vol(P, k, j, i) : sph_area1(P, k, j, i);
+  return (what == 12 || what == 13) ? CG_OK : CG_ERR_ARG;
+}
+
+int or_sph_forcing(const cg_block* blk, const cg_thermo* th, const cg_sphere* sp, double wght,
+                   const double* w6, double* du6) {
+  Grid g(*blk);
+  if (g.nmass != 0 || g.nx2 != g.nx3) return CG_ERR_ARG;
+  const size_t ps = (size_t)g.N * g.ncell;
+  for (int p = 1; p <= 6; ++p) {
+    SphPanel P = sph_panel(*blk, p);
+    sph_forcing(g, P, *th, *sp, p, wght, w6 + (p - 1) * ps, du6 + (p - 1) * ps);
+  }
+  return CG_OK;
+}
+
+int or_sph_link(int p, int s, int* q, int* sq, int* inv) {
+  SphLink L = sph_link(p, s);
+  *q = L.q;
+  *sq = L.sq;
+  *inv = L.inv;
+  return CG_OK;
+}
+
+void or_sph_latlon(int p, double X, double Y, double* lat, double* lon) {
+  sph_latlon(p, X, Y, lat, lon);
+}
+
+void or_sph_rll_from_abp(double X, double Y, int block, double v2, double v3, double* U,
+                         double* V) {
+  sph_rll_from_abp(X, Y, block, v2, v3, U, V);
+}
+
+void or_sph_abp_from_rll(double X, double Y, int block, double U, double V, double* v2,
+                         double* v3) {
+  sph_abp_from_rll(X, Y, block, U, V, v2, v3);
 }
 
 }  // extern "C"

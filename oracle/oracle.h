@@ -100,6 +100,28 @@ int or_roe_matrices(const cg_block* blk, const cg_thermo* th, double gm1,
                     const double* wl, const double* wr, double* abs_a,
                     double* prim_out);
 
+/* Cubed sphere (oracle_sphere.inc): six panels of the block `blk` (nx1
+ * levels from radius x1min, nx2 = nx3 cells per panel side, dry), u and w
+ * six consecutive AthenaArrays.  or_sph_step: nsteps SSPRK(3,3) steps;
+ * or_sph_fill_ghosts: w's x2 / x3 ghosts from the neighbours' u and the x1
+ * walls; or_sph_prim_to_cons: interior u from w. */
+int or_sph_step(const cg_block* blk, const cg_thermo* th, const cg_config* cfg,
+                const cg_sphere* sp, double dt, int nsteps, double* u, double* w);
+int or_sph_fill_ghosts(const cg_block* blk, const cg_thermo* th, const double* u,
+                       double* w);
+int or_sph_prim_to_cons(const cg_block* blk, const cg_thermo* th, const double* w,
+                        double* u);
+/* Coriolis and Held-Suarez terms of the six panels added to du6 over wght */
+int or_sph_forcing(const cg_block* blk, const cg_thermo* th, const cg_sphere* sp, double wght,
+                   const double* w6, double* du6);
+int or_sph_metric(const cg_block* blk, int p, int what, double* out);
+int or_sph_link(int p, int s, int* q, int* sq, int* inv);
+void or_sph_latlon(int p, double X, double Y, double* lat, double* lon);
+void or_sph_rll_from_abp(double X, double Y, int block, double v2, double v3, double* U,
+                         double* V);
+void or_sph_abp_from_rll(double X, double Y, int block, double U, double V, double* v2,
+                         double* v3);
+
 /* Saturation-guess network (oracle_mlp.cpp).  Parameters: or_mlp_nparam()
  * floats, layout W1 (64x4), b1, W2 (64x64), b2, W3 (64x64), b3, W4 (3x64),
  * b4.  emulate = 1 rounds as the device does (bf16 operands, fp32 sums). */

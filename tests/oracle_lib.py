@@ -35,6 +35,19 @@ _SIGS = {
     "or_saturation_adjust": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "or_step": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "or_kessler": (_I, [_P, _P, _P, _D, _P]),
+    "or_sph_step": (_I, [_P, _P, _P, _P, _D, _I, _P, _P]),
+    "or_sph_fill_ghosts": (_I, [_P, _P, _P, _P]),
+    "or_sph_prim_to_cons": (_I, [_P, _P, _P, _P]),
+    "or_sph_forcing": (_I, [_P, _P, _P, _D, _P, _P]),
+    "or_sph_metric": (_I, [_P, _I, _I, _P]),
+    "or_sph_link": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                         ctypes.POINTER(_I)]),
+    "or_sph_latlon": (None, [_I, _D, _D, ctypes.POINTER(_D),
+                             ctypes.POINTER(_D)]),
+    "or_sph_rll_from_abp": (None, [_D, _D, _I, _D, _D, ctypes.POINTER(_D),
+                                   ctypes.POINTER(_D)]),
+    "or_sph_abp_from_rll": (None, [_D, _D, _I, _D, _D, ctypes.POINTER(_D),
+                                   ctypes.POINTER(_D)]),
     "or_step_omp": (_I, [_P, _P, _P, _D, _I, _P, _P, _I]),
     "or_sums": (_I, [_P, _P, _P]),
     "or_new_block_dt": (_D, [_P, _P, _P, _P, _D]),
@@ -274,3 +287,73 @@ def mlp_pairs(th, rho, ie, y, npairs, seed):
                            npairs, seed, ptr(x), ptr(t), ptr(it))
     assert r == 0
     return x, t, it
+
+
+# ------------------------------------------------------------ cubed sphere
+def sph_prim_to_cons(blk, th, w6):
+    w6 = np.ascontiguousarray(w6, np.float64)
+    u6 = np.zeros_like(w6)
+    assert lib().or_sph_prim_to_cons(ref(blk), ref(th), ptr(w6), ptr(u6)) == 0
+    return u6
+
+
+def sph_fill_ghosts(blk, th, u6, w6):
+    """w6 with its x2 / x3 ghosts filled from the neighbours' u6 and the x1
+    walls."""
+    u6 = np.ascontiguousarray(u6, np.float64)
+    w6 = np.array(w6, dtype=np.float64, copy=True)
+    assert lib().or_sph_fill_ghosts(ref(blk), ref(th), ptr(u6), ptr(w6)) == 0
+    return w6
+
+
+def sph_step(blk, th, cfg, sp, dt, nsteps, u6, w6):
+    u6 = np.array(u6, dtype=np.float64, copy=True)
+    w6 = np.array(w6, dtype=np.float64, copy=True)
+    r = lib().or_sph_step(ref(blk), ref(th), ref(cfg), ref(sp), dt, nsteps,
+                          ptr(u6), ptr(w6))
+    assert r == 0, r
+    return u6, w6
+
+
+def sph_metric(blk, p, what):
+    """Metric array `what` of panel p (1-6): (nc, nc) tables for 0-11, the
+    interior (nx3, nx2, nx1) volumes (12) or lower x1-face areas (13)."""
+    if what < 12:
+        out = np.zeros((blk.nx2 + 6, blk.nx2 + 6))
+    else:
+        out = np.zeros((blk.nx3, blk.nx2, blk.nx1))
+    assert lib().or_sph_metric(ref(blk), p, what, ptr(out)) == 0
+    return out
+
+
+def sph_link(p, s):
+    q, sq, inv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().or_sph_link(p, s, ctypes.byref(q), ctypes.byref(sq), ctypes.byref(inv))
+    return q.value, sq.value, inv.value
+
+
+def sph_latlon(p, X, Y):
+    lat, lon = ctypes.c_double(), ctypes.c_double()
+    lib().or_sph_latlon(p, X, Y, ctypes.byref(lat), ctypes.byref(lon))
+    return lat.value, lon.value
+
+
+def sph_rll_from_abp(X, Y, block, v2, v3):
+    U, V = ctypes.c_double(), ctypes.c_double()
+    lib().or_sph_rll_from_abp(X, Y, block, v2, v3, ctypes.byref(U), ctypes.byref(V))
+    return U.value, V.value
+
+
+def sph_abp_from_rll(X, Y, block, U, V):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().or_sph_abp_from_rll(X, Y, block, U, V, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def sph_forcing(blk, th, sp, wght, w6):
+    """Coriolis + Held-Suarez terms of du over the step wght."""
+    w6 = np.ascontiguousarray(w6, np.float64)
+    du = np.zeros_like(w6)
+    assert lib().or_sph_forcing(ref(blk), ref(th), ref(sp), wght, ptr(w6), ptr(du)) == 0
+    return du
+
