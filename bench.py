@@ -719,6 +719,67 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     return out
 
 
+def bench_sphere(ctx, steps, warmup, dt=60.0, nx=(64, 96, 96), acc_steps=20):
+    """Config 3: the cubed sphere of six 96 x 96 panels of 64 levels with the
+    Held-Suarez forcing and rotation, 3 x 3 implicit x1 columns (hs94.inp),
+    fixed step dt, in the plain FP64 form and the FP32-flux form (x2 / x3
+    face states and Riemann fluxes in FP32).  ms per step (graph replay),
+    per-kernel event averages, the state after the timed steps, and the FP32
+    form's error against FP64 after acc_steps steps."""
+    blk, th, cfg = ca.pgen_setup(3, *nx)
+    sp = ca.pgen_sphere()
+    w = ca.pgen_sphere_init(2024, blk, th)
+    cells = 6 * blk.ncells()
+    sk, sj, si = blk.interior()
+    I = (slice(None), slice(None), sk, sj, si)
+    log("cubed sphere 6x%dx%dx%d, dt=%.1f s" % (blk.nx2, blk.nx3, blk.nx1, dt))
+    forms, finals = {}, {}
+    for name, prec in (("plain_fp64", ca.PREC_FP64), ("fp32_faces", ca.PREC_FP32)):
+        cfg.precision = prec
+        run = ca.SphereRun(ctx, blk, th, cfg, sp)
+        try:
+            run.set_all(w)
+            run.step(dt, warmup, graph=USE_GRAPH)
+            t0 = time.perf_counter()
+            run.step(dt, steps, graph=USE_GRAPH)
+            t1 = time.perf_counter()
+            ms = 1e3 * (t1 - t0) / steps
+            u1 = run.cons_all()
+            run.set_all(w)
+            run.step(dt, acc_steps, graph=USE_GRAPH)
+            finals[name] = run.cons_all()
+            run.set_timing(True)
+            nt = max(2, min(steps, 5))
+            run.step(dt, nt, graph=False)
+            stats = {k: (v[0] / v[1], v[1] / nt) for k, v in run.kernel_stats().items()}
+            run.set_timing(False)
+        finally:
+            run.close()
+        vmax = float(np.abs(u1[:, 1:4][:, :, sk, sj, si] / u1[:, 0:1][:, :, sk, sj, si]).max())
+        forms[name] = {"ms_per_step": round(ms, 4),
+                       "cell_updates_per_s": cells / (ms * 1e-3),
+                       "finite": bool(np.isfinite(u1[I]).all()),
+                       "max_momentum_over_density": round(vmax, 4),
+                       "kernels_avg_ms": {k: round(v[0], 4) for k, v in stats.items()},
+                       "kernel_launches_per_step": {k: round(v[1], 2)
+                                                    for k, v in stats.items()}}
+        log("sphere %s: %.3f ms/step, %.3e cell-updates/s"
+            % (name, ms, forms[name]["cell_updates_per_s"]))
+    a, b = finals["fp32_faces"][I], finals["plain_fp64"][I]
+    names = ["density", "momentum1", "momentum2", "momentum3", "energy"]
+    err = {names[n]: float(np.linalg.norm(a[:, n] - b[:, n]) / np.linalg.norm(b[:, n]))
+           for n in range(5)}
+    err["steps"] = acc_steps
+    forms["fp32_faces"]["rel_l2_vs_fp64"] = err
+    forms["fp32_faces"]["speedup_vs_plain"] = round(
+        forms["plain_fp64"]["ms_per_step"] / forms["fp32_faces"]["ms_per_step"], 3)
+    return {"workload": "cubed sphere 6 x %d x %d x %d (R 6.371e6 m, 25 km deep), "
+                        "WENO5 + HLLC with panel frames, 3x3 implicit x1, "
+                        "Held-Suarez forcing, Coriolis, SSPRK(3,3)"
+                        % (blk.nx2, blk.nx3, blk.nx1),
+            "cells": cells, "dt": dt, "steps": steps, "forms": forms}
+
+
 # the launch shapes the autotuner picks between on one MI355X: column solve
 # with 4 waves per workgroup on one workgroup per compute unit, sweeps fused
 # per group or in two launches; 32 x 8 face tiles marching 16 levels
@@ -783,6 +844,8 @@ def main():
     ap.add_argument("--mlp", action="store_true",
                     help="short fixed run of the network for rocprofv3 "
                          "(pairs, a few training steps, 3 forwards of 8M)")
+    ap.add_argument("--sphere", action="store_true",
+                    help="only the cubed sphere (config 3)")
     ap.add_argument("--lines", action="store_true",
                     help="only the dry box with implicit x2/x3 lines "
                          "(implicit_flag 15)")
@@ -801,6 +864,11 @@ def main():
     if args.mlp:
         mlp_run(args.nx, 200)
         return
+    if args.sphere:
+        ctx = ca.Context(0)
+        print(json.dumps(bench_sphere(ctx, max(1, args.steps), args.warmup)))
+        ctx.close()
+        return
     if args.lines:
         ctx = ca.Context(0)
         print(json.dumps(bench_lines(ctx, args.nx, max(1, args.steps),
@@ -811,8 +879,9 @@ def main():
     ctx = ca.Context(0)
     blk, cfg, dt, cells, forms, acc = bench_box(ctx, args.nx, args.steps,
                                                 args.warmup, args.acc_steps)
-    moist = batch = lines = host = None
+    moist = batch = lines = host = sphere = None
     if not args.quick:
+        sphere = bench_sphere(ctx, max(2, args.steps // 2), 1)
         lines = bench_lines(ctx, args.nx, max(2, args.steps // 5), 1)
         host = bench_host_solve(ctx, args.nx)
         moist = bench_moist(ctx, args.nx, args.steps, args.warmup,
@@ -886,6 +955,7 @@ def main():
         "batch": batch,
         "implicit_lines": lines,
         "implicit_solve_host": host,
+        "cubed_sphere": sphere,
     }
     print(json.dumps(out))
 

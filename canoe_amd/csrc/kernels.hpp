@@ -62,13 +62,11 @@ CG_HD void outflow_ghost_w(const KThermo& th, const double* wb, double psv_b,
 // column (Decomposition::ChangeToTemperature, :40-111).  One column per
 // lane.
 template <int NM>
-__global__ void __launch_bounds__(256)
-    k_hydrostatic_scan(KGrid g, KThermo th, const double* __restrict__ w,
-                       double* __restrict__ psf, double* __restrict__ psv) {
+__device__ __forceinline__ void scan_column(const KGrid& g, const KThermo& th,
+                                            const double* __restrict__ w,
+                                            double* __restrict__ psf,
+                                            double* __restrict__ psv, long long t) {
   constexpr int IPR = 4 + NM;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long ncol = (long long)g.nx2 * g.nx3;
-  if (t >= ncol) return;
   int j = g.js + (int)(t % g.nx2);
   int k = g.ks + (int)(t / g.nx2);
   const double grav = g.grav, dx = g.dx1;
@@ -103,19 +101,32 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <int NM>
+__global__ void __launch_bounds__(256)
+    k_hydrostatic_scan(KGrid g, KThermo th, const double* __restrict__ w,
+                       double* __restrict__ psf, double* __restrict__ psv) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx2 * g.nx3) return;
+  scan_column<NM>(g, th, w, psf, psv, t);
+}
+
 // ---------------------------------------------------------------------------
 // K1a: x1 face fluxes with the hydrostatic decomposition around the
 // reconstruction (patches/19.calculate_fluxes.patch:23-47;
 // change_to_temperature.cpp:107-110 and :155-164).  One lane per face.
+// One x1 face of item t; a non-null hc is the (k, j) table of cos(theta)
+// with which the tangential momentum fluxes are projected to covariant
+// components afterwards (cubed-sphere panels, hllc_transform.cpp:235-272).
 template <int NM, int ORDER>
-__global__ void __launch_bounds__(256)
-    k_flux_x1(KGrid g, KThermo th, int rsolver, int decomp,
-              const double* __restrict__ w, const double* __restrict__ psf,
-              const double* __restrict__ psv, double* __restrict__ f1) {
+__device__ __forceinline__ void flux_x1_face(const KGrid& g, const KThermo& th, int rsolver,
+                                             int decomp, const double* __restrict__ w,
+                                             const double* __restrict__ psf,
+                                             const double* __restrict__ psv,
+                                             double* __restrict__ f1, long long t,
+                                             const double* __restrict__ hc = nullptr,
+                                             int hnc = 0) {
   constexpr int N = 5 + NM, IPR = 4 + NM;
   constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
   int i, j, k;
   g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
   i += g.is;
@@ -175,8 +186,25 @@ __global__ void __launch_bounds__(256)
   }
   double fx[N];
   riemann<NM, 0, double>(th, rsolver, wl, wr, fx);
+  if (hc) {
+    constexpr int IVY = 2 + NM, IVZ = 3 + NM;
+    const double cth = hc[(long long)k * hnc + j];
+    const double ty = fx[IVY], tz = fx[IVZ];
+    fx[IVY] = ty + tz * cth;
+    fx[IVZ] = tz + ty * cth;
+  }
 #pragma unroll
   for (int n = 0; n < N; ++n) f1[g.f[0].at(n, k, j, i)] = fx[n];
+}
+
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_flux_x1(KGrid g, KThermo th, int rsolver, int decomp,
+              const double* __restrict__ w, const double* __restrict__ psf,
+              const double* __restrict__ psv, double* __restrict__ f1) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  flux_x1_face<NM, ORDER>(g, th, rsolver, decomp, w, psf, psv, f1, t);
 }
 
 // K1b/K1c: x2 and x3 face fluxes.  One lane per face.

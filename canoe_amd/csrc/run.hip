@@ -284,18 +284,14 @@ void sphere_fill(cg_run* r) {
   SphRun& S = *r->sph;
   hipStream_t st = r->ctx->stream;
   KTimer* tm = r->tm.on ? &r->tm : nullptr;
+  const KGrid& g = S.k.g;
+  const long long ncol = (long long)g.nx2 * g.nx3;
   TIMED(tm, KID_BC_X2,
         (k_sph_ghosts<<<nblk((long long)S.nghost * S.k.nr, 256), 256, 0, st>>>(
             S.k, S.d_ptr, S.d_ghosts, S.nghost)));
-  for (RunBlock& b : r->blocks) {
-    const KGrid& g = b.g;
-    TIMED(tm, KID_BC_X1,
-          (k_bc_x1<0><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, st>>>(g, b.w)));
-    if (g.grav != 0.0)
-      TIMED(tm, KID_SCAN,
-            (k_hydrostatic_scan<0><<<nblk((long long)g.nx2 * g.nx3, 256), 256, 0, st>>>(
-                g, r->kt, b.w, b.psf, b.psv)));
-  }
+  TIMED(tm, KID_BC_X1, (k_sph_bc_x1<<<dim3(nblk(ncol * NG, 256), 6), 256, 0, st>>>(S.k)));
+  if (g.grav != 0.0)
+    TIMED(tm, KID_SCAN, (k_sph_scan<<<dim3(nblk(ncol, 256), 6), 256, 0, st>>>(S.k, r->kt)));
   S.dirty = false;
 }
 
@@ -312,26 +308,31 @@ int sphere_stage(cg_run* r, int s) {
   const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   const long long nf1 = (long long)(g.nx1 + 1) * g.nx2 * g.nx3;
   const long long nface = (long long)ks.nr * ks.n * (ks.n + 1);
-  for (RunBlock& b : r->blocks)
-    TIMED(tm, KID_FLUX_X1,
-          (k_flux_x1<0, 5><<<nblk(nf1, 256), 256, 0, st>>>(g, r->kt, CG_RS_HLLC,
-                                                           g.grav != 0.0 ? 1 : 0, b.w, b.psf,
-                                                           b.psv, b.f1)));
-  TIMED(tm, KID_FLUX_X1, (k_sph_x1_project<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks)));
-  if (r->cfg.order_h == 5) {
-    TIMED(tm, KID_FLUX_X2, (k_sph_faces<5, 1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
-    TIMED(tm, KID_FLUX_X3, (k_sph_faces<5, 2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
-  } else {
-    TIMED(tm, KID_FLUX_X2, (k_sph_faces<2, 1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
-    TIMED(tm, KID_FLUX_X3, (k_sph_faces<2, 2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks)));
-  }
-  TIMED(tm, KID_FLUX_X2,
-        (k_sph_exchange<<<nblk((long long)S.nedge * ks.nr, 256), 256, 0, st>>>(ks, S.d_edges,
-                                                                             S.nedge)));
-  TIMED(tm, KID_FLUX_X2,
-        (k_sph_face_flux<1><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks, r->kt)));
-  TIMED(tm, KID_FLUX_X3,
-        (k_sph_face_flux<2><<<dim3(nblk(nface, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  TIMED(tm, KID_FLUX_X1, (k_sph_flux_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  // x2 / x3 faces: FP64, or FP32 reconstruction and Riemann fluxes (the
+  // FP32-flux form; x1 fluxes, divergence, sources and the column solve stay
+  // FP64)
+  const dim3 gf(nblk(nface, 256), 6);
+  const unsigned ge = nblk((long long)S.nedge * ks.nr, 256);
+#define CG_SPH_FACES(R)                                                                   \
+  do {                                                                                    \
+    if (r->cfg.order_h == 5) {                                                            \
+      TIMED(tm, KID_FLUX_X2, (k_sph_faces<5, 1, R><<<gf, 256, 0, st>>>(ks)));             \
+      TIMED(tm, KID_FLUX_X3, (k_sph_faces<5, 2, R><<<gf, 256, 0, st>>>(ks)));             \
+    } else {                                                                              \
+      TIMED(tm, KID_FLUX_X2, (k_sph_faces<2, 1, R><<<gf, 256, 0, st>>>(ks)));             \
+      TIMED(tm, KID_FLUX_X3, (k_sph_faces<2, 2, R><<<gf, 256, 0, st>>>(ks)));             \
+    }                                                                                     \
+    TIMED(tm, KID_FLUX_X2,                                                                \
+          (k_sph_exchange<R><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));               \
+    TIMED(tm, KID_FLUX_X2, (k_sph_face_flux<1, R><<<gf, 256, 0, st>>>(ks, r->kt)));       \
+    TIMED(tm, KID_FLUX_X3, (k_sph_face_flux<2, R><<<gf, 256, 0, st>>>(ks, r->kt)));       \
+  } while (0)
+  if (r->cfg.precision == CG_PREC_FP32)
+    CG_SPH_FACES(float);
+  else
+    CG_SPH_FACES(double);
+#undef CG_SPH_FACES
   TIMED(tm, KID_DIVERGENCE,
         (k_sph_divergence<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, sbeta[s], r->d_dt)));
   if (r->cfg.implicit_flag & 1) {
@@ -766,12 +767,13 @@ int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
   for (int s = 2; s < 6; ++s)
     if (panel->bc[s] != CG_BC_PANEL)
       return set_err(ctx, CG_ERR_ARG, "x2 / x3 sides of a panel must be CG_BC_PANEL");
-  if (cfg->precision != CG_PREC_FP64 || cfg->rsolver != CG_RS_HLLC ||
+  if ((cfg->precision != CG_PREC_FP64 && cfg->precision != CG_PREC_FP32) ||
+      cfg->rsolver != CG_RS_HLLC ||
       (cfg->implicit_flag != 0 && cfg->implicit_flag != 1 && cfg->implicit_flag != 9) ||
       cfg->satadj != CG_SATADJ_OFF || cfg->visc != 0.0 || cfg->heat_flux != 0.0 ||
       kessler_on(*cfg))
     return set_err(ctx, CG_ERR_ARG,
-                   "cubed sphere: FP64, HLLC, implicit_flag 0 / 1 / 9, no adjustment, "
+                   "cubed sphere: HLLC, implicit_flag 0 / 1 / 9, no adjustment, "
                    "diffusion, heating or microphysics");
   if (!(panel->x1min > 0.0)) return set_err(ctx, CG_ERR_ARG, "x1min is the inner radius");
   // six blocks; their x2 / x3 flags are never read by the sphere's kernels
@@ -781,10 +783,12 @@ int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
   cg_config c = *cfg;
   c.order_v = 5;  // Weno5X1 under CUBED_SPHERE (patches/19.calculate_fluxes.patch:29-31)
   c.implicit_flag = 0;  // the batch's own column solve is not used
+  c.precision = CG_PREC_FP64;  // FP64 flux arrays; the FP32 form only changes x2 / x3 faces
   cg_run* r = nullptr;
   int e = cg_batch_create(ctx, 6, blks.data(), th, &c, &r);
   if (e) return e;
   r->cfg.implicit_flag = cfg->implicit_flag;
+  r->cfg.precision = cfg->precision;
   r->sph = new SphRun();
   SphRun& S = *r->sph;
   S.sp = *sp;
@@ -835,6 +839,8 @@ int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
       k.f1[p] = rb.f1;
       k.f2[p] = rb.f2;
       k.f3[p] = rb.f3;
+      k.psf[p] = rb.psf;
+      k.psv[p] = rb.psv;
       hp.u[p] = rb.u;
       hp.w[p] = rb.w;
       hp.kj[p] = k.kj[p];

@@ -42,6 +42,8 @@ struct KSph {
   double* f1[6];
   double* f2[6];
   double* f3[6];
+  double* psf[6];
+  double* psv[6];
   double* lr;  // face states: 4 buffers (l2, r2, l3, r3) per panel, lrn each
   long long lrn;
   const double* x1f;
@@ -117,28 +119,40 @@ __global__ void __launch_bounds__(256) k_sph_prim_to_cons(KSph s, int p) {
   for (int v = 0; v < 5; ++v) s.u[p][c0 + v * g.c.n] = u[v];
 }
 
-// covariant projection of the x1 face fluxes' tangential momenta
-__global__ void __launch_bounds__(256) k_sph_x1_project(KSph s) {
+// x1 face fluxes of the six panels (decomposition, WENO5, HLLC) with the
+// covariant projection of the tangential momentum fluxes
+__global__ void __launch_bounds__(256) k_sph_flux_x1(KSph s, KThermo th) {
   const int p = blockIdx.y;
   const KGrid& g = s.g;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
-  int i, j, k;
-  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
-  i += g.is;
-  j += g.js;
-  k += g.ks;
-  const double cth = skj(s, p, cg_host::SKJ_HC, k, j);
-  double* f = s.f1[p];
-  const long long a = g.f[0].at(2, k, j, i), b = g.f[0].at(3, k, j, i);
-  double ty = f[a], tz = f[b];
-  f[a] = ty + tz * cth;
-  f[b] = tz + ty * cth;
+  flux_x1_face<0, 5>(g, th, CG_RS_HLLC, g.grav != 0.0 ? 1 : 0, s.w[p], s.psf[p], s.psv[p],
+                     s.f1[p], t, s.kj[p] + (long long)cg_host::SKJ_HC * s.nc * s.nc, s.nc);
+}
+
+// hydrostatic face and cell pressures of the six panels' columns
+__global__ void __launch_bounds__(256) k_sph_scan(KSph s, KThermo th) {
+  const int p = blockIdx.y;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)s.g.nx2 * s.g.nx3) return;
+  scan_column<0>(s.g, th, s.w[p], s.psf[p], s.psv[p], t);
+}
+
+// reflecting x1 walls of the six panels' columns
+__global__ void __launch_bounds__(256) k_sph_bc_x1(KSph s) {
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx2 * g.nx3 * NG) return;
+  int gh, j, k;
+  g.decode(t, NG, g.nx2, g.nx3, gh, j, k);
+  bc_pair(s.w[p], g.c.n, 5, 1, g.bc[0], g.bc[1], g.at(0, k + g.ks, j + g.js, g.is),
+          g.at(0, k + g.ks, j + g.js, g.ie), g.c.i, gh + 1);
 }
 
 // x2 (DIR = 1) or x3 (DIR = 2) face states of every panel into the face
 // buffers; ORDER 5 (WENO5) or 2 (PLM)
-template <int ORDER, int DIR>
+template <int ORDER, int DIR, class R = double>
 __global__ void __launch_bounds__(256) k_sph_faces(KSph s) {
   constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
   const int p = blockIdx.y;
@@ -164,26 +178,27 @@ __global__ void __launch_bounds__(256) k_sph_faces(KSph s) {
   const long long st = (DIR == 1) ? g.c.j : g.c.k;
   const long long base = g.at(0, k, j, i);
   const double* w = s.w[p];
+  R* lr = reinterpret_cast<R*>(s.lr);
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    double q[6];
+    R q[6];
 #pragma unroll
-    for (int m = LO; m <= HI; ++m) q[m] = w[base + v * g.c.n + (m - 3) * st];
-    double qm, qp, wl, wr;
+    for (int m = LO; m <= HI; ++m) q[m] = (R)w[base + v * g.c.n + (m - 3) * st];
+    R qm, qp, wl, wr;
     if (ORDER == 5) {
-      weno5_cell<double>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
+      weno5_cell<R>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
       wl = qp;
-      weno5_cell<double>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
+      weno5_cell<R>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
       wr = qm;
     } else {
-      plm_cell<double>(q[1], q[2], q[3], &qm, &qp);
+      plm_cell<R>(q[1], q[2], q[3], &qm, &qp);
       wl = qp;
-      plm_cell<double>(q[2], q[3], q[4], &qm, &qp);
+      plm_cell<R>(q[2], q[3], q[4], &qm, &qp);
       wr = qm;
     }
     const int bl = (DIR == 1) ? 0 : 2;
-    s.lr[slr(s, p, bl, v, ii, t, f)] = wl;
-    s.lr[slr(s, p, bl + 1, v, ii, t, f)] = wr;
+    lr[slr(s, p, bl, v, ii, t, f)] = wl;
+    lr[slr(s, p, bl + 1, v, ii, t, f)] = wr;
   }
 }
 
@@ -191,6 +206,7 @@ __global__ void __launch_bounds__(256) k_sph_faces(KSph s) {
 // the neighbour's inner state there, its contravariant velocity carried into
 // this panel's frame through (U, V).  Outer and inner slots are distinct, so the items are
 // independent.
+template <class R = double>
 __global__ void __launch_bounds__(256)
     k_sph_exchange(KSph s, const SphEdge* __restrict__ edges, int nedge) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,19 +219,20 @@ __global__ void __launch_bounds__(256)
   const int fq = (e.sq % 2 == 0) ? 0 : n;
   const int bp = (e.s < 2) ? ((e.s == 0) ? 0 : 1) : ((e.s == 2) ? 2 : 3);
   const int fp = (e.s % 2 == 0) ? 0 : n;
-  double q[5];
+  R* lr = reinterpret_cast<R*>(s.lr);
+  R q[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) q[v] = s.lr[slr(s, e.q, bq, v, ii, e.tq, fq)];
-  const double U = e.r00 * q[2] + e.r01 * q[3];
-  const double V = e.r10 * q[2] + e.r11 * q[3];
-  q[2] = e.a00 * U + e.a01 * V;
-  q[3] = e.a10 * U + e.a11 * V;
+  for (int v = 0; v < 5; ++v) q[v] = lr[slr(s, e.q, bq, v, ii, e.tq, fq)];
+  const R U = (R)e.r00 * q[2] + (R)e.r01 * q[3];
+  const R V = (R)e.r10 * q[2] + (R)e.r11 * q[3];
+  q[2] = (R)e.a00 * U + (R)e.a01 * V;
+  q[3] = (R)e.a10 * U + (R)e.a11 * V;
 #pragma unroll
-  for (int v = 0; v < 5; ++v) s.lr[slr(s, e.p, bp, v, ii, e.t, fp)] = q[v];
+  for (int v = 0; v < 5; ++v) lr[slr(s, e.p, bp, v, ii, e.t, fp)] = q[v];
 }
 
 // HLLC at x2 (DIR = 1) / x3 (DIR = 2) faces in the face's orthonormal frame
-template <int DIR>
+template <int DIR, class R = double>
 __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   const int p = blockIdx.y;
   const KGrid& g = s.g;
@@ -238,19 +255,23 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   const int j = (DIR == 1) ? g.js + f : g.js + t;
   const int k = (DIR == 1) ? g.ks + t : g.ks + f;
   const int bl = (DIR == 1) ? 0 : 2;
-  double a[5], b[5];
+  const R* lr = reinterpret_cast<const R*>(s.lr);
+  R a[5], b[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    a[v] = s.lr[slr(s, p, bl, v, ii, t, f)];
-    b[v] = s.lr[slr(s, p, bl + 1, v, ii, t, f)];
+    a[v] = lr[slr(s, p, bl, v, ii, t, f)];
+    b[v] = lr[slr(s, p, bl + 1, v, ii, t, f)];
   }
-  const double cf = skj(s, p, DIR == 1 ? cg_host::SKJ_COSF2 : cg_host::SKJ_COSF3, k, j);
-  const double sf = skj(s, p, DIR == 1 ? cg_host::SKJ_SINF2 : cg_host::SKJ_SINF3, k, j);
-  const double hc = skj(s, p, DIR == 1 ? cg_host::SKJ_HF2 : cg_host::SKJ_HF3, k, j);
-  const double gi = 1.0 / (sf * sf);
+  // the face metric, the frame transforms and their products in the form's
+  // arithmetic type (the metric itself is FP64 on the host)
+  const R cf = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_COSF2 : cg_host::SKJ_COSF3, k, j);
+  const R sf = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_SINF2 : cg_host::SKJ_SINF3, k, j);
+  const R hc = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_HF2 : cg_host::SKJ_HF3, k, j);
+  const R one = (R)1.0;
+  const R gi = one / (sf * sf);
   if (DIR == 1) {  // PrimToLocal2
-    const double T22 = 1.0 / sqrt(gi), T32 = cf / sqrt(1.0), T33 = sqrt(1.0);
-    double uu2 = a[2], uu3 = a[3];
+    const R T22 = one / sqrt(gi), T32 = cf / sqrt(one), T33 = sqrt(one);
+    R uu2 = a[2], uu3 = a[3];
     a[2] = T22 * uu2;
     a[3] = T32 * uu2 + T33 * uu3;
     uu2 = b[2];
@@ -258,8 +279,8 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
     b[2] = T22 * uu2;
     b[3] = T32 * uu2 + T33 * uu3;
   } else {  // PrimToLocal3
-    const double T22 = sqrt(1.0), T23 = cf / sqrt(1.0), T33 = 1.0 / sqrt(gi);
-    double uu2 = a[2], uu3 = a[3];
+    const R T22 = sqrt(one), T23 = cf / sqrt(one), T33 = one / sqrt(gi);
+    R uu2 = a[2], uu3 = a[3];
     a[2] = T22 * uu2 + T23 * uu3;
     a[3] = T33 * uu3;
     uu2 = b[2];
@@ -267,26 +288,26 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
     b[2] = T22 * uu2 + T23 * uu3;
     b[3] = T33 * uu3;
   }
-  double fx[5];
-  riemann<0, DIR, double>(th, CG_RS_HLLC, a, b, fx);
-  const double txx = fx[1], txy = fx[2], txz = fx[3];
+  R fx[5];
+  riemann<0, DIR, R>(th, CG_RS_HLLC, a, b, fx);
+  const R txx = fx[1], txy = fx[2], txz = fx[3];
   if (DIR == 1) {  // FluxToGlobal2
-    const double T22 = sqrt(gi), T32 = -sqrt(gi) * cf / 1.0, T33 = 1.0 / sqrt(1.0);
-    fx[1] = 1.0 * txx;
+    const R T22 = sqrt(gi), T32 = -sqrt(gi) * cf / one, T33 = one / sqrt(one);
+    fx[1] = one * txx;
     fx[2] = T22 * txy;
     fx[3] = T32 * txy + T33 * txz;
   } else {  // FluxToGlobal3
-    const double T22 = 1.0 / sqrt(1.0), T23 = -cf / 1.0 * sqrt(gi), T33 = sqrt(gi);
-    fx[1] = 1.0 * txx;
+    const R T22 = one / sqrt(one), T23 = -cf / one * sqrt(gi), T33 = sqrt(gi);
+    fx[1] = one * txx;
     fx[2] = T22 * txy + T23 * txz;
     fx[3] = T33 * txz;
   }
-  const double ty = fx[2], tz = fx[3];
+  const R ty = fx[2], tz = fx[3];
   fx[2] = ty + tz * hc;
   fx[3] = tz + ty * hc;
   double* F = (DIR == 1) ? s.f2[p] : s.f3[p];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = fx[v];
+  for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = (double)fx[v];
 }
 
 // areas and volume of a panel cell (gnomonic_equiangle.cpp:221-260, 407-416)
