@@ -217,7 +217,7 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
     cfg->satadj = (config == 4) ? CG_SATADJ_NEWTON : CG_SATADJ_OFF;
   } else if (config == 3) {
     // one panel of the cubed sphere (hs94.inp: R = 6.371e6 m, 25 km deep,
-    // g = 9.81; BASELINE's 6 x 96 x 96 x 64)
+    // g = 9.81; six panels of 96 x 96 x 64)
     blk->nx1 = nx1 > 0 ? nx1 : 64;
     blk->nx2 = nx2 > 0 ? nx2 : 96;
     blk->nx3 = nx3 > 0 ? nx3 : blk->nx2;

@@ -480,7 +480,9 @@ int cg_run_autotune(cg_run* run, double dt, cg_tune* out);
  * terms of the panel metric, covariant momenta in u and contravariant
  * velocities in w (patches/26.adiabatic_hydro.patch), the x1 column solve
  * with the panel areas (implicit_flag 0, 1 or 9), gravity and the Coriolis /
- * Held-Suarez forcing of cg_sphere.  Dry gas, FP64.  The cg_run_* calls
+ * Held-Suarez forcing of cg_sphere.  Dry gas; cfg->precision CG_PREC_FP32
+ * computes the x2 / x3 face states and Riemann fluxes in FP32 (x1 fluxes,
+ * divergence, column solve and state in FP64).  The cg_run_* calls
  * taking a block index address panel b (0-5: panels 1-6); the time-step,
  * check, history, debug, sums and autotune calls return CG_ERR_ARG. */
 int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
@@ -501,7 +503,7 @@ int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
 int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
                  const cg_thermo* th, const cg_config* cfg, double* w);
 int cg_apply_bc_host(const cg_block* blk, double* w);
-/* Config 3 (BASELINE's cubed-sphere GCM step; examples/2023-Chen-exo3/
+/* Config 3 (the cubed-sphere Held-Suarez GCM step; examples/2023-Chen-exo3/
  * hs94.cpp, hs94.inp): cg_pgen_setup(3, ...) gives the panel block (nx1 = 64
  * levels over 25 km from R = 6.371e6 m, 96 x 96 cells per panel, g = 9.81),
  * dry air and the run settings; cg_pgen_sphere fills the rotation and
