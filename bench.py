@@ -195,14 +195,18 @@ def algorithmic(blk, cfg):
     """Algorithmic HBM bytes per launch of each operation: per-unit figures
     x units per launch.  Flux divergence: read w once, write du once,
     (s_in + 8) N bytes per cell, s_in = 8 in both forms (the FP32-flux
-    kernels read the FP64 state w; no FP32 copy of it is stored); implicit
-    solve: read w, read du, write du, 24 N bytes per cell."""
+    kernels read the FP64 state w; no FP32 copy of it is stored).  Column
+    solve of a run: the launch solves the x1 columns and, in its downward
+    sweep, applies the stage (u, u1 and du combined, ConservedToPrimitive)
+    and writes the next stage's hydrostatic pressures, so one launch reads
+    w, du and u (and u1 at stages 2 and 3) and writes u, w, psf and psv:
+    (8 (5 + 2/3) N + 16) bytes per cell, averaged over the three stages."""
     N = blk.N
     cells = blk.ncells()
     s_in = 8
     return {
         "flux_divergence": (s_in + 8) * N * cells,
-        "implicit_solve": 24 * N * cells,
+        "implicit_solve": int(round((8 * (5 + 2.0 / 3.0) * N + 16) * cells)),
         "update": 32 * N * cells,   # read u, u1, du; write u (+ w)
     }
 

@@ -70,6 +70,7 @@ int cg_create(int device, cg_ctx** out) {
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(int)) != hipSuccess ||
       hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess ||  // the null-stream fill is done
       fixed_scratch_address(device, &c->fixed_scratch) != hipSuccess) {
     if (c->d_status) (void)hipFree(c->d_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);

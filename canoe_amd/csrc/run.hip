@@ -557,6 +557,8 @@ int ensure_tiles(cg_run* r) {
   HIPCHK(r->ctx, hipMalloc((void**)&r->d_tiles, sizeof(KTile) * tiles.size()));
   HIPCHK(r->ctx, hipMemcpy(r->d_tiles, tiles.data(), sizeof(KTile) * tiles.size(),
                            hipMemcpyHostToDevice));
+  // the copy runs on the null stream: done before the next launch reads it
+  HIPCHK(r->ctx, hipDeviceSynchronize());
   return CG_OK;
 }
 
@@ -740,6 +742,10 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
   }
   r->tm.s = ctx->stream;
   if (ok) ok = hipEventCreate(&r->tm.e0) == hipSuccess && hipEventCreate(&r->tm.e1) == hipSuccess;
+  // the fills and table copies above are queued on the null stream, which
+  // the context's non-blocking stream does not wait for: finish them before
+  // any later work (set_prim's relayout, the first step) can be overwritten
+  if (ok) ok = hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     cg_run_destroy(r);
     return set_err(ctx, CG_ERR_HIP, "allocation failed");
@@ -847,6 +853,8 @@ int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
     }
     ok = hipMemcpy(S.d_ptr, &hp, sizeof(KSphPtrs), hipMemcpyHostToDevice) == hipSuccess;
   }
+  // null-stream fills and copies finish before work on the context's stream
+  if (ok) ok = hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     cg_run_destroy(r);
     return set_err(ctx, CG_ERR_HIP, "allocation failed");
