@@ -202,6 +202,7 @@ int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
       (cfg->order_v != 2 && cfg->order_v != 5))
     return set_err(ctx, CG_ERR_ARG, "order must be 2 or 5");
   KGrid g = make_grid(*blk, 0);
+  g.status = ctx->d_status;  // check_reconstruct of every face
   KThermo kt = make_thermo(*th);
   long long ncol = (long long)g.nx2 * g.nx3;
   AsyncScratch psf_s(ctx->stream), psv_s(ctx->stream);
@@ -214,8 +215,7 @@ int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
   else
     launch_fluxes<3>(g, kt, *cfg, w, psf, psv, flux1, flux2, flux3, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  return CG_OK;
+  return check_status(ctx);
 }
 
 int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,

@@ -97,6 +97,9 @@ inline int check_status(cg_ctx* ctx) {
   if (st) {
     HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (st & 4)
+      return set_err(ctx, CG_ERR_NEGATIVE,
+                     "negative reconstructed face density or pressure");
     if (st & 1) return set_err(ctx, CG_ERR_PIVOT, "zero pivot in implicit solve");
     if (st & 2)
       return set_err(ctx, CG_ERR_CONVERGE, "saturation Newton did not converge");

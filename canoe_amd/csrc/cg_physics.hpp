@@ -38,6 +38,8 @@ struct KGrid {
   long long ps_i, ps_c;
   double dx1, dx2, dx3, grav;
   int bc[6];
+  // status word of the face checks (null: not checked)
+  int* status = nullptr;
   CG_HD long long at(int n, int k, int j, int i) const { return c.at(n, k, j, i); }
   CG_HD long long pcol(int k, int j) const {
     return (long long)(k - ks) * nx2 + (j - js);

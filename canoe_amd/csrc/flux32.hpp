@@ -366,6 +366,7 @@ __global__ void __launch_bounds__(256)
     wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
   }
   float fx[N];
+  face_check<NM, float>(g.status, wl, wr, pb);
   riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
 #pragma unroll
   for (int n = 0; n < N; ++n) f1[g.f[0].at(n, k, j, i)] = fx[n];
@@ -403,6 +404,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
   float fx[N];
+  face_check<NM, float>(g.status, wl, wr, (float)pr0);
   riemann_p<NM, DIR>(th, rsolver, wl, wr, (float)pr0, fx);
 #pragma unroll
   for (int n = 0; n < N; ++n) fh[g.f[DIR].at(n, k, j, i)] = fx[n];
@@ -504,6 +506,7 @@ __global__ void __launch_bounds__(256)
   const int f = c;  // face between cells c-1 and c
   if (lane < 3 || lane > 61 || f > g.je + 1) return;
   float fx[N];
+  face_check<NM, float>(g.status, ql, qm, (float)prd);
   riemann_p<NM, 1>(th, rsolver, ql, qm, (float)prd, fx);
 #pragma unroll
   for (int n = 0; n < N; ++n) f2[g.f[1].at(n, k, f, i)] = fx[n];
@@ -565,6 +568,7 @@ __global__ void __launch_bounds__(256)
     float qp[N];
     cell(qm, qp);
     float fx[N];
+    face_check<NM, float>(g.status, qpp, qm, (float)prd);
     riemann_p<NM, 2>(th, rsolver, qpp, qm, (float)prd, fx);
 #pragma unroll
     for (int n = 0; n < N; ++n) f3[g.f[2].at(n, f, j, i)] = fx[n];
@@ -627,6 +631,7 @@ __global__ void __launch_bounds__(256)
       wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
       wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
     }
+    face_check<NM, float>(g.status, wl, wr, pb);
     riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
   };
 #pragma unroll
@@ -760,6 +765,7 @@ __device__ __forceinline__ void mix_fused_tile(
       wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
       wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
     }
+    face_check<NM, float>(g.status, wl, wr, pb);
     riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
   };
 #pragma unroll
@@ -840,6 +846,7 @@ __device__ __forceinline__ void mix_fused_tile(
         wl[n] = L.qp[n][lq];
         wr[n] = L.qm[n][rq];
       }
+      face_check<NM, float>(g.status, wl, wr, (float)prd);
       riemann_p<NM, 0>(th, rsolver, wl, wr, (float)prd, fx);
 #pragma unroll
       for (int n = 0; n < N; ++n) L.fx[n][e] = fx[n];

@@ -12,6 +12,22 @@
 
 namespace cg {
 
+// check_reconstruct (src/checks.cpp:79-97), run after the face states of
+// every direction are reconstructed (patches/19.calculate_fluxes.patch:45,
+// 73, 117): a negative face density or pressure (pb + the stored pressure,
+// pb the base pressure of a perturbation form) sets bit 4 of the status
+// word, which the host reports as CG_ERR_NEGATIVE.  NaN passes, as in the
+// reference.
+constexpr int CG_STATUS_FACE = 4;
+template <int NM, class R>
+__device__ __forceinline__ void face_check(int* status, const R* wl, const R* wr,
+                                           R pb) {
+  constexpr int IPR = 4 + NM;
+  if (status && (wl[0] < R(0) || wr[0] < R(0) || wl[IPR] + pb < R(0) ||
+                 wr[IPR] + pb < R(0)))
+    atomicOr(status, CG_STATUS_FACE);
+}
+
 // R = Rd (1 + sum_vap q inv_mu_ratio_m1) of the decomposition
 // (src/snap/decomposition/change_to_temperature.cpp:100-104)
 template <int NM>
@@ -184,6 +200,7 @@ __device__ __forceinline__ void flux_x1_face(const KGrid& g, const KThermo& th, 
     if (wl[IPR] < 0.) wl[IPR] = ps;
     wl[0] = wl[IPR] / (wl[0] * gamma);
   }
+  face_check<NM, double>(g.status, wl, wr, 0.0);
   double fx[N];
   riemann<NM, 0, double>(th, rsolver, wl, wr, fx);
   if (hc) {
@@ -246,6 +263,7 @@ __global__ void __launch_bounds__(256)
       wr[n] = qm;
     }
   }
+  face_check<NM, double>(g.status, wl, wr, 0.0);
   double fx[N];
   riemann<NM, DIR, double>(th, rsolver, wl, wr, fx);
 #pragma unroll

@@ -676,6 +676,7 @@ int cg_batch_create(cg_ctx* ctx, int nblocks, const cg_block* blks,
     RunBlock& rb = r->blocks[b];
     rb.blk = blks[b];
     rb.g = make_grid(blks[b], 1);
+    rb.g.status = ctx->d_status;  // face checks of the flux kernels
     rb.ga = make_grid(blks[b], 0);
     ok = alloc_block(rb, r->N, *cfg);
   }

@@ -288,6 +288,7 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
     b[2] = T22 * uu2 + T23 * uu3;
     b[3] = T33 * uu3;
   }
+  face_check<0, R>(g.status, a, b, R(0));
   R fx[5];
   riemann<0, DIR, R>(th, CG_RS_HLLC, a, b, fx);
   const R txx = fx[1], txy = fx[2], txz = fx[3];

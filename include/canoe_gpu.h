@@ -167,7 +167,11 @@ int cg_riemann(cg_ctx* ctx, const cg_thermo* th, int solver, int ivx,
  * change_to_temperature.cpp:113-127 and :150-153; runs apply that rewrite
  * where the stage reads those ghosts).  Fluxes are computed on every face
  * of interior cells.  flux2 / flux3 are not read or written for a direction
- * without cells (nx2 = 1 / nx3 = 1) and may then be null. */
+ * without cells (nx2 = 1 / nx3 = 1) and may then be null.  Every face state
+ * is checked as check_reconstruct does (src/checks.cpp:79-97, called at
+ * patches/19.calculate_fluxes.patch:45, 73, 117): a negative reconstructed
+ * density or pressure returns CG_ERR_NEGATIVE (the fluxes are still
+ * written), where the reference built with glog aborts. */
 int cg_calculate_fluxes(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,
                         const cg_config* cfg, const double* w, double* flux1,
                         double* flux2, double* flux3);
@@ -340,7 +344,8 @@ int cg_run_get_cons_host(cg_run* run, double* u_host);
 int cg_run_get_prim_host(cg_run* run, double* w_host);
 /* nsteps steps of size dt (use_graph != 0: replay the captured graph).  The
  * stage kernels read the step from device memory, so one captured graph
- * serves every dt. */
+ * serves every dt.  CG_ERR_NEGATIVE: a negative reconstructed face density
+ * or pressure in some stage (the face checks of cg_calculate_fluxes). */
 int cg_run_step(cg_run* run, double dt, int nsteps, int use_graph);
 /* One SSPRK(3,3) stage (0, 1 or 2) of the run's blocks: the per-stage tasks
  * of ImplicitHydroTasks (src/tasklist/implicit_hydro_tasks.cpp:134-313) --

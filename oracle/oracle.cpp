@@ -17,6 +17,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 #include <cmath>
 #include <cstdint>
@@ -422,7 +423,22 @@ void restore_outflow_ghosts(const Grid& g, const cg_block& blk,
 }
 
 // ---------------------------------------------------------------------------
-// Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch:19-185), pure.
+// check_reconstruct (src/checks.cpp:79-97), called after the face states of
+// each direction (patches/19.calculate_fluxes.patch:45, 73, 117): the faces
+// il..iu of wl / wr (entry n * ld + i) with a negative density or pressure,
+// counted once per face into g_face_failures.
+std::atomic<long long> g_face_failures{0};
+void check_reconstruct(int IPR, const double* wl, const double* wr, size_t ld,
+                       int il, int iu) {
+  long long n = 0;
+  for (int i = il; i <= iu; ++i)
+    if (wl[i] < 0. || wr[i] < 0. || wl[IPR * ld + i] < 0. || wr[IPR * ld + i] < 0.)
+      ++n;
+  if (n) g_face_failures += n;
+}
+
+// Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch:19-185), pure
+// apart from the face-check count.
 void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
                       const cg_config& cfg, const double* w, double* f1,
                       double* f2, double* f3) {
@@ -474,6 +490,7 @@ void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
           wl[0 * (nc1 + 1) + i] = *wlp / (wl[0 * (nc1 + 1) + i] * gamma);
         }
       }
+      check_reconstruct(IPR, wl.data(), wr.data(), nc1 + 1, g.is, g.ie + 1);
       std::vector<double> fx((size_t)N * (nc1 + 1));
       riemann(th, cfg.rsolver, g.IVX, g.is, g.ie + 1, wl.data(), wr.data(),
               nc1 + 1, fx.data(), nc1 + 1);
@@ -499,6 +516,7 @@ void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
             wr[n * nf + (j - g.js)] = tr[j];
           }
         }
+        check_reconstruct(IPR, wl.data(), wr.data(), nf, 0, nf - 1);
         riemann(th, cfg.rsolver, g.IVY, 0, nf - 1, wl.data(), wr.data(), nf,
                 fx.data(), nf);
         for (int n = 0; n < N; ++n)
@@ -525,6 +543,7 @@ void calculate_fluxes(const Grid& g, const cg_block& blk, const cg_thermo& th,
             wr[n * nf + (k - g.ks)] = tr[k];
           }
         }
+        check_reconstruct(IPR, wl.data(), wr.data(), nf, 0, nf - 1);
         riemann(th, cfg.rsolver, g.IVZ, 0, nf - 1, wl.data(), wr.data(), nf,
                 fx.data(), nf);
         for (int n = 0; n < N; ++n)
@@ -1823,9 +1842,12 @@ int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
                         const cg_config* cfg, const double* w, double* flux1,
                         double* flux2, double* flux3) {
   Grid g(*blk);
+  g_face_failures = 0;
   calculate_fluxes(g, *blk, *th, *cfg, w, flux1, flux2, flux3);
   return CG_OK;
 }
+
+long long or_face_failures(void) { return g_face_failures.load(); }
 
 int or_restore_outflow_ghosts(const cg_block* blk, const cg_thermo* th,
                               double* w) {

@@ -44,6 +44,9 @@ int or_decompose_column(const cg_block* blk, const cg_thermo* th,
 int or_calculate_fluxes(const cg_block* blk, const cg_thermo* th,
                         const cg_config* cfg, const double* w, double* flux1,
                         double* flux2, double* flux3);
+/* faces of the last or_calculate_fluxes with a negative density or pressure
+ * (check_reconstruct, src/checks.cpp:79-97) */
+long long or_face_failures(void);
 int or_restore_outflow_ghosts(const cg_block* blk, const cg_thermo* th,
                               double* w);
 int or_add_flux_divergence(const cg_block* blk, double wght,

@@ -25,6 +25,7 @@ _SIGS = {
                         ctypes.c_ssize_t]),
     "or_decompose_column": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "or_calculate_fluxes": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "or_face_failures": (ctypes.c_longlong, []),
     "or_restore_outflow_ghosts": (_I, [_P, _P, _P]),
     "or_add_flux_divergence": (_I, [_P, _D, _P, _P, _P, _P]),
     "or_add_sources": (_I, [_P, _P, _P, _D, _P, _P]),
@@ -119,6 +120,12 @@ def calculate_fluxes(blk, th, cfg, w):
     lib().or_calculate_fluxes(ref(blk), ref(th), ref(cfg), ptr(w), ptr(fs[0]),
                               ptr(fs[1]), ptr(fs[2]))
     return fs
+
+
+def face_failures():
+    """Faces of the last calculate_fluxes with a negative reconstructed
+    density or pressure (check_reconstruct)."""
+    return int(lib().or_face_failures())
 
 
 def restore_outflow_ghosts(blk, th, w):
