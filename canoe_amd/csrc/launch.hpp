@@ -214,6 +214,14 @@ inline KLines make_lines(const KGrid& g, int dir) {
   L.sra = area * s2v;
   L.sda = (area - area) * s2v;
   L.grav = (dir == 0) ? g.grav : 0.0;
+  L.td0 = t0;
+  L.td1 = t1;
+  L.t0s = st[t0];
+  L.t1s = st[t1];
+  L.nc1 = g.nc1;
+  L.nc2 = g.nc2;
+  L.nc3 = g.nc3;
+  L.c = g.c;
   return L;
 }
 
@@ -271,12 +279,25 @@ inline int check_implicit_flag(cg_ctx* ctx, const cg_block& b, int flag) {
     return set_err(ctx, CG_ERR_ARG, "implicit_flag must be in 0..15 (got %d)", flag);
   bool per[3];
   periodic_dirs(b, per);
-  // x2 / x3 lines between walls with species: the reference weights the
-  // species by w at the transposed index (forward_backward.hpp:109-115)
-  if (b.nmass > 0 && (((flag & 2) && b.nx2 > 1 && !per[1]) ||
-                      ((flag & 4) && b.nx3 > 1 && !per[2])))
-    return set_err(ctx, CG_ERR_ARG,
-                   "x2/x3 implicit lines with species need periodic ends");
+  // x2 / x3 lines between walls with species weight the species by w at the
+  // shuffled index (w_shuffled, forward_backward.hpp:109-115): the largest
+  // such entry must lie inside w (the reference reads past its end otherwise)
+  if (b.nmass > 0) {
+    const long long nc1 = b.nx1 + 2 * NG, nc2 = b.nx2 > 1 ? b.nx2 + 2 * NG : 1,
+                    nc3 = b.nx3 > 1 ? b.nx3 + 2 * NG : 1;
+    const long long ie = NG + b.nx1 - 1, je = (b.nx2 > 1 ? NG : 0) + b.nx2 - 1,
+                    ke = (b.nx3 > 1 ? NG : 0) + b.nx3 - 1;
+    const long long size = (long long)(5 + b.nmass) * nc3 * nc2 * nc1;
+    auto flat = [&](long long K, long long J, long long I) {
+      return ((b.nmass * nc3 + K) * nc2 + J) * nc1 + I;
+    };
+    if (((flag & 2) && b.nx2 > 1 && !per[1] && flat(ie, ke, je) >= size) ||
+        ((flag & 4) && b.nx3 > 1 && !per[2] && flat(je, ie, ke) >= size))
+      return set_err(ctx, CG_ERR_ARG,
+                     "x2/x3 implicit lines with species between walls: the "
+                     "reference's species weights would read past the end of w "
+                     "for this block");
+  }
   return CG_OK;
 }
 

@@ -25,10 +25,42 @@ struct KLines {
   long long org0, sf, ss, sl, sn;
   double sla, sra, sda;  // aleft / (2 vol), aright / (2 vol), their difference
   double grav;           // the x1 lines' gravity, 0 on x2 / x3 lines
+  // cell indices of line t: direction td0 runs from t0s with t % nf,
+  // td1 from t1s with t / nf; nc* and c describe w for the shuffled read
+  int td0, td1, t0s, t1s;
+  int nc1, nc2, nc3;
+  Strides c;
   CG_HD long long org(long long t) const {
     return org0 + (t % nf) * sf + (t / nf) * ss;
   }
 };
+
+// w_n at the entry the reference's BackwardSubstitution reads for a cell
+// of an x2 / x3 line (forward_backward.hpp:109-115): it indexes w with the
+// loop indices (k, j, i) of the shuffled line buffer (solve_implicit_3d.cpp:
+// 28-31, 59-62) while w itself is not shuffled, so the read is the
+// AthenaArray entry ((n nc3 + K) nc2 + J) nc1 + I with (K, J, I) = (x1, x3,
+// x2) of the cell on x2 lines and (x2, x1, x3) on x3 lines.  That entry,
+// decoded into (n, k, j, i) of w's own layout; the host checks that it lies
+// inside w.
+template <int DIR>
+__device__ inline double w_shuffled(const KLines& L, const double* __restrict__ w,
+                                    int n, long long t, int i) {
+  const int a0 = L.t0s + (int)(t % L.nf), a1 = L.t1s + (int)(t / L.nf);
+  int x[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[d] = (d == DIR) ? i : (d == L.td0 ? a0 : a1);
+  long long K, J, I;
+  if (DIR == 1) {
+    K = x[0]; J = x[2]; I = x[1];
+  } else {
+    K = x[1]; J = x[0]; I = x[2];
+  }
+  const long long f = ((n * (long long)L.nc3 + K) * L.nc2 + J) * L.nc1 + I;
+  const long long r = f / L.nc1, r2 = r / L.nc2;
+  return w[L.c.at((int)(r2 / L.nc3), (int)(r2 % L.nc3), (int)(r % L.nc2),
+                  (int)(f % L.nc1))];
+}
 
 // Scratch doubles of one level: walls a'_i, delta_i (B x B + B); periodic
 // ends ainv_i, gamma_i, c_i (B x B each) and zeta_i (B).
@@ -192,8 +224,8 @@ struct LineAsm {
 
 // du of a cell from the line's solution d (frame of DIR).  Walls: the
 // write-back of BackwardSubstitution (forward_backward.hpp:100-124), the
-// total-density increment split over the species by w_n (x1 lines only:
-// x2 / x3 lines with species are rejected on the host).  Periodic: the
+// total-density increment split over the species by w_n (the cell's own
+// on x1 lines, w_shuffled on x2 / x3 lines; cell i of line t).  Periodic: the
 // write-back of PeriodicBackwardSubstitution (periodic_forward_backward.hpp:
 // 216-229), species unchanged.  x2 / x3 lines store IDN, the normal momentum
 // and IEN only, the variables the reference copies out of its line buffer
@@ -201,7 +233,8 @@ struct LineAsm {
 template <int NM, int B, int DIR, bool PER>
 __device__ inline void line_write_back(const Vt<B>& d, const double* __restrict__ w,
                                        double* __restrict__ du, long long ci,
-                                       long long sn) {
+                                       long long sn, const KLines& L, long long t,
+                                       int i) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   constexpr int vn = IVX + DIR, vt1 = IVX + (1 + DIR) % 3, vt2 = IVX + (2 + DIR) % 3;
   double o[N];
@@ -212,10 +245,9 @@ __device__ inline void line_write_back(const Vt<B>& d, const double* __restrict_
 #pragma unroll
     for (int n = 1; n < IVX; ++n) dens += o[n];
     dens = d.v[0] - dens;
-    if (DIR == 0) {
 #pragma unroll
-      for (int n = 1; n < IVX; ++n) o[n] += dens * w[ci + n * sn];
-    }
+    for (int n = 1; n < IVX; ++n)
+      o[n] += dens * (DIR == 0 ? w[ci + n * sn] : w_shuffled<DIR>(L, w, n, t, i));
   }
   o[0] = d.v[0];
   if (B == 5) {
@@ -387,7 +419,8 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < B; ++r) d.v[r] = d.v[r] - ad.v[r];
     }
     dn = d;
-    line_write_back<NM, B, DIR, false>(d, w, du, o + (long long)(lo + p) * L.sl, L.sn);
+    line_write_back<NM, B, DIR, false>(d, w, du, o + (long long)(lo + p) * L.sl, L.sn,
+                                       L, t, lo + p);
   }
 }
 
@@ -555,7 +588,8 @@ __global__ void __launch_bounds__(64)
     mvec<B>(ai, z, xl);
   }
   if (!ok) atomicOr(status, 1);
-  line_write_back<NM, B, DIR, true>(xl, w, du, o + (long long)(lo + n - 1) * L.sl, L.sn);
+  line_write_back<NM, B, DIR, true>(xl, w, du, o + (long long)(lo + n - 1) * L.sl, L.sn,
+                                    L, t, lo + n - 1);
   Vt<B> xn = xl;
   Mt<B> zero;
 #pragma unroll
@@ -578,7 +612,8 @@ __global__ void __launch_bounds__(64)
     for (int r = 0; r < B; ++r) tv.v[r] = (cur[(OZ + r) * st] - ux.v[r]) - gx.v[r];
     mvec<B>(ai, tv, x);
     xn = x;
-    line_write_back<NM, B, DIR, true>(x, w, du, o + (long long)(lo + p) * L.sl, L.sn);
+    line_write_back<NM, B, DIR, true>(x, w, du, o + (long long)(lo + p) * L.sl, L.sn, L,
+                                      t, lo + p);
   }
 }
 

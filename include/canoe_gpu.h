@@ -193,8 +193,13 @@ int cg_add_flux_divergence(cg_ctx* ctx, const cg_block* blk, double wght,
  * is solved by the cyclic recurrences of PeriodicForwardSweep /
  * PeriodicBackwardSubstitution (periodic_forward_backward.hpp:14-232), the
  * others with reflecting walls folded in.  x2 / x3 lines write back du of
- * IDN, the normal momentum and IEN only, as the reference does; with
- * species they need periodic ends (CG_ERR_ARG otherwise).  `w` must have
+ * IDN, the normal momentum and IEN only, as the reference does.  Between
+ * walls their density increment is split over the species with w read where
+ * the reference reads it: at the loop indices of its shuffled line buffer
+ * (forward_backward.hpp:109-115 with solve_implicit_3d.cpp:28-31, 59-62),
+ * i.e. the AthenaArray entry ((n nc3 + K) nc2 + J) nc1 + I, (K, J, I) =
+ * (x1, x3, x2) of the cell on x2 lines and (x2, x1, x3) on x3 lines;
+ * CG_ERR_ARG where that entry lies past the end of w.  `w` must have
  * the ghost cells of every solved direction filled.  du is updated in
  * place; all lines of a direction in one launch. */
 int cg_implicit_solve(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,

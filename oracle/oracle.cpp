@@ -1161,9 +1161,14 @@ bool correction_line(const Grid& g, const cg_block& blk, const cg_thermo& th,
       for (int n = 1; n < IVX; ++n) dens += o[n];
       dens = delta[i].v[0] - dens;
       o[0] = delta[i].v[0];
-      // the species weights are w at the line's own cell (x1 lines; x2 / x3
-      // lines with species are rejected by implicit_solve)
-      for (int n = 1; n < IVX; ++n) o[n] += dens * w[at(n, i)];
+      // the species weights: BackwardSubstitution reads w(n, k, j, i) with
+      // the loop indices of the line buffer du_ (forward_backward.hpp:
+      // 109-115), which for x2 / x3 lines are shuffled (solve_implicit_3d.cpp:
+      // 28-31, 59-62) while w is not: the AthenaArray entry
+      // ((n nc3 + k) nc2 + j) nc1 + i of this line's (k, j) and cell i
+      // (in range: implicit_solve checks the largest such index)
+      for (int n = 1; n < IVX; ++n)
+        o[n] += dens * (dir == 0 ? w[at(n, i)] : w[g.at(n, k, j, i)]);
       if (B == 5) {
         o[vn] = delta[i].v[1];
         o[vt1] = delta[i].v[2];
@@ -1226,8 +1231,12 @@ int implicit_solve(const Grid& g, const cg_block& blk, const cg_thermo& th,
                        blk.bc[2] == CG_BC_PERIODIC && blk.bc[3] == CG_BC_PERIODIC,
                        blk.bc[4] == CG_BC_PERIODIC && blk.bc[5] == CG_BC_PERIODIC};
   // x2 / x3 walls with species: the reference weights the species by w at
-  // the shuffled (transposed) index (forward_backward.hpp:109-115)
-  if (g.nmass > 0 && (((flag & 2) && g.f2 && !per[1]) || ((flag & 4) && g.f3 && !per[2])))
+  // the shuffled index (forward_backward.hpp:109-115, see correction_line);
+  // a block where that index runs past the end of w is rejected
+  if (g.nmass > 0 && (((flag & 2) && g.f2 && !per[1] &&
+                       g.at(g.nmass, g.ie, g.ke, g.je) >= (size_t)g.N * g.ncell) ||
+                      ((flag & 4) && g.f3 && !per[2] &&
+                       g.at(g.nmass, g.je, g.ie, g.ke) >= (size_t)g.N * g.ncell)))
     return CG_ERR_ARG;
   int err = CG_OK;
   const int dirs[3] = {2, 1, 0};

@@ -99,11 +99,29 @@ def test_moist_periodic_lines(ctx):
     _check(ctx, blk, th, w, du, 0.5, 15)
 
 
-def test_species_with_walled_lines_rejected(ctx):
+@pytest.mark.parametrize("flag", [10, 12, 14, 2, 4, 6])
+def test_species_with_walled_lines(ctx, flag):
+    """Moist block, walls at both x2 and x3 ends: the species weights read w
+    at the shuffled index of the reference's line buffer (w_shuffled)."""
     blk, th, cfg, w = perturbed_state(4, (8, 10, 6), vel=10.0)
+    bc = list(blk.bc)
+    bc[2] = bc[3] = bc[4] = bc[5] = ca.BC_REFLECT
+    _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
+    du = np.zeros(blk.shape())
+    sk, sj, si = blk.interior()
+    du[:, sk, sj, si] = np.random.default_rng(5).normal(
+        size=du[:, sk, sj, si].shape) * 1e-3
+    _say("moist walled lines flag=%d" % flag)
+    _check(ctx, blk, th, w, du, 0.5, flag)
+
+
+def test_species_lines_past_the_end_of_w_rejected(ctx):
+    blk, th, cfg, w = perturbed_state(4, (8, 10, 1), vel=10.0)
     bc = list(blk.bc)
     bc[2] = bc[3] = ca.BC_REFLECT
     _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
     with pytest.raises(ca.CanoeGpuError):
         ca.ImplicitSolver(ctx, blk, th, 10).solve_implicit_3d(
             np.zeros(blk.shape()), w, 0.5)
@@ -134,6 +152,33 @@ def test_run_with_lines_matches_oracle(ctx, flag, graph):
     _say("  relerr %.3e" % e)
     assert e <= TOL
     assert dtg == orl.new_block_dt(blk, th, cfg, wg, 0.5)
+
+
+def test_moist_run_with_walled_lines(ctx):
+    """A moist run with walls at the x2 ends and x2 lines + x1 columns in
+    every stage (the species weights of the x2 lines read the run's resident
+    w at the shuffled index) against the oracle's step."""
+    blk, th, cfg, w = perturbed_state(4, (12, 10, 8), vel=1.0, dp=1e-3)
+    bc = list(blk.bc)
+    bc[2] = bc[3] = ca.BC_REFLECT
+    _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
+    cfg.implicit_flag = 11
+    dt = orl.new_block_dt(blk, th, cfg, w, 0.5)
+    _say("moist run, walled x2 lines, flag 11")
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        run.step(dt, 3, graph=True)
+        ug = run.cons()
+    finally:
+        run.close()
+    u0 = orl.prim_to_cons(blk, th, w)
+    uo, _ = orl.step(blk, th, cfg, dt, 3, u0, w)
+    sk, sj, si = blk.interior()
+    e = relerr(ug[:, sk, sj, si], uo[:, sk, sj, si])
+    _say("  relerr %.3e" % e)
+    assert e <= TOL
 
 
 def test_mixed_run_with_lines(ctx):

@@ -17,6 +17,11 @@ periodic_forward_backward.hpp:14-232).
   momentum rows still telescope.
 * x2 / x3 lines change only IDN, the normal momentum and IEN; the directions
   run in the order x3, x2, x1.
+* Walled x2 / x3 lines of a moist block weight the species by w at the
+  shuffled index of the reference's line buffer: a change of w_n at that
+  entry (an x1 ghost that nothing else reads) moves exactly one cell's
+  density increment, by the amount the write-back's algebra predicts; a
+  block where the entry runs past the end of w is rejected.
 """
 import numpy as np
 import pytest
@@ -188,16 +193,57 @@ def test_directions_run_x3_then_x2_then_x1():
     assert not np.array_equal(all3, other)
 
 
-def test_species_need_periodic_x2_x3_lines():
-    blk, th, cfg, w = perturbed_state(4, (6, 8, 6), vel=10.0)
-    du = np.zeros(blk.shape())
-    sk, sj, si = blk.interior()
-    du[:, sk, sj, si] = np.random.default_rng(2).normal(
-        size=du[:, sk, sj, si].shape) * 1e-3
-    _, r = orl.implicit_solve(blk, th, du, w, 0.5, 10)  # periodic x2: fine
-    assert r == 0
+def _moist_walls(dims, seed=2):
+    blk, th, cfg, w = perturbed_state(4, dims, vel=10.0)
     bc = list(blk.bc)
     bc[2] = bc[3] = ca.BC_REFLECT
     _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
+    du = np.zeros(blk.shape())
+    sk, sj, si = blk.interior()
+    du[:, sk, sj, si] = np.random.default_rng(seed).normal(
+        size=du[:, sk, sj, si].shape) * 1e-3
+    return blk, th, w, du
+
+
+def test_walled_species_lines_read_w_at_the_shuffled_index():
+    """BackwardSubstitution splits the density increment over the species
+    with w(n, k, j, i) indexed by the shuffled line buffer's loop indices
+    (forward_backward.hpp:109-115, solve_implicit_3d.cpp:59-62): for the x2
+    line cell (x1, x2, x3) = (5, 9, 6) that is the entry w[n, 5, 6, 9]
+    (x3 = 5, x2 = 6, x1 = 9), an x1 ghost no x2 line reads otherwise.
+    Raising w_n there by D changes only that cell's density increment, by
+    -dens D, with dens = (du_IDN' - du_IDN) / (1 - sum_n w_n(entry)) from the
+    write-back's own algebra (du_IDN' = dens + du_IDN - dens sum_n w_n)."""
+    blk, th, w, du = _moist_walls((6, 8, 7))
+    d0, r = orl.implicit_solve(blk, th, du, w, 0.5, 10)
+    assert r == 0
+    cell = (0, 6, 9, 5)
+    wT = np.array([w[n, 5, 6, 9] for n in range(1, 4)])
+    dens = (d0[cell] - du[cell]) / (1.0 - wT.sum())
+    for n, D in ((1, 1e-3), (3, -2e-3)):
+        w1 = w.copy()
+        w1[n, 5, 6, 9] += D
+        d1, r = orl.implicit_solve(blk, th, du, w1, 0.5, 10)
+        assert r == 0
+        diff = d1 - d0
+        changed = np.argwhere(diff != 0)
+        assert [tuple(c) for c in changed] == [cell]
+        np.testing.assert_allclose(diff[cell], -dens * D, rtol=1e-9)
+
+
+def test_walled_species_lines_past_the_end_of_w_rejected():
+    # 2D x1-x2 block: the shuffled index of the x2 lines' species weights
+    # runs past the end of w (the reference reads out of bounds there)
+    blk, th, cfg, w = perturbed_state(4, (8, 10, 1), vel=10.0)
+    bc = list(blk.bc)
+    bc[2] = bc[3] = ca.BC_REFLECT
+    _set_bc(blk, bc)
+    w = ca.apply_bc_host(blk, w)
+    du = np.zeros(blk.shape())
     _, r = orl.implicit_solve(blk, th, du, w, 0.5, 10)
     assert r == ca.ERR_ARG
+    bc[2] = bc[3] = ca.BC_PERIODIC  # periodic lines weight no species
+    _set_bc(blk, bc)
+    _, r = orl.implicit_solve(blk, th, du, ca.apply_bc_host(blk, w), 0.5, 10)
+    assert r == 0
