@@ -17,6 +17,8 @@
 // normal-momentum flux difference, which is the part that balances gravity.
 #pragma once
 
+#include <type_traits>
+
 #include "cg_timer.hpp"
 #include "kernels.hpp"
 
@@ -810,28 +812,36 @@ __device__ __forceinline__ void mix_fused_tile(
     }
     __syncthreads();
     // (2) face states: x2 for cells j0-1 .. j0+FT_J of the interior rows,
-    //     x3 for cells k0-1 .. k0+FT_K of the interior columns, as one
-    //     item list split evenly over the workgroup.  The velocity slots
-    //     are rotated so that the normal component comes first (x2: v2, v3,
-    //     v1; x3: v3, v1, v2), the order in which the Riemann solver of
-    //     that direction reads them: one code path serves both directions.
-    for (int e = t; e < NQ; e += NT) {
-      const bool x2 = e < NQ2;
-      const int e3 = e - NQ2;
-      const int pj = x2 ? (e % C2) + FT_H - 1 : (e3 % FT_J) + FT_H;
-      const int pk = x2 ? (e / C2) + FT_H : (e3 / FT_J) + FT_H - 1;
-      const int st = x2 ? 1 : FT_PJ;
-      const int rot = x2 ? 1 : 2;
+    //     x3 for cells k0-1 .. k0+FT_K of the interior columns, as two item
+    //     loops with the direction's strides known at compile time; the x3
+    //     items start one wavefront further on, so that the wavefront that
+    //     took the x2 remainder is not also given the x3 remainder (no
+    //     wavefront does more than one item beyond the even share).  The
+    //     velocity slots are rotated so that the normal component comes
+    //     first (x2: v2, v3, v1; x3: v3, v1, v2), the order in which the
+    //     Riemann solver of that direction reads them: one code path serves
+    //     both directions.
+    auto states = [&](auto dirc, int e, int pj, int pk) {
+      constexpr int D = decltype(dirc)::value;  // 1: x2, 2: x3
+      constexpr int st = (D == 1) ? 1 : FT_PJ;
       const float* pc0 = &plane[0][pk][pj];
 #pragma unroll
       for (int n = 0; n < N; ++n) {
-        const int sv = (n >= IVX && n < IVX + 3) ? IVX + (n - IVX + rot) % 3 : n;
+        const int sv = (n >= IVX && n < IVX + 3) ? IVX + (n - IVX + D) % 3 : n;
         const float* pv = pc0 + sv * (FT_PK * FT_PJ);
         float qm, qp;
         recon(pv[-2 * st], pv[-st], pv[0], pv[st], pv[2 * st], &qm, &qp);
         L.qm[n][e] = qm;
         L.qp[n][e] = qp;
       }
+    };
+    for (int e = t; e < NQ2; e += NT)
+      states(std::integral_constant<int, 1>(), e, (e % C2) + FT_H - 1, (e / C2) + FT_H);
+    {
+      constexpr int R2 = ((NQ2 % NT) + 63) / 64 * 64;
+      for (int e3 = (t + NT - R2) % NT; e3 < NQ - NQ2; e3 += NT)
+        states(std::integral_constant<int, 2>(), NQ2 + e3, (e3 % FT_J) + FT_H,
+               (e3 / FT_J) + FT_H - 1);
     }
     __syncthreads();
     // (3) x2 / x3 Riemann fluxes of the tile's faces, one item list
