@@ -535,6 +535,8 @@ def bench_moist(ctx, nx, steps, warmup, acc_steps, npairs, train_steps):
     out["chosen"] = (min(ok, key=lambda n: forms[n]["ms_per_step"]) if ok
                      else "plain_fp64_newton")
     out["forms"] = forms
+    out["kessler"] = bench_kessler(ctx, blk, th, cfg, w0, dt, steps, warmup,
+                                   forms["mixed_newton"]["tune"])
     net.close()
     dx.free()
     dtt.free()
@@ -723,6 +725,46 @@ def bench_batch(ctx, seed, nblocks, steps, warmup, acc_steps):
     return out
 
 
+# Kessler rates of examples/2019-Li-snap/earth-precip.yaml:44-53 and
+# water-kessler.yaml:4-18 (s^-1), precipitation fall speed (m/s)
+KESSLER_RATES = dict(k_auto=0.01, k_accr=0.02, k_evap=0.1, v_fall=8.0)
+
+
+def bench_kessler(ctx, blk, th, cfg, w0, dt, steps, warmup, tune):
+    """The moist box's chosen form (FP32 fluxes, plain Newton) with Kessler
+    warm rain and precipitation sedimentation after the last stage
+    (k_kessler: one lane per column sweeping down, one read and one write of
+    u and w per cell, 32 N algorithmic bytes per cell).  ms per step, the
+    kernel's event-timed average and its HBM rate."""
+    cfg.precision = ca.PREC_FP32
+    cfg.satadj = ca.SATADJ_NEWTON
+    for k, v in KESSLER_RATES.items():
+        setattr(cfg, k, v)
+    t = ca.Tune(*[tune[k] for k, _ in ca.Tune._fields_]) if tune else None
+    try:
+        ms, stats, _, med = time_form(ctx, [blk], th, cfg, [w0], dt, steps,
+                                      warmup, tune=t)
+    finally:
+        for k in KESSLER_RATES:
+            setattr(cfg, k, 0.0)
+    cells = blk.ncells()
+    kk = stats.get("k_kessler", (0.0, 0))
+    alg = 32 * blk.N * cells
+    ach = alg / (kk[0] * 1e-3) / 1e9 if kk[0] > 0 else None
+    log("kessler: %.3f ms/step, k_kessler %.4f ms" % (ms, kk[0]))
+    return {"workload": "moist box %dx%dx%d, FP32 fluxes + plain Newton, "
+                        "Kessler rates %s" % (blk.nx1, blk.nx2, blk.nx3,
+                                              KESSLER_RATES),
+            "ms_per_step": round(ms, 4),
+            "ms_per_step_median_of_20": round(med, 4),
+            "k_kessler_ms": round(kk[0], 4),
+            "k_kessler_launches_per_step": round(kk[1], 2),
+            "algorithmic_bytes": alg,
+            "achieved_GBs": round(ach, 1) if ach else None,
+            "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
+            "kernels_avg_ms": {k: round(v[0], 4) for k, v in stats.items()}}
+
+
 def bench_sphere(ctx, steps, warmup, dt=60.0, nx=(64, 96, 96), acc_steps=20):
     """Config 3: the cubed sphere of six 96 x 96 panels of 64 levels with the
     Held-Suarez forcing and rotation, 3 x 3 implicit x1 columns (hs94.inp),
@@ -850,6 +892,8 @@ def main():
                          "(pairs, a few training steps, 3 forwards of 8M)")
     ap.add_argument("--sphere", action="store_true",
                     help="only the cubed sphere (config 3)")
+    ap.add_argument("--kessler", action="store_true",
+                    help="only the moist box with Kessler microphysics")
     ap.add_argument("--lines", action="store_true",
                     help="only the dry box with implicit x2/x3 lines "
                          "(implicit_flag 15)")
@@ -871,6 +915,15 @@ def main():
     if args.sphere:
         ctx = ca.Context(0)
         print(json.dumps(bench_sphere(ctx, max(1, args.steps), args.warmup)))
+        ctx.close()
+        return
+    if args.kessler:
+        ctx = ca.Context(0)
+        blk, th, cfg = ca.pgen_setup(4, *args.nx)
+        w0 = ca.pgen_init(4, 1, blk, th, cfg)
+        print(json.dumps(bench_kessler(ctx, blk, th, cfg, w0,
+                                       cfl_dt(blk, th, cfg, w0),
+                                       max(1, args.steps), args.warmup, None)))
         ctx.close()
         return
     if args.lines:
