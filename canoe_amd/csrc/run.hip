@@ -335,29 +335,36 @@ int sphere_stage(cg_run* r, int s) {
 #undef CG_SPH_FACES
   TIMED(tm, KID_DIVERGENCE,
         (k_sph_divergence<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, sbeta[s], r->d_dt)));
-  if (r->cfg.implicit_flag & 1) {
-    const long long ncol = (long long)g.nx2 * g.nx3;
-    if (r->cfg.implicit_flag & 8) {
-      TIMED(tm, KID_IMPLICIT_FWD,
-            (k_sph_implicit_fwd<5><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(
-                ks, r->kt, sbeta[s], r->d_dt, S.arch, S.arch_panel, r->ctx->d_status)));
-      TIMED(tm, KID_IMPLICIT_BWD,
-            (k_sph_implicit_bwd<5><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(ks, S.arch,
-                                                                         S.arch_panel)));
-    } else {
-      TIMED(tm, KID_IMPLICIT_FWD,
-            (k_sph_implicit_fwd<3><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(
-                ks, r->kt, sbeta[s], r->d_dt, S.arch, S.arch_panel, r->ctx->d_status)));
-      TIMED(tm, KID_IMPLICIT_BWD,
-            (k_sph_implicit_bwd<3><<<dim3(nblk(ncol, 64), 6), 64, 0, st>>>(ks, S.arch,
-                                                                         S.arch_panel)));
-    }
-  }
   KUpdate up;
   up.stage = s;
   up.g1 = sg1[s];
   up.g2 = sg2[s];
   up.satadj = 0;
+  if (r->cfg.implicit_flag & 1) {
+    // forward sweep, then one downward pass per column: back substitution,
+    // stage update, ConservedToPrimitive, x1 walls, next stage's psf / psv;
+    // then the x2 / x3 ghosts from the neighbour panels
+    const long long ncol = (long long)g.nx2 * g.nx3;
+    const dim3 gc(nblk(ncol, 64), 6);
+    if (r->cfg.implicit_flag & 8) {
+      TIMED(tm, KID_IMPLICIT_FWD,
+            (k_sph_implicit_fwd<5><<<gc, 64, 0, st>>>(ks, r->kt, sbeta[s], r->d_dt, S.arch,
+                                                      S.arch_panel, r->ctx->d_status)));
+      TIMED(tm, KID_IMPLICIT_BWD,
+            (k_sph_bwd_update<5><<<gc, 64, 0, st>>>(ks, up, S.arch, S.arch_panel)));
+    } else {
+      TIMED(tm, KID_IMPLICIT_FWD,
+            (k_sph_implicit_fwd<3><<<gc, 64, 0, st>>>(ks, r->kt, sbeta[s], r->d_dt, S.arch,
+                                                      S.arch_panel, r->ctx->d_status)));
+      TIMED(tm, KID_IMPLICIT_BWD,
+            (k_sph_bwd_update<3><<<gc, 64, 0, st>>>(ks, up, S.arch, S.arch_panel)));
+    }
+    TIMED(tm, KID_BC_X2,
+          (k_sph_ghosts<<<nblk((long long)S.nghost * ks.nr, 256), 256, 0, st>>>(
+              ks, S.d_ptr, S.d_ghosts, S.nghost)));
+    S.dirty = false;
+    return CG_OK;
+  }
   TIMED(tm, KID_UPDATE, (k_sph_update<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, up)));
   sphere_fill(r);
   return CG_OK;

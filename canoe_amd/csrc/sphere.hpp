@@ -2,9 +2,9 @@
 // gnomonic-equiangle panels (src/exo3/gnomonic_equiangle.cpp), each a block
 // of the run in the level-major layout, dry gas, FP64.
 //
-// Per stage: x1 fluxes (k_flux_x1: decomposition, WENO5, HLLC) with the
-// covariant projection of the tangential momentum fluxes (k_sph_x1_project,
-// hllc_transform.cpp:235-272); x2 / x3 face states of every panel
+// Per stage: x1 fluxes (k_sph_flux_x1: decomposition, WENO5, HLLC) with the
+// covariant projection of the tangential momentum fluxes
+// (hllc_transform.cpp:235-272); x2 / x3 face states of every panel
 // (k_sph_faces); the panel-edge exchange of face states (k_sph_exchange,
 // cubed_sphere.cpp:250-565); HLLC in the face's orthonormal frame
 // (k_sph_face_flux: PrimToLocal2/3, FluxToGlobal2/3 of
@@ -12,9 +12,10 @@
 // divergence with the panel's areas and volumes, the geometric source terms
 // (gnomonic_equiangle.cpp:831-877), gravity and the Held-Suarez / Coriolis
 // forcing of examples/2023-Chen-exo3/hs94.cpp:56-151 (k_sph_divergence);
-// the x1 column solve with the panel's areas (k_sph_implicit_fwd and
-// k_implicit_bwd); the stage update with ConservedToPrimitive for covariant
-// momenta (k_sph_update, patches/26.adiabatic_hydro.patch:21-46); the ghost
+// the x1 column solve with the panel's areas (k_sph_implicit_fwd, then
+// k_sph_bwd_update, which also applies the stage, the x1 walls and the
+// scan); the stage update with ConservedToPrimitive for covariant momenta
+// (k_sph_update, patches/26.adiabatic_hydro.patch:21-46); the ghost
 // zones of x2 / x3 interpolated from the neighbour panels (k_sph_ghosts,
 // cubed_sphere_utility.cpp:80-400); x1 walls and the hydrostatic scan.
 // Arithmetic is written in the order of the oracle's restatement so that
@@ -448,9 +449,17 @@ __global__ void __launch_bounds__(64)
     atomicOr(status, 1);
 }
 
+// x1 columns of panel blockIdx.y after the forward sweep, top down: the
+// back substitution and write-back (forward_backward.hpp:86-124), the stage
+// update and ConservedToPrimitive of k_sph_update, the reflecting x1 ghosts
+// of k_sph_bc_x1 and the next stage's hydrostatic face and cell pressures of
+// k_sph_scan, each cell as its new state appears (one pass over the column;
+// each step's arithmetic is that of the separate kernel, which the explicit
+// step still uses).
 template <int B>
 __global__ void __launch_bounds__(64)
-    k_sph_implicit_bwd(KSph s, const double* __restrict__ arch, long long arch_panel) {
+    k_sph_bwd_update(KSph s, KUpdate up, const double* __restrict__ arch,
+                     long long arch_panel) {
   constexpr int AS = B * B + B;
   const int p = blockIdx.y;
   const KGrid& g = s.g;
@@ -460,23 +469,137 @@ __global__ void __launch_bounds__(64)
   const int j = g.js + (int)(col % g.nx2);
   const int k = g.ks + (int)(col / g.nx2);
   const double* ap = arch + p * arch_panel;
-  double* du = s.du[p];
-  const long long sn = g.c.n;
-  Vt<B> dn;
-  for (int i = g.ie; i >= g.is; --i) {
-    Vt<B> d;
-    back_step<B>(ap, (long long)(i - g.is) * AS, ncol, col, i == g.ie, dn, d);
-    dn = d;
-    const long long ci = g.at(0, k, j, i);
-    double duo[5], wsp[5], dun[5];
+  const double* du = s.du[p];
+  double* u = s.u[p];
+  double* u1 = s.u1[p];
+  double* w = s.w[p];
+  double* psf = s.psf[p];
+  double* psv = s.psv[p];
+  const long long sn = g.c.n, si = g.c.i;
+  const long long c0 = g.at(0, k, j, 0);
+  const long long pc = col * g.ps_c;
+  const double cth = skj(s, p, cg_host::SKJ_COSC, k, j);
+  const int is = g.is, ie = g.ie;
+  const double grav = g.grav, dx = g.dx1;
+  const bool hs = grav != 0.0;
+  double pf_next = 0.0, pf_top = 0.0;
+  // the archive and state of a level are requested two levels ahead of its
+  // use (three buffers), so that their latency overlaps the levels above
+  struct Level {
+    Mt<B> a;
+    Vt<B> dl;
+    double duo[5], uo[5], u1o[5];
+  };
+  auto fetch = [&](Level& L, int i) {
+    const long long lb = (long long)(i - is) * AS;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      duo[n] = du[ci + n * sn];
-      wsp[n] = 0.0;
+    for (int r = 0; r < B; ++r) L.dl.v[r] = ap[(lb + B * B + r) * ncol + col];
+    if (i != ie) {
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int q = 0; q < B; ++q) L.a.a[r][q] = ap[(lb + r * B + q) * ncol + col];
     }
-    write_back<0, B>(d, duo, wsp, dun);
+    const long long ci = c0 + i * si;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) du[ci + n * sn] = dun[n];
+    for (int v = 0; v < 5; ++v) {
+      L.duo[v] = du[ci + v * sn];
+      L.uo[v] = u[ci + v * sn];
+      L.u1o[v] = (up.stage == 0) ? 0.0 : u1[ci + v * sn];
+    }
+  };
+  Vt<B> dn;
+  auto level = [&](const Level& L, int i) {
+    // back substitution (forward_backward.hpp:86-97)
+    Vt<B> d;
+    if (i == ie) {
+      d = L.dl;
+    } else {
+      Vt<B> ad;
+      mvec<B>(L.a, dn, ad);
+#pragma unroll
+      for (int r = 0; r < B; ++r) d.v[r] = L.dl.v[r] - ad.v[r];
+    }
+    dn = d;
+    const long long ci = c0 + i * si;
+    double wsp[5], dun[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wsp[v] = 0.0;
+    write_back<0, B>(d, L.duo, wsp, dun);
+    double uc[5], wc[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      double x;
+      if (up.stage == 0) {
+        x = 0.0 + 1.0 * L.uo[v];
+        u1[ci + v * sn] = x;
+      } else {
+        x = up.g1 * L.uo[v] + up.g2 * L.u1o[v];
+      }
+      uc[v] = x + dun[v];
+    }
+    sph_cons_to_prim(s.gm1, cth, uc, wc);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      u[ci + v * sn] = uc[v];
+      w[ci + v * sn] = wc[v];
+    }
+    // reflecting x1 walls: ghost m mirrors the cell m from the wall
+    const int mt = ie - i, mb = i - is;
+    if (mt < NG) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        w[c0 + v * sn + (ie + 1 + mt) * si] = (v == 1) ? -1.0 * wc[v] : wc[v];
+    }
+    if (mb < NG) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        w[c0 + v * sn + (is - 1 - mb) * si] = (v == 1) ? -1.0 * wc[v] : wc[v];
+    }
+    if (hs) {
+      if (i == ie) {
+        const double RdTv = wc[4] / wc[0];
+        pf_next = wc[4] * exp_d(grav * dx / (2 * RdTv));
+        pf_top = pf_next;
+        psf[pc + (ie + 1) * g.ps_i] = pf_next;
+      }
+      const double pfi = pf_next - grav * wc[0] * dx;
+      psf[pc + i * g.ps_i] = pfi;
+      psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
+      pf_next = pfi;
+    }
+  };
+  Level L0, L1, L2;
+  fetch(L0, ie);
+  if (ie - 1 >= is) fetch(L1, ie - 1);
+  for (int i = ie; i >= is; i -= 3) {
+    if (i - 2 >= is) fetch(L2, i - 2);
+    level(L0, i);
+    if (i - 1 < is) break;
+    if (i - 3 >= is) fetch(L0, i - 3);
+    level(L1, i - 1);
+    if (i - 2 < is) break;
+    if (i - 4 >= is) fetch(L1, i - 4);
+    level(L2, i - 2);
+  }
+  if (hs) {
+    // ghost levels: below from psf(is) down, above from psf(ie + 1) up
+    const double gb = (g.bc[0] == 0) ? -grav : grav;
+    const double gt = (g.bc[1] == 0) ? -grav : grav;
+    double cur = pf_next;
+    for (int gh = 1; gh <= NG; ++gh) {
+      const double lo = cur - gb * w[c0 + (is - gh) * si] * dx;
+      psf[pc + (is - gh) * g.ps_i] = lo;
+      psv[pc + (is - gh) * g.ps_i] = log_mean(lo, cur);
+      cur = lo;
+    }
+    cur = pf_top;
+    for (int gh = 1; gh <= NG; ++gh) {
+      const double hi = cur + gt * w[c0 + (ie + gh) * si] * dx;
+      psf[pc + (ie + gh + 1) * g.ps_i] = hi;
+      psv[pc + (ie + gh) * g.ps_i] = log_mean(cur, hi);
+      cur = hi;
+    }
   }
 }
 
