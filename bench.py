@@ -734,8 +734,10 @@ def bench_kessler(ctx, blk, th, cfg, w0, dt, steps, warmup, tune):
     """The moist box's chosen form (FP32 fluxes, plain Newton) with Kessler
     warm rain and precipitation sedimentation after the last stage
     (k_kessler: one lane per column sweeping down, one read and one write of
-    u and w per cell, 32 N algorithmic bytes per cell).  ms per step, the
-    kernel's event-timed average and its HBM rate."""
+    u and w per cell, and, with the box's reflecting x1 walls, the column's
+    x1 ghosts and next-stage hydrostatic pressures psf / psv in the same
+    sweep: 32 N + 16 algorithmic bytes per cell).  ms per step, the kernel's
+    event-timed average and its HBM rate."""
     cfg.precision = ca.PREC_FP32
     cfg.satadj = ca.SATADJ_NEWTON
     for k, v in KESSLER_RATES.items():
@@ -749,7 +751,7 @@ def bench_kessler(ctx, blk, th, cfg, w0, dt, steps, warmup, tune):
             setattr(cfg, k, 0.0)
     cells = blk.ncells()
     kk = stats.get("k_kessler", (0.0, 0))
-    alg = 32 * blk.N * cells
+    alg = (32 * blk.N + 16) * cells
     ach = alg / (kk[0] * 1e-3) / 1e9 if kk[0] > 0 else None
     log("kessler: %.3f ms/step, k_kessler %.4f ms" % (ms, kk[0]))
     return {"workload": "moist box %dx%dx%d, FP32 fluxes + plain Newton, "

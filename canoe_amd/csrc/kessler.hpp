@@ -26,18 +26,28 @@ struct KKessler {
 };
 
 // One column (k, j): u (conserved) in / out, w (primitive) out, interior
-// cells only.  dt is the full step.
-template <int NM>
+// cells only.  dt is the full step.  FUSE (reflecting or outflow x1 sides):
+// the column's x1 ghosts of w (those of k_bc_x1) and its hydrostatic face
+// and cell pressures psf / psv at column offset pc (those of
+// k_hydrostatic_scan) as each new cell state appears, top down.
+template <int NM, bool FUSE = false>
 __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th,
                                                const KKessler& kz, double dt,
                                                double* __restrict__ u,
-                                               double* __restrict__ w, int k, int j) {
+                                               double* __restrict__ w, int k, int j,
+                                               double* __restrict__ psf = nullptr,
+                                               double* __restrict__ psv = nullptr,
+                                               long long pc = 0) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
   const int rv_ = 1 + th.iv, rc_ = 1 + th.ic, rp_ = 1 + kz.ip;
   const double cvp = th.cvd * (1. + th.cv_ratio_m1[kz.ip]);
   double cf = dt * kz.v_fall / g.dx1;
   if (cf > 1.) cf = 1.;
-  const long long sn = g.c.n;
+  const long long sn = g.c.n, si = g.c.i;
+  const long long cb = g.at(0, k, j, 0);
+  const int is = g.is, ie = g.ie;
+  const double grav = g.grav, dx = g.dx1;
+  double pf_next = 0., pf_top = 0.;
   double in_p = 0., in_m0 = 0., in_m1 = 0., in_m2 = 0., in_e = 0.;
   for (int i = g.ie; i >= g.is; --i) {
     const long long c0 = g.at(0, k, j, i);
@@ -102,11 +112,67 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
       u[c0 + n * sn] = uc[n];
       w[c0 + n * sn] = wc[n];
     }
+    if (FUSE) {
+      // x1 ghosts (bc_pair: reflecting sides mirror, normal velocity
+      // negated; outflow sides copy the boundary cell)
+      const int mt = ie - i, mb = i - is;
+      if (mt < NG && g.bc[1] == 0) {
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+          w[cb + n * sn + (ie + 1 + mt) * si] = ((n == IVX) ? -1.0 : 1.0) * wc[n];
+      }
+      if (mt == 0 && g.bc[1] == 2) {
+        for (int gh = 1; gh <= NG; ++gh)
+#pragma unroll
+          for (int n = 0; n < N; ++n) w[cb + n * sn + (ie + gh) * si] = wc[n];
+      }
+      if (mb < NG && g.bc[0] == 0) {
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+          w[cb + n * sn + (is - 1 - mb) * si] = ((n == IVX) ? -1.0 : 1.0) * wc[n];
+      }
+      if (mb == 0 && g.bc[0] == 2) {
+        for (int gh = 1; gh <= NG; ++gh)
+#pragma unroll
+          for (int n = 0; n < N; ++n) w[cb + n * sn + (is - gh) * si] = wc[n];
+      }
+      // hydrostatic face and cell pressures (k_hydrostatic_scan)
+      if (grav != 0.0) {
+        if (i == ie) {
+          const double RdTv = wc[IEN] / wc[0];
+          pf_next = wc[IEN] * exp_d(grav * dx / (2 * RdTv));
+          pf_top = pf_next;
+          psf[pc + (ie + 1) * g.ps_i] = pf_next;
+        }
+        const double pfi = pf_next - grav * wc[0] * dx;
+        psf[pc + i * g.ps_i] = pfi;
+        psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
+        pf_next = pfi;
+      }
+    }
     in_p = out_p;
     in_m0 = out_m0;
     in_m1 = out_m1;
     in_m2 = out_m2;
     in_e = out_e;
+  }
+  if (FUSE && grav != 0.0) {
+    const double gb = (g.bc[0] == 0) ? -grav : grav;
+    const double gt = (g.bc[1] == 0) ? -grav : grav;
+    double cur = pf_next;
+    for (int gh = 1; gh <= NG; ++gh) {
+      const double lo = cur - gb * w[cb + (is - gh) * si] * dx;
+      psf[pc + (is - gh) * g.ps_i] = lo;
+      psv[pc + (is - gh) * g.ps_i] = log_mean(lo, cur);
+      cur = lo;
+    }
+    cur = pf_top;
+    for (int gh = 1; gh <= NG; ++gh) {
+      const double hi = cur + gt * w[cb + (ie + gh) * si] * dx;
+      psf[pc + (ie + gh + 1) * g.ps_i] = hi;
+      psv[pc + (ie + gh) * g.ps_i] = log_mean(cur, hi);
+      cur = hi;
+    }
   }
 }
 
@@ -120,6 +186,22 @@ __global__ void __launch_bounds__(64)
   const int j = g.js + (int)(t % g.nx2);
   const int k = g.ks + (int)(t / g.nx2);
   kessler_column<NM>(g, th, kz, dtp ? dt * dtp[0] : dt, u, w, k, j);
+}
+
+// k_kessler followed, in the same column sweep, by the column's x1 ghosts
+// and hydrostatic pressures (runs whose x1 sides are walls or outflow).
+template <int NM>
+__global__ void __launch_bounds__(64)
+    k_kessler_fill(KGrid g, KThermo th, KKessler kz, double dt,
+                   const double* __restrict__ dtp, double* __restrict__ u,
+                   double* __restrict__ w, double* __restrict__ psf,
+                   double* __restrict__ psv) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx2 * g.nx3) return;
+  const int j = g.js + (int)(t % g.nx2);
+  const int k = g.ks + (int)(t / g.nx2);
+  kessler_column<NM, true>(g, th, kz, dtp ? dt * dtp[0] : dt, u, w, k, j, psf, psv,
+                           t * g.ps_c);
 }
 
 }  // namespace cg

@@ -337,4 +337,15 @@ void launch_kessler(const KGrid& g, const KThermo& th, const KKessler& kz, doubl
                                                                           u, w)));
 }
 
+// launch_kessler with the column's x1 ghosts and hydrostatic pressures in
+// the same sweep (x1 sides that are walls or outflow)
+template <int NM>
+void launch_kessler_fill(const KGrid& g, const KThermo& th, const KKessler& kz, double dt,
+                         const double* dtp, double* u, double* w, double* psf,
+                         double* psv, hipStream_t s, KTimer* tm = nullptr) {
+  TIMED(tm, KID_KESSLER,
+        (k_kessler_fill<NM><<<nblk((long long)g.nx2 * g.nx3, 64), 64, 0, s>>>(
+            g, th, kz, dt, dtp, u, w, psf, psv)));
+}
+
 }  // namespace cg_host

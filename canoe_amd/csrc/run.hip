@@ -244,8 +244,15 @@ int run_stage(cg_run* r, int s) {
       if (guided)
         TIMED(tm, KID_SATADJ,
               (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
-      if (kes) launch_kessler<NM>(g, r->kt, r->kz, 1.0, r->d_dt, b.u, b.w, st, tm);
-      if (guided || kes) {
+      // the Kessler sweep also refills the x1 ghosts and the hydrostatic
+      // pressures where the x1 sides are walls or outflow
+      const bool kfill = kes && g.bc[0] != CG_BC_PERIODIC && g.bc[1] != CG_BC_PERIODIC;
+      if (kfill)
+        launch_kessler_fill<NM>(g, r->kt, r->kz, 1.0, r->d_dt, b.u, b.w, b.psf, b.psv, st,
+                                tm);
+      else if (kes)
+        launch_kessler<NM>(g, r->kt, r->kz, 1.0, r->d_dt, b.u, b.w, st, tm);
+      if ((guided || kes) && !kfill) {
         // changed cells change w: x1 ghosts and hydrostatic pressures again
         TIMED(tm, KID_BC_X1,
               (k_bc_x1<NM><<<nblk((long long)g.nx2 * g.nx3 * NG, 256), 256, 0, st>>>(g, b.w)));
