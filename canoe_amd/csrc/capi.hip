@@ -379,10 +379,13 @@ int cg_calculate_fluxes_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* 
   if (!e && g.f2) e = stage_in(ctx, f2, flux2, f2_doubles(g, N));
   if (!e && g.f3) e = stage_in(ctx, f3, flux3, f3_doubles(g, N));
   if (!e) e = cg_calculate_fluxes(ctx, blk, th, cfg, dw.p, f1.p, f2.p, f3.p);
+  // a failed face check still returns the fluxes, as the device call does
+  const int fe = e;
+  if (e == CG_ERR_NEGATIVE) e = CG_OK;
   if (!e) e = stage_out(ctx, flux1, f1);
   if (!e && g.f2) e = stage_out(ctx, flux2, f2);
   if (!e && g.f3) e = stage_out(ctx, flux3, f3);
-  return e;
+  return e ? e : fe;
 }
 
 int cg_implicit_solve_host(cg_ctx* ctx, const cg_block* blk, const cg_thermo* th,

@@ -86,3 +86,18 @@ def test_run_step_reports_negative_faces(ctx, prec):
         run.step(0.01, 1, graph=True)
     finally:
         run.close()
+
+
+def test_host_variant_reports_and_returns_fluxes(ctx):
+    blk, th, cfg, w = _state([(4, 6, 7, 9, -1.0)])
+    fs = [np.full(s, 7.0) for s in blk.flux_shapes()]
+    rc = ca.lib().cg_calculate_fluxes_host(
+        ctx.h, ca._ref(blk), ca._ref(th), ca._ref(cfg), ca._ptr(w), ca._ptr(fs[0]),
+        ca._ptr(fs[1]), ca._ptr(fs[2]))
+    assert rc == CG_ERR_NEGATIVE
+    fo = orl.calculate_fluxes(blk, th, cfg, w)
+    sk, sj, si = blk.interior()
+    for d, (a, b) in enumerate(zip(fs, fo)):
+        s = [slice(None), sk, sj, si]
+        s[3 - d] = slice(s[3 - d].start, s[3 - d].stop + 1)
+        np.testing.assert_array_equal(a[tuple(s)], b[tuple(s)])

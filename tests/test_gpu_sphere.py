@@ -108,3 +108,23 @@ def test_sphere_fp32_faces_close_to_fp64(ctx):
     _say("sphere fp32 faces vs fp64, rel L2 per variable: %s" % rows)
     assert max(rows[0], rows[4]) < 1e-6 and max(rows[1:4]) < 1e-3
 
+
+
+def test_sphere_reports_negative_faces(ctx):
+    """The face checks (check_reconstruct) of the sphere's x1 and panel
+    faces: a cell of negative pressure makes the step return
+    CG_ERR_NEGATIVE, in both forms."""
+    blk, th, cfg, sp, w = _state()
+    sk, sj, si = blk.interior()
+    w[2, 4, sk.start + 3, sj.start + 4, si.start + 7] = -1.0
+    for prec in (ca.PREC_FP64, ca.PREC_FP32):
+        cfg.precision = prec
+        _say("sphere negative cell, precision %d" % prec)
+        run = ca.SphereRun(ctx, blk, th, cfg, sp)
+        try:
+            run.set_all(w)
+            with pytest.raises(ca.CanoeGpuError) as e:
+                run.step(100.0, 1, graph=True)
+            assert e.value.code == 4
+        finally:
+            run.close()
