@@ -119,3 +119,29 @@ def test_run_with_kessler_mixed_close_to_fp64(ctx):
     e = relerr(out[1][:, sk, sj, si], out[0][:, sk, sj, si])
     _say("kessler mixed vs fp64 relerr %.3e" % e)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("bcs", [(ca.BC_REFLECT, ca.BC_OUTFLOW), (ca.BC_OUTFLOW, ca.BC_REFLECT)])
+def test_run_with_kessler_outflow_x1_equals_oracle(ctx, bcs):
+    """The Kessler sweep of an implicit run also writes the column's x1
+    ghosts (here an outflow side: copies of the boundary cell) and the next
+    stage's hydrostatic pressures; the steps equal the oracle's."""
+    blk, th, cfg, w = _moist()
+    blk.bc[0], blk.bc[1] = bcs
+    w = ca.apply_bc_host(blk, w)
+    dt = cfl_dt(blk, th, cfg, w, 0.5)
+    _say("kessler run, x1 bcs %s" % (bcs,))
+    run = ca.Run(ctx, blk, th, cfg)
+    try:
+        run.set_prim(w)
+        run.step(dt, 3, graph=True)
+        ug, wg = run.cons(), run.prim()
+    finally:
+        run.close()
+    u = orl.prim_to_cons(blk, th, w)
+    uo, wo = orl.step(blk, th, cfg, dt, 3, u, w)
+    sk, sj, si = blk.interior()
+    eu = relerr(ug[:, sk, sj, si], uo[:, sk, sj, si])
+    ew = relerr(wg[:, sk, sj, si], wo[:, sk, sj, si])
+    _say("  relerr u %.3e w %.3e" % (eu, ew))
+    assert eu <= 1e-12 and ew <= 1e-12
