@@ -951,13 +951,14 @@ __device__ inline void back_step(const double* __restrict__ arch,
                                  long long lb, long long as, long long ao,
                                  bool top, const Vt<B>& dn, Vt<B>& d) {
 #pragma unroll
-  for (int r = 0; r < B; ++r) d.v[r] = arch[(lb + B * B + r) * as + ao];
+  for (int r = 0; r < B; ++r) d.v[r] = __builtin_nontemporal_load(&arch[(lb + B * B + r) * as + ao]);
   if (!top) {
     Mt<B> a;
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int q = 0; q < B; ++q) a.a[r][q] = arch[(lb + r * B + q) * as + ao];
+      for (int q = 0; q < B; ++q)
+        a.a[r][q] = __builtin_nontemporal_load(&arch[(lb + r * B + q) * as + ao]);
     Vt<B> ad;
     mvec<B>(a, dn, ad);
 #pragma unroll
@@ -1152,15 +1153,19 @@ __device__ __forceinline__ void bwd_update_column(
     Vt<B> dl;
     double duo[N], wsp[N], uo[N], u1o[N];
   };
+  // the archive is read once: non-temporal loads keep it from displacing
+  // the state in the caches
   auto fetch = [&](Level& L, int i) {
     const long long lb = (long long)(i - is) * AS;
 #pragma unroll
-    for (int r = 0; r < B; ++r) L.dl.v[r] = arch[(lb + B * B + r) * as + ao];
+    for (int r = 0; r < B; ++r)
+      L.dl.v[r] = __builtin_nontemporal_load(&arch[(lb + B * B + r) * as + ao]);
     if (i != ie) {
 #pragma unroll
       for (int r = 0; r < B; ++r)
 #pragma unroll
-        for (int q = 0; q < B; ++q) L.a.a[r][q] = arch[(lb + r * B + q) * as + ao];
+        for (int q = 0; q < B; ++q)
+          L.a.a[r][q] = __builtin_nontemporal_load(&arch[(lb + r * B + q) * as + ao]);
     }
     const long long ci = c0 + i * si;
 #pragma unroll

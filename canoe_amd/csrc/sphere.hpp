@@ -493,12 +493,14 @@ __global__ void __launch_bounds__(64)
   auto fetch = [&](Level& L, int i) {
     const long long lb = (long long)(i - is) * AS;
 #pragma unroll
-    for (int r = 0; r < B; ++r) L.dl.v[r] = ap[(lb + B * B + r) * ncol + col];
+    for (int r = 0; r < B; ++r)
+      L.dl.v[r] = __builtin_nontemporal_load(&ap[(lb + B * B + r) * ncol + col]);
     if (i != ie) {
 #pragma unroll
       for (int r = 0; r < B; ++r)
 #pragma unroll
-        for (int q = 0; q < B; ++q) L.a.a[r][q] = ap[(lb + r * B + q) * ncol + col];
+        for (int q = 0; q < B; ++q)
+          L.a.a[r][q] = __builtin_nontemporal_load(&ap[(lb + r * B + q) * ncol + col]);
     }
     const long long ci = c0 + i * si;
 #pragma unroll
