@@ -6,11 +6,20 @@
 // diagnostics; at the end the history totals and the zone-cycles report of
 // src/mesh_destroy.cpp:38-57.  The block and its initial state come from
 // the benchmark configurations (1: dry density current, 2: dry stratified
-// box, 4: moist box, 5: one seeded block of the uneven-column batch).
+// box, 3: the cubed sphere of six panels with the Held-Suarez forcing,
+// 4: moist box, 5: one seeded block of the uneven-column batch).
 //
 //   canoe_step <config> [--nx nx1 nx2 nx3] [--nlim N] [--tlim T] [--cfl C]
 //              [--fp32] [--no-graph] [--seed S] [--out K] [--hst] [--dbg]
-//              [--nc] [--basename NAME] [--implicit-flag F]
+//              [--nc] [--basename NAME] [--implicit-flag F] [--kessler]
+//              [--dt DT]
+//
+// --kessler turns on Kessler warm rain and precipitation sedimentation with
+// the rates of examples/2019-Li-snap/earth-precip.yaml:44-53 and
+// water-kessler.yaml:4-18 (moist configurations).  Config 3 advances the
+// six panels with the fixed step of hs94.inp (--dt, default 60 s) and
+// reports the panels' total mass; the other options that need a single
+// block (--hst, --dbg, --nc, --cfl) do not apply to it.
 //
 // The cycles run on the device through cg_run_advance: each step is
 // followed by the patched Hydro::NewBlockTimeStep reduced over the resident
@@ -34,12 +43,6 @@
 
 #include "../../include/canoe_gpu.h"
 
-extern "C" {
-int cg_pgen_setup(int config, int nx1, int nx2, int nx3, cg_block* blk,
-                  cg_thermo* th, cg_config* cfg);
-int cg_pgen_init(int config, uint64_t seed, const cg_block* blk,
-                 const cg_thermo* th, const cg_config* cfg, double* w);
-}
 
 namespace {
 
@@ -66,7 +69,8 @@ void usage() {
                "usage: canoe_step <config 1|2|4|5> [--nx nx1 nx2 nx3] "
                "[--nlim N] [--tlim T] [--cfl C] [--fp32] "
                "[--no-graph] [--seed S] [--out K] [--hst] [--dbg] "
-               "[--nc] [--basename NAME] [--implicit-flag F]\n");
+               "[--nc] [--basename NAME] [--implicit-flag F] [--kessler] "
+               "[--dt DT]\n");
 }
 
 int check(cg_ctx* ctx, int rc, const char* what) {
@@ -74,6 +78,74 @@ int check(cg_ctx* ctx, int rc, const char* what) {
     std::fprintf(stderr, "canoe_step: %s failed (%d): %s\n", what, rc,
                  ctx ? cg_last_error(ctx) : "");
   return rc;
+}
+
+// Config 3: the six panels of the cubed sphere (cg_sphere_create) advanced by
+// nlim fixed steps of dt, replayed from the run's graph; every out_every
+// cycles the density summed over the panels' interior cells.
+int run_sphere(const cg_block& blk, const cg_thermo& th, const cg_config& cfg,
+               uint64_t seed, long long nlim, long long out_every, double dt,
+               bool graph) {
+  Dims d(blk);
+  cg_sphere sp;
+  std::vector<double> w6((size_t)6 * d.N * d.sn);
+  if (cg_pgen_sphere(&sp) != CG_OK ||
+      cg_pgen_sphere_init(seed, &blk, &th, w6.data()) != CG_OK)
+    return 1;
+  cg_ctx* ctx = nullptr;
+  int rc = cg_create(0, &ctx);
+  if (rc != CG_OK) {
+    std::fprintf(stderr, "canoe_step: no HIP device (%d)\n", rc);
+    return 1;
+  }
+  cg_run* run = nullptr;
+  int err = check(ctx, cg_sphere_create(ctx, &blk, &th, &cfg, &sp, &run),
+                  "cg_sphere_create");
+  for (int p = 0; p < 6 && !err; ++p)
+    err = check(ctx, cg_run_block_set_prim_host(run, p, &w6[(size_t)p * d.N * d.sn]),
+                "set_prim");
+  if (err) {
+    if (run) cg_run_destroy(run);
+    cg_destroy(ctx);
+    return 1;
+  }
+  std::printf("canoe_step: config 3, 6 panels of %d x %d cells and %d levels, %s, %s\n",
+              blk.nx2, blk.nx3, blk.nx1,
+              cfg.precision == CG_PREC_FP32 ? "FP32 x2/x3 faces" : "FP64", cg_version());
+  std::vector<double> u((size_t)d.N * d.sn);
+  auto mass = [&](double* out) -> int {
+    double m = 0.0;
+    for (int p = 0; p < 6; ++p) {
+      if (check(ctx, cg_run_block_get_cons_host(run, p, u.data()), "get_cons")) return 1;
+      for (int k = d.ks; k < d.ks + blk.nx3; ++k)
+        for (int j = d.js; j < d.js + blk.nx2; ++j)
+          for (int i = d.is; i < d.is + blk.nx1; ++i) m += u[d.at(0, k, j, i)];
+    }
+    *out = m;
+    return 0;
+  };
+  double m0 = 0.0, m1 = 0.0;
+  err = mass(&m0);
+  long long ncycle = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  while (!err && ncycle < nlim) {
+    std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, ncycle * dt, dt);
+    const long long n = std::min(out_every, nlim - ncycle);
+    if ((err = check(ctx, cg_run_step(run, dt, (int)n, graph ? 1 : 0), "cg_run_step"))) break;
+    ncycle += n;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (!err) err = mass(&m1);
+  const double wall = std::chrono::duration<double>(t1 - t0).count();
+  const double zc = 6.0 * blk.nx1 * blk.nx2 * blk.nx3 * (double)ncycle;
+  std::printf("cycle=%lld time=%.6e dt=%.6e\n", ncycle, ncycle * dt, dt);
+  std::printf("panel density sum (initial -> final): %.16e -> %.16e\n", m0, m1);
+  std::printf("zone-cycles = %.0f\n", zc);
+  std::printf("wall time = %.6f s\n", wall);
+  std::printf("zone-cycles/wall_second = %.6e\n", wall > 0 ? zc / wall : 0.0);
+  cg_run_destroy(run);
+  cg_destroy(ctx);
+  return err ? 1 : 0;
 }
 
 }  // namespace
@@ -89,6 +161,8 @@ int main(int argc, char** argv) {
   double tlim = 1e300, cfl = 0.9;
   bool fp32 = false, graph = true, hst = false, dbg = false, nc = false;
   int iflag = -1;  // <hydro> implicit_flag (implicit_solver.cpp:47)
+  bool kessler = false;
+  double fixed_dt = 60.0;  // config 3: hs94.inp's step
   std::string base = "canoe";
   uint64_t seed = 1;
   for (int a = 2; a < argc; ++a) {
@@ -133,6 +207,11 @@ int main(int argc, char** argv) {
     } else if (s == "--implicit-flag") {
       need(1);
       iflag = std::atoi(argv[++a]);
+    } else if (s == "--kessler") {
+      kessler = true;
+    } else if (s == "--dt") {
+      need(1);
+      fixed_dt = std::atof(argv[++a]);
     } else {
       usage();
       return 2;
@@ -147,6 +226,13 @@ int main(int argc, char** argv) {
   }
   if (fp32) cfg.precision = CG_PREC_FP32;
   if (iflag >= 0) cfg.implicit_flag = iflag;
+  if (kessler) {
+    cfg.k_auto = 0.01;
+    cfg.k_accr = 0.02;
+    cfg.k_evap = 0.1;
+    cfg.v_fall = 8.0;
+  }
+  if (config == 3) return run_sphere(blk, th, cfg, seed, nlim, out_every, fixed_dt, graph);
   Dims d(blk);
   std::vector<double> w((size_t)d.N * d.sn);
   if (cg_pgen_init(config, seed, &blk, &th, &cfg, w.data()) != CG_OK) return 1;
