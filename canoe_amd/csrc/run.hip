@@ -321,24 +321,24 @@ int sphere_stage(cg_run* r, int s) {
   // FP64)
   const dim3 gf(nblk(nface, 256), 6);
   const unsigned ge = nblk((long long)S.nedge * ks.nr, 256);
-#define CG_SPH_FACES(R)                                                                   \
+#define CG_SPH_FACES(O, R)                                                                  \
   do {                                                                                    \
-    if (r->cfg.order_h == 5) {                                                            \
-      TIMED(tm, KID_FLUX_X2, (k_sph_faces<5, 1, R><<<gf, 256, 0, st>>>(ks)));             \
-      TIMED(tm, KID_FLUX_X3, (k_sph_faces<5, 2, R><<<gf, 256, 0, st>>>(ks)));             \
-    } else {                                                                              \
-      TIMED(tm, KID_FLUX_X2, (k_sph_faces<2, 1, R><<<gf, 256, 0, st>>>(ks)));             \
-      TIMED(tm, KID_FLUX_X3, (k_sph_faces<2, 2, R><<<gf, 256, 0, st>>>(ks)));             \
-    }                                                                                     \
     TIMED(tm, KID_FLUX_X2,                                                                \
-          (k_sph_exchange<R><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));               \
-    TIMED(tm, KID_FLUX_X2, (k_sph_face_flux<1, R><<<gf, 256, 0, st>>>(ks, r->kt)));       \
-    TIMED(tm, KID_FLUX_X3, (k_sph_face_flux<2, R><<<gf, 256, 0, st>>>(ks, r->kt)));       \
+          (k_sph_edge_states<O, R><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));         \
+    TIMED(tm, KID_FLUX_X2, (k_sph_face_flux<O, 1, R><<<gf, 256, 0, st>>>(ks, r->kt)));    \
+    TIMED(tm, KID_FLUX_X3, (k_sph_face_flux<O, 2, R><<<gf, 256, 0, st>>>(ks, r->kt)));    \
   } while (0)
-  if (r->cfg.precision == CG_PREC_FP32)
-    CG_SPH_FACES(float);
-  else
-    CG_SPH_FACES(double);
+  if (r->cfg.precision == CG_PREC_FP32) {
+    if (r->cfg.order_h == 5)
+      CG_SPH_FACES(5, float);
+    else
+      CG_SPH_FACES(2, float);
+  } else {
+    if (r->cfg.order_h == 5)
+      CG_SPH_FACES(5, double);
+    else
+      CG_SPH_FACES(2, double);
+  }
 #undef CG_SPH_FACES
   TIMED(tm, KID_DIVERGENCE,
         (k_sph_divergence<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, sbeta[s], r->d_dt)));

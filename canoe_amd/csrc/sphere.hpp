@@ -4,10 +4,10 @@
 //
 // Per stage: x1 fluxes (k_sph_flux_x1: decomposition, WENO5, HLLC) with the
 // covariant projection of the tangential momentum fluxes
-// (hllc_transform.cpp:235-272); x2 / x3 face states of every panel
-// (k_sph_faces); the panel-edge exchange of face states (k_sph_exchange,
-// cubed_sphere.cpp:250-565); HLLC in the face's orthonormal frame
-// (k_sph_face_flux: PrimToLocal2/3, FluxToGlobal2/3 of
+// (hllc_transform.cpp:235-272); the panel-edge exchange of face states
+// (k_sph_edge_states, cubed_sphere.cpp:250-565: the neighbours' inner states
+// at the edge faces); x2 / x3 face states and HLLC in the face's
+// orthonormal frame (k_sph_face_flux: PrimToLocal2/3, FluxToGlobal2/3 of
 // gnomonic_equiangle.cpp:559-828, then the covariant projection); flux
 // divergence with the panel's areas and volumes, the geometric source terms
 // (gnomonic_equiangle.cpp:831-877), gravity and the Held-Suarez / Coriolis
@@ -45,7 +45,7 @@ struct KSph {
   double* f3[6];
   double* psf[6];
   double* psv[6];
-  double* lr;  // face states: 4 buffers (l2, r2, l3, r3) per panel, lrn each
+  double* lr;  // edge buffer: neighbours' inner states at the panel-edge faces
   long long lrn;
   const double* x1f;
   const double* x1v;
@@ -63,15 +63,6 @@ struct KSphPtrs {
 
 __device__ __forceinline__ double skj(const KSph& s, int p, int a, int k, int j) {
   return s.kj[p][(long long)a * s.nc * s.nc + (long long)k * s.nc + j];
-}
-
-// face-state buffer b (0: l2, 1: r2, 2: l3, 3: r3) of panel p: variable v,
-// level ii (0-based), along index t, face f
-__device__ __forceinline__ long long slr(const KSph& s, int p, int b, int v, int ii, int t,
-                                         int f) {
-  const long long base = ((long long)p * 4 + b) * s.lrn;
-  if (b < 2) return base + (((long long)v * s.nr + ii) * s.n + t) * (s.n + 1) + f;
-  return base + (((long long)v * s.nr + ii) * (s.n + 1) + f) * s.n + t;
 }
 
 // ConservedToPrimitive / PrimitiveToConserved with covariant momenta
@@ -151,89 +142,91 @@ __global__ void __launch_bounds__(256) k_sph_bc_x1(KSph s) {
           g.at(0, k + g.ks, j + g.js, g.ie), g.c.i, gh + 1);
 }
 
-// x2 (DIR = 1) or x3 (DIR = 2) face states of every panel into the face
-// buffers; ORDER 5 (WENO5) or 2 (PLM)
-template <int ORDER, int DIR, class R = double>
-__global__ void __launch_bounds__(256) k_sph_faces(KSph s) {
-  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  const int p = blockIdx.y;
+// One face state of panel p at level ii: x2 (DIR = 1) face f between cells
+// j = js + f - 1 and js + f of row k = ks + t, or x3 (DIR = 2) face f between
+// cells k = ks + f - 1 and ks + f of column j = js + t; RIGHT false: the left
+// state (upper face of the cell below, patches/19.calculate_fluxes.patch
+// wl_), true: the right state (lower face of the cell above, wr_).  ORDER 5
+// (WENO5) or 2 (PLM), in the form's arithmetic type R.
+template <int ORDER, int DIR, bool RIGHT, class R>
+__device__ __forceinline__ void sph_face_state(const KSph& s, int p, int f, int t, int ii,
+                                               R* out) {
   const KGrid& g = s.g;
-  const int n = s.n;
-  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= (long long)s.nr * n * (n + 1)) return;
-  int t, f, ii;
-  if (DIR == 1) {
-    f = (int)(item % (n + 1));
-    const long long r = item / (n + 1);
-    t = (int)(r % n);
-    ii = (int)(r / n);
-  } else {
-    t = (int)(item % n);
-    const long long r = item / n;
-    f = (int)(r % (n + 1));
-    ii = (int)(r / (n + 1));
-  }
   const int i = g.is + ii;
   const int j = (DIR == 1) ? g.js + f : g.js + t;
   const int k = (DIR == 1) ? g.ks + t : g.ks + f;
   const long long st = (DIR == 1) ? g.c.j : g.c.k;
   const long long base = g.at(0, k, j, i);
   const double* w = s.w[p];
-  R* lr = reinterpret_cast<R*>(s.lr);
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    R q[6];
-#pragma unroll
-    for (int m = LO; m <= HI; ++m) q[m] = (R)w[base + v * g.c.n + (m - 3) * st];
-    R qm, qp, wl, wr;
+    const double* c = w + base + v * g.c.n;
+    R qm, qp;
     if (ORDER == 5) {
-      weno5_cell<R>(q[0], q[1], q[2], q[3], q[4], &qm, &qp);
-      wl = qp;
-      weno5_cell<R>(q[1], q[2], q[3], q[4], q[5], &qm, &qp);
-      wr = qm;
+      if (!RIGHT) {
+        weno5_cell<R>((R)c[-3 * st], (R)c[-2 * st], (R)c[-st], (R)c[0], (R)c[st], &qm, &qp);
+        out[v] = qp;
+      } else {
+        weno5_cell<R>((R)c[-2 * st], (R)c[-st], (R)c[0], (R)c[st], (R)c[2 * st], &qm, &qp);
+        out[v] = qm;
+      }
     } else {
-      plm_cell<R>(q[1], q[2], q[3], &qm, &qp);
-      wl = qp;
-      plm_cell<R>(q[2], q[3], q[4], &qm, &qp);
-      wr = qm;
+      if (!RIGHT) {
+        plm_cell<R>((R)c[-2 * st], (R)c[-st], (R)c[0], &qm, &qp);
+        out[v] = qp;
+      } else {
+        plm_cell<R>((R)c[-st], (R)c[0], (R)c[st], &qm, &qp);
+        out[v] = qm;
+      }
     }
-    const int bl = (DIR == 1) ? 0 : 2;
-    lr[slr(s, p, bl, v, ii, t, f)] = wl;
-    lr[slr(s, p, bl + 1, v, ii, t, f)] = wr;
   }
 }
 
-// Panel-edge exchange: a panel's outer face state at each edge face becomes
-// the neighbour's inner state there, its contravariant velocity carried into
-// this panel's frame through (U, V).  Outer and inner slots are distinct, so the items are
-// independent.
-template <class R = double>
+// edge buffer entry of panel-edge item e (= (p 4 + side) n + t), variable v,
+// level ii: the neighbour's inner state at that face in this panel's frame
+__device__ __forceinline__ long long sph_eb(const KSph& s, long long e, int v, int ii) {
+  return (e * 5 + v) * s.nr + ii;
+}
+
+// Panel-edge exchange (SaveLR3DValues / SynchronizeFluxes*, cubed_sphere.cpp:
+// 250-565): for every panel-edge face, the neighbour q's inner state there
+// (its own reconstruction next to the shared edge), its contravariant
+// velocity carried into this panel's frame through (U, V) at the face, into
+// the edge buffer.  Only the edge faces are reconstructed here; every other
+// face state is formed where its flux is.
+template <int ORDER, class R = double>
 __global__ void __launch_bounds__(256)
-    k_sph_exchange(KSph s, const SphEdge* __restrict__ edges, int nedge) {
+    k_sph_edge_states(KSph s, const SphEdge* __restrict__ edges, int nedge) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= (long long)nedge * s.nr) return;
-  const SphEdge e = edges[item / s.nr];
+  const long long ei = item / s.nr;
+  const SphEdge e = edges[ei];
   const int ii = (int)(item % s.nr);
   const int n = s.n;
-  // inner state of q at its side sq: the state from its cell next to the edge
-  const int bq = (e.sq < 2) ? ((e.sq == 0) ? 1 : 0) : ((e.sq == 2) ? 3 : 2);
-  const int fq = (e.sq % 2 == 0) ? 0 : n;
-  const int bp = (e.s < 2) ? ((e.s == 0) ? 0 : 1) : ((e.s == 2) ? 2 : 3);
-  const int fp = (e.s % 2 == 0) ? 0 : n;
-  R* lr = reinterpret_cast<R*>(s.lr);
+  // the inner state of q at its side sq: the right state of its first face
+  // on a lower side, the left state of its last face on an upper side
   R q[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) q[v] = lr[slr(s, e.q, bq, v, ii, e.tq, fq)];
+  switch (e.sq) {
+    case 0: sph_face_state<ORDER, 1, true, R>(s, e.q, 0, e.tq, ii, q); break;
+    case 1: sph_face_state<ORDER, 1, false, R>(s, e.q, n, e.tq, ii, q); break;
+    case 2: sph_face_state<ORDER, 2, true, R>(s, e.q, 0, e.tq, ii, q); break;
+    default: sph_face_state<ORDER, 2, false, R>(s, e.q, n, e.tq, ii, q); break;
+  }
   const R U = (R)e.r00 * q[2] + (R)e.r01 * q[3];
   const R V = (R)e.r10 * q[2] + (R)e.r11 * q[3];
   q[2] = (R)e.a00 * U + (R)e.a01 * V;
   q[3] = (R)e.a10 * U + (R)e.a11 * V;
+  R* eb = reinterpret_cast<R*>(s.lr);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) lr[slr(s, e.p, bp, v, ii, e.t, fp)] = q[v];
+  for (int v = 0; v < 5; ++v) eb[sph_eb(s, ei, v, ii)] = q[v];
 }
 
-// HLLC at x2 (DIR = 1) / x3 (DIR = 2) faces in the face's orthonormal frame
-template <int DIR, class R = double>
+// x2 (DIR = 1) / x3 (DIR = 2) faces of every panel: the two face states
+// (reconstructed here, or at a panel edge the outer one from the edge
+// buffer), HLLC in the face's orthonormal frame (PrimToLocal2/3, FluxToGlobal2/3
+// of gnomonic_equiangle.cpp:559-828) and the covariant projection of the
+// tangential momentum fluxes (hllc_transform.cpp:235-272)
+template <int ORDER, int DIR, class R = double>
 __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   const int p = blockIdx.y;
   const KGrid& g = s.g;
@@ -255,13 +248,21 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   const int i = g.is + ii;
   const int j = (DIR == 1) ? g.js + f : g.js + t;
   const int k = (DIR == 1) ? g.ks + t : g.ks + f;
-  const int bl = (DIR == 1) ? 0 : 2;
-  const R* lr = reinterpret_cast<const R*>(s.lr);
+  const R* eb = reinterpret_cast<const R*>(s.lr);
   R a[5], b[5];
+  if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
+    const long long e = ((long long)p * 4 + (DIR == 1 ? 0 : 2)) * n + t;
 #pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    a[v] = lr[slr(s, p, bl, v, ii, t, f)];
-    b[v] = lr[slr(s, p, bl + 1, v, ii, t, f)];
+    for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, e, v, ii)];
+  } else {
+    sph_face_state<ORDER, DIR, false, R>(s, p, f, t, ii, a);
+  }
+  if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
+    const long long e = ((long long)p * 4 + (DIR == 1 ? 1 : 3)) * n + t;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, e, v, ii)];
+  } else {
+    sph_face_state<ORDER, DIR, true, R>(s, p, f, t, ii, b);
   }
   // the face metric, the frame transforms and their products in the form's
   // arithmetic type (the metric itself is FP64 on the host)
