@@ -221,49 +221,18 @@ __global__ void __launch_bounds__(256)
   for (int v = 0; v < 5; ++v) eb[sph_eb(s, ei, v, ii)] = q[v];
 }
 
-// x2 (DIR = 1) / x3 (DIR = 2) faces of every panel: the two face states
-// (reconstructed here, or at a panel edge the outer one from the edge
-// buffer), HLLC in the face's orthonormal frame (PrimToLocal2/3, FluxToGlobal2/3
-// of gnomonic_equiangle.cpp:559-828) and the covariant projection of the
-// tangential momentum fluxes (hllc_transform.cpp:235-272)
-template <int ORDER, int DIR, class R = double>
-__global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
-  const int p = blockIdx.y;
+// The x2 (DIR = 1) / x3 (DIR = 2) face f of panel p (row / column t, level
+// ii) from its left and right states: HLLC in the face's orthonormal frame
+// (PrimToLocal2/3, FluxToGlobal2/3 of gnomonic_equiangle.cpp:559-828) and the
+// covariant projection of the tangential momentum fluxes
+// (hllc_transform.cpp:235-272), stored in f2 / f3.
+template <int DIR, class R>
+__device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& th, int p,
+                                                 int f, int t, int ii, R* a, R* b) {
   const KGrid& g = s.g;
-  const int n = s.n;
-  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= (long long)s.nr * n * (n + 1)) return;
-  int t, f, ii;
-  if (DIR == 1) {
-    f = (int)(item % (n + 1));
-    const long long r = item / (n + 1);
-    t = (int)(r % n);
-    ii = (int)(r / n);
-  } else {
-    t = (int)(item % n);
-    const long long r = item / n;
-    f = (int)(r % (n + 1));
-    ii = (int)(r / (n + 1));
-  }
   const int i = g.is + ii;
   const int j = (DIR == 1) ? g.js + f : g.js + t;
   const int k = (DIR == 1) ? g.ks + t : g.ks + f;
-  const R* eb = reinterpret_cast<const R*>(s.lr);
-  R a[5], b[5];
-  if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
-    const long long e = ((long long)p * 4 + (DIR == 1 ? 0 : 2)) * n + t;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, e, v, ii)];
-  } else {
-    sph_face_state<ORDER, DIR, false, R>(s, p, f, t, ii, a);
-  }
-  if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
-    const long long e = ((long long)p * 4 + (DIR == 1 ? 1 : 3)) * n + t;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, e, v, ii)];
-  } else {
-    sph_face_state<ORDER, DIR, true, R>(s, p, f, t, ii, b);
-  }
   // the face metric, the frame transforms and their products in the form's
   // arithmetic type (the metric itself is FP64 on the host)
   const R cf = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_COSF2 : cg_host::SKJ_COSF3, k, j);
@@ -311,6 +280,46 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   double* F = (DIR == 1) ? s.f2[p] : s.f3[p];
 #pragma unroll
   for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = (double)fx[v];
+}
+
+// x2 (DIR = 1) / x3 (DIR = 2) faces of every panel, one lane per face: the
+// two face states reconstructed here (at a panel edge the outer one from the
+// edge buffer), then sph_face_riemann
+template <int ORDER, int DIR, class R = double>
+__global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
+  const int p = blockIdx.y;
+  const int n = s.n;
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= (long long)s.nr * n * (n + 1)) return;
+  int t, f, ii;
+  if (DIR == 1) {
+    f = (int)(item % (n + 1));
+    const long long r = item / (n + 1);
+    t = (int)(r % n);
+    ii = (int)(r / n);
+  } else {
+    t = (int)(item % n);
+    const long long r = item / n;
+    f = (int)(r % (n + 1));
+    ii = (int)(r / (n + 1));
+  }
+  const R* eb = reinterpret_cast<const R*>(s.lr);
+  R a[5], b[5];
+  if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
+    const long long e = ((long long)p * 4 + (DIR == 1 ? 0 : 2)) * n + t;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, e, v, ii)];
+  } else {
+    sph_face_state<ORDER, DIR, false, R>(s, p, f, t, ii, a);
+  }
+  if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
+    const long long e = ((long long)p * 4 + (DIR == 1 ? 1 : 3)) * n + t;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, e, v, ii)];
+  } else {
+    sph_face_state<ORDER, DIR, true, R>(s, p, f, t, ii, b);
+  }
+  sph_face_riemann<DIR, R>(s, th, p, f, t, ii, a, b);
 }
 
 // areas and volume of a panel cell (gnomonic_equiangle.cpp:221-260, 407-416)
