@@ -770,8 +770,9 @@ def bench_kessler(ctx, blk, th, cfg, w0, dt, steps, warmup, tune):
 def bench_sphere(ctx, steps, warmup, dt=60.0, nx=(64, 96, 96), acc_steps=20):
     """Config 3: the cubed sphere of six 96 x 96 panels of 64 levels with the
     Held-Suarez forcing and rotation, 3 x 3 implicit x1 columns (hs94.inp),
-    fixed step dt, in the plain FP64 form and the FP32-flux form (x2 / x3
-    face states and Riemann fluxes in FP32).  ms per step (graph replay),
+    fixed step dt, in the plain FP64 form and the FP32-flux form (face
+    states and Riemann fluxes of all three directions in FP32, the x1
+    decomposition and face pressures in FP64).  ms per step (graph replay),
     per-kernel event averages, the state after the timed steps, and the FP32
     form's error against FP64 after acc_steps steps."""
     blk, th, cfg = ca.pgen_setup(3, *nx)

@@ -324,18 +324,20 @@ __device__ __forceinline__ double decomp32(const KGrid& g, const KThermo& th,
   return r;
 }
 
-// x1 faces in float: decomposition in FP64 (T, p - psv), reconstruction and
-// LMARS/HLLC in float about pb = psf(i).
+// x1 face t (of the nx1 + 1 faces of each interior column) in float:
+// decomposition in FP64 (T, p - psv), reconstruction and LMARS/HLLC in
+// float about pb = psf(i); the normal momentum flux excludes pb (the caller
+// adds the face pressures in FP64).  Sets the face's (k, j, i).
 template <int NM, int ORDER>
-__global__ void __launch_bounds__(256)
-    k_flux32_x1(KGrid g, KThermo th, int rsolver, int decomp,
-                const double* __restrict__ w, const double* __restrict__ psf,
-                const double* __restrict__ psv, float* __restrict__ f1) {
+__device__ __forceinline__ void flux32_x1_face(const KGrid& g, const KThermo& th,
+                                               int rsolver, int decomp,
+                                               const double* __restrict__ w,
+                                               const double* __restrict__ psf,
+                                               const double* __restrict__ psv,
+                                               long long t, int& i, int& j, int& k,
+                                               float* fx) {
   constexpr int N = 5 + NM, IPR = 4 + NM;
   constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
-  int i, j, k;
   g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
   i += g.is;
   j += g.js;
@@ -367,9 +369,21 @@ __global__ void __launch_bounds__(256)
     wr[0] = fdiv(pb + wr[IPR], wr[0] * gam);
     wl[0] = fdiv(pb + wl[IPR], wl[0] * gam);
   }
-  float fx[N];
   face_check<NM, float>(g.status, wl, wr, pb);
   riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
+}
+
+template <int NM, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_flux32_x1(KGrid g, KThermo th, int rsolver, int decomp,
+                const double* __restrict__ w, const double* __restrict__ psf,
+                const double* __restrict__ psv, float* __restrict__ f1) {
+  constexpr int N = 5 + NM;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  int i, j, k;
+  float fx[N];
+  flux32_x1_face<NM, ORDER>(g, th, rsolver, decomp, w, psf, psv, t, i, j, k, fx);
 #pragma unroll
   for (int n = 0; n < N; ++n) f1[g.f[0].at(n, k, j, i)] = fx[n];
 }

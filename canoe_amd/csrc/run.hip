@@ -315,10 +315,16 @@ int sphere_stage(cg_run* r, int s) {
   const long long ncell = (long long)g.nx1 * g.nx2 * g.nx3;
   const long long nf1 = (long long)(g.nx1 + 1) * g.nx2 * g.nx3;
   const long long nface = (long long)ks.nr * ks.n * (ks.n + 1);
-  TIMED(tm, KID_FLUX_X1, (k_sph_flux_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
-  // x2 / x3 faces: FP64, or FP32 reconstruction and Riemann fluxes (the
-  // FP32-flux form; x1 fluxes, divergence, sources and the column solve stay
-  // FP64)
+  // faces: FP64, or FP32 reconstruction and Riemann fluxes (the FP32-flux
+  // form; the x1 decomposition, the face pressures of the x1 momentum flux,
+  // the divergence, sources and the column solve stay FP64)
+  const bool f32 = r->cfg.precision == CG_PREC_FP32;
+  const int psplit = (f32 && g.grav != 0.0) ? 1 : 0;
+  if (f32)
+    TIMED(tm, KID_FLUX_X1,
+          (k_sph_flux32_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  else
+    TIMED(tm, KID_FLUX_X1, (k_sph_flux_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
   const dim3 gf(nblk(nface, 256), 6);
   const unsigned ge = nblk((long long)S.nedge * ks.nr, 256);
 #define CG_SPH_FACES(O, R)                                                                  \
@@ -328,7 +334,7 @@ int sphere_stage(cg_run* r, int s) {
     TIMED(tm, KID_FLUX_X2, (k_sph_face_flux<O, 1, R><<<gf, 256, 0, st>>>(ks, r->kt)));    \
     TIMED(tm, KID_FLUX_X3, (k_sph_face_flux<O, 2, R><<<gf, 256, 0, st>>>(ks, r->kt)));    \
   } while (0)
-  if (r->cfg.precision == CG_PREC_FP32) {
+  if (f32) {
     if (r->cfg.order_h == 5)
       CG_SPH_FACES(5, float);
     else
@@ -340,8 +346,12 @@ int sphere_stage(cg_run* r, int s) {
       CG_SPH_FACES(2, double);
   }
 #undef CG_SPH_FACES
-  TIMED(tm, KID_DIVERGENCE,
-        (k_sph_divergence<<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(ks, sbeta[s], r->d_dt)));
+  if (f32)
+    TIMED(tm, KID_DIVERGENCE, (k_sph_divergence<float><<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(
+                                  ks, sbeta[s], r->d_dt, psplit)));
+  else
+    TIMED(tm, KID_DIVERGENCE, (k_sph_divergence<double><<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(
+                                  ks, sbeta[s], r->d_dt, psplit)));
   KUpdate up;
   up.stage = s;
   up.g1 = sg1[s];

@@ -22,6 +22,7 @@
 // the two agree to rounding.
 #pragma once
 #include "cg_physics.hpp"
+#include "flux32.hpp"
 #include "kernels.hpp"
 #include "sphere_geom.hpp"
 
@@ -120,6 +121,28 @@ __global__ void __launch_bounds__(256) k_sph_flux_x1(KSph s, KThermo th) {
   if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
   flux_x1_face<0, 5>(g, th, CG_RS_HLLC, g.grav != 0.0 ? 1 : 0, s.w[p], s.psf[p], s.psv[p],
                      s.f1[p], t, s.kj[p] + (long long)cg_host::SKJ_HC * s.nc * s.nc, s.nc);
+}
+
+// The same x1 faces in float (the FP32-flux form): decomposition in FP64,
+// WENO5 and HLLC in float about the face's hydrostatic pressure psf, which
+// the normal momentum flux leaves out (k_sph_divergence adds the face
+// pressures times the face areas in FP64), then the covariant projection.
+__global__ void __launch_bounds__(256) k_sph_flux32_x1(KSph s, KThermo th) {
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
+  int i, j, k;
+  float fx[5];
+  flux32_x1_face<0, 5>(g, th, CG_RS_HLLC, g.grav != 0.0 ? 1 : 0, s.w[p], s.psf[p], s.psv[p],
+                       t, i, j, k, fx);
+  const float hc = (float)skj(s, p, cg_host::SKJ_HC, k, j);
+  const float ty = fx[2], tz = fx[3];
+  fx[2] = ty + tz * hc;
+  fx[3] = tz + ty * hc;
+  float* F1 = reinterpret_cast<float*>(s.f1[p]);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) F1[g.f[0].at(v, k, j, i)] = fx[v];
 }
 
 // hydrostatic face and cell pressures of the six panels' columns
@@ -277,9 +300,10 @@ __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& t
   const R ty = fx[2], tz = fx[3];
   fx[2] = ty + tz * hc;
   fx[3] = tz + ty * hc;
-  double* F = (DIR == 1) ? s.f2[p] : s.f3[p];
+  // stored in the form's type (the FP32 form's fluxes take half the bytes)
+  R* F = reinterpret_cast<R*>((DIR == 1) ? s.f2[p] : s.f3[p]);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = (double)fx[v];
+  for (int v = 0; v < 5; ++v) F[g.f[DIR].at(v, k, j, i)] = fx[v];
 }
 
 // x2 (DIR = 1) / x3 (DIR = 2) faces of every panel, one lane per face: the
@@ -338,17 +362,21 @@ __device__ __forceinline__ double sph_v(const KSph& s, int p, int k, int j, int 
          s.x1v[i] * s.x1v[i] * skj(s, p, cg_host::SKJ_SINC, k, j);
 }
 
-// du = -wght div F (areas, volume) + geometric terms + gravity + forcing
+// du = -wght div F (areas, volume) + geometric terms + gravity + forcing;
+// FT: the type the face fluxes are stored in (double, or float in the
+// FP32-flux form); psplit: the x1 fluxes leave the face pressures psf out of
+// the normal momentum flux (k_sph_flux32_x1), added here in FP64
+template <class FT>
 __global__ void __launch_bounds__(256)
-    k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp) {
+    k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp, int psplit) {
   const int p = blockIdx.y;
   int i, j, k;
   if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
   const KGrid& g = s.g;
   const double wght = beta * dtp[0];
-  const double* F1 = s.f1[p];
-  const double* F2 = s.f2[p];
-  const double* F3 = s.f3[p];
+  const FT* F1 = reinterpret_cast<const FT*>(s.f1[p]);
+  const FT* F2 = reinterpret_cast<const FT*>(s.f2[p]);
+  const FT* F3 = reinterpret_cast<const FT*>(s.f3[p]);
   const double a1l = sph_a1(s, p, k, j, i), a1r = sph_a1(s, p, k, j, i + 1);
   const double a2l = sph_a2(s, p, k, j, i), a2r = sph_a2(s, p, k, j + 1, i);
   const double a3l = sph_a3(s, p, k, j, i), a3r = sph_a3(s, p, k + 1, j, i);
@@ -356,9 +384,14 @@ __global__ void __launch_bounds__(256)
   double d[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    double dflx = a1r * F1[g.f[0].at(v, k, j, i + 1)] - a1l * F1[g.f[0].at(v, k, j, i)];
-    dflx += a2r * F2[g.f[1].at(v, k, j + 1, i)] - a2l * F2[g.f[1].at(v, k, j, i)];
-    dflx += a3r * F3[g.f[2].at(v, k + 1, j, i)] - a3l * F3[g.f[2].at(v, k, j, i)];
+    double dflx = a1r * (double)F1[g.f[0].at(v, k, j, i + 1)] -
+                  a1l * (double)F1[g.f[0].at(v, k, j, i)];
+    if (v == 1 && psplit) {
+      const double* pf = s.psf[p] + g.pcol(k, j) * g.ps_c;
+      dflx += a1r * pf[(i + 1) * g.ps_i] - a1l * pf[i * g.ps_i];
+    }
+    dflx += a2r * (double)F2[g.f[1].at(v, k, j + 1, i)] - a2l * (double)F2[g.f[1].at(v, k, j, i)];
+    dflx += a3r * (double)F3[g.f[2].at(v, k + 1, j, i)] - a3l * (double)F3[g.f[2].at(v, k, j, i)];
     d[v] = 0.0 - wght * dflx / vol;
   }
   const long long c0 = g.at(0, k, j, i);

@@ -491,8 +491,10 @@ int cg_run_autotune(cg_run* run, double dt, cg_tune* out);
  * velocities in w (patches/26.adiabatic_hydro.patch), the x1 column solve
  * with the panel areas (implicit_flag 0, 1 or 9), gravity and the Coriolis /
  * Held-Suarez forcing of cg_sphere.  Dry gas; cfg->precision CG_PREC_FP32
- * computes the x2 / x3 face states and Riemann fluxes in FP32 (x1 fluxes,
- * divergence, column solve and state in FP64).  The cg_run_* calls
+ * computes the face states and Riemann fluxes of all three directions in
+ * FP32 and keeps the fluxes in FP32 (the x1 decomposition, the face
+ * pressures of the x1 momentum flux, divergence, column solve and state in
+ * FP64).  The cg_run_* calls
  * taking a block index address panel b (0-5: panels 1-6); the time-step,
  * check, history, debug, sums and autotune calls return CG_ERR_ARG. */
 int cg_sphere_create(cg_ctx* ctx, const cg_block* panel, const cg_thermo* th,
