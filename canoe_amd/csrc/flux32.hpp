@@ -324,38 +324,35 @@ __device__ __forceinline__ double decomp32(const KGrid& g, const KThermo& th,
   return r;
 }
 
-// x1 face t (of the nx1 + 1 faces of each interior column) in float:
-// decomposition in FP64 (T, p - psv), reconstruction and LMARS/HLLC in
-// float about pb = psf(i); the normal momentum flux excludes pb (the caller
-// adds the face pressures in FP64).  Sets the face's (k, j, i).
-template <int NM, int ORDER>
-__device__ __forceinline__ void flux32_x1_face(const KGrid& g, const KThermo& th,
-                                               int rsolver, int decomp,
-                                               const double* __restrict__ w,
-                                               const double* __restrict__ psf,
+// Window slot m of an x1 march: cell i of column (k, j) as floats, with T
+// and the perturbation p - psv (an FP64 difference rounded to float) in
+// place of density and pressure when decomp; rdv[m] = its R (decomp32).
+template <int NM>
+__device__ __forceinline__ void x1_window_cell(const KGrid& g, const KThermo& th,
+                                               int decomp, const double* __restrict__ w,
                                                const double* __restrict__ psv,
-                                               long long t, int& i, int& j, int& k,
-                                               float* fx) {
+                                               long long pc, int k, int j, int i,
+                                               float (*q)[6], double* rdv, int m) {
   constexpr int N = 5 + NM, IPR = 4 + NM;
-  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
-  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
-  i += g.is;
-  j += g.js;
-  k += g.ks;
-  const long long pc = g.pcol(k, j) * g.ps_c;
-  float q[N][6];
-  double rdv[6];
+  double wc[N];
 #pragma unroll
-  for (int m = LO; m <= HI; ++m) {
-    double wc[N];
+  for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i)];
 #pragma unroll
-    for (int n = 0; n < N; ++n) wc[n] = w[g.at(n, k, j, i - 3 + m)];
-#pragma unroll
-    for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
-    if (decomp)  // T in float; the perturbation p - psv in FP64 before rounding
-      rdv[m] = decomp32<NM>(g, th, w, g.at(0, k, j, 0), psv, pc, i - 3 + m, wc,
-                            q[0][m], q[IPR][m]);
-  }
+  for (int n = 0; n < N; ++n) q[n][m] = (float)wc[n];
+  if (decomp)  // T in float; the perturbation p - psv in FP64 before rounding
+    rdv[m] = decomp32<NM>(g, th, w, g.at(0, k, j, 0), psv, pc, i, wc, q[0][m], q[IPR][m]);
+}
+
+// x1 face i from the window q / rdv of cells i-3 .. i+2 (x1_window_cell):
+// reconstruction and LMARS/HLLC in float about pb = psf(i), pb left out of
+// the normal momentum flux.
+template <int NM, int ORDER>
+__device__ __forceinline__ void x1_face_from_window(const KGrid& g, const KThermo& th,
+                                                    int rsolver, int decomp,
+                                                    const double* __restrict__ psf,
+                                                    long long pc, int i, float (*q)[6],
+                                                    const double* rdv, float* fx) {
+  constexpr int N = 5 + NM, IPR = 4 + NM;
   float wl[N], wr[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) recon_pair_f<ORDER>(q[n], &wl[n], &wr[n]);
@@ -371,6 +368,33 @@ __device__ __forceinline__ void flux32_x1_face(const KGrid& g, const KThermo& th
   }
   face_check<NM, float>(g.status, wl, wr, pb);
   riemann_p<NM, 0>(th, rsolver, wl, wr, pb, fx);
+}
+
+// x1 face t (of the nx1 + 1 faces of each interior column) in float:
+// decomposition in FP64 (T, p - psv), reconstruction and LMARS/HLLC in
+// float about pb = psf(i); the normal momentum flux excludes pb (the caller
+// adds the face pressures in FP64).  Sets the face's (k, j, i).
+template <int NM, int ORDER>
+__device__ __forceinline__ void flux32_x1_face(const KGrid& g, const KThermo& th,
+                                               int rsolver, int decomp,
+                                               const double* __restrict__ w,
+                                               const double* __restrict__ psf,
+                                               const double* __restrict__ psv,
+                                               long long t, int& i, int& j, int& k,
+                                               float* fx) {
+  constexpr int N = 5 + NM;
+  constexpr int LO = (ORDER == 5) ? 0 : 1, HI = (ORDER == 5) ? 5 : 4;
+  g.decode(t, g.nx1 + 1, g.nx2, g.nx3, i, j, k);
+  i += g.is;
+  j += g.js;
+  k += g.ks;
+  const long long pc = g.pcol(k, j) * g.ps_c;
+  float q[N][6];
+  double rdv[6];
+#pragma unroll
+  for (int m = LO; m <= HI; ++m)
+    x1_window_cell<NM>(g, th, decomp, w, psv, pc, k, j, i - 3 + m, q, rdv, m);
+  x1_face_from_window<NM, ORDER>(g, th, rsolver, decomp, psf, pc, i, q, rdv, fx);
 }
 
 template <int NM, int ORDER>

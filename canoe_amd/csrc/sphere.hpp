@@ -184,6 +184,18 @@ __device__ __forceinline__ void sph_face_state(const KSph& s, int p, int f, int 
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double* c = w + base + v * g.c.n;
+    if constexpr (std::is_same<R, float>::value) {
+      // the FP32 form: only the needed side, with the fast reciprocals of
+      // the box's FP32 faces
+      const float a = (float)c[-2 * st], b = (float)c[-st], m = (float)c[0],
+                  d = (float)c[st];
+      if (ORDER == 5)
+        out[v] = RIGHT ? weno5_up_f((float)c[2 * st], d, m, b, a)
+                       : weno5_up_f((float)c[-3 * st], a, b, m, d);
+      else
+        out[v] = RIGHT ? plm_up_f(d, m, b) : plm_up_f(a, b, m);
+      continue;
+    }
     R qm, qp;
     if (ORDER == 5) {
       if (!RIGHT) {
