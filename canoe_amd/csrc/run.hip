@@ -55,6 +55,7 @@ struct SphRun {
   SphEdge* d_edges = nullptr;
   int nedge = 0;
   bool dirty = true;             // ghosts and hydrostatic pressures to refresh
+  int x1seg = 8;                 // faces per lane of the FP32 x1 march
 };
 
 struct cg_run {
@@ -320,9 +321,11 @@ int sphere_stage(cg_run* r, int s) {
   // the divergence, sources and the column solve stay FP64)
   const bool f32 = r->cfg.precision == CG_PREC_FP32;
   const int psplit = (f32 && g.grav != 0.0) ? 1 : 0;
-  if (f32)
-    TIMED(tm, KID_FLUX_X1,
-          (k_sph_flux32_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
+  if (f32) {
+    const long long nx1l = (long long)g.nx2 * g.nx3 * ((g.nx1 + 1 + S.x1seg - 1) / S.x1seg);
+    TIMED(tm, KID_FLUX_X1, (k_sph_flux32_x1m<<<dim3(nblk(nx1l, 256), 6), 256, 0, st>>>(
+                               ks, r->kt, S.x1seg)));
+  }
   else
     TIMED(tm, KID_FLUX_X1, (k_sph_flux_x1<<<dim3(nblk(nf1, 256), 6), 256, 0, st>>>(ks, r->kt)));
   const dim3 gf(nblk(nface, 256), 6);

@@ -126,23 +126,52 @@ __global__ void __launch_bounds__(256) k_sph_flux_x1(KSph s, KThermo th) {
 // The same x1 faces in float (the FP32-flux form): decomposition in FP64,
 // WENO5 and HLLC in float about the face's hydrostatic pressure psf, which
 // the normal momentum flux leaves out (k_sph_divergence adds the face
-// pressures times the face areas in FP64), then the covariant projection.
-__global__ void __launch_bounds__(256) k_sph_flux32_x1(KSph s, KThermo th) {
+// pressures times the face areas in FP64), then the covariant projection;
+// stored as float.  Marched up column segments of `seg` faces (one lane per
+// column segment, lanes along j): each cell's float window entry and
+// decomposition are formed once per segment instead of once for every face
+// whose stencil holds it (the faces equal the per-face flux32_x1_face's).
+__global__ void __launch_bounds__(256) k_sph_flux32_x1m(KSph s, KThermo th, int seg) {
   const int p = blockIdx.y;
   const KGrid& g = s.g;
+  const long long ncol = (long long)g.nx2 * g.nx3;
+  const int nseg = (g.nx1 + 1 + seg - 1) / seg;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)(g.nx1 + 1) * g.nx2 * g.nx3) return;
-  int i, j, k;
-  float fx[5];
-  flux32_x1_face<0, 5>(g, th, CG_RS_HLLC, g.grav != 0.0 ? 1 : 0, s.w[p], s.psf[p], s.psv[p],
-                       t, i, j, k, fx);
+  if (t >= ncol * nseg) return;
+  const long long col = t % ncol;
+  const int sg = (int)(t / ncol);
+  const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
+  const int f0 = g.is + sg * seg;
+  const int f1 = min(f0 + seg, g.ie + 2);  // faces f0 .. f1-1
+  const int decomp = g.grav != 0.0 ? 1 : 0;
+  const double* w = s.w[p];
+  const double* psv = s.psv[p];
+  const long long pc = g.pcol(k, j) * g.ps_c;
   const float hc = (float)skj(s, p, cg_host::SKJ_HC, k, j);
-  const float ty = fx[2], tz = fx[3];
-  fx[2] = ty + tz * hc;
-  fx[3] = tz + ty * hc;
   float* F1 = reinterpret_cast<float*>(s.f1[p]);
+  // window: cells f-3 .. f+2 of face f
+  float q[5][6];
+  double rdv[6];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) F1[g.f[0].at(v, k, j, i)] = fx[v];
+  for (int m = 0; m < 6; ++m) x1_window_cell<0>(g, th, decomp, w, psv, pc, k, j, f0 - 3 + m, q, rdv, m);
+  for (int f = f0; f < f1; ++f) {
+    float fx[5];
+    x1_face_from_window<0, 5>(g, th, CG_RS_HLLC, decomp, s.psf[p], pc, f, q, rdv, fx);
+    const float ty = fx[2], tz = fx[3];
+    fx[2] = ty + tz * hc;
+    fx[3] = tz + ty * hc;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F1[g.f[0].at(v, k, j, f)] = fx[v];
+    if (f + 1 < f1) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v][m] = q[v][m + 1];
+        rdv[m] = rdv[m + 1];
+      }
+      x1_window_cell<0>(g, th, decomp, w, psv, pc, k, j, f + 3, q, rdv, 5);
+    }
+  }
 }
 
 // hydrostatic face and cell pressures of the six panels' columns
@@ -377,7 +406,7 @@ __device__ __forceinline__ double sph_v(const KSph& s, int p, int k, int j, int 
 // du = -wght div F (areas, volume) + geometric terms + gravity + forcing;
 // FT: the type the face fluxes are stored in (double, or float in the
 // FP32-flux form); psplit: the x1 fluxes leave the face pressures psf out of
-// the normal momentum flux (k_sph_flux32_x1), added here in FP64
+// the normal momentum flux (k_sph_flux32_x1m), added here in FP64
 template <class FT>
 __global__ void __launch_bounds__(256)
     k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp, int psplit) {
