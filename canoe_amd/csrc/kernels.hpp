@@ -749,8 +749,9 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
   Mt<B> ap;  // a'_{i-1}
   Vt<B> dl;  // delta_{i-1}
   bool ok = true;
-  // the next level's w and this level's du are loaded one level ahead, so
-  // that their latency overlaps the block algebra of the current level
+  // the next level's w and this level's du (and, on a panel, the next
+  // level's radii) are loaded one level ahead, so that their latency
+  // overlaps the block algebra of the current level
   double wn[N], dun[N];
   auto load_du = [&](double* dst, int i) {
 #pragma unroll
@@ -758,6 +759,24 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
   };
   load(wn, is + 1);
   load_du(dun, is);
+  double xf0 = 0.0, xf1 = 0.0, xv = 0.0, nxf = 0.0, nxv = 0.0;
+  if (GEO) {
+    xf0 = geo->x1f[is];
+    xf1 = geo->x1f[is + 1];
+    xv = geo->x1v[is];
+  }
+  // archive of level lvl: a'_lvl and delta_lvl
+  auto archive = [&](int lvl) {
+    const long long lb = (long long)(lvl - is) * AS;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int q = 0; q < B; ++q)
+        __builtin_nontemporal_store(ap.a[r][q], &arch[(lb + r * B + q) * as + ao]);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+      __builtin_nontemporal_store(dl.v[r], &arch[(lb + B * B + r) * as + ao]);
+  };
   for (int i = is; i <= ie; ++i) {
     double dui[N];
 #pragma unroll
@@ -765,9 +784,18 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
       wp[n] = wn[n];
       dui[n] = dun[n];
     }
-    if (i < ie) {
-      load(wn, i + 2);
-      load_du(dun, i + 1);
+    // The previous level's archive and the next level's loads are issued
+    // together, right after this level's data have arrived: a wait for
+    // loaded data also waits for every store issued before it, so each
+    // gets a whole level of block algebra to complete.  The loads run at
+    // every level (the last reloads levels ie + 1 and ie, unused) so that no
+    // branch joins before their values are read.
+    if (i > is) archive(i - 1);
+    load(wn, min(i + 2, ie + 1));
+    load_du(dun, min(i + 1, ie));
+    if (GEO) {
+      nxf = geo->x1f[min(i + 2, ie + 1)];
+      nxv = geo->x1v[min(i + 1, ie)];
     }
     if (i == ie && rw && g.bc[1] == 2)
       outflow_ghost_w<NM>(th, wc, psvr[pc + ie * g.ps_i],
@@ -777,9 +805,9 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
     flux_jac<NM, B>(gm1, wp, dp);
     double sl = sl0, sr = sr0, sd = sd0;
     if (GEO) {
-      const double al = geo->d2a * geo->d3a * geo->x1f[i] * geo->x1f[i] * geo->sinc;
-      const double ar = geo->d2a * geo->d3a * geo->x1f[i + 1] * geo->x1f[i + 1] * geo->sinc;
-      const double v = geo->dx1 * geo->d2a * geo->d3a * geo->x1v[i] * geo->x1v[i] * geo->sinc;
+      const double al = geo->d2a * geo->d3a * xf0 * xf0 * geo->sinc;
+      const double ar = geo->d2a * geo->d3a * xf1 * xf1 * geo->sinc;
+      const double v = geo->dx1 * geo->d2a * geo->d3a * xv * xv * geo->sinc;
       const double s2 = 1. / (2. * v);
       sl = al * s2;
       sr = ar * s2;
@@ -910,15 +938,6 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
     ap = c;
 #pragma unroll
     for (int r = 0; r < B; ++r) dl.v[r] = t[r];
-    const long long lb = (long long)(i - is) * AS;
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-#pragma unroll
-      for (int q = 0; q < B; ++q)
-        __builtin_nontemporal_store(ap.a[r][q], &arch[(lb + r * B + q) * as + ao]);
-#pragma unroll
-    for (int r = 0; r < B; ++r)
-      __builtin_nontemporal_store(dl.v[r], &arch[(lb + B * B + r) * as + ao]);
     Am = Ap;
     dm = dc;
     dc = dp;
@@ -927,7 +946,13 @@ __device__ __forceinline__ bool fwd_column(const KGrid& g, const KThermo& th,
       wm[n] = wc[n];
       wc[n] = wp[n];
     }
+    if (GEO) {
+      xf0 = xf1;
+      xf1 = nxf;
+      xv = nxv;
+    }
   }
+  archive(ie);
   return ok;
 }
 
