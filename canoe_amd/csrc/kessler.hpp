@@ -49,15 +49,74 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
   const double grav = g.grav, dx = g.dx1;
   double pf_next = 0., pf_top = 0.;
   double in_p = 0., in_m0 = 0., in_m1 = 0., in_m2 = 0., in_e = 0.;
-  for (int i = g.ie; i >= g.is; --i) {
+  // The next level's u is loaded a level ahead, and a level's stores are
+  // issued at the next level, together with that load, once the level's own
+  // data have been used: a wait for loaded data also waits for every store
+  // issued before it, so stores at the end of a level would expose their
+  // write latency at the start of the next.
+  struct Out {
+    int i;
+    double uc[N], wc[N];
+    double pfi, pvi, ptop;
+  };
+  Out pend;
+  auto flush = [&](const Out& o) {
+    const long long c0 = g.at(0, k, j, o.i);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      u[c0 + n * sn] = o.uc[n];
+      w[c0 + n * sn] = o.wc[n];
+    }
+    if (FUSE) {
+      // x1 ghosts (bc_pair: reflecting sides mirror, normal velocity
+      // negated; outflow sides copy the boundary cell)
+      const int mt = ie - o.i, mb = o.i - is;
+      if (mt < NG && g.bc[1] == 0) {
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+          w[cb + n * sn + (ie + 1 + mt) * si] = ((n == IVX) ? -1.0 : 1.0) * o.wc[n];
+      }
+      if (mt == 0 && g.bc[1] == 2) {
+        for (int gh = 1; gh <= NG; ++gh)
+#pragma unroll
+          for (int n = 0; n < N; ++n) w[cb + n * sn + (ie + gh) * si] = o.wc[n];
+      }
+      if (mb < NG && g.bc[0] == 0) {
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+          w[cb + n * sn + (is - 1 - mb) * si] = ((n == IVX) ? -1.0 : 1.0) * o.wc[n];
+      }
+      if (mb == 0 && g.bc[0] == 2) {
+        for (int gh = 1; gh <= NG; ++gh)
+#pragma unroll
+          for (int n = 0; n < N; ++n) w[cb + n * sn + (is - gh) * si] = o.wc[n];
+      }
+      if (grav != 0.0) {
+        if (o.i == ie) psf[pc + (ie + 1) * g.ps_i] = o.ptop;
+        psf[pc + o.i * g.ps_i] = o.pfi;
+        psv[pc + o.i * g.ps_i] = o.pvi;
+      }
+    }
+  };
+  double un[N];
+  auto load_u = [&](int i) {
     const long long c0 = g.at(0, k, j, i);
+#pragma unroll
+    for (int n = 0; n < N; ++n) un[n] = u[c0 + n * sn];
+  };
+  load_u(ie);
+  for (int i = g.ie; i >= g.is; --i) {
     double uc[N], wc[N];
 #pragma unroll
-    for (int n = 0; n < N; ++n) uc[n] = u[c0 + n * sn];
+    for (int n = 0; n < N; ++n) uc[n] = un[n];
     double rho = uc[0];
 #pragma unroll
     for (int n = 1; n <= NM; ++n) rho += uc[n];
     cons_to_prim_cell<NM>(th, uc, wc);
+    // the level above's stores and the level below's load (at the last
+    // level a reload of level is, unused)
+    if (i < ie) flush(pend);
+    load_u(max(i - 1, is));
     const double T = temperature_d<NM>(th, wc);
     // species picked by runtime index from registers: select chains
     double rc = 0., rp = 0., rv = 0.;
@@ -107,48 +166,24 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
     uc[IVX + 2] = (uc[IVX + 2] - out_m2) + in_m2;
     uc[IEN] = (uc[IEN] - out_e) + in_e;
     cons_to_prim_cell<NM>(th, uc, wc);
+    pend.i = i;
 #pragma unroll
     for (int n = 0; n < N; ++n) {
-      u[c0 + n * sn] = uc[n];
-      w[c0 + n * sn] = wc[n];
+      pend.uc[n] = uc[n];
+      pend.wc[n] = wc[n];
     }
-    if (FUSE) {
-      // x1 ghosts (bc_pair: reflecting sides mirror, normal velocity
-      // negated; outflow sides copy the boundary cell)
-      const int mt = ie - i, mb = i - is;
-      if (mt < NG && g.bc[1] == 0) {
-#pragma unroll
-        for (int n = 0; n < N; ++n)
-          w[cb + n * sn + (ie + 1 + mt) * si] = ((n == IVX) ? -1.0 : 1.0) * wc[n];
+    // hydrostatic face and cell pressures (k_hydrostatic_scan)
+    if (FUSE && grav != 0.0) {
+      if (i == ie) {
+        const double RdTv = wc[IEN] / wc[0];
+        pf_next = wc[IEN] * exp_d(grav * dx / (2 * RdTv));
+        pf_top = pf_next;
+        pend.ptop = pf_next;
       }
-      if (mt == 0 && g.bc[1] == 2) {
-        for (int gh = 1; gh <= NG; ++gh)
-#pragma unroll
-          for (int n = 0; n < N; ++n) w[cb + n * sn + (ie + gh) * si] = wc[n];
-      }
-      if (mb < NG && g.bc[0] == 0) {
-#pragma unroll
-        for (int n = 0; n < N; ++n)
-          w[cb + n * sn + (is - 1 - mb) * si] = ((n == IVX) ? -1.0 : 1.0) * wc[n];
-      }
-      if (mb == 0 && g.bc[0] == 2) {
-        for (int gh = 1; gh <= NG; ++gh)
-#pragma unroll
-          for (int n = 0; n < N; ++n) w[cb + n * sn + (is - gh) * si] = wc[n];
-      }
-      // hydrostatic face and cell pressures (k_hydrostatic_scan)
-      if (grav != 0.0) {
-        if (i == ie) {
-          const double RdTv = wc[IEN] / wc[0];
-          pf_next = wc[IEN] * exp_d(grav * dx / (2 * RdTv));
-          pf_top = pf_next;
-          psf[pc + (ie + 1) * g.ps_i] = pf_next;
-        }
-        const double pfi = pf_next - grav * wc[0] * dx;
-        psf[pc + i * g.ps_i] = pfi;
-        psv[pc + i * g.ps_i] = log_mean(pfi, pf_next);
-        pf_next = pfi;
-      }
+      const double pfi = pf_next - grav * wc[0] * dx;
+      pend.pfi = pfi;
+      pend.pvi = log_mean(pfi, pf_next);
+      pf_next = pfi;
     }
     in_p = out_p;
     in_m0 = out_m0;
@@ -156,6 +191,7 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
     in_m2 = out_m2;
     in_e = out_e;
   }
+  flush(pend);
   if (FUSE && grav != 0.0) {
     const double gb = (g.bc[0] == 0) ? -grav : grav;
     const double gt = (g.bc[1] == 0) ? -grav : grav;
