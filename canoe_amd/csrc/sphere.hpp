@@ -41,6 +41,8 @@ struct KSph {
   double* w[6];
   double* u1[6];
   double* du[6];
+  // face fluxes: doubles, or floats in the FP32-flux form (a run keeps one
+  // precision, so each buffer is only ever read as the type it was written)
   double* f1[6];
   double* f2[6];
   double* f3[6];
