@@ -338,10 +338,19 @@ int sphere_stage(cg_run* r, int s) {
     TIMED(tm, KID_FLUX_X3, (k_sph_face_flux<O, 2, R><<<gf, 256, 0, st>>>(ks, r->kt)));    \
   } while (0)
   if (f32) {
-    if (r->cfg.order_h == 5)
-      CG_SPH_FACES(5, float);
-    else
-      CG_SPH_FACES(2, float);
+    // FP32 form: the x2 and x3 faces of 32 x 8 column tiles at one level,
+    // the cells' face values shared through LDS
+    const unsigned nt = (unsigned)(((ks.n + SF_J - 1) / SF_J) * ((ks.n + SF_K - 1) / SF_K) *
+                                   ks.nr);
+    if (r->cfg.order_h == 5) {
+      TIMED(tm, KID_FLUX_X2,
+            (k_sph_edge_states<5, float><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));
+      TIMED(tm, KID_FLUX_X3, (k_sph_faces32<5><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(ks, r->kt)));
+    } else {
+      TIMED(tm, KID_FLUX_X2,
+            (k_sph_edge_states<2, float><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));
+      TIMED(tm, KID_FLUX_X3, (k_sph_faces32<2><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(ks, r->kt)));
+    }
   } else {
     if (r->cfg.order_h == 5)
       CG_SPH_FACES(5, double);

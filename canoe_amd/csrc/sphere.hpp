@@ -305,9 +305,20 @@ __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& t
   const R sf = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_SINF2 : cg_host::SKJ_SINF3, k, j);
   const R hc = (R)skj(s, p, DIR == 1 ? cg_host::SKJ_HF2 : cg_host::SKJ_HF3, k, j);
   const R one = (R)1.0;
-  const R gi = one / (sf * sf);
+  // gi = 1 / sin^2 of the face angle, rg its square root, irg 1 / rg (the
+  // FP32 form with the fast reciprocal and square root)
+  R gi, rg, irg;
+  if constexpr (std::is_same<R, float>::value) {
+    gi = frcp(sf * sf);
+    rg = fsqrt(gi);
+    irg = frcp(rg);
+  } else {
+    gi = one / (sf * sf);
+    rg = sqrt(gi);
+    irg = one / sqrt(gi);
+  }
   if (DIR == 1) {  // PrimToLocal2
-    const R T22 = one / sqrt(gi), T32 = cf / sqrt(one), T33 = sqrt(one);
+    const R T22 = irg, T32 = cf / sqrt(one), T33 = sqrt(one);
     R uu2 = a[2], uu3 = a[3];
     a[2] = T22 * uu2;
     a[3] = T32 * uu2 + T33 * uu3;
@@ -316,7 +327,7 @@ __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& t
     b[2] = T22 * uu2;
     b[3] = T32 * uu2 + T33 * uu3;
   } else {  // PrimToLocal3
-    const R T22 = sqrt(one), T23 = cf / sqrt(one), T33 = one / sqrt(gi);
+    const R T22 = sqrt(one), T23 = cf / sqrt(one), T33 = irg;
     R uu2 = a[2], uu3 = a[3];
     a[2] = T22 * uu2 + T23 * uu3;
     a[3] = T33 * uu3;
@@ -327,15 +338,18 @@ __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& t
   }
   face_check<0, R>(g.status, a, b, R(0));
   R fx[5];
-  riemann<0, DIR, R>(th, CG_RS_HLLC, a, b, fx);
+  if constexpr (std::is_same<R, float>::value)
+    riemann_p<0, DIR>(th, CG_RS_HLLC, a, b, 0.f, fx);  // the FP32 faces' HLLC
+  else
+    riemann<0, DIR, R>(th, CG_RS_HLLC, a, b, fx);
   const R txx = fx[1], txy = fx[2], txz = fx[3];
   if (DIR == 1) {  // FluxToGlobal2
-    const R T22 = sqrt(gi), T32 = -sqrt(gi) * cf / one, T33 = one / sqrt(one);
+    const R T22 = rg, T32 = -rg * cf / one, T33 = one / sqrt(one);
     fx[1] = one * txx;
     fx[2] = T22 * txy;
     fx[3] = T32 * txy + T33 * txz;
   } else {  // FluxToGlobal3
-    const R T22 = one / sqrt(one), T23 = -cf / one * sqrt(gi), T33 = sqrt(gi);
+    const R T22 = one / sqrt(one), T23 = -cf / one * rg, T33 = rg;
     fx[1] = one * txx;
     fx[2] = T22 * txy + T23 * txz;
     fx[3] = T33 * txz;
@@ -387,6 +401,112 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
     sph_face_state<ORDER, DIR, true, R>(s, p, f, t, ii, b);
   }
   sph_face_riemann<DIR, R>(s, th, p, f, t, ii, a, b);
+}
+
+// The FP32 form's x2 and x3 faces of a tile of SF_J x SF_K columns of one
+// panel at one level: the tile's plane with a 3-cell halo is staged in LDS
+// as floats, each cell's two face values per direction come from one WENO5
+// (or PLM) evaluation (weno5_both_f: the smoothness indicators are shared),
+// and each face's HLLC in its orthonormal frame runs from LDS
+// (sph_face_riemann); at a panel edge the outer state is the neighbour's
+// from the edge buffer.  A face on the boundary between two tiles is
+// stored by the tile it opens (the panel's last face by the last tile).
+constexpr int SF_J = 32, SF_K = 8;
+template <int ORDER>
+__global__ void __launch_bounds__(SF_J * SF_K) k_sph_faces32(KSph s, KThermo th) {
+  constexpr int H = 3, PJ = SF_J + 2 * H, PK = SF_K + 2 * H, NT = SF_J * SF_K;
+  constexpr int C2 = SF_J + 2, C3 = SF_K + 2;  // cells with face values
+  constexpr int NQ2 = SF_K * C2, NQ3 = C3 * SF_J;
+  constexpr int NF2 = SF_K * (SF_J + 1), NF = NF2 + (SF_K + 1) * SF_J;
+  __shared__ float plane[5][PK][PJ];
+  __shared__ float ql[5][NF], qr[5][NF];  // left / right state of each face
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const int n = s.n;
+  const int ntj = (n + SF_J - 1) / SF_J, ntk = (n + SF_K - 1) / SF_K;
+  const int tj = blockIdx.x % ntj, tk = (blockIdx.x / ntj) % ntk;
+  const int ii = blockIdx.x / (ntj * ntk);
+  const int j0 = tj * SF_J, k0 = tk * SF_K;  // first face / cell of the tile
+  const int i = g.is + ii;
+  const int t = threadIdx.x;
+  const double* w = s.w[p];
+  // (1) the plane of cells j0-3 .. j0+SF_J+2, k0-3 .. k0+SF_K+2
+  for (int e = t; e < PJ * PK; e += NT) {
+    const int ej = e % PJ, ek = e / PJ;
+    const int gj = min(g.js + j0 - H + ej, g.nc2 - 1);
+    const int gk = min(g.ks + k0 - H + ek, g.nc3 - 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) plane[v][ek][ej] = (float)w[g.at(v, gk, gj, i)];
+  }
+  __syncthreads();
+  // (2) face values: cell c of a row (column) puts its upper value into the
+  // left state of tile face c and its lower value into the right state of
+  // tile face c - 1
+  auto values = [&](int st, int pk, int pj, int fl, int fr) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const float* c = &plane[v][pk][pj];
+      float qm, qp;
+      if (ORDER == 5) {
+        weno5_both_f(c[-2 * st], c[-st], c[0], c[st], c[2 * st], &qm, &qp);
+      } else {
+        qp = plm_up_f(c[-st], c[0], c[st]);
+        qm = plm_up_f(c[st], c[0], c[-st]);
+      }
+      if (fl >= 0) ql[v][fl] = qp;
+      if (fr >= 0) qr[v][fr] = qm;
+    }
+  };
+  for (int e = t; e < NQ2; e += NT) {  // x2: rows k0 .. k0+SF_K-1
+    const int c = e % C2, r = e / C2, f0 = r * (SF_J + 1);
+    values(1, r + H, c + H - 1, c <= SF_J ? f0 + c : -1, c >= 1 ? f0 + c - 1 : -1);
+  }
+  for (int e = t; e < NQ3; e += NT) {  // x3: columns j0 .. j0+SF_J-1
+    const int q = e % SF_J, c = e / SF_J;
+    values(PJ, c + H - 1, q + H, c <= SF_K ? NF2 + c * SF_J + q : -1,
+           c >= 1 ? NF2 + (c - 1) * SF_J + q : -1);
+  }
+  __syncthreads();
+  // (3) the faces
+  const float* eb = reinterpret_cast<const float*>(s.lr);
+  for (int e = t; e < NF; e += NT) {
+    const bool x2 = e < NF2;
+    int f, tt, fc, last;
+    if (x2) {
+      fc = e % (SF_J + 1);
+      f = j0 + fc;
+      tt = k0 + e / (SF_J + 1);
+      last = SF_J;
+    } else {
+      const int e3 = e - NF2;
+      fc = e3 / SF_J;
+      f = k0 + fc;
+      tt = j0 + e3 % SF_J;
+      last = SF_K;
+    }
+    if (tt >= n || f > n || (fc == last && f != n)) continue;
+    float a[5], b[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      a[v] = ql[v][e];
+      b[v] = qr[v][e];
+    }
+    const int side = x2 ? 0 : 2;
+    if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
+      const long long ei = ((long long)p * 4 + side) * n + tt;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, ei, v, ii)];
+    }
+    if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
+      const long long ei = ((long long)p * 4 + side + 1) * n + tt;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, ei, v, ii)];
+    }
+    if (x2)
+      sph_face_riemann<1, float>(s, th, p, f, tt, ii, a, b);
+    else
+      sph_face_riemann<2, float>(s, th, p, f, tt, ii, a, b);
+  }
 }
 
 // areas and volume of a panel cell (gnomonic_equiangle.cpp:221-260, 407-416)
