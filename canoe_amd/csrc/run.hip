@@ -342,14 +342,17 @@ int sphere_stage(cg_run* r, int s) {
     // the cells' face values shared through LDS
     const unsigned nt = (unsigned)(((ks.n + SF_J - 1) / SF_J) * ((ks.n + SF_K - 1) / SF_K) *
                                    ks.nr);
+    // with the flux divergence, sources and forcing of the tile's cells fused
     if (r->cfg.order_h == 5) {
       TIMED(tm, KID_FLUX_X2,
             (k_sph_edge_states<5, float><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));
-      TIMED(tm, KID_FLUX_X3, (k_sph_faces32<5><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(ks, r->kt)));
+      TIMED(tm, KID_DIVERGENCE, (k_sph_faces32<5, true><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(
+                                    ks, r->kt, sbeta[s], r->d_dt, psplit)));
     } else {
       TIMED(tm, KID_FLUX_X2,
             (k_sph_edge_states<2, float><<<ge, 256, 0, st>>>(ks, S.d_edges, S.nedge)));
-      TIMED(tm, KID_FLUX_X3, (k_sph_faces32<2><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(ks, r->kt)));
+      TIMED(tm, KID_DIVERGENCE, (k_sph_faces32<2, true><<<dim3(nt, 6), SF_J * SF_K, 0, st>>>(
+                                    ks, r->kt, sbeta[s], r->d_dt, psplit)));
     }
   } else {
     if (r->cfg.order_h == 5)
@@ -358,10 +361,7 @@ int sphere_stage(cg_run* r, int s) {
       CG_SPH_FACES(2, double);
   }
 #undef CG_SPH_FACES
-  if (f32)
-    TIMED(tm, KID_DIVERGENCE, (k_sph_divergence<float><<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(
-                                  ks, sbeta[s], r->d_dt, psplit)));
-  else
+  if (!f32)
     TIMED(tm, KID_DIVERGENCE, (k_sph_divergence<double><<<dim3(nblk(ncell, 256), 6), 256, 0, st>>>(
                                   ks, sbeta[s], r->d_dt, psplit)));
   KUpdate up;

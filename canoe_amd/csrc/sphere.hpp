@@ -294,7 +294,8 @@ __global__ void __launch_bounds__(256)
 // (hllc_transform.cpp:235-272), stored in f2 / f3.
 template <int DIR, class R>
 __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& th, int p,
-                                                 int f, int t, int ii, R* a, R* b) {
+                                                 int f, int t, int ii, R* a, R* b,
+                                                 R* out = nullptr) {
   const KGrid& g = s.g;
   const int i = g.is + ii;
   const int j = (DIR == 1) ? g.js + f : g.js + t;
@@ -357,6 +358,11 @@ __device__ __forceinline__ void sph_face_riemann(const KSph& s, const KThermo& t
   const R ty = fx[2], tz = fx[3];
   fx[2] = ty + tz * hc;
   fx[3] = tz + ty * hc;
+  if (out) {  // to the caller (k_sph_faces32 with the divergence fused)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = fx[v];
+    return;
+  }
   // stored in the form's type (the FP32 form's fluxes take half the bytes)
   R* F = reinterpret_cast<R*>((DIR == 1) ? s.f2[p] : s.f3[p]);
 #pragma unroll
@@ -403,112 +409,6 @@ __global__ void __launch_bounds__(256) k_sph_face_flux(KSph s, KThermo th) {
   sph_face_riemann<DIR, R>(s, th, p, f, t, ii, a, b);
 }
 
-// The FP32 form's x2 and x3 faces of a tile of SF_J x SF_K columns of one
-// panel at one level: the tile's plane with a 3-cell halo is staged in LDS
-// as floats, each cell's two face values per direction come from one WENO5
-// (or PLM) evaluation (weno5_both_f: the smoothness indicators are shared),
-// and each face's HLLC in its orthonormal frame runs from LDS
-// (sph_face_riemann); at a panel edge the outer state is the neighbour's
-// from the edge buffer.  A face on the boundary between two tiles is
-// stored by the tile it opens (the panel's last face by the last tile).
-constexpr int SF_J = 32, SF_K = 8;
-template <int ORDER>
-__global__ void __launch_bounds__(SF_J * SF_K) k_sph_faces32(KSph s, KThermo th) {
-  constexpr int H = 3, PJ = SF_J + 2 * H, PK = SF_K + 2 * H, NT = SF_J * SF_K;
-  constexpr int C2 = SF_J + 2, C3 = SF_K + 2;  // cells with face values
-  constexpr int NQ2 = SF_K * C2, NQ3 = C3 * SF_J;
-  constexpr int NF2 = SF_K * (SF_J + 1), NF = NF2 + (SF_K + 1) * SF_J;
-  __shared__ float plane[5][PK][PJ];
-  __shared__ float ql[5][NF], qr[5][NF];  // left / right state of each face
-  const int p = blockIdx.y;
-  const KGrid& g = s.g;
-  const int n = s.n;
-  const int ntj = (n + SF_J - 1) / SF_J, ntk = (n + SF_K - 1) / SF_K;
-  const int tj = blockIdx.x % ntj, tk = (blockIdx.x / ntj) % ntk;
-  const int ii = blockIdx.x / (ntj * ntk);
-  const int j0 = tj * SF_J, k0 = tk * SF_K;  // first face / cell of the tile
-  const int i = g.is + ii;
-  const int t = threadIdx.x;
-  const double* w = s.w[p];
-  // (1) the plane of cells j0-3 .. j0+SF_J+2, k0-3 .. k0+SF_K+2
-  for (int e = t; e < PJ * PK; e += NT) {
-    const int ej = e % PJ, ek = e / PJ;
-    const int gj = min(g.js + j0 - H + ej, g.nc2 - 1);
-    const int gk = min(g.ks + k0 - H + ek, g.nc3 - 1);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) plane[v][ek][ej] = (float)w[g.at(v, gk, gj, i)];
-  }
-  __syncthreads();
-  // (2) face values: cell c of a row (column) puts its upper value into the
-  // left state of tile face c and its lower value into the right state of
-  // tile face c - 1
-  auto values = [&](int st, int pk, int pj, int fl, int fr) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const float* c = &plane[v][pk][pj];
-      float qm, qp;
-      if (ORDER == 5) {
-        weno5_both_f(c[-2 * st], c[-st], c[0], c[st], c[2 * st], &qm, &qp);
-      } else {
-        qp = plm_up_f(c[-st], c[0], c[st]);
-        qm = plm_up_f(c[st], c[0], c[-st]);
-      }
-      if (fl >= 0) ql[v][fl] = qp;
-      if (fr >= 0) qr[v][fr] = qm;
-    }
-  };
-  for (int e = t; e < NQ2; e += NT) {  // x2: rows k0 .. k0+SF_K-1
-    const int c = e % C2, r = e / C2, f0 = r * (SF_J + 1);
-    values(1, r + H, c + H - 1, c <= SF_J ? f0 + c : -1, c >= 1 ? f0 + c - 1 : -1);
-  }
-  for (int e = t; e < NQ3; e += NT) {  // x3: columns j0 .. j0+SF_J-1
-    const int q = e % SF_J, c = e / SF_J;
-    values(PJ, c + H - 1, q + H, c <= SF_K ? NF2 + c * SF_J + q : -1,
-           c >= 1 ? NF2 + (c - 1) * SF_J + q : -1);
-  }
-  __syncthreads();
-  // (3) the faces
-  const float* eb = reinterpret_cast<const float*>(s.lr);
-  for (int e = t; e < NF; e += NT) {
-    const bool x2 = e < NF2;
-    int f, tt, fc, last;
-    if (x2) {
-      fc = e % (SF_J + 1);
-      f = j0 + fc;
-      tt = k0 + e / (SF_J + 1);
-      last = SF_J;
-    } else {
-      const int e3 = e - NF2;
-      fc = e3 / SF_J;
-      f = k0 + fc;
-      tt = j0 + e3 % SF_J;
-      last = SF_K;
-    }
-    if (tt >= n || f > n || (fc == last && f != n)) continue;
-    float a[5], b[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      a[v] = ql[v][e];
-      b[v] = qr[v][e];
-    }
-    const int side = x2 ? 0 : 2;
-    if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
-      const long long ei = ((long long)p * 4 + side) * n + tt;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, ei, v, ii)];
-    }
-    if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
-      const long long ei = ((long long)p * 4 + side + 1) * n + tt;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, ei, v, ii)];
-    }
-    if (x2)
-      sph_face_riemann<1, float>(s, th, p, f, tt, ii, a, b);
-    else
-      sph_face_riemann<2, float>(s, th, p, f, tt, ii, a, b);
-  }
-}
-
 // areas and volume of a panel cell (gnomonic_equiangle.cpp:221-260, 407-416)
 __device__ __forceinline__ double sph_a1(const KSph& s, int p, int k, int j, int i) {
   return skj(s, p, cg_host::SKJ_D2A, k, j) * skj(s, p, cg_host::SKJ_D3A, k, j) * s.x1f[i] *
@@ -525,21 +425,17 @@ __device__ __forceinline__ double sph_v(const KSph& s, int p, int k, int j, int 
          s.x1v[i] * s.x1v[i] * skj(s, p, cg_host::SKJ_SINC, k, j);
 }
 
-// du = -wght div F (areas, volume) + geometric terms + gravity + forcing;
-// FT: the type the face fluxes are stored in (double, or float in the
-// FP32-flux form); psplit: the x1 fluxes leave the face pressures psf out of
-// the normal momentum flux (k_sph_flux32_x1m), added here in FP64
-template <class FT>
-__global__ void __launch_bounds__(256)
-    k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp, int psplit) {
-  const int p = blockIdx.y;
-  int i, j, k;
-  if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+// du of cell (k, j, i) of panel p = -wght div F (areas, volume) + geometric
+// terms + gravity + forcing, from its six face fluxes (f1l / f1r the lower /
+// upper x1 face, then x2, x3); psplit: the x1 fluxes leave the face
+// pressures psf out of the normal momentum flux (k_sph_flux32_x1m), added
+// here in FP64
+__device__ __forceinline__ void sph_cell_du(const KSph& s, int p, int k, int j, int i,
+                                            double wght, const double* f1l,
+                                            const double* f1r, const double* f2l,
+                                            const double* f2r, const double* f3l,
+                                            const double* f3r, int psplit) {
   const KGrid& g = s.g;
-  const double wght = beta * dtp[0];
-  const FT* F1 = reinterpret_cast<const FT*>(s.f1[p]);
-  const FT* F2 = reinterpret_cast<const FT*>(s.f2[p]);
-  const FT* F3 = reinterpret_cast<const FT*>(s.f3[p]);
   const double a1l = sph_a1(s, p, k, j, i), a1r = sph_a1(s, p, k, j, i + 1);
   const double a2l = sph_a2(s, p, k, j, i), a2r = sph_a2(s, p, k, j + 1, i);
   const double a3l = sph_a3(s, p, k, j, i), a3r = sph_a3(s, p, k + 1, j, i);
@@ -547,14 +443,13 @@ __global__ void __launch_bounds__(256)
   double d[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    double dflx = a1r * (double)F1[g.f[0].at(v, k, j, i + 1)] -
-                  a1l * (double)F1[g.f[0].at(v, k, j, i)];
+    double dflx = a1r * f1r[v] - a1l * f1l[v];
     if (v == 1 && psplit) {
       const double* pf = s.psf[p] + g.pcol(k, j) * g.ps_c;
       dflx += a1r * pf[(i + 1) * g.ps_i] - a1l * pf[i * g.ps_i];
     }
-    dflx += a2r * (double)F2[g.f[1].at(v, k, j + 1, i)] - a2l * (double)F2[g.f[1].at(v, k, j, i)];
-    dflx += a3r * (double)F3[g.f[2].at(v, k + 1, j, i)] - a3l * (double)F3[g.f[2].at(v, k, j, i)];
+    dflx += a2r * f2r[v] - a2l * f2l[v];
+    dflx += a3r * f3r[v] - a3l * f3l[v];
     d[v] = 0.0 - wght * dflx / vol;
   }
   const long long c0 = g.at(0, k, j, i);
@@ -627,6 +522,170 @@ __global__ void __launch_bounds__(256)
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) s.du[p][c0 + v * g.c.n] = d[v];
+}
+
+// sph_cell_du of every cell, the face fluxes read from f1 / f2 / f3, stored
+// as FT (double, or float in the FP32-flux form)
+template <class FT>
+__global__ void __launch_bounds__(256)
+    k_sph_divergence(KSph s, double beta, const double* __restrict__ dtp, int psplit) {
+  const int p = blockIdx.y;
+  int i, j, k;
+  if (!sph_cell(s.g, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const KGrid& g = s.g;
+  const FT* F1 = reinterpret_cast<const FT*>(s.f1[p]);
+  const FT* F2 = reinterpret_cast<const FT*>(s.f2[p]);
+  const FT* F3 = reinterpret_cast<const FT*>(s.f3[p]);
+  double f1l[5], f1r[5], f2l[5], f2r[5], f3l[5], f3r[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    f1l[v] = (double)F1[g.f[0].at(v, k, j, i)];
+    f1r[v] = (double)F1[g.f[0].at(v, k, j, i + 1)];
+    f2l[v] = (double)F2[g.f[1].at(v, k, j, i)];
+    f2r[v] = (double)F2[g.f[1].at(v, k, j + 1, i)];
+    f3l[v] = (double)F3[g.f[2].at(v, k, j, i)];
+    f3r[v] = (double)F3[g.f[2].at(v, k + 1, j, i)];
+  }
+  sph_cell_du(s, p, k, j, i, beta * dtp[0], f1l, f1r, f2l, f2r, f3l, f3r, psplit);
+}
+
+// The FP32 form's x2 and x3 faces of a tile of SF_J x SF_K columns of one
+// panel at one level: the tile's plane with a 3-cell halo is staged in LDS
+// as floats, each cell's two face values per direction come from one WENO5
+// (or PLM) evaluation (weno5_both_f: the smoothness indicators are shared),
+// and each face's HLLC in its orthonormal frame runs from LDS
+// (sph_face_riemann); at a panel edge the outer state is the neighbour's
+// from the edge buffer.  DIV false: the fluxes are stored in f2 / f3, a face
+// on the boundary between two tiles by the tile it opens (the panel's last
+// face by the last tile).  DIV true: each tile keeps all its faces' fluxes
+// in LDS and each lane finishes its cell with sph_cell_du (the x1 fluxes
+// from f1, stage weight beta dtp[0]): the x2 / x3 fluxes never leave the
+// chip.
+constexpr int SF_J = 32, SF_K = 8;
+template <int ORDER, bool DIV>
+__global__ void __launch_bounds__(SF_J * SF_K)
+    k_sph_faces32(KSph s, KThermo th, double beta, const double* __restrict__ dtp,
+                  int psplit) {
+  constexpr int H = 3, PJ = SF_J + 2 * H, PK = SF_K + 2 * H, NT = SF_J * SF_K;
+  constexpr int C2 = SF_J + 2, C3 = SF_K + 2;  // cells with face values
+  constexpr int NQ2 = SF_K * C2, NQ3 = C3 * SF_J;
+  constexpr int NF2 = SF_K * (SF_J + 1), NF = NF2 + (SF_K + 1) * SF_J;
+  __shared__ float plane[5][PK][PJ];
+  __shared__ float ql[5][NF], qr[5][NF];  // left / right state of each face
+  __shared__ float fxs[DIV ? 5 : 1][DIV ? NF : 1];  // face fluxes (DIV)
+  const int p = blockIdx.y;
+  const KGrid& g = s.g;
+  const int n = s.n;
+  const int ntj = (n + SF_J - 1) / SF_J, ntk = (n + SF_K - 1) / SF_K;
+  const int tj = blockIdx.x % ntj, tk = (blockIdx.x / ntj) % ntk;
+  const int ii = blockIdx.x / (ntj * ntk);
+  const int j0 = tj * SF_J, k0 = tk * SF_K;  // first face / cell of the tile
+  const int i = g.is + ii;
+  const int t = threadIdx.x;
+  const double* w = s.w[p];
+  // (1) the plane of cells j0-3 .. j0+SF_J+2, k0-3 .. k0+SF_K+2
+  for (int e = t; e < PJ * PK; e += NT) {
+    const int ej = e % PJ, ek = e / PJ;
+    const int gj = min(g.js + j0 - H + ej, g.nc2 - 1);
+    const int gk = min(g.ks + k0 - H + ek, g.nc3 - 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) plane[v][ek][ej] = (float)w[g.at(v, gk, gj, i)];
+  }
+  __syncthreads();
+  // (2) face values: cell c of a row (column) puts its upper value into the
+  // left state of tile face c and its lower value into the right state of
+  // tile face c - 1
+  auto values = [&](int st, int pk, int pj, int fl, int fr) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const float* c = &plane[v][pk][pj];
+      float qm, qp;
+      if (ORDER == 5) {
+        weno5_both_f(c[-2 * st], c[-st], c[0], c[st], c[2 * st], &qm, &qp);
+      } else {
+        qp = plm_up_f(c[-st], c[0], c[st]);
+        qm = plm_up_f(c[st], c[0], c[-st]);
+      }
+      if (fl >= 0) ql[v][fl] = qp;
+      if (fr >= 0) qr[v][fr] = qm;
+    }
+  };
+  for (int e = t; e < NQ2; e += NT) {  // x2: rows k0 .. k0+SF_K-1
+    const int c = e % C2, r = e / C2, f0 = r * (SF_J + 1);
+    values(1, r + H, c + H - 1, c <= SF_J ? f0 + c : -1, c >= 1 ? f0 + c - 1 : -1);
+  }
+  for (int e = t; e < NQ3; e += NT) {  // x3: columns j0 .. j0+SF_J-1
+    const int q = e % SF_J, c = e / SF_J;
+    values(PJ, c + H - 1, q + H, c <= SF_K ? NF2 + c * SF_J + q : -1,
+           c >= 1 ? NF2 + (c - 1) * SF_J + q : -1);
+  }
+  __syncthreads();
+  // (3) the faces
+  const float* eb = reinterpret_cast<const float*>(s.lr);
+  for (int e = t; e < NF; e += NT) {
+    const bool x2 = e < NF2;
+    int f, tt, fc, last;
+    if (x2) {
+      fc = e % (SF_J + 1);
+      f = j0 + fc;
+      tt = k0 + e / (SF_J + 1);
+      last = SF_J;
+    } else {
+      const int e3 = e - NF2;
+      fc = e3 / SF_J;
+      f = k0 + fc;
+      tt = j0 + e3 % SF_J;
+      last = SF_K;
+    }
+    if (tt >= n || f > n || (!DIV && fc == last && f != n)) continue;
+    float a[5], b[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      a[v] = ql[v][e];
+      b[v] = qr[v][e];
+    }
+    const int side = x2 ? 0 : 2;
+    if (f == 0) {  // lower panel edge: the outer (left) state is the neighbour's
+      const long long ei = ((long long)p * 4 + side) * n + tt;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a[v] = eb[sph_eb(s, ei, v, ii)];
+    }
+    if (f == n) {  // upper panel edge: the outer (right) state is the neighbour's
+      const long long ei = ((long long)p * 4 + side + 1) * n + tt;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) b[v] = eb[sph_eb(s, ei, v, ii)];
+    }
+    float fo[5];
+    float* out = DIV ? fo : nullptr;
+    if (x2)
+      sph_face_riemann<1, float>(s, th, p, f, tt, ii, a, b, out);
+    else
+      sph_face_riemann<2, float>(s, th, p, f, tt, ii, a, b, out);
+    if (DIV) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fxs[v][e] = fo[v];
+    }
+  }
+  if (!DIV) return;
+  __syncthreads();
+  // (4) the cell of this lane: its x2 / x3 face fluxes from LDS, its x1 ones
+  // from f1
+  const int cj = t % SF_J, ck = t / SF_J;
+  if (j0 + cj >= n || k0 + ck >= n) return;
+  const int J = g.js + j0 + cj, K = g.ks + k0 + ck;
+  const float* F1 = reinterpret_cast<const float*>(s.f1[p]);
+  double f1l[5], f1r[5], f2l[5], f2r[5], f3l[5], f3r[5];
+  const int e2 = ck * (SF_J + 1) + cj, e3 = NF2 + ck * SF_J + cj;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    f1l[v] = (double)F1[g.f[0].at(v, K, J, i)];
+    f1r[v] = (double)F1[g.f[0].at(v, K, J, i + 1)];
+    f2l[v] = (double)fxs[v][e2];
+    f2r[v] = (double)fxs[v][e2 + 1];
+    f3l[v] = (double)fxs[v][e3];
+    f3r[v] = (double)fxs[v][e3 + SF_J];
+  }
+  sph_cell_du(s, p, K, J, i, beta * dtp[0], f1l, f1r, f2l, f2r, f3l, f3r, psplit);
 }
 
 // x1 columns of panel blockIdx.y: forward sweep with the panel geometry
