@@ -1,6 +1,6 @@
 // sphere.hpp -- device kernels of the cubed-sphere run (config 3): six
 // gnomonic-equiangle panels (src/exo3/gnomonic_equiangle.cpp), each a block
-// of the run in the level-major layout, dry gas, FP64.
+// of the run in the level-major layout, dry gas, FP64 state.
 //
 // Per stage: x1 fluxes (k_sph_flux_x1: decomposition, WENO5, HLLC) with the
 // covariant projection of the tangential momentum fluxes
@@ -11,15 +11,19 @@
 // gnomonic_equiangle.cpp:559-828, then the covariant projection); flux
 // divergence with the panel's areas and volumes, the geometric source terms
 // (gnomonic_equiangle.cpp:831-877), gravity and the Held-Suarez / Coriolis
-// forcing of examples/2023-Chen-exo3/hs94.cpp:56-151 (k_sph_divergence);
-// the x1 column solve with the panel's areas (k_sph_implicit_fwd, then
-// k_sph_bwd_update, which also applies the stage, the x1 walls and the
-// scan); the stage update with ConservedToPrimitive for covariant momenta
-// (k_sph_update, patches/26.adiabatic_hydro.patch:21-46); the ghost
-// zones of x2 / x3 interpolated from the neighbour panels (k_sph_ghosts,
-// cubed_sphere_utility.cpp:80-400); x1 walls and the hydrostatic scan.
-// Arithmetic is written in the order of the oracle's restatement so that
-// the two agree to rounding.
+// forcing of examples/2023-Chen-exo3/hs94.cpp:56-151 (k_sph_divergence,
+// sph_cell_du); the x1 column solve with the panel's areas
+// (k_sph_implicit_fwd, then k_sph_bwd_update, which also applies the stage,
+// the x1 walls and the scan); the stage update with ConservedToPrimitive
+// for covariant momenta (k_sph_update, patches/26.adiabatic_hydro.patch:
+// 21-46); the ghost zones of x2 / x3 interpolated from the neighbour panels
+// (k_sph_ghosts, cubed_sphere_utility.cpp:80-400); x1 walls and the
+// hydrostatic scan.  The FP32-flux form replaces the face kernels and the
+// divergence with k_sph_flux32_x1m (x1 faces in float, marched up column
+// segments) and k_sph_faces32 (x2 / x3 faces of column tiles through LDS,
+// fused with the divergence); its state, sources and column solve stay
+// FP64.  The FP64 form's arithmetic is written in the order of the oracle's
+// restatement so that the two agree to rounding.
 #pragma once
 #include "cg_physics.hpp"
 #include "flux32.hpp"
