@@ -6,9 +6,11 @@ direct launches: bar 1e-12 (the radial momentum is the small residual of a
 near-hydrostatic balance, so rounding differences grow by about 1e6 over
 three steps; the device therefore follows the oracle's arithmetic, the
 (U, V) route of the reference included, and agrees to rounding).  The run
-conserves the six panels' total mass to 1e-13; the FP32-flux form (x2 / x3
-face states and Riemann fluxes in FP32) stays within 1e-5 of FP64; and
-unsupported settings are rejected."""
+conserves the six panels' total mass to 1e-13; the FP32-flux form (face
+states and Riemann fluxes of all three directions in FP32, the x2 / x3 faces
+of 32 x 8 column tiles) stays within 1e-6 of FP64 in density and energy and
+1e-3 in the small momenta, on panels that fill a tile partly, exactly and
+with a remainder; and unsupported settings are rejected."""
 import numpy as np
 import pytest
 
@@ -87,8 +89,10 @@ def test_sphere_rejects_unsupported_settings(ctx):
             ca.SphereRun(ctx, b, th, c, sp)
 
 
-def test_sphere_fp32_faces_close_to_fp64(ctx):
-    blk, th, cfg, sp, w = _state()
+@pytest.mark.parametrize("n", [8, 12, 36])
+def test_sphere_fp32_faces_close_to_fp64(ctx, n):
+    blk, th, cfg, sp, w = _state(n=n)
+    _say("sphere fp32 faces, panels of %d x %d" % (n, n))
     out = []
     for prec in (ca.PREC_FP64, ca.PREC_FP32):
         cfg.precision = prec
