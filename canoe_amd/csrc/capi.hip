@@ -99,6 +99,10 @@ int cg_destroy(cg_ctx* ctx) {
   if (ctx->t0) hipEventDestroy(ctx->t0);
   if (ctx->t1) hipEventDestroy(ctx->t1);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  // hand the pool's unused memory (kept between calls, see cg_create) back
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess)
+    (void)hipMemPoolTrimTo(pool, 0);
   delete ctx;
   return CG_OK;
 }
