@@ -609,7 +609,8 @@ def bench_host_solve(ctx, nx, reps=3):
     # the parts of a call: the same transfers between pageable host memory
     # and device buffers allocated once (du and w in, du out), and the
     # device solve on those buffers (event-timed); the rest of the call is
-    # its own allocation of staging and scratch memory
+    # its staging and scratch memory (taken from the device's pool, which the
+    # context keeps between calls) and host overhead
     L = ca.lib()
     dd, dw = ctx.array(du), ctx.array(w)
     d = du.copy()
