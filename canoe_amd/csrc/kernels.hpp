@@ -1058,10 +1058,94 @@ __global__ void __launch_bounds__(64)
 // implicit_hydro_tasks.cpp:146-160), u += du (:277-283), saturation
 // adjustment on the last stage (UpdateAllConserved, :290-313),
 // ConservedToPrimitive.  Returns the new u (uc) and w (wc).
+// Kessler warm rain and precipitation sedimentation: rates and fall speed
+// (examples/2019-Li-snap/earth-precip.yaml:44-53, water-kessler.yaml:4-18),
+// ip the 0-based species of the precipitation
+struct KKessler {
+  double k_auto, k_accr, k_evap, v_fall;
+  int ip;
+};
+
+// One cell of the top-down Kessler sweep: uc (conserved state), wc (its
+// primitive state) and rho (total density) in; the cloud -> precipitation
+// conversion, the evaporation of precipitation into sub-saturated air and
+// the sedimentation exchange applied to uc, wc.  fin: the precipitation
+// mass, momenta and energy falling in from the cell above; fout: what falls
+// out of this cell (nothing at the bottom level: it does not cross the
+// wall).  dt the full step, cf = min(1, dt v_fall / dx1), cvp the
+// precipitation's cv.
+template <int NM>
+__device__ __forceinline__ void kessler_cell(const KThermo& th, const KKessler& kz,
+                                             double dt, double cf, double cvp, bool bottom,
+                                             double rho, double* uc, double* wc,
+                                             const double* fin, double* fout) {
+  constexpr int IVX = 1 + NM, IEN = 4 + NM;
+  const int rv_ = 1 + th.iv, rc_ = 1 + th.ic, rp_ = 1 + kz.ip;
+  const double T = temperature_d<NM>(th, wc);
+  // species picked by runtime index from registers: select chains
+  double rc = 0., rp = 0., rv = 0.;
+#pragma unroll
+  for (int n = 1; n <= NM; ++n) {
+    if (n == rc_) rc = uc[n];
+    if (n == rp_) rp = uc[n];
+    if (n == rv_) rv = uc[n];
+  }
+  double a = dt * (kz.k_auto + kz.k_accr * (rp / rho)) * rc;
+  if (a > rc) a = rc;
+  rc = rc - a;
+  rp = rp + a;
+  const double rs = qsat_d(th, T, rho) * rho;
+  const double def = rs - rv;
+  double b = 0.;
+  if (def > 0. && rp > 0.) {
+    b = dt * kz.k_evap * rp * (def / rs);
+    if (b > rp) b = rp;
+    if (b > def) b = def;
+  }
+  rp = rp - b;
+  rv = rv + b;
+#pragma unroll
+  for (int n = 1; n <= NM; ++n) {
+    if (n == rc_) uc[n] = rc;
+    if (n == rp_) uc[n] = rp;
+    if (n == rv_) uc[n] = rv;
+  }
+  uc[IEN] = uc[IEN] - th.L * b;
+  double out_p = 0., out_m0 = 0., out_m1 = 0., out_m2 = 0., out_e = 0.;
+  if (!bottom && cf > 0.) {
+    cons_to_prim_cell<NM>(th, uc, wc);
+    const double T2 = temperature_d<NM>(th, wc);
+    const double v1 = wc[IVX], v2 = wc[IVX + 1], v3 = wc[IVX + 2];
+    out_p = cf * rp;
+    out_m0 = out_p * v1;
+    out_m1 = out_p * v2;
+    out_m2 = out_p * v3;
+    out_e = out_p * (cvp * T2 + 0.5 * (v1 * v1 + v2 * v2 + v3 * v3));
+  }
+#pragma unroll
+  for (int n = 1; n <= NM; ++n)
+    if (n == rp_) uc[n] = (uc[n] - out_p) + fin[0];
+  uc[IVX] = (uc[IVX] - out_m0) + fin[1];
+  uc[IVX + 1] = (uc[IVX + 1] - out_m1) + fin[2];
+  uc[IVX + 2] = (uc[IVX + 2] - out_m2) + fin[3];
+  uc[IEN] = (uc[IEN] - out_e) + fin[4];
+  cons_to_prim_cell<NM>(th, uc, wc);
+  fout[0] = out_p;
+  fout[1] = out_m0;
+  fout[2] = out_m1;
+  fout[3] = out_m2;
+  fout[4] = out_e;
+}
+
 struct KUpdate {
   int stage;  // 0, 1, 2
   double g1, g2;
   int satadj;
+  // Kessler microphysics in the downward sweep (its KES instances): the
+  // rates and the full step, kdt or kdt * kdtp[0] when kdtp is given
+  KKessler kz{};
+  double kdt = 0.0;
+  const double* kdtp = nullptr;
 };
 
 // uo / u1o: the cell's u and u1 (u1o unused on stage 0, which writes u1).
@@ -1150,9 +1234,12 @@ __global__ void __launch_bounds__(256)
 // the write-back into du, the end-of-stage update of u and w, the x1 ghost
 // cells of w (reflecting / outflow) and the hydrostatic face and cell
 // pressures of the next stage (the same recurrences as k_hydrostatic_scan,
-// evaluated as the new state appears top-down).
+// evaluated as the new state appears top-down).  KES (last stage of a run
+// with Kessler microphysics, x1 sides walls or outflow): each new cell
+// state passes through kessler_cell before it is stored, which is the top-
+// down sweep of kessler_column run as the update reaches each level.
 // One column: c0 as in fwd_column, pc = offset of the column in psf / psv.
-template <int NM, int B>
+template <int NM, int B, bool KES = false>
 __device__ __forceinline__ void bwd_update_column(
     const KGrid& g, const KThermo& th, const KUpdate& up,
     const double* __restrict__ arch, long long as, long long ao,
@@ -1171,6 +1258,16 @@ __device__ __forceinline__ void bwd_update_column(
   double rtop[NG];    // density of the top three cells (ie, ie-1, ie-2)
   double pf_top[NG + 1];  // psf(ie+1) .. psf(ie+4)
   double pf_next = 0.0;   // psf(i+1)
+  // Kessler: the full step, the fall fraction, the precipitation's cv and
+  // what falls into the current level from the level above
+  double kdt = 0.0, kcf = 0.0, kcvp = 0.0;
+  double kin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if constexpr (KES) {
+    kdt = up.kdtp ? up.kdt * up.kdtp[0] : up.kdt;
+    kcf = kdt * up.kz.v_fall / dx;
+    if (kcf > 1.) kcf = 1.;
+    kcvp = th.cvd * (1. + th.cv_ratio_m1[up.kz.ip]);
+  }
   // the archive and the state of the levels below are loaded two levels
   // ahead, so that their latency overlaps the current levels' updates
   struct Level {
@@ -1220,6 +1317,15 @@ __device__ __forceinline__ void bwd_update_column(
       double dun[N], uc[N], wc[N];
       write_back<NM, B>(d, cur.duo, cur.wsp, dun);
       update_cell_pre<NM>(th, up, cur.uo, cur.u1o, u1, ci, sn, dun, uc, wc, status);
+      if constexpr (KES) {
+        double rho = uc[0];
+#pragma unroll
+        for (int n = 1; n <= NM; ++n) rho += uc[n];
+        double kout[5];
+        kessler_cell<NM>(th, up.kz, kdt, kcf, kcvp, i == is, rho, uc, wc, kin, kout);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) kin[n] = kout[n];
+      }
 #pragma unroll
       for (int n = 0; n < N; ++n) {
         // streamed: read again only by the next stage's kernels
@@ -1388,7 +1494,7 @@ __global__ void __launch_bounds__(64 * WPB)
   }
 }
 
-template <int NM, int B, int WPB>
+template <int NM, int B, int WPB, bool KES = false>
 __global__ void __launch_bounds__(64 * WPB)
     k_implicit_bwd_update_batch(const KBlockPtr* __restrict__ bp,
                                 const KGroup* __restrict__ grp, int ngroups,
@@ -1406,7 +1512,7 @@ __global__ void __launch_bounds__(64 * WPB)
     if (col < (long long)g.nx2 * g.nx3) {
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       const KBlockPtr& p = bp[b];
-      bwd_update_column<NM, B>(g, th, up, arch + aoff, (long long)g.nx2 * g.nx3,
+      bwd_update_column<NM, B, KES>(g, th, up, arch + aoff, (long long)g.nx2 * g.nx3,
                                col, gptr(p.du),
                                gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
                                gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
@@ -1419,7 +1525,7 @@ __global__ void __launch_bounds__(64 * WPB)
 // the columns of a group depend on no other group within a stage, and the
 // top levels' archive, written last by the forward sweep, is read first by
 // the downward sweep while still in the caches.
-template <int NM, int B, int WPB>
+template <int NM, int B, int WPB, bool KES = false>
 __global__ void __launch_bounds__(64 * WPB)
     k_implicit_fused_batch(const KBlockPtr* __restrict__ bp,
                            const KGroup* __restrict__ grp, int ngroups,
@@ -1442,7 +1548,7 @@ __global__ void __launch_bounds__(64 * WPB)
       if (!fwd_column<NM, B>(g, th, dt, gptr(p.w), gptr(p.du), g.at(0, k, j, 0),
                              arch + aoff, ncol, col, gptr(p.psv), col * g.ps_c))
         atomicOr(status, 1);
-      bwd_update_column<NM, B>(g, th, up, arch + aoff, ncol, col, gptr(p.du),
+      bwd_update_column<NM, B, KES>(g, th, up, arch + aoff, ncol, col, gptr(p.du),
                                gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
                                gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
                                status);

@@ -13,17 +13,14 @@
 // E -= L b (the moist internal energy ie - rho L (q_c + q_p) is kept), then
 // a fraction c = min(1, dt v_fall / dx1) of the cell's precipitation falls
 // into the cell below with the cell's velocity and its own sensible energy
-// cv_p T + |v|^2 / 2; nothing crosses the walls.
+// cv_p T + |v|^2 / 2; nothing crosses the walls.  The per-cell step,
+// kessler_cell, lives in kernels.hpp: the implicit downward sweep applies it
+// as well, in the last stage of runs whose x1 sides are walls or outflow.
 #pragma once
 #include "cg_physics.hpp"
 #include "kernels.hpp"
 
 namespace cg {
-
-struct KKessler {
-  double k_auto, k_accr, k_evap, v_fall;
-  int ip;  // 0-based species of the precipitation
-};
 
 // One column (k, j): u (conserved) in / out, w (primitive) out, interior
 // cells only.  dt is the full step.  FUSE (reflecting or outflow x1 sides):
@@ -39,7 +36,6 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
                                                double* __restrict__ psv = nullptr,
                                                long long pc = 0) {
   constexpr int N = 5 + NM, IVX = 1 + NM, IEN = 4 + NM;
-  const int rv_ = 1 + th.iv, rc_ = 1 + th.ic, rp_ = 1 + kz.ip;
   const double cvp = th.cvd * (1. + th.cv_ratio_m1[kz.ip]);
   double cf = dt * kz.v_fall / g.dx1;
   if (cf > 1.) cf = 1.;
@@ -48,7 +44,7 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
   const int is = g.is, ie = g.ie;
   const double grav = g.grav, dx = g.dx1;
   double pf_next = 0., pf_top = 0.;
-  double in_p = 0., in_m0 = 0., in_m1 = 0., in_m2 = 0., in_e = 0.;
+  double fin[5] = {0., 0., 0., 0., 0.};
   // The next level's u is loaded a level ahead, and a level's stores are
   // issued at the next level, together with that load, once the level's own
   // data have been used: a wait for loaded data also waits for every store
@@ -117,55 +113,8 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
     // level a reload of level is, unused)
     if (i < ie) flush(pend);
     load_u(max(i - 1, is));
-    const double T = temperature_d<NM>(th, wc);
-    // species picked by runtime index from registers: select chains
-    double rc = 0., rp = 0., rv = 0.;
-#pragma unroll
-    for (int n = 1; n <= NM; ++n) {
-      if (n == rc_) rc = uc[n];
-      if (n == rp_) rp = uc[n];
-      if (n == rv_) rv = uc[n];
-    }
-    double a = dt * (kz.k_auto + kz.k_accr * (rp / rho)) * rc;
-    if (a > rc) a = rc;
-    rc = rc - a;
-    rp = rp + a;
-    const double rs = qsat_d(th, T, rho) * rho;
-    const double def = rs - rv;
-    double b = 0.;
-    if (def > 0. && rp > 0.) {
-      b = dt * kz.k_evap * rp * (def / rs);
-      if (b > rp) b = rp;
-      if (b > def) b = def;
-    }
-    rp = rp - b;
-    rv = rv + b;
-#pragma unroll
-    for (int n = 1; n <= NM; ++n) {
-      if (n == rc_) uc[n] = rc;
-      if (n == rp_) uc[n] = rp;
-      if (n == rv_) uc[n] = rv;
-    }
-    uc[IEN] = uc[IEN] - th.L * b;
-    double out_p = 0., out_m0 = 0., out_m1 = 0., out_m2 = 0., out_e = 0.;
-    if (i > g.is && cf > 0.) {
-      cons_to_prim_cell<NM>(th, uc, wc);
-      const double T2 = temperature_d<NM>(th, wc);
-      const double v1 = wc[IVX], v2 = wc[IVX + 1], v3 = wc[IVX + 2];
-      out_p = cf * rp;
-      out_m0 = out_p * v1;
-      out_m1 = out_p * v2;
-      out_m2 = out_p * v3;
-      out_e = out_p * (cvp * T2 + 0.5 * (v1 * v1 + v2 * v2 + v3 * v3));
-    }
-#pragma unroll
-    for (int n = 1; n <= NM; ++n)
-      if (n == rp_) uc[n] = (uc[n] - out_p) + in_p;
-    uc[IVX] = (uc[IVX] - out_m0) + in_m0;
-    uc[IVX + 1] = (uc[IVX + 1] - out_m1) + in_m1;
-    uc[IVX + 2] = (uc[IVX + 2] - out_m2) + in_m2;
-    uc[IEN] = (uc[IEN] - out_e) + in_e;
-    cons_to_prim_cell<NM>(th, uc, wc);
+    double fout[5];
+    kessler_cell<NM>(th, kz, dt, cf, cvp, i == is, rho, uc, wc, fin, fout);
     pend.i = i;
 #pragma unroll
     for (int n = 0; n < N; ++n) {
@@ -185,11 +134,8 @@ __device__ __forceinline__ void kessler_column(const KGrid& g, const KThermo& th
       pend.pvi = log_mean(pfi, pf_next);
       pf_next = pfi;
     }
-    in_p = out_p;
-    in_m0 = out_m0;
-    in_m1 = out_m1;
-    in_m2 = out_m2;
-    in_e = out_e;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) fin[n] = fout[n];
   }
   flush(pend);
   if (FUSE && grav != 0.0) {

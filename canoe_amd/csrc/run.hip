@@ -128,13 +128,13 @@ unsigned implicit_grid(const cg_run* r) {
 }
 
 // beta: the stage weight; the sweep's step is beta times the device step.
-template <int NM, int B, int WPB>
+template <int NM, int B, int WPB, bool KES>
 void launch_col_batch(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   hipStream_t s = r->ctx->stream;
   const unsigned gr = implicit_grid(r);
   if (r->tune.implicit_fused) {
     TIMED(tm, KID_IMPLICIT_FUSED,
-          (k_implicit_fused_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+          (k_implicit_fused_batch<NM, B, WPB, KES><<<gr, 64 * WPB, 0, s>>>(
               r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, up, r->arch,
               r->ctx->d_status)));
     return;
@@ -144,22 +144,35 @@ void launch_col_batch(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
             r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, r->arch,
             r->ctx->d_status)));
   TIMED(tm, KID_IMPLICIT_BWD,
-        (k_implicit_bwd_update_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+        (k_implicit_bwd_update_batch<NM, B, WPB, KES><<<gr, 64 * WPB, 0, s>>>(
             r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
 }
 
-template <int NM>
-void launch_columns(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
+template <int NM, bool KES>
+void launch_columns_k(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   if (r->cfg.implicit_flag & 8) {
     if (r->tune.implicit_wpb == 4)
-      launch_col_batch<NM, 5, 4>(r, beta, up, tm);
+      launch_col_batch<NM, 5, 4, KES>(r, beta, up, tm);
     else if (r->tune.implicit_wpb == 2)
-      launch_col_batch<NM, 5, 2>(r, beta, up, tm);
+      launch_col_batch<NM, 5, 2, KES>(r, beta, up, tm);
     else
-      launch_col_batch<NM, 5, 1>(r, beta, up, tm);
+      launch_col_batch<NM, 5, 1, KES>(r, beta, up, tm);
   } else {
-    launch_col_batch<NM, 3, 1>(r, beta, up, tm);
+    launch_col_batch<NM, 3, 1, KES>(r, beta, up, tm);
   }
+}
+
+// kes: the sweep also applies the Kessler microphysics (up.kz, up.kdt,
+// up.kdtp set; NM = 3 only)
+template <int NM>
+void launch_columns(cg_run* r, double beta, const KUpdate& up, KTimer* tm, bool kes) {
+  if constexpr (NM == 3) {
+    if (kes) {
+      launch_columns_k<NM, true>(r, beta, up, tm);
+      return;
+    }
+  }
+  launch_columns_k<NM, false>(r, beta, up, tm);
 }
 
 // Stage s of a step whose size the kernels read from r->d_dt[0].
@@ -239,12 +252,25 @@ int run_stage(cg_run* r, int s) {
   // precipitation species)
   const bool kes = (s == 2) && r->kes && NM == 3;
   if (imp) {
-    launch_columns<NM>(r, sbeta[s], up, tm);
+    // with the adjustment in the update (not the guided pass) and every
+    // block's x1 sides walls or outflow, the downward sweep applies the
+    // microphysics to each new cell state before it fills the x1 ghosts and
+    // the hydrostatic pressures
+    bool kcol = kes && !guided;
+    for (const RunBlock& b : r->blocks)
+      kcol = kcol && b.g.bc[0] != CG_BC_PERIODIC && b.g.bc[1] != CG_BC_PERIODIC;
+    if (kcol) {
+      up.kz = r->kz;
+      up.kdt = 1.0;
+      up.kdtp = r->d_dt;
+    }
+    launch_columns<NM>(r, sbeta[s], up, tm, kcol);
     for (RunBlock& b : r->blocks) {
       const KGrid& g = b.g;
       if (guided)
         TIMED(tm, KID_SATADJ,
               (launch_satadj_run_mlp(NM, g, r->kt, r->mlp, b.u, b.w, r->ctx->d_status, st)));
+      if (kcol) continue;
       // the Kessler sweep also refills the x1 ghosts and the hydrostatic
       // pressures where the x1 sides are walls or outflow
       const bool kfill = kes && g.bc[0] != CG_BC_PERIODIC && g.bc[1] != CG_BC_PERIODIC;
