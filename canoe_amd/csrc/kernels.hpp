@@ -1494,13 +1494,15 @@ __global__ void __launch_bounds__(64 * WPB)
   }
 }
 
-template <int NM, int B, int WPB, bool KES = false>
-__global__ void __launch_bounds__(64 * WPB)
-    k_implicit_bwd_update_batch(const KBlockPtr* __restrict__ bp,
-                                const KGroup* __restrict__ grp, int ngroups,
-                                KThermo th, KUpdate up,
-                                const double* __restrict__ arch,
-                                int* __restrict__ status) {
+// The downward sweeps of the groups taken by one wave (KES: with the
+// Kessler microphysics of the last stage).
+template <int NM, int B, int WPB, bool KES>
+__device__ __forceinline__ void bwd_update_groups(const KBlockPtr* __restrict__ bp,
+                                                  const KGroup* __restrict__ grp,
+                                                  int ngroups, const KThermo& th,
+                                                  const KUpdate& up,
+                                                  const double* __restrict__ arch,
+                                                  int* __restrict__ status) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int q = blockIdx.x * WPB + wv; q < ngroups; q += gridDim.x * WPB) {
@@ -1513,25 +1515,45 @@ __global__ void __launch_bounds__(64 * WPB)
       const int j = g.js + (int)(col % g.nx2), k = g.ks + (int)(col / g.nx2);
       const KBlockPtr& p = bp[b];
       bwd_update_column<NM, B, KES>(g, th, up, arch + aoff, (long long)g.nx2 * g.nx3,
-                               col, gptr(p.du),
-                               gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
-                               gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
-                               status);
+                                    col, gptr(p.du),
+                                    gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
+                                    gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
+                                    status);
     }
   }
+}
+
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_bwd_update_batch(const KBlockPtr* __restrict__ bp,
+                                const KGroup* __restrict__ grp, int ngroups,
+                                KThermo th, KUpdate up,
+                                const double* __restrict__ arch,
+                                int* __restrict__ status) {
+  bwd_update_groups<NM, B, WPB, false>(bp, grp, ngroups, th, up, arch, status);
+}
+
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_bwd_update_kessler_batch(const KBlockPtr* __restrict__ bp,
+                                        const KGroup* __restrict__ grp, int ngroups,
+                                        KThermo th, KUpdate up,
+                                        const double* __restrict__ arch,
+                                        int* __restrict__ status) {
+  bwd_update_groups<NM, B, WPB, true>(bp, grp, ngroups, th, up, arch, status);
 }
 
 // Forward sweep and downward sweep of each group back to back in one wave:
 // the columns of a group depend on no other group within a stage, and the
 // top levels' archive, written last by the forward sweep, is read first by
 // the downward sweep while still in the caches.
-template <int NM, int B, int WPB, bool KES = false>
-__global__ void __launch_bounds__(64 * WPB)
-    k_implicit_fused_batch(const KBlockPtr* __restrict__ bp,
-                           const KGroup* __restrict__ grp, int ngroups,
-                           KThermo th, double dt, const double* __restrict__ dtp,
-                           KUpdate up, double* __restrict__ arch,
-                           int* __restrict__ status) {
+template <int NM, int B, int WPB, bool KES>
+__device__ __forceinline__ void fused_groups(const KBlockPtr* __restrict__ bp,
+                                             const KGroup* __restrict__ grp, int ngroups,
+                                             const KThermo& th, double dt,
+                                             const double* __restrict__ dtp,
+                                             const KUpdate& up, double* __restrict__ arch,
+                                             int* __restrict__ status) {
   if (dtp) dt = dt * dtp[0];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1549,11 +1571,32 @@ __global__ void __launch_bounds__(64 * WPB)
                              arch + aoff, ncol, col, gptr(p.psv), col * g.ps_c))
         atomicOr(status, 1);
       bwd_update_column<NM, B, KES>(g, th, up, arch + aoff, ncol, col, gptr(p.du),
-                               gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
-                               gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
-                               status);
+                                    gptr(p.u), gptr(p.u1), gptr(p.w), gptr(p.psf),
+                                    gptr(p.psv), g.at(0, k, j, 0), col * g.ps_c,
+                                    status);
     }
   }
+}
+
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_fused_batch(const KBlockPtr* __restrict__ bp,
+                           const KGroup* __restrict__ grp, int ngroups,
+                           KThermo th, double dt, const double* __restrict__ dtp,
+                           KUpdate up, double* __restrict__ arch,
+                           int* __restrict__ status) {
+  fused_groups<NM, B, WPB, false>(bp, grp, ngroups, th, dt, dtp, up, arch, status);
+}
+
+// The same with the Kessler microphysics in the downward sweep (last stage).
+template <int NM, int B, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_implicit_fused_kessler_batch(const KBlockPtr* __restrict__ bp,
+                                   const KGroup* __restrict__ grp, int ngroups,
+                                   KThermo th, double dt, const double* __restrict__ dtp,
+                                   KUpdate up, double* __restrict__ arch,
+                                   int* __restrict__ status) {
+  fused_groups<NM, B, WPB, true>(bp, grp, ngroups, th, dt, dtp, up, arch, status);
 }
 
 // Streaming copy of n 16-byte words (cg_memcpy_d2d): grid-stride loop.

@@ -133,19 +133,30 @@ void launch_col_batch(cg_run* r, double beta, const KUpdate& up, KTimer* tm) {
   hipStream_t s = r->ctx->stream;
   const unsigned gr = implicit_grid(r);
   if (r->tune.implicit_fused) {
-    TIMED(tm, KID_IMPLICIT_FUSED,
-          (k_implicit_fused_batch<NM, B, WPB, KES><<<gr, 64 * WPB, 0, s>>>(
-              r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, up, r->arch,
-              r->ctx->d_status)));
+    if constexpr (KES)
+      TIMED(tm, KID_IMPLICIT_FUSED,
+            (k_implicit_fused_kessler_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+                r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, up, r->arch,
+                r->ctx->d_status)));
+    else
+      TIMED(tm, KID_IMPLICIT_FUSED,
+            (k_implicit_fused_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+                r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, up, r->arch,
+                r->ctx->d_status)));
     return;
   }
   TIMED(tm, KID_IMPLICIT_FWD,
         (k_implicit_fwd_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
             r->d_bp, r->d_grp, r->ngroups, r->kt, beta, r->d_dt, r->arch,
             r->ctx->d_status)));
-  TIMED(tm, KID_IMPLICIT_BWD,
-        (k_implicit_bwd_update_batch<NM, B, WPB, KES><<<gr, 64 * WPB, 0, s>>>(
-            r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
+  if constexpr (KES)
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd_update_kessler_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+              r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
+  else
+    TIMED(tm, KID_IMPLICIT_BWD,
+          (k_implicit_bwd_update_batch<NM, B, WPB><<<gr, 64 * WPB, 0, s>>>(
+              r->d_bp, r->d_grp, r->ngroups, r->kt, up, r->arch, r->ctx->d_status)));
 }
 
 template <int NM, bool KES>

@@ -1,8 +1,10 @@
 """GPU parity of the Kessler microphysics and precipitation sedimentation
-(k_kessler): cg_microphysics on caller arrays and moist runs that apply it
-after the last stage equal the oracle bit for bit (FP64 form, graph and
-direct, with and without the implicit x1 solve); the FP32-flux form stays
-within 1e-5 of the FP64 form; water is conserved by the operator."""
+(k_kessler, and the last stage's downward sweep of implicit runs):
+cg_microphysics on caller arrays and moist runs that apply it after the last
+stage equal the oracle (FP64 form, graph and direct, with and without the
+implicit x1 solve); the sweep's operator equals the column kernel's bit for
+bit; the FP32-flux form stays within 1e-5 of the FP64 form; water is
+conserved by the operator."""
 import numpy as np
 import pytest
 
@@ -119,6 +121,47 @@ def test_run_with_kessler_mixed_close_to_fp64(ctx):
     e = relerr(out[1][:, sk, sj, si], out[0][:, sk, sj, si])
     _say("kessler mixed vs fp64 relerr %.3e" % e)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("prec", [ca.PREC_FP64, ca.PREC_FP32])
+@pytest.mark.parametrize("implicit_fused", [1, 0])
+def test_sweep_microphysics_equals_the_column_operator(ctx, prec, implicit_fused):
+    """Implicit runs with wall x1 sides apply the operator per cell inside
+    the last stage's downward sweep: one step with it equals, bit for bit,
+    one step without it followed by cg_microphysics (the separate column
+    kernel) on that step's state, in both flux forms and with the forward
+    and downward sweeps in one launch or in two."""
+    blk, th, cfg, w = _moist()
+    cfg.precision = prec
+    dt = cfl_dt(blk, th, cfg, w, 0.5)
+    _say("sweep microphysics prec=%d fused=%d dt=%.4f" % (prec, implicit_fused, dt))
+    outs = []
+    for on in (True, False):
+        for k, v in RATES.items():
+            setattr(cfg, k, v if on else 0.0)
+        run = ca.Run(ctx, blk, th, cfg)
+        try:
+            run.set_tune(implicit_fused=implicit_fused)
+            run.set_prim(w)
+            run.step(dt, 1, graph=False)
+            outs.append((run.cons(), run.prim()))
+        finally:
+            run.close()
+    for k, v in RATES.items():
+        setattr(cfg, k, v)
+    du, dw = ctx.array(outs[1][0]), ctx.array(outs[1][1])
+    try:
+        assert ca.lib().cg_microphysics(ctx.h, ca._ref(blk), ca._ref(th),
+                                        ca._ref(cfg), dt, du.ptr, dw.ptr) == ca.OK
+        ug, wg = du.download(), dw.download()
+    finally:
+        du.free()
+        dw.free()
+    sk, sj, si = blk.interior()
+    np.testing.assert_array_equal(outs[0][0][:, sk, sj, si], ug[:, sk, sj, si])
+    np.testing.assert_array_equal(outs[0][1][:, sk, sj, si], wg[:, sk, sj, si])
+    # the operator changed the state
+    assert not np.array_equal(outs[0][0], outs[1][0])
 
 
 @pytest.mark.parametrize("bcs", [(ca.BC_REFLECT, ca.BC_OUTFLOW), (ca.BC_OUTFLOW, ca.BC_REFLECT)])
