@@ -9,6 +9,10 @@ the oracle is too slow to compare cell by cell.
   wrong tile, group or halo index breaks it.
   With the periodic x2 / x3 implicit lines (implicit_flag 15) a shifted
   line starts its cyclic recurrence at another cell: equal to round-off.
+* The cubed sphere (six 96 x 96 panels of 64 levels) from a state uniform on
+  each level and at rest: the four equatorial panels, which a quarter turn
+  about the polar axis maps onto each other in the same local coordinates,
+  stay equal to round-off, and the total mass is conserved.
 * Conservation: mass to round-off with reflecting walls and periodic sides;
   the moist box keeps its total water through transport and the saturation
   adjustment (the implicit correction's species split does not: by the
@@ -18,6 +22,7 @@ import numpy as np
 import pytest
 
 import canoe_amd as ca
+import oracle_lib as orl
 from helpers import cfl_dt, relerr
 
 pytestmark = pytest.mark.gpu
@@ -199,3 +204,59 @@ def test_full_moist_box_kessler_translation_and_mass(ctx, prec):
     assert abs(m1 / m0 - 1) < 1e-13
     # the operator acted: precipitation changed by more than round-off
     assert abs(s1[3] / s0[3] - 1) > 1e-6
+
+
+# equatorial panels (0-based blocks) of GetLatLon (cubed_sphere.cpp:78-89)
+EQUATORIAL = (1, 3, 5, 2)
+
+
+@pytest.mark.parametrize("prec", [ca.PREC_FP64, ca.PREC_FP32])
+def test_full_sphere_longitude_symmetry_and_mass(ctx, prec):
+    """Config 3 at its size (3 x 3 implicit x1 columns, Held-Suarez forcing,
+    rotation), 5 steps of 60 s from each level's mean state at rest: the
+    forcing and the Coriolis parameter depend on latitude only, so the
+    equatorial panels stay equal to each other.  Not bit for bit: the polar
+    panels, whose faces in the two horizontal directions run different
+    code, feed them ghosts that agree only to round-off.  FP64: density and
+    energy within 1e-12 of their maxima, momenta within 1e-12 of rho c;
+    total mass to 1e-13.  FP32 faces: density and energy within 1e-7, and
+    mass to 1e-8 (each panel computes the flux of a panel edge in its own
+    frame, as the reference does, so the two sides of an edge agree to FP32
+    rounding; measured 2.7e-10 after 5 steps)."""
+    blk, th, cfg = ca.pgen_setup(3)
+    cfg.precision = prec
+    sp = ca.pgen_sphere()
+    w = ca.pgen_sphere_init(1, blk, th)
+    sk, sj, si = blk.interior()
+    for n in range(w.shape[1]):
+        lvl = w[0, n][sk, sj, si].mean(axis=(0, 1))
+        w[:, n][:, sk, sj, si] = 0.0 if n in (1, 2, 3) else lvl
+    _say("full sphere 6x96x96x64 prec=%d" % prec)
+    run = ca.SphereRun(ctx, blk, th, cfg, sp)
+    try:
+        run.set_all(w)
+        u0 = run.cons_all()
+        run.step(60.0, 5, graph=True)
+        u1 = run.cons_all()
+    finally:
+        run.close()
+    I = (slice(None), slice(None), sk, sj, si)
+    assert np.all(np.isfinite(u1[I]))
+    vol = np.stack([orl.sph_metric(blk, p, 12) for p in range(1, 7)])
+    m0 = (u0[:, 0][:, sk, sj, si] * vol).sum()
+    m1 = (u1[:, 0][:, sk, sj, si] * vol).sum()
+    ref = u1[EQUATORIAL[0]][:, sk, sj, si]
+    rho_c = np.abs(ref[0]).max() * 340.0
+    scale = [np.abs(ref[0]).max(), rho_c, rho_c, rho_c, np.abs(ref[4]).max()]
+    worst = np.zeros(5)
+    for p in EQUATORIAL[1:]:
+        d = np.abs(u1[p][:, sk, sj, si] - ref).max(axis=(1, 2, 3))
+        worst = np.maximum(worst, d / scale)
+    _say("  mass %.3e; panel differences %s" % (m1 / m0 - 1,
+                                                " ".join("%.2e" % x for x in worst)))
+    if prec == ca.PREC_FP64:
+        assert abs(m1 / m0 - 1) < 1e-13
+        assert worst.max() < 1e-12
+    else:
+        assert abs(m1 / m0 - 1) < 1e-8
+        assert max(worst[0], worst[4]) < 1e-7
